@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain HIP closed search (spectra/s) on one MI355X.
+
+Workload (default): the proteome-scale plain search -- 20,000 synthetic proteins, 1 missed cleavage, Met-ox <= 2,
+reversed decoys (~3.1M peptidoforms), 100,000 synthetic spectra (charges 2/3/4, b/y 2+ for z >= 3), precursor
++-10 ppm, fragment +-0.02 Th, top-K 50 + pair closure. A step = one pass of the search hot path over the whole
+batch with the raw spectra already in HBM: K0 peak view -> hypotheses -> K1 (index) + top-K + closure -> K2
+exact rescoring -> rank 1. The index build is timed separately and not part of a step.
+
+Prints ONE JSON line (rank 0) with the metric, per-kernel HIP-event times, the roofline of the dominant kernel
+(algorithmic bytes per launch / its average duration) and the CPU baseline (the C++ oracle's search, OpenMP, on a
+bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vulcan-proteomics_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "spectra/s (index+rescore) plain vs learned prefilter at ≥99.5% top-1 agreement"
+HBM_PEAK_GBS = 8000.0
+STATIC = {"C": 57.021464}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(ds, gi, n, l1):
+    """Per-launch algorithmic HBM bytes of K1 (fused top-K) and K2 on this batch.
+    K1: sum_h [4*matched_h + 8*peaks_s(h)*(maxfz_h - 1)] + 4*peaks_s + 12*n_cand_s + 29 per hypothesis
+        (postings that must be read, a bin-bound pair per (peak, z) lookup, peak masses, the candidate list,
+        hypothesis window/charge in, scored/matched out).
+    K2: sum_c (21 + L_c) + 8*peaks_s + 37 per candidate (seq_off 8 + len 1 + mask 8 + mass32 4 + residues L,
+        peak mass+intensity, candidate row/hypothesis/maxfz in, 28-byte result out)."""
+    import numpy as np
+    hyp_off = ds.hyp["hyp_off"][: n + 1].cpu().numpy()
+    nh = int(hyp_off[-1])
+    pk_off = ds.peaks["off"][: n + 1].cpu().numpy()
+    npk = np.diff(pk_off)
+    hspec = ds.hyp["spec"][:nh].cpu().numpy()
+    mfz = ds.hyp["maxfz"][:nh].cpu().numpy().astype(np.int64)
+    matched = ds.matched[:nh].cpu().numpy().astype(np.int64)
+    nc = ds.n_cand[:n].cpu().numpy().astype(np.int64)
+    k1 = int((4 * matched + 8 * npk[hspec] * (mfz - 1)).sum() + 4 * npk.sum() + 12 * nc.sum() + 29 * nh)
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    rows = ds.c_row[: n * ds.stride].cpu().numpy()[m]
+    k2 = int((21 + l1.length[rows].astype(np.int64)).sum() + 8 * npk[nc > 0].sum() + 37 * len(rows))
+    return {"k1_topk": k1, "k2": k2, "n_hyp": nh, "n_rescored": int(len(rows)), "matched": int(matched.sum())}
+
+
+def cpu_baseline(vp, l1, sp, target_s: float):
+    """The oracle's whole-search (OpenMP over spectra) on a bounded prefix of the same spectra."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from vulcan.gpu.host import Spectra
+    hix = O.HostIndex(l1)
+
+    def take(n):
+        off = sp.offsets[: n + 1]
+        return Spectra(sp.run, sp.scan[:n], sp.index[:n], sp.rt[:n], sp.precursor_mz[:n], sp.charge[:n],
+                       sp.mz[: off[-1]], sp.intensity[: off[-1]], off.copy())
+    n0 = min(2000, len(sp))
+    t0 = time.perf_counter()
+    O.search(vp, hix, take(n0))
+    dt = time.perf_counter() - t0
+    n = int(min(len(sp), max(n0, n0 * target_s / max(dt, 1e-6))))
+    sub = take(n)
+    t0 = time.perf_counter()
+    res = O.search(vp, hix, sub)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "spectra/s", "cores": int(O.lib().or_max_threads()), "kind": "port",
+            "sample": f"first {n} of {len(sp)} spectra of the same workload, {dt:.1f} s, OpenMP C++ oracle "
+                      f"(K0, hypotheses, per-row K1, top-K + closure, K2, rank 1)",
+            "cpu_model": _cpu_model()}, res, n
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cfg", type=int, default=1)
+    ap.add_argument("--n-spectra", type=int, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the rank-1 comparison with the CPU sample")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run the u32 copy kernel on this many MiB (HBM counter calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU sharding "
+              f"design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from vulcan.gpu import _lib as K
+    from vulcan.gpu import engine as E
+    from vulcan.index import synth
+    dev = torch.device("cuda:0")
+    if a.calib_copy_mib:
+        n = a.calib_copy_mib * 1024 * 1024 // 4
+        src = torch.ones(n, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            K.check(K.lib().vh_copy_u32(K.ptr(src), K.ptr(dst), n, K.stream()), "copy")
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_each_way": n * 4, "launches": 5}))
+        return
+    w = synth.WORKLOADS[a.cfg]
+    t0 = time.perf_counter()
+    l1, lab = synth.build_index(a.cfg)
+    log(f"table: {len(l1)} peptidoforms ({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    sp, truth, _ = synth.spectra(l1, lab, a.cfg, n=a.n_spectra)
+    n = len(sp)
+    log(f"spectra: {n} ({time.perf_counter() - t0:.1f} s)")
+    p = w.search_params()
+    gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
+    log(f"index: {gi.n_postings} postings, {gi.n_buckets} buckets x {gi.n_bins} bins, build {gi.build_s:.3f} s")
+    raw = E.upload_spectra(sp, dev)
+    ds = E.DeviceSearch(gi, p, n)
+    for _ in range(a.warmup):
+        ds.run(raw)
+    torch.cuda.synchronize()
+    log("warmup done")
+    ev: dict = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ds.run(raw, ev)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"timed {a.steps} steps: {wall:.3f} s")
+    stage_ms = {}
+    for k, lst in ev.items():
+        pairs = list(zip(lst[0::2], lst[1::2]))
+        stage_ms[k] = float(np.mean([s.elapsed_time(e) for s, e in pairs]))
+    if int(ds.status.item()) != 0:
+        raise RuntimeError(f"k1_topk status {int(ds.status.item())}")
+    sel = ds.sel[:n].cpu().numpy()
+    rows = ds.c_row[: n * ds.stride].cpu().numpy()
+    best = np.where(sel >= 0, rows[np.maximum(sel, 0)], -1)
+    sig = truth >= 0
+    recovery = float(np.mean(best[sig] == truth[sig]))
+    ab = algorithmic_bytes(ds, gi, n, l1)
+    dom = max(("k1_topk", "k2"), key=lambda k: stage_ms[k])
+    achieved = ab[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{dom}_cfg{a.cfg}_traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        traffic = tj.get("hbm_bytes_per_launch")
+    value = n * a.steps / wall
+    out = {"metric": METRIC, "value": value, "unit": "spectra/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": w.name, "n_spectra": n, "n_peptidoforms": len(l1), "n_postings": gi.n_postings,
+                      "topk": p.topk, "pair_closure": True, "precursor_ppm": list(p.precursor_ppm),
+                      "fragment_da": list(p.fragment_ppm), "max_fragment_charge": p.max_fragment_charge,
+                      "parallelism": "single-gpu"},
+           "stage_ms": stage_ms, "index_build_s": gi.build_s,
+           "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                        "algorithmic_bytes_per_launch": ab[dom]},
+           "counts": ab, "top1_truth_recovery": recovery}
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        cb, res, ncpu = cpu_baseline(ds.vp, l1, sp, a.cpu_seconds)
+        out["cpu_baseline"] = cb
+        if not a.no_check:
+            out["rank1_equal_cpu_sample"] = bool(np.array_equal(best[:ncpu], res["row"][:ncpu]))
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

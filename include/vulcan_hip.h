@@ -1,0 +1,180 @@
+/* C ABI of libvulcan_hip.so: the MI355X (gfx950) DDA search hot path.
+ *
+ * Drop-in boundary for the reference's ctypes surface vulcan/gpu/_lib.py:22-35 (launchers in
+ * vulcan/gpu/kernels.cu:379-474). Differences by design:
+ *   - no process-global state: the residue tables, N-term delta and tolerance unit that the reference sets
+ *     with vk_set_params / vk_set_nterm / vk_set_tol_da (kernels.cu:381-389, cudaMemcpyToSymbol) travel in a
+ *     VhParams passed by pointer to every call, so concurrent callers with different settings are safe;
+ *   - every buffer is caller-owned device memory (torch tensors in the Python layer); scratch memory comes
+ *     from a caller workspace whose size a *_workspace_bytes query returns; nothing here allocates;
+ *   - every call is asynchronous on the given stream (a hipStream_t passed as void*) and never synchronizes,
+ *     so a whole search step can be captured into a hipGraph;
+ *   - return value: 0 = ok, > 0 = hipError_t, < 0 = VH_E_* argument/overflow error (reference: cudaError_t,
+ *     _lib.py:74-76 turns nonzero into RuntimeError).
+ * u32/u64 data travel as raw bit patterns (the reference passes u32 as int32 tensors, engine.py:38-41).
+ */
+#ifndef VULCAN_HIP_H
+#define VULCAN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VH_ABI_VERSION 3
+
+#define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
+#define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
+#define VH_E_WORKSPACE (-3)  /* workspace smaller than the *_workspace_bytes query */
+#define VH_E_LIMIT (-4)      /* a documented size limit is exceeded (rows, bins, hypotheses per spectrum) */
+
+#define VH_MAX_HYP_PER_SPEC 64   /* hypotheses of one spectrum handled by one K1 wave */
+#define VH_K2_MAX_PEAKS 512      /* K2 looks at the first 512 mass-sorted peaks (kernels.cu:260, 278) */
+#define VH_MAX_PEPTIDE_LEN 63    /* residues per peptidoform (mask bits 0..62; bit 63 = N-term mod) */
+
+/* Search parameters. Reference: SearchParams (vulcan/gpu/host.py:15-44) plus the constant-memory state of
+ * kernels.cu:19-38 (FragParams, c_nterm, c_tol_da). */
+typedef struct VhParams {
+  float mono[26];             /* Sage f32 residue masses by letter (engine.py:30-32) */
+  float stat[26];             /* static mod per residue, f32 (engine.py:74-76) */
+  float var[26];              /* variable mod per residue, f32 (engine.py:78-82) */
+  float nterm;                /* variable peptide-N-term delta, mask bit 63 (kernels.cu:24-30) */
+  int32_t tol_da;             /* fragment unit: 0 ppm, 1 Da (kernels.cu:31-38) */
+  float tol_lo, tol_hi;       /* fragment tolerance bounds (fragment_ppm) */
+  float fmin, fmax;           /* fragment_min_mz / fragment_max_mz */
+  int32_t min_ion_index;      /* index filter of K1 (kernels.cu:67-68) */
+  float prec_lo, prec_hi;     /* precursor_ppm */
+  int32_t iso_lo, iso_hi;     /* isotope_errors */
+  int32_t z_lo, z_hi;         /* precursor_charge (tried when the spectrum has none) */
+  int32_t max_fragment_charge;/* 0 = None (fragment charges up to z-1) */
+  int32_t max_peaks;          /* K0 top peaks per spectrum */
+  int32_t min_peaks;          /* K0 validity */
+  int32_t min_matched_peaks;  /* ranking filter (engine.py:368) */
+  int32_t topk;               /* K1 top-K per spectrum (engine.py:291-296) */
+  int32_t k1_threshold;       /* K1 append threshold (kernels.cu:171) */
+  int32_t pair_closure;       /* add partner rows (engine.py:297-301) */
+  int32_t tile;               /* K1 LDS histogram slots per wave: 256..4096, power of two */
+  int32_t reserved[8];
+} VhParams;
+
+/* Device view of the peptidoform table (vulcan/index/build.py:13-21) and the fragment index.
+ * Fragment index: rows are cut into buckets of 2^bucket_log2 consecutive rows (the table is sorted by
+ * precursor mass, so a bucket is a precursor-mass interval); inside a bucket, postings are grouped by
+ * fragment bin = (f32 bit pattern of the fragment mass >> bin_shift) - bin_base and, inside a bin, ordered
+ * by row. offsets[bucket * n_bins + bin] .. offsets[... + 1] delimit a (bucket, bin) posting list.
+ * A posting is (row - bucket start) << bin_shift | (f32 bits & (2^bin_shift - 1)), i.e. it holds the exact
+ * f32 fragment mass, so index lookups compare exactly what Sage compares (no lossy quantisation). */
+typedef struct VhIndexView {
+  int64_t n_rows;
+  const int64_t* seq_off;
+  const uint8_t* length;
+  const uint64_t* mask;
+  const float* mass32;
+  const uint8_t* residues;
+  const int64_t* partner;
+  int32_t bucket_log2;
+  int32_t bin_shift;
+  uint32_t bin_base;
+  int32_t n_bins;
+  int64_t n_buckets;
+  const uint32_t* offsets;
+  const uint32_t* entries;
+  int64_t n_postings;
+} VhIndexView;
+
+/* ---------------------------------------------------------------------------------------------- misc */
+int vh_version(void);
+/* sizeof(VhParams) (which = 0) or sizeof(VhIndexView) (which = 1), to check a binding's struct layout. */
+size_t vh_abi_sizeof(int which);
+/* Scratch bytes of the prefix scans used inside vh_k0 / vh_hypotheses / vh_frag_count for n items. */
+size_t vh_scan_workspace_bytes(int64_t n);
+/* Plain copy kernel (n u32 words), used to calibrate HBM byte counters. No reference counterpart. */
+int vh_copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, void* stream);
+
+/* ---------------------------------------------------------------------------------------------- index */
+/* Per-row indexed-fragment count, then inclusive scan: post_off[0] = 0, post_off[i+1] = sum of counts of
+ * rows 0..i. Replaces vk_frag_count (kernels.cu:53-71, 391-399) and the cumsum of engine.py:109-128. */
+int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, void* ws, size_t ws_bytes,
+                  void* stream);
+/* Scratch bytes of vh_index_build for n_postings keys. */
+size_t vh_index_workspace_bytes(int64_t n_postings, int64_t n_buckets, int32_t n_bins);
+/* Emit 64-bit keys ((bucket * n_bins + bin) << 32 | posting), LSD radix sort them on the device, write
+ * ix->entries (n_postings u32) and ix->offsets (n_buckets * n_bins + 1 u32). Replaces vk_frag_emit
+ * (kernels.cu:73-101, 401-411) and the torch.sort / bincount / cumsum CSR assembly of engine.py:129-148. */
+int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
+                   uint32_t* entries, void* ws, size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------------- K0 */
+/* Peak view (engine.py:171-210): keep fmin <= m/z <= fmax (not NaN), mass = m/z - PROTON (f32), top
+ * max_peaks by intensity desc (ties: mass desc, then input order), then mass asc; TIC = f64 sum -> f32;
+ * valid = kept >= min_peaks. Output is CSR: pk_off (n_spec+1); capacity = pk_capacity peaks. */
+int vh_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* intensity, const int64_t* raw_off,
+          int64_t* pk_off, float* pk_mass, float* pk_int, float* tic, uint8_t* valid, int64_t pk_capacity,
+          void* ws, size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------------- hypotheses */
+/* Precursor hypotheses (host.py:64-91) on the device: order (spectrum, charge, isotope), f32 arithmetic
+ * mz = pmz - PROTON, pm = mz*z, M = pm - iso*NEUTRON, window = M + M*ppm/1e6, rows = searchsorted left/right
+ * on mass32; maxfz = max(min(z, max_fragment_charge+1), 2). hyp_off (n_spec+1) groups them by spectrum. */
+int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz, const int8_t* charge,
+                  const uint8_t* valid, const float* mass32, int64_t n_rows, int64_t* hyp_off, int32_t* h_spec,
+                  int8_t* h_z, int8_t* h_iso, float* h_pm, float* h_M, int64_t* h_lo, int64_t* h_hi,
+                  int8_t* h_maxfz, int64_t hyp_capacity, void* ws, size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------------- K1 */
+/* Candidate generation, append form (run_k1, engine.py:250-281; k1_kernel, kernels.cu:112-190): for every
+ * hypothesis h and row r of its window, count(h, r) = #(indexed fragment of r, peak, z < maxfz) with
+ * tol_bound(peak*z, lo) <= fragment <= tol_bound(peak*z, hi); append (h, r, count) when count >= threshold
+ * (unordered; *n_out may exceed cap, then only cap entries are written); hyp_scored[h] = #rows with
+ * count > 0, hyp_matched[h] = sum of counts (written, not accumulated). One wave64 per spectrum. */
+int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off, const float* pk_mass,
+          const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo, const int64_t* h_hi, int64_t cap,
+          unsigned long long* n_out, int32_t* out_hyp, int32_t* out_row, int32_t* out_cnt, uint32_t* hyp_scored,
+          unsigned long long* hyp_matched, void* stream);
+
+/* Fused candidate generation + top-K + pair closure (run_k1 + select_topk, engine.py:250-312): per spectrum,
+ * the top-K (count desc, row desc, hypothesis asc) of its appended entries, their partner rows (count -1),
+ * deduplicated by row (lowest hypothesis, highest count), written in row order at s*cand_stride.
+ * n_cand[s] = number of candidates (<= cand_stride). status[0] is set nonzero if a spectrum exceeded a
+ * limit (its list is then empty). */
+int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off,
+               const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo,
+               const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp,
+               int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
+               void* stream);
+
+/* Brute-force counts for given (hypothesis, row) pairs without the index (k1_brute_kernel,
+ * kernels.cu:195-232): the definition K1 must equal. */
+int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32_t* c_hyp, const int32_t* c_row,
+                const int32_t* h_spec, const int8_t* h_maxfz, const int64_t* pk_off, const float* pk_mass,
+                int32_t* out_cnt, void* stream);
+
+/* ---------------------------------------------------------------------------------------------- K2 */
+/* Exact b/y rescoring (k2_kernel, kernels.cu:263-376): all b_k / y_k (k = 0..L-2) at fragment charges
+ * 1..maxfz-1 against the spectrum's first 512 mass-sorted peaks; most intense peak in the tolerance window
+ * (ties: the later peak); matched counts, summed intensities, intensity-weighted ppm, longest consecutive
+ * ladders; hyperscore = ln((sb+1)(sy+1)) + lnfact(mb) + lnfact(my) (255 if not finite).
+ * Groups of candidates share a spectrum: group g has spectrum grp_spec[g] (NULL: g), first candidate
+ * grp_start[g] (NULL: g*stride) and grp_count[g] candidates. maxfz of a candidate = h_maxfz[c_hyp[c]].
+ * c_shift (NULL = closed search) = observed - candidate mass: shifted + unshifted ions with the delta placed
+ * on the best residue (vk_k2_shift, kernels.cu:291-363, 461-472). One wave64 per group. */
+int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* grp_spec,
+          const int64_t* grp_start, const int32_t* grp_count, int32_t stride, const int32_t* c_row,
+          const int32_t* c_hyp, const int8_t* h_maxfz, const float* c_shift, const int64_t* pk_off,
+          const float* pk_mass, const float* pk_int, double* o_hs, float* o_sb, float* o_sy, float* o_ppm,
+          int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, void* stream);
+
+/* ---------------------------------------------------------------------------------------------- ranking */
+/* PSM ranking (psm_table, engine.py:366-385): per group, candidates with mb + my >= min_matched_peaks
+ * ordered by (hyperscore desc, row asc); o_sel[g*report + i] = candidate index of rank i+1 (-1 if none),
+ * o_next[g*report + i] = hyperscore of the following candidate in that order (0 if none). */
+int vh_rank(const VhParams* p, int32_t n_groups, const int64_t* grp_start, const int32_t* grp_count,
+            int32_t stride, const int32_t* c_row, const double* hs, const int16_t* mb, const int16_t* my,
+            int32_t report, int64_t* o_sel, double* o_next, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VULCAN_HIP_H */

@@ -1,0 +1,56 @@
+"""The C-ABI library loads on a CPU-only host and exports every symbol include/vulcan_hip.h declares;
+the ctypes struct mirror matches the C layout (no GPU calls)."""
+import ctypes as C
+import os
+import re
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "vulcan_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|size_t)\s+(vh_\w+)\(", text, flags=re.M)))
+
+
+def test_header_declares_the_hot_path():
+    names = declared_symbols()
+    for need in ("vh_frag_count", "vh_index_build", "vh_k0", "vh_hypotheses", "vh_k1", "vh_k1_topk", "vh_k1_brute",
+                 "vh_k2", "vh_rank", "vh_version", "vh_abi_sizeof"):
+        assert need in names
+
+
+def test_library_exports_every_declared_symbol():
+    from vulcan.gpu import _lib
+    L = _lib.load()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    assert set(declared_symbols()) <= set(_lib._SIGS), "binding lacks a signature"
+
+
+def test_abi_layout_and_version():
+    from vulcan.gpu import _lib
+    from vulcan.gpu._abi import VhIndexView, VhParams
+    L = _lib.load()
+    assert L.vh_abi_sizeof(0) == C.sizeof(VhParams)
+    assert L.vh_abi_sizeof(1) == C.sizeof(VhIndexView)
+    m = re.search(r"#define VH_ABI_VERSION (\d+)", open(HEADER).read())
+    assert L.vh_version() == int(m.group(1))
+
+
+def test_argument_errors_are_reported_without_a_gpu():
+    """Null / inconsistent arguments are rejected on the host before any launch (negative VH_E_* codes)."""
+    from vulcan.gpu import _lib
+    from vulcan.gpu._abi import VhIndexView, make_params
+    from vulcan.gpu.host import SearchParams
+    L = _lib.load()
+    vp = make_params(SearchParams(), {}, {})
+    ix = VhIndexView()
+    assert L.vh_k1_topk(C.byref(vp), C.byref(ix), 1, None, None, None, None, None, None, 2, None, None, None, None,
+                        None, None, None, None) == -1
+    assert L.vh_k2(C.byref(vp), C.byref(ix), 1, None, None, None, 0, None, None, None, None, None, None, None, None,
+                   None, None, None, None, None, None, None, None) == -1
+    vp.max_peaks = 4096
+    assert L.vh_k0(C.byref(vp), 1, None, None, C.c_void_p(8), C.c_void_p(8), None, None, C.c_void_p(8),
+                   C.c_void_p(8), 10, None, 0, None) == -4

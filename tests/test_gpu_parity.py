@@ -1,0 +1,324 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (bit-exact for counts, candidate lists, matched ions and
+f32 sums; hyperscore within 1e-12 relative: the device and host f64 log may differ in the last ulp)."""
+import numpy as np
+import pytest
+
+from conftest import STATIC
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("needs a ROCm GPU", allow_module_level=True)
+
+HS_RTOL = 1e-12
+
+
+def _O():
+    from oracle import oracle as O
+    return O
+
+
+class Env:
+    def __init__(self, d, **over):
+        from vulcan.gpu import engine as E
+        from vulcan.gpu._abi import make_params
+        self.d = d
+        self.p = d["p"]
+        for k, v in over.items():
+            setattr(self.p, k, v)
+        self.l1 = d["l1"]
+        self.gi = E.GpuIndex(self.l1, self.p, d["static"], d["var"])
+        self.vp = make_params(self.p, d["static"], d["var"])
+        O = _O()
+        self.hix = O.HostIndex(self.l1)
+        self.pk = O.k0(self.vp, d["sp"])
+        from vulcan.gpu._abi import max_hyp_per_spectrum
+        self.hyp = O.hypotheses(self.vp, d["sp"], self.pk["valid"], self.l1.mass32, max_hyp_per_spectrum(self.p))
+        self.k1 = O.k1_counts(self.vp, self.hix, self.pk, self.hyp)
+
+
+@pytest.fixture(scope="module")
+def env0(cfg0):
+    print("[gpu] building cfg0 env", flush=True)
+    return Env(cfg0)
+
+
+@pytest.fixture(scope="module")
+def env1(cfg1_small):
+    print("[gpu] building cfg1-small env", flush=True)
+    return Env(cfg1_small)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+# ------------------------------------------------------------------------------------------------ index
+def _postings_numpy(l1, vp, gi):
+    """Posting keys of every row by f32 accumulation (np.add/subtract.accumulate are sequential)."""
+    rm0 = np.array([np.float32(vp.mono[i]) + np.float32(vp.stat[i]) for i in range(26)], np.float32)
+    rm1 = np.array([np.float32(vp.mono[i]) + np.float32(vp.var[i]) for i in range(26)], np.float32)
+    keys = []
+    mii, fmin, fmax = vp.min_ion_index, np.float32(vp.fmin), np.float32(vp.fmax)
+    for i in range(len(l1)):
+        L = int(l1.length[i])
+        s = l1.residues[l1.seq_off[i]:l1.seq_off[i] + L].astype(np.int64) - 65
+        bits = (int(l1.mask[i]) >> np.arange(L)) & 1
+        rm = np.where(bits[:L - 1] == 1, rm1[s[:L - 1]], rm0[s[:L - 1]]).astype(np.float32)
+        b = np.add.accumulate(rm, dtype=np.float32)
+        y = np.subtract.accumulate(np.concatenate([[l1.mass32[i]], rm]).astype(np.float32), dtype=np.float32)[1:]
+        k = np.arange(L - 1)
+        for f, ok in ((b, k + 1 > mii), (y, L - 1 - k > mii)):
+            for j in range(L - 1):
+                if ok[j] and fmin <= f[j] <= fmax:
+                    fb = int(np.array([f[j]], np.float32).view(np.uint32)[0])
+                    bucket = i >> gi.bucket_log2
+                    local = i & ((1 << gi.bucket_log2) - 1)
+                    lin = bucket * gi.n_bins + ((fb >> gi.bin_shift) - gi.bin_base)
+                    keys.append((lin << 32) | (local << gi.bin_shift) | (fb & ((1 << gi.bin_shift) - 1)))
+    return np.array(keys, np.uint64)
+
+
+def test_index_build_matches_numpy(env0):
+    """Device emission + LSD radix sort + directory == numpy emission + stable sort on the (bucket, bin) part,
+    on the first 3,000 rows (a sub-table keeps the Python loop short)."""
+    from vulcan.gpu import engine as E
+    from vulcan.index.build import L1Index
+    l1 = env0.l1
+    n = 3000
+    sub = L1Index(arrays={"pf_id": l1.pf_id[:n], "mass32": l1.mass32[:n], "seq_off": l1.seq_off[:n],
+                          "length": l1.length[:n], "mask": l1.mask[:n], "n_var": l1.n_var[:n],
+                          "missed": l1.missed[:n], "partner": np.arange(n), "residues": l1.residues})
+    p = env0.p
+    p2 = type(p)(**{**p.to_dict(), "chunk_size": 512})
+    gi = E.GpuIndex(sub, p2, STATIC, {})
+    keys = _postings_numpy(sub, gi.vp, gi)
+    assert gi.n_postings == len(keys)
+    o = np.argsort(keys >> np.uint64(32), kind="stable")
+    ks = keys[o]
+    assert np.array_equal(_np(gi.entries).view(np.uint32)[:len(ks)], (ks & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+    hi = (ks >> np.uint64(32)).astype(np.int64)
+    want = np.searchsorted(hi, np.arange(gi.n_buckets * gi.n_bins + 1), side="left")
+    assert np.array_equal(_np(gi.offsets).view(np.uint32).astype(np.int64), want)
+
+
+# ------------------------------------------------------------------------------------------------ K0 / hyps
+def test_k0_parity(env0):
+    from vulcan.gpu import engine as E
+    st = E.Stats()
+    pk = E.k0_preprocess(env0.d["sp"], env0.p, env0.gi.dev, st, env0.gi)
+    o = env0.pk
+    assert np.array_equal(_np(pk["off"]), o["off"])
+    m = int(o["off"][-1])
+    assert np.array_equal(_np(pk["mass"])[:m], o["mass"])
+    assert np.array_equal(_np(pk["int"])[:m], o["int"])
+    assert np.array_equal(pk["valid"], o["valid"].astype(bool))
+    assert np.allclose(_np(pk["tic"]), o["tic"], rtol=1e-6)
+
+
+def test_k0_edge_cases_parity(env0):
+    """Empty spectra, NaN m/z and intensities, ties, bounds, and spectra with > 512 in-range peaks (global path)."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu.host import SearchParams, Spectra
+    rng = np.random.default_rng(11)
+    peaks = []
+    for n in (0, 3, 150, 151, 513, 900, 1200):
+        mz = rng.uniform(140, 2010, n).astype(np.float32)
+        it = rng.integers(1, 4, n).astype(np.float32)
+        if n > 10:
+            mz[:5] = mz[5:10]
+            mz[3] = np.nan
+            it[4] = np.nan
+            mz[6], mz[7] = 150.0, 2000.0
+        peaks.append((mz, it))
+    off = np.zeros(len(peaks) + 1, np.int64)
+    np.cumsum([len(m) for m, _ in peaks], out=off[1:])
+    sp = Spectra("e", np.arange(len(peaks), dtype=np.uint32), np.arange(len(peaks), dtype=np.uint32),
+                 np.zeros(len(peaks)), np.full(len(peaks), 600.0), np.full(len(peaks), 2, np.int8),
+                 np.concatenate([m for m, _ in peaks]), np.concatenate([i for _, i in peaks]), off)
+    p = SearchParams(max_peaks=150, min_peaks=10)
+    from vulcan.gpu._abi import make_params
+    vp = make_params(p, STATIC, {})
+    o = _O().k0(vp, sp)
+    pk = E.k0_preprocess(sp, p, env0.gi.dev, E.Stats(), None)
+    m = int(o["off"][-1])
+    assert np.array_equal(_np(pk["off"]), o["off"])
+    assert np.array_equal(_np(pk["mass"])[:m], o["mass"])
+    assert np.array_equal(_np(pk["int"])[:m], o["int"], equal_nan=True)
+    assert np.array_equal(pk["valid"], o["valid"].astype(bool))
+
+
+def test_hypotheses_parity(env1):
+    from vulcan.gpu import engine as E
+    st = E.Stats()
+    pk = E.k0_preprocess(env1.d["sp"], env1.p, env1.gi.dev, st, env1.gi)
+    hd = E.device_hypotheses(env1.gi, pk["raw"], pk, env1.p)
+    h = E.hyp_to_host(hd)
+    for k in ("hyp_off", "spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz"):
+        assert np.array_equal(h[k], env1.hyp[k]), k
+
+
+# ------------------------------------------------------------------------------------------------ K1
+def _device_peaks(env):
+    from vulcan.gpu import engine as E
+    return E.k0_preprocess(env.d["sp"], env.p, env.gi.dev, E.Stats(), env.gi)
+
+
+@pytest.mark.parametrize("which", ["env0", "env1"])
+def test_k1_append_equals_oracle(which, request):
+    from vulcan.gpu import engine as E
+    env = request.getfixturevalue(which)
+    pk = _device_peaks(env)
+    o_h, o_r, o_c, scored, matched, d_spec, d_maxfz = E.run_k1(env.gi, env.hyp, pk, env.p, E.Stats())
+    O = _O()
+    hh, rr, cc = O.k1_append(env.hyp, env.k1, env.p.k1_threshold)
+    a = sorted(zip(_np(o_h).tolist(), _np(o_r).tolist(), _np(o_c).tolist()))
+    b = sorted(zip(hh.tolist(), rr.tolist(), cc.tolist()))
+    assert len(a) > 100 and a == b
+    assert np.array_equal(_np(scored).astype(np.uint32), env.k1["scored"])
+    assert np.array_equal(_np(matched).astype(np.uint64), env.k1["matched"])
+
+
+def test_k1_brute_device_equals_oracle(env1):
+    from vulcan.gpu import engine as E
+    pk = _device_peaks(env1)
+    hh, rr, cc = _O().k1_append(env1.hyp, env1.k1, 0)
+    sel = np.random.default_rng(4).choice(len(hh), min(len(hh), 20000), replace=False)
+    dev = env1.gi.dev
+    bc = E.k1_brute(env1.gi, env1.hyp, pk, torch.from_numpy(hh[sel]).to(dev), torch.from_numpy(rr[sel]).to(dev),
+                    env1.p)
+    assert np.array_equal(_np(bc), cc[sel])
+
+
+def _topk_device(env, **over):
+    from vulcan.gpu import engine as E
+    p = type(env.p)(**{**env.p.to_dict(), **over})
+    ds = E.DeviceSearch(env.gi, p, len(env.d["sp"]))
+    raw = E.upload_spectra(env.d["sp"], env.gi.dev)
+    ds.run(raw)
+    torch.cuda.synchronize()
+    assert int(ds.status.item()) == 0
+    return ds, p
+
+
+@pytest.mark.parametrize("over", [{}, {"topk": 3}, {"topk": 3, "tile": 64}, {"topk": 200, "tile": 256}])
+@pytest.mark.parametrize("which", ["env0", "env1"])
+def test_k1_topk_fused_equals_oracle(which, over, request):
+    from vulcan.gpu._abi import make_params
+    env = request.getfixturevalue(which)
+    ds, p = _topk_device(env, **over)
+    vp = make_params(p, env.d["static"], env.d["var"])
+    tk = _O().topk(vp, env.hix, env.hyp, env.k1, ds.stride)
+    n = len(env.d["sp"])
+    nc = _np(ds.n_cand)[:n]
+    assert np.array_equal(nc, tk["n_cand"])
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    for dv, ov in ((ds.c_row, tk["row"]), (ds.c_hyp, tk["hyp"]), (ds.c_cnt, tk["cnt"])):
+        assert np.array_equal(_np(dv)[:n * ds.stride][m], ov[m])
+    nh = len(env.hyp["spec"])
+    assert np.array_equal(_np(ds.scored)[:nh].astype(np.uint32), env.k1["scored"])
+    assert np.array_equal(_np(ds.matched)[:nh].astype(np.uint64), env.k1["matched"])
+
+
+def test_select_topk_torch_equals_oracle(env1):
+    from vulcan.gpu import engine as E
+    pk = _device_peaks(env1)
+    o_h, o_r, o_c, scored, matched, d_spec, d_maxfz = E.run_k1(env1.gi, env1.hyp, pk, env1.p, E.Stats())
+    h, r, c = E.select_topk(o_h, o_r, o_c, d_spec, env1.gi, env1.p)
+    stride = 2 * env1.p.topk
+    tk = _O().topk(env1.vp, env1.hix, env1.hyp, env1.k1, stride)
+    n = len(env1.d["sp"])
+    m = (np.arange(stride)[None, :] < tk["n_cand"][:, None]).ravel()
+    assert np.array_equal(_np(r), tk["row"][m])
+    assert np.array_equal(_np(h), tk["hyp"][m])
+    assert np.array_equal(_np(c), tk["cnt"][m])
+    del n
+
+
+# ------------------------------------------------------------------------------------------------ K2 / rank
+@pytest.mark.parametrize("which", ["env0", "env1"])
+def test_k2_and_rank_equal_oracle(which, request):
+    env = request.getfixturevalue(which)
+    ds, p = _topk_device(env)
+    O = _O()
+    n = len(env.d["sp"])
+    nc = _np(ds.n_cand)[:n]
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    rows = _np(ds.c_row)[:n * ds.stride][m]
+    hyps = _np(ds.c_hyp)[:n * ds.stride][m]
+    start = np.concatenate([[0], np.cumsum(nc)[:-1]])
+    ref = O.k2(env.vp, env.hix, env.pk, rows, hyps, env.hyp["maxfz"], grp_start=start, grp_count=nc)
+    assert len(rows) > 1000
+    for k in ("matched_b", "matched_y", "longest_b", "longest_y", "summed_b", "summed_y"):
+        assert np.array_equal(_np(ds.k2[k])[:n * ds.stride][m], ref[k]), k
+    assert np.array_equal(_np(ds.k2["ppm"])[:n * ds.stride][m], ref["ppm"], equal_nan=True)
+    hs = _np(ds.k2["hyperscore"])[:n * ds.stride][m]
+    assert np.allclose(hs, ref["hyperscore"], rtol=HS_RTOL, atol=0)
+    sel, nxt = O.rank(env.vp, start, nc, rows, ref["hyperscore"], ref["matched_b"], ref["matched_y"], 1)
+    dsel = _np(ds.sel)[:n]
+    flat_of = np.flatnonzero(m)          # oracle index -> strided device index
+    want = np.where(sel >= 0, flat_of[np.maximum(sel, 0)], -1)
+    assert np.array_equal(dsel, want)
+    assert np.allclose(_np(ds.nxt)[:n], nxt, rtol=HS_RTOL, atol=0)
+
+
+def test_search_end_to_end(env0):
+    from vulcan.gpu import engine as E
+    O = _O()
+    tbl, st, dbg = E.search(env0.d["sp"], env0.gi, env0.l1, env0.p)
+    ref = O.search(env0.vp, env0.hix, env0.d["sp"])
+    got = dict(zip(tbl.column("spec_idx").to_pylist(), tbl.column("pf_row").to_pylist()))
+    want = {i: int(r) for i, r in enumerate(ref["row"]) if r >= 0}
+    assert got == want
+    hs = dict(zip(tbl.column("spec_idx").to_pylist(), tbl.column("hyperscore").to_pylist()))
+    assert all(abs(hs[i] - ref["hs"][i]) <= HS_RTOL * abs(ref["hs"][i]) for i in want)
+    truth = env0.d["truth"]
+    sig = truth >= 0
+    assert np.mean([got.get(i) == truth[i] for i in np.flatnonzero(sig)]) > 0.9
+    assert st.counts["n_rescored"] == int(ref["n_cand"].sum())
+
+
+def test_search_report5_matches_oracle_ranking(env1):
+    from vulcan.gpu import engine as E
+    tbl, _, dbg = E.search(env1.d["sp"], env1.gi, env1.l1, env1.p, report=5)
+    O = _O()
+    k2 = {k: _np(v) for k, v in dbg["k2"].items()}
+    spec = k2["spec"]
+    us, cnt = np.unique(spec, return_counts=True)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    rows = _np(dbg["r"]).astype(np.int32)
+    sel, nxt = O.rank(env1.vp, start, cnt, rows, k2["hyperscore"], k2["matched_b"], k2["matched_y"], 5)
+    want = [(int(spec[i]), int(rows[i])) for i in sel if i >= 0]
+    got = list(zip(tbl.column("spec_idx").to_pylist(), tbl.column("pf_row").to_pylist()))
+    assert got == want
+
+
+def test_deterministic_rerun(env1):
+    a, _ = _topk_device(env1)
+    ra = [_np(t).copy() for t in (a.n_cand, a.c_row, a.c_hyp, a.c_cnt, a.k2["hyperscore"], a.sel)]
+    b, _ = _topk_device(env1)
+    rb = [_np(t) for t in (b.n_cand, b.c_row, b.c_hyp, b.c_cnt, b.k2["hyperscore"], b.sel)]
+    n = len(env1.d["sp"])
+    nc = ra[0][:n]
+    m = (np.arange(a.stride)[None, :] < nc[:, None]).ravel()
+    assert np.array_equal(ra[0][:n], rb[0][:n])
+    for x, y in zip(ra[1:5], rb[1:5]):
+        assert np.array_equal(x[:n * a.stride][m], y[:n * a.stride][m])
+    assert np.array_equal(ra[5][:n], rb[5][:n])
+
+
+def test_label_blindness_gpu():
+    from vulcan.gpu import engine as E
+    from vulcan.index import synth
+    from vulcan.index.build import build_tables
+    w = synth.WORKLOADS[1]
+    prots = synth.proteome(300, 77)
+    a, la = build_tables(prots, w.index_params())
+    b, _ = build_tables(prots, w.index_params(), shuffle_rows_seed=3, swap_labels=True)
+    sp, _, _ = synth.spectra(a, la, 1, n=300, seed=9)
+    p = w.search_params()
+    outs = []
+    for l1 in (a, b):
+        gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
+        outs.append(E.search(sp, gi, l1, p, report=5)[0])
+    assert outs[0].equals(outs[1])

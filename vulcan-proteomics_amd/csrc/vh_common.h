@@ -1,0 +1,81 @@
+// Shared device helpers for the gfx950 search kernels (wave64, no FMA contraction: built with
+// -ffp-contract=off so every f32 operation rounds exactly like the reference's Sage arithmetic).
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vulcan_hip.h"
+
+#define VH_PROTON 1.0072764f    // host.py:10
+#define VH_NEUTRON 1.00335f     // host.py:11
+#define VH_WAVE 64
+
+// Per-call device parameters (by value). Residue masses are pre-added on the host in f32
+// (mono + static, mono + var: the same single rounding as kernels.cu:47-51).
+struct DevP {
+  float rm[2][26];
+  float nterm;
+  int tol_da;
+  float tol_lo, tol_hi, fmin, fmax;
+  int min_ion_index;
+  float prec_lo, prec_hi;
+  int iso_lo, iso_hi, z_lo, z_hi, max_fragment_charge;
+  int max_peaks, min_peaks, min_matched_peaks, topk, k1_threshold, pair_closure, tile;
+};
+
+struct DevIdx {
+  int64_t n_rows;
+  const int64_t* seq_off;
+  const uint8_t* length;
+  const uint64_t* mask;
+  const float* mass32;
+  const uint8_t* residues;
+  const int64_t* partner;
+  int bucket_log2, bin_shift;
+  uint32_t bin_base;
+  int n_bins;
+  int64_t n_buckets;
+  const uint32_t* offsets;
+  const uint32_t* entries;
+};
+
+DevP vh_make_devp(const VhParams* p);
+DevIdx vh_make_devidx(const VhIndexView* ix);
+int vh_scan_inclusive_i64(int64_t* a, int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
+int vh_scan_exclusive_u32(uint32_t* a, int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
+size_t vh_scan_ws_u32(int64_t n);
+
+#define VH_CHECK_LAUNCH() do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return (int)e_; } while (0)
+
+// kernels.cu:36-38: ppm bound = m + (m*tol)/1e6 (each op rounded), Da bound = m + tol
+__device__ __forceinline__ float tol_bound(float m, float tol, int da) {
+  return da ? (m + tol) : (m + (m * tol) / 1000000.0f);
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Order all LDS traffic of this wave before/after (waves own private LDS regions; no block barrier needed).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t ballot(bool b) { return __ballot(b); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ULL << lane_id()) - 1ULL; }
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t fbits(float x) { return __float_as_uint(x); }
+
+// Residue table in LDS: rm[bit][letter] (bit = variable-mod bit of the mask).
+__device__ __forceinline__ void load_rm(float* lds_rm, const DevP& P) {
+  for (int i = threadIdx.x; i < 52; i += blockDim.x) lds_rm[i] = (&P.rm[0][0])[i];
+}

@@ -1,0 +1,244 @@
+// Fragment inverted index on the device: per-row fragment count, key emission, LSD radix sort, CSR directory.
+//
+// Replaces frag_count_kernel / frag_emit_kernel (vulcan/gpu/kernels.cu:53-101) and the torch.sort + bincount +
+// cumsum assembly of GpuIndex.__init__ (vulcan/gpu/engine.py:101-148). Key = (bucket * n_bins + bin) << 32 |
+// posting, where bucket = row >> bucket_log2 (rows are sorted by precursor mass), bin = f32 bits >> bin_shift
+// (minus the bin of fragment_min_mz) and posting = local row << bin_shift | low f32 bits. The f32 bit pattern
+// of a positive float is order-preserving, so every posting keeps the exact fragment mass.
+// Only the high 32 bits are sorted (stable LSD, 8-bit digits): inside a (bucket, bin) list the postings keep
+// their emission order, i.e. rows ascending, which K1 binary-searches.
+#include "vh_common.h"
+
+// ------------------------------------------------------------------------------------------------ count / emit
+__global__ __launch_bounds__(256) void frag_count_kernel(DevP P, DevIdx X, int64_t* post_off) {
+  __shared__ float rm[52];
+  load_rm(rm, P);
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) post_off[0] = 0;
+  if (i >= X.n_rows) return;
+  const uint8_t* s = X.residues + X.seq_off[i];
+  int L = X.length[i];
+  uint64_t mk = X.mask[i];
+  float nt = (mk >> 63) ? P.nterm : 0.f;
+  float b = nt, y = X.mass32[i] - nt;
+  int c = 0;
+  for (int k = 0; k < L - 1; ++k) {
+    float r = rm[((mk >> k) & 1ULL) * 26 + (s[k] - 'A')];
+    b = b + r;
+    y = y - r;
+    if (k + 1 > P.min_ion_index && b >= P.fmin && b <= P.fmax) ++c;
+    if (L - 1 - k > P.min_ion_index && y >= P.fmin && y <= P.fmax) ++c;
+  }
+  post_off[i + 1] = c;
+}
+
+__device__ __forceinline__ uint64_t posting_key(float f, uint64_t bucket, uint32_t local, const DevIdx& X) {
+  uint32_t bits = fbits(f);
+  uint64_t bin = (uint64_t)((bits >> X.bin_shift) - X.bin_base);
+  uint32_t low = bits & ((1u << X.bin_shift) - 1u);
+  return ((bucket * (uint64_t)X.n_bins + bin) << 32) | (uint64_t)((local << X.bin_shift) | low);
+}
+
+__global__ __launch_bounds__(256) void frag_emit_kernel(DevP P, DevIdx X, const int64_t* post_off, uint64_t* keys) {
+  __shared__ float rm[52];
+  load_rm(rm, P);
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= X.n_rows) return;
+  const uint8_t* s = X.residues + X.seq_off[i];
+  int L = X.length[i];
+  uint64_t mk = X.mask[i];
+  float nt = (mk >> 63) ? P.nterm : 0.f;
+  float b = nt, y = X.mass32[i] - nt;
+  uint64_t bucket = (uint64_t)i >> X.bucket_log2;
+  uint32_t local = (uint32_t)(i & ((1LL << X.bucket_log2) - 1));
+  int64_t o = post_off[i];
+  for (int k = 0; k < L - 1; ++k) {
+    float r = rm[((mk >> k) & 1ULL) * 26 + (s[k] - 'A')];
+    b = b + r;
+    y = y - r;
+    if (k + 1 > P.min_ion_index && b >= P.fmin && b <= P.fmax) keys[o++] = posting_key(b, bucket, local, X);
+    if (L - 1 - k > P.min_ion_index && y >= P.fmin && y <= P.fmax) keys[o++] = posting_key(y, bucket, local, X);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ radix sort
+// Tile = 4 waves x 16 steps x 64 lanes = 4096 keys; wave w owns positions [w*1024, (w+1)*1024) of the tile and
+// processes them step-major, so per-wave running digit counters give stable ranks.
+#define RS_THREADS 256
+#define RS_STEPS 16
+#define RS_TILE (RS_THREADS * RS_STEPS)
+
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool ok) {
+  uint64_t peers = ballot(ok);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    bool bit = (d >> b) & 1u;
+    uint64_t m = ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                                 int shift, uint32_t* hist, int64_t n_tiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int w = threadIdx.x >> 6;
+  int64_t base = (int64_t)blockIdx.x * RS_TILE + w * (RS_STEPS * 64);
+  for (int st = 0; st < RS_STEPS; ++st) {
+    int64_t j = base + st * 64 + lane_id();
+    bool ok = j < n;
+    uint32_t d = ok ? (uint32_t)(keys[j] >> shift) & 255u : 0u;
+    uint64_t peers = match_digit(d, ok);
+    if (ok && __popcll(peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t* __restrict__ kin,
+                                                                   uint64_t* __restrict__ kout, int64_t n, int shift,
+                                                                   const uint32_t* __restrict__ hist_scan,
+                                                                   int64_t n_tiles) {
+  __shared__ uint32_t wcnt[4][256];
+  __shared__ uint32_t goff[256];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  for (int i = threadIdx.x; i < 4 * 256; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
+  goff[threadIdx.x] = hist_scan[(int64_t)threadIdx.x * n_tiles + blockIdx.x];
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * RS_TILE + w * (RS_STEPS * 64);
+  uint64_t k[RS_STEPS];
+  uint32_t rank[RS_STEPS];
+#pragma unroll
+  for (int st = 0; st < RS_STEPS; ++st) {
+    int64_t j = base + st * 64 + lane;
+    bool ok = j < n;
+    uint64_t key = ok ? kin[j] : 0ULL;
+    uint32_t d = (uint32_t)(key >> shift) & 255u;
+    uint64_t peers = match_digit(d, ok);
+    uint32_t before = wcnt[w][d];
+    wave_sync();
+    if (ok && __popcll(peers & lanemask_lt()) == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    wave_sync();
+    k[st] = key;
+    rank[st] = before + (uint32_t)__popcll(peers & lanemask_lt());
+  }
+  __syncthreads();
+  {
+    int d = threadIdx.x;
+    uint32_t acc = 0;
+    for (int ww = 0; ww < 4; ++ww) {
+      uint32_t c = wcnt[ww][d];
+      wcnt[ww][d] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int st = 0; st < RS_STEPS; ++st) {
+    int64_t j = base + st * 64 + lane;
+    if (j < n) {
+      uint32_t d = (uint32_t)(k[st] >> shift) & 255u;
+      kout[(int64_t)goff[d] + wcnt[w][d] + rank[st]] = k[st];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ directory
+__global__ __launch_bounds__(256) void directory_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                        uint32_t* offsets, uint32_t* entries, int64_t n_dir) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n == 0) {
+    for (int64_t b = i; b <= n_dir; b += (int64_t)gridDim.x * blockDim.x) offsets[b] = 0;
+    return;
+  }
+  if (i >= n) return;
+  uint64_t kk = keys[i];
+  int64_t hi = (int64_t)(kk >> 32);
+  int64_t prev = i ? (int64_t)(keys[i - 1] >> 32) : -1;
+  for (int64_t b = prev + 1; b <= hi; ++b) offsets[b] = (uint32_t)i;
+  entries[i] = (uint32_t)kk;
+  if (i == n - 1)
+    for (int64_t b = hi + 1; b <= n_dir; ++b) offsets[b] = (uint32_t)n;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int64_t n_tiles_of(int64_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+extern "C" {
+
+size_t vh_index_workspace_bytes(int64_t n_postings, int64_t n_buckets, int32_t n_bins) {
+  (void)n_buckets;
+  (void)n_bins;
+  int64_t nt = n_tiles_of(n_postings > 0 ? n_postings : 1);
+  size_t hist = (size_t)256 * nt;
+  return 2 * align256((size_t)(n_postings > 0 ? n_postings : 1) * 8) + align256(hist * 4) +
+         align256(vh_scan_ws_u32(hist));
+}
+
+int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, void* ws, size_t ws_bytes,
+                  void* stream) {
+  if (!p || !ix || !post_off || ix->n_rows < 0) return VH_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  DevP P = vh_make_devp(p);
+  DevIdx X = vh_make_devidx(ix);
+  int64_t n = ix->n_rows;
+  unsigned grid = (unsigned)((n + 255) / 256);
+  frag_count_kernel<<<grid ? grid : 1, 256, 0, st>>>(P, X, post_off);
+  VH_CHECK_LAUNCH();
+  return vh_scan_inclusive_i64(post_off + 1, n, ws, ws_bytes, st);
+}
+
+int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
+                   uint32_t* entries, void* ws, size_t ws_bytes, void* stream) {
+  if (!p || !ix || !post_off || !offsets || (ix->n_postings && !entries)) return VH_E_ARG;
+  if (ix->bin_shift < 4 || ix->bin_shift > 12 || ix->bucket_log2 < 0 || ix->bucket_log2 + ix->bin_shift > 32)
+    return VH_E_ARG;
+  if (ix->n_postings < 0 || ix->n_postings >= (1LL << 32)) return VH_E_LIMIT;
+  int64_t n_dir = ix->n_buckets * (int64_t)ix->n_bins;
+  if (n_dir <= 0 || n_dir >= (1LL << 32)) return VH_E_LIMIT;
+  if (ix->n_buckets < ((ix->n_rows + (1LL << ix->bucket_log2) - 1) >> ix->bucket_log2)) return VH_E_ARG;
+  if (ws_bytes < vh_index_workspace_bytes(ix->n_postings, ix->n_buckets, ix->n_bins)) return VH_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  DevP P = vh_make_devp(p);
+  DevIdx X = vh_make_devidx(ix);
+  int64_t n = ix->n_postings;
+  char* w = (char*)ws;
+  size_t kb = align256((size_t)(n > 0 ? n : 1) * 8);
+  uint64_t* ka = (uint64_t*)w;
+  uint64_t* kb2 = (uint64_t*)(w + kb);
+  int64_t nt = n_tiles_of(n > 0 ? n : 1);
+  size_t hist_n = (size_t)256 * nt;
+  uint32_t* hist = (uint32_t*)(w + 2 * kb);
+  void* sws = w + 2 * kb + align256(hist_n * 4);
+  size_t sws_bytes = ws_bytes - (2 * kb + align256(hist_n * 4));
+  unsigned grid = (unsigned)((ix->n_rows + 255) / 256);
+  if (ix->n_rows) frag_emit_kernel<<<grid, 256, 0, st>>>(P, X, post_off, ka);
+  VH_CHECK_LAUNCH();
+  int nbits = 0;
+  while (nbits < 32 && ((n_dir - 1) >> nbits) != 0) ++nbits;
+  uint64_t* src = ka;
+  uint64_t* dst = kb2;
+  if (n > 0) {
+    for (int shift = 32; shift < 32 + nbits; shift += 8) {
+      rs_upsweep_kernel<<<(unsigned)nt, RS_THREADS, 0, st>>>(src, n, shift, hist, nt);
+      int rc = vh_scan_exclusive_u32(hist, (int64_t)hist_n, sws, sws_bytes, st);
+      if (rc) return rc;
+      rs_downsweep_kernel<<<(unsigned)nt, RS_THREADS, 0, st>>>(src, dst, n, shift, hist, nt);
+      VH_CHECK_LAUNCH();
+      uint64_t* t = src;
+      src = dst;
+      dst = t;
+    }
+  }
+  unsigned g2 = (unsigned)((n > 0 ? n : 1024) + 255) / 256;
+  directory_kernel<<<g2, 256, 0, st>>>(src, n, offsets, entries, n_dir);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,466 @@
+// K1 candidate generation on the fragment index, fused with the per-spectrum top-K and pair closure.
+//
+// Semantics (the reference's k1_kernel, vulcan/gpu/kernels.cu:112-190, and select_topk, engine.py:284-312):
+//   count(h, r) = #(indexed fragment f of row r, peak p, fragment charge z < maxfz(h)) with
+//                 tol_bound(mass_p*z, lo) <= f <= tol_bound(mass_p*z, hi), for r in h's precursor window;
+//   appended if count >= k1_threshold; scored[h] = #(r: count > 0); matched[h] = sum of counts.
+//   Top-K per spectrum over its appended entries by (count clamped at 32767 desc, row desc; identical keys from
+//   two hypotheses: lower hypothesis first), + partner rows (count -1), dedupe by row (lowest hypothesis,
+//   highest count), row order.
+// Design: one wave64 per spectrum. All (hypothesis, row) pairs of the spectrum's windows get a slot in an LDS
+// u32 histogram (tiled when they exceed `tile` slots). Each lane takes peaks; for each (peak, z) the exact f32
+// key range [bits(lo bound), bits(hi bound)] selects the bins of every bucket a window touches; inside a bin the
+// postings are row-ordered, so a binary search finds the window's first row, the scan stops at its last row,
+// and matching postings increment their slot with an LDS atomic. The scan of the histogram feeds a wave-private
+// top-K buffer (ballot-compacted inserts, pruned by the current K-th key, bitonic compaction when full).
+// Closure and dedupe are one more bitonic sort. Nothing leaves the wave but the final candidate list.
+#include "vh_common.h"
+
+#define K1_MAXSEG 64
+
+struct Seg {
+  int hl, r0, r1, base, mfz, pad0, pad1, pad2;
+};
+
+__device__ void bitonic_sort_u64(uint64_t* a, int n, bool descending) {
+  const int lane = lane_id();
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < n; i += 64) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          uint64_t x = a[i], y = a[ixj];
+          bool up = ((i & k) == 0) != descending;   // ascending run in this block
+          if (up ? (x > y) : (x < y)) {
+            a[i] = y;
+            a[ixj] = x;
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+// sort the buffer descending, keep the best K; returns the new fill and updates the pruning threshold
+__device__ __forceinline__ int topk_compact(uint64_t* buf, int capb, int nbuf, int K, uint64_t& thresh) {
+  for (int i = nbuf + lane_id(); i < capb; i += 64) buf[i] = 0ULL;
+  wave_sync();
+  bitonic_sort_u64(buf, capb, true);
+  int n = nbuf < K ? nbuf : K;
+  thresh = n == K ? buf[K - 1] : 0ULL;
+  wave_sync();
+  return n;
+}
+
+struct K1Out {
+  // append mode
+  int64_t cap;
+  unsigned long long* n_out;
+  int32_t *out_hyp, *out_row, *out_cnt;
+  // top-K mode
+  int cand_stride, capb, clos_n;
+  int32_t *n_cand, *c_row, *c_hyp, *c_cnt;
+  uint32_t* hyp_scored;
+  unsigned long long* hyp_matched;
+  int32_t* status;
+  int wave_bytes, hist_bytes;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, const int64_t* __restrict__ pk_off,
+                                                 const float* __restrict__ pk_mass,
+                                                 const int64_t* __restrict__ hyp_off,
+                                                 const int8_t* __restrict__ h_maxfz,
+                                                 const int64_t* __restrict__ h_lo,
+                                                 const int64_t* __restrict__ h_hi, K1Out O) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  const int s = blockIdx.x * (blockDim.x >> 6) + w;
+  if (s >= n_spec) return;   // a whole wave leaves; no block-level barrier is used below
+  unsigned char* wb = smem + (size_t)w * O.wave_bytes;
+  uint32_t* hist = (uint32_t*)wb;
+  Seg* segs = (Seg*)(wb + O.hist_bytes);
+  uint32_t* hsc = (uint32_t*)(segs + K1_MAXSEG);
+  unsigned long long* hmt = (unsigned long long*)(hsc + 64);
+  uint64_t* buf = (uint64_t*)(hmt + 64);
+
+  const int64_t h0 = hyp_off[s];
+  const int nh = (int)(hyp_off[s + 1] - h0);
+  if (nh == 0 || nh > 64) {
+    if (MODE == 1 && lane == 0) O.n_cand[s] = 0;
+    if (nh > 64 && lane == 0) atomicOr(O.status, 1);
+    return;
+  }
+  int my_lo = 0, my_hi = 0, my_mfz = 0;
+  if (lane < nh) {
+    my_lo = (int)h_lo[h0 + lane];
+    my_hi = (int)h_hi[h0 + lane];
+    my_mfz = h_maxfz[h0 + lane];
+  }
+  hsc[lane] = 0;
+  hmt[lane] = 0;
+  const int64_t p0 = pk_off[s];
+  const int np = (int)(pk_off[s + 1] - p0);
+  const int T = P.tile;
+  const int S = X.bin_shift, LB = X.bucket_log2;
+  const uint32_t lowmask = (1u << S) - 1u;
+  const int base = (int)X.bin_base, n_bins = X.n_bins;
+  const int B = 1 << LB;
+  const int K = P.topk;
+  int nbuf = 0;
+  uint64_t thresh = 0;
+  bool clamped = false;
+
+  int hc = 0;
+  int rc = __shfl(my_lo, 0);
+  for (;;) {
+    // ---- plan one tile of (hypothesis, row-range) segments (wave-uniform)
+    int used = 0, nseg = 0, zmax = 0;
+    while (hc < nh && used < T && nseg < K1_MAXSEG) {
+      int hi_c = __shfl(my_hi, hc);
+      if (rc >= hi_c) {
+        ++hc;
+        if (hc < nh) rc = __shfl(my_lo, hc);
+        continue;
+      }
+      int take = hi_c - rc < T - used ? hi_c - rc : T - used;
+      int mf = __shfl(my_mfz, hc);
+      if (lane == 0) {
+        Seg g;
+        g.hl = hc; g.r0 = rc; g.r1 = rc + take; g.base = used; g.mfz = mf;
+        g.pad0 = g.pad1 = g.pad2 = 0;
+        segs[nseg] = g;
+      }
+      zmax = mf > zmax ? mf : zmax;
+      ++nseg;
+      used += take;
+      rc += take;
+    }
+    if (nseg == 0) break;
+    for (int j = lane; j < used; j += 64) hist[j] = 0;
+    wave_sync();
+    // ---- accumulate shared-peak counts
+    for (int p = lane; p < np; p += 64) {
+      const float pm = pk_mass[p0 + p];
+      for (int z = 1; z < zmax; ++z) {
+        float m = pm * (float)z;
+        float a = tol_bound(m, P.tol_lo, P.tol_da);
+        float b = tol_bound(m, P.tol_hi, P.tol_da);
+        if (b < P.fmin || a > P.fmax) continue;
+        uint32_t ka = a > 0.f ? fbits(a) : 0u;
+        uint32_t kb = fbits(b);
+        int bl = (int)(ka >> S) - base;
+        int bh = (int)(kb >> S) - base;
+        bl = bl < 0 ? 0 : bl;
+        bh = bh > n_bins - 1 ? n_bins - 1 : bh;
+        if (bl > bh) continue;
+        for (int q = 0; q < nseg; ++q) {
+          const Seg g = segs[q];
+          if (z >= g.mfz) continue;
+          const int bk0 = g.r0 >> LB, bk1 = (g.r1 - 1) >> LB;
+          for (int bk = bk0; bk <= bk1; ++bk) {
+            const int rb = bk << LB;
+            const uint32_t lo_l = (uint32_t)(g.r0 - rb > 0 ? g.r0 - rb : 0);
+            const uint32_t hi_l = (uint32_t)(g.r1 - rb < B ? g.r1 - rb : B);
+            const uint32_t* off = X.offsets + (size_t)bk * (size_t)n_bins;
+            uint32_t e1 = off[bl];
+            for (int bin = bl; bin <= bh; ++bin) {
+              const uint32_t e0 = e1;
+              e1 = off[bin + 1];
+              if (e0 == e1) continue;
+              uint32_t L = e0, R = e1;
+              while (L < R) {
+                uint32_t M = (L + R) >> 1;
+                if ((X.entries[M] >> S) < lo_l) L = M + 1; else R = M;
+              }
+              const uint32_t hikey = (uint32_t)(bin + base) << S;
+              for (uint32_t e = L; e < e1; ++e) {
+                const uint32_t v = X.entries[e];
+                const uint32_t loc = v >> S;
+                if (loc >= hi_l) break;
+                const uint32_t key = hikey | (v & lowmask);
+                if (key >= ka && key <= kb) atomicAdd(&hist[g.base + rb + (int)loc - g.r0], 1u);
+              }
+            }
+          }
+        }
+      }
+    }
+    wave_sync();
+    // ---- histogram scan: per-hypothesis statistics, append or top-K insert
+    for (int q = 0; q < nseg; ++q) {
+      const Seg g = segs[q];
+      const int len = g.r1 - g.r0;
+      uint32_t sc = 0;
+      unsigned long long mt = 0;
+      for (int j0 = 0; j0 < len; j0 += 64) {
+        const int j = j0 + lane;
+        const bool act = j < len;
+        const uint32_t c = act ? hist[g.base + j] : 0u;
+        sc += c > 0u;
+        mt += c;
+        const bool flag = act && (int)c >= P.k1_threshold;
+        if (MODE == 0) {
+          uint64_t bal = ballot(flag);
+          if (bal) {
+            unsigned long long bp = 0;
+            if (lane == 0) bp = atomicAdd(O.n_out, (unsigned long long)__popcll(bal));
+            bp = __shfl(bp, 0);
+            if (flag) {
+              unsigned long long pos = bp + (unsigned long long)__popcll(bal & lanemask_lt());
+              if ((int64_t)pos < O.cap) {
+                O.out_hyp[pos] = (int32_t)(h0 + g.hl);
+                O.out_row[pos] = g.r0 + j;
+                O.out_cnt[pos] = (int32_t)c;
+              }
+            }
+          }
+        } else {
+          uint64_t key = 0;
+          if (flag) {
+            uint32_t cc = c < 32767u ? c : 32767u;
+            clamped |= c > 32767u;
+            key = ((uint64_t)cc << 48) | ((uint64_t)(uint32_t)(g.r0 + j) << 20) | (uint64_t)(0xFFFFF - g.hl);
+          }
+          bool ins = flag && key > thresh;
+          uint64_t bal = ballot(ins);
+          int cnt = __popcll(bal);
+          if (cnt) {
+            if (nbuf + cnt > O.capb) {
+              nbuf = topk_compact(buf, O.capb, nbuf, K, thresh);
+              ins = flag && key > thresh;
+              bal = ballot(ins);
+              cnt = __popcll(bal);
+            }
+            if (ins) buf[nbuf + __popcll(bal & lanemask_lt())] = key;
+            nbuf += cnt;
+            wave_sync();
+          }
+        }
+      }
+      sc = wave_sum(sc);
+      mt = wave_sum(mt);
+      if (lane == 0) {
+        hsc[g.hl] += sc;
+        hmt[g.hl] += mt;
+      }
+    }
+    wave_sync();
+  }
+  if (lane < nh) {
+    O.hyp_scored[h0 + lane] = hsc[lane];
+    O.hyp_matched[h0 + lane] = hmt[lane];
+  }
+  if (MODE == 0) return;
+
+  // ---- final top-K, pair closure, dedupe by row
+  if (ballot(clamped)) {
+    if (lane == 0) atomicOr(O.status, 2);
+  }
+  const int ntop = topk_compact(buf, O.capb, nbuf, K, thresh);
+  uint64_t* ck = (uint64_t*)wb;                      // reuses the histogram area
+  int32_t* cv = (int32_t*)(ck + O.clos_n);
+  const int ncl = P.pair_closure ? 2 * ntop : ntop;
+  for (int i = lane; i < O.clos_n; i += 64) {
+    uint64_t v = ~0ULL;
+    if (i < ntop) {
+      uint64_t key = buf[i];
+      uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
+      uint32_t hl = 0xFFFFFu - (uint32_t)(key & 0xFFFFFULL);
+      cv[i] = (int32_t)(key >> 48);
+      v = ((uint64_t)row << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
+    } else if (i < ncl) {
+      uint64_t key = buf[i - ntop];
+      uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
+      uint32_t hl = 0xFFFFFu - (uint32_t)(key & 0xFFFFFULL);
+      uint32_t pr = (uint32_t)X.partner[row];
+      cv[i] = -1;
+      v = ((uint64_t)pr << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
+    }
+    ck[i] = v;
+  }
+  wave_sync();
+  bitonic_sort_u64(ck, O.clos_n, false);
+  int nout = 0;
+  int64_t ob = (int64_t)s * O.cand_stride;
+  for (int i0 = 0; i0 < ncl; i0 += 64) {
+    const int i = i0 + lane;
+    bool first = false;
+    uint32_t row = 0, hl = 0;
+    int32_t cc = -1;
+    if (i < ncl) {
+      uint64_t v = ck[i];
+      row = (uint32_t)(v >> 32);
+      hl = (uint32_t)((v >> 24) & 0xFFu);
+      first = i == 0 || (uint32_t)(ck[i - 1] >> 32) != row;
+      if (first) {
+        cc = cv[v & 0xFFFFFFu];
+        for (int j = i + 1; j < ncl && (uint32_t)(ck[j] >> 32) == row; ++j) {
+          int32_t c2 = cv[ck[j] & 0xFFFFFFu];
+          cc = c2 > cc ? c2 : cc;
+        }
+      }
+    }
+    uint64_t bal = ballot(first);
+    if (first) {
+      int pos = nout + __popcll(bal & lanemask_lt());
+      O.c_row[ob + pos] = (int32_t)row;
+      O.c_hyp[ob + pos] = (int32_t)(h0 + hl);
+      O.c_cnt[ob + pos] = cc;
+    }
+    nout += __popcll(bal);
+  }
+  if (lane == 0) O.n_cand[s] = nout;
+}
+
+// ------------------------------------------------------------------------------------------------ brute force
+// k1_brute_kernel (kernels.cu:195-232): one thread per (hypothesis, row) pair, fragments generated on the fly,
+// every peak and charge compared with plain f32 bounds.
+__global__ __launch_bounds__(256) void k1_brute_kernel(DevP P, DevIdx X, int64_t n, const int32_t* __restrict__ c_hyp,
+                                                       const int32_t* __restrict__ c_row,
+                                                       const int32_t* __restrict__ h_spec,
+                                                       const int8_t* __restrict__ h_maxfz,
+                                                       const int64_t* __restrict__ pk_off,
+                                                       const float* __restrict__ pk_mass, int32_t* out_cnt) {
+  __shared__ float rm[52];
+  load_rm(rm, P);
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int h = c_hyp[i], r = c_row[i];
+  int s = h_spec[h], maxfz = h_maxfz[h];
+  int64_t q0 = pk_off[s], q1 = pk_off[s + 1];
+  const uint8_t* sq = X.residues + X.seq_off[r];
+  int L = X.length[r];
+  uint64_t mk = X.mask[r];
+  float nt = (mk >> 63) ? P.nterm : 0.f;
+  float b = nt, y = X.mass32[r] - nt;
+  int cnt = 0;
+  for (int k = 0; k < L - 1; ++k) {
+    float rmv = rm[((mk >> k) & 1ULL) * 26 + (sq[k] - 'A')];
+    b = b + rmv;
+    y = y - rmv;
+    for (int t = 0; t < 2; ++t) {
+      float f = t == 0 ? b : y;
+      bool ok = t == 0 ? (k + 1 > P.min_ion_index) : (L - 1 - k > P.min_ion_index);
+      if (!ok || f < P.fmin || f > P.fmax) continue;
+      for (int64_t q = q0; q < q1; ++q)
+        for (int z = 1; z < maxfz; ++z) {
+          float m = pk_mass[q] * (float)z;
+          float a = tol_bound(m, P.tol_lo, P.tol_da);
+          float bd = tol_bound(m, P.tol_hi, P.tol_da);
+          if (f >= a && f <= bd) ++cnt;
+        }
+    }
+  }
+  out_cnt[i] = cnt;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ULL; }
+
+// ------------------------------------------------------------------------------------------------ launchers
+static int pow2_at_least(int x) {
+  int n = 1;
+  while (n < x) n <<= 1;
+  return n;
+}
+
+static int k1_check(const VhParams* p, const VhIndexView* ix) {
+  if (!p || !ix || !ix->offsets || !ix->entries) return VH_E_ARG;
+  if (p->tile < 64 || p->tile > 8192 || (p->tile & (p->tile - 1))) return VH_E_ARG;
+  if (ix->n_rows >= (1LL << 28)) return VH_E_LIMIT;
+  if (ix->bin_shift < 4 || ix->bin_shift > 12 || ix->bucket_log2 + ix->bin_shift > 32) return VH_E_ARG;
+  return 0;
+}
+
+static void k1_geometry(const VhParams* p, int mode, K1Out& O, int& waves) {
+  int K = p->topk > 0 ? p->topk : 1;
+  O.capb = mode ? pow2_at_least(K + 64) : 0;
+  O.clos_n = mode ? pow2_at_least((2 * K) > 64 ? 2 * K : 64) : 0;
+  int hb = p->tile * 4;
+  int cb = O.clos_n * 12;
+  O.hist_bytes = ((hb > cb ? hb : cb) + 15) & ~15;
+  O.wave_bytes = O.hist_bytes + K1_MAXSEG * (int)sizeof(Seg) + 64 * 4 + 64 * 8 + O.capb * 8;
+  waves = 65536 / O.wave_bytes;
+  waves = waves < 1 ? 1 : (waves > 4 ? 4 : waves);
+}
+
+extern "C" {
+
+int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off, const float* pk_mass,
+          const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo, const int64_t* h_hi, int64_t cap,
+          unsigned long long* n_out, int32_t* out_hyp, int32_t* out_row, int32_t* out_cnt, uint32_t* hyp_scored,
+          unsigned long long* hyp_matched, void* stream) {
+  int rc = k1_check(p, ix);
+  if (rc) return rc;
+  if (n_spec < 0 || !n_out || !hyp_scored || !hyp_matched || cap < 0) return VH_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  zero_u64_kernel<<<1, 1, 0, st>>>(n_out);
+  if (n_spec == 0) return 0;
+  K1Out O = {};
+  O.cap = cap;
+  O.n_out = n_out;
+  O.out_hyp = out_hyp;
+  O.out_row = out_row;
+  O.out_cnt = out_cnt;
+  O.hyp_scored = hyp_scored;
+  O.hyp_matched = hyp_matched;
+  int waves;
+  k1_geometry(p, 0, O, waves);
+  if (O.wave_bytes > 65536) return VH_E_LIMIT;
+  unsigned g = (unsigned)((n_spec + waves - 1) / waves);
+  k1_kernel<0><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
+                                                                      pk_off, pk_mass, hyp_off, h_maxfz, h_lo, h_hi,
+                                                                      O);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off,
+               const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo,
+               const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp,
+               int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
+               void* stream) {
+  int rc = k1_check(p, ix);
+  if (rc) return rc;
+  if (n_spec < 0 || !n_cand || !c_row || !c_hyp || !c_cnt || !hyp_scored || !hyp_matched || !status)
+    return VH_E_ARG;
+  if (p->topk < 1 || p->topk > 1024 || (p->pair_closure && !ix->partner)) return VH_E_ARG;
+  if (cand_stride < (p->pair_closure ? 2 * p->topk : p->topk)) return VH_E_CAPACITY;
+  if (n_spec == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  K1Out O = {};
+  O.cand_stride = cand_stride;
+  O.n_cand = n_cand;
+  O.c_row = c_row;
+  O.c_hyp = c_hyp;
+  O.c_cnt = c_cnt;
+  O.hyp_scored = hyp_scored;
+  O.hyp_matched = hyp_matched;
+  O.status = status;
+  int waves;
+  k1_geometry(p, 1, O, waves);
+  if (O.wave_bytes > 65536) return VH_E_LIMIT;
+  unsigned g = (unsigned)((n_spec + waves - 1) / waves);
+  k1_kernel<1><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
+                                                                      pk_off, pk_mass, hyp_off, h_maxfz, h_lo, h_hi,
+                                                                      O);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32_t* c_hyp, const int32_t* c_row,
+                const int32_t* h_spec, const int8_t* h_maxfz, const int64_t* pk_off, const float* pk_mass,
+                int32_t* out_cnt, void* stream) {
+  if (!p || !ix || n < 0) return VH_E_ARG;
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  unsigned g = (unsigned)((n + 255) / 256);
+  k1_brute_kernel<<<g, 256, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n, c_hyp, c_row, h_spec, h_maxfz, pk_off,
+                                     pk_mass, out_cnt);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // extern "C"

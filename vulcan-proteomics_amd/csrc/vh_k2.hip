@@ -1,0 +1,226 @@
+// K2 exact b/y rescoring and PSM ranking.
+//
+// K2 restates k2_kernel (vulcan/gpu/kernels.cu:237-376): for every candidate, all b_k / y_k (k = 0..L-2, no
+// index filter) at fragment charges 1..maxfz-1, window [tol_bound(f/z, lo), tol_bound(f/z, hi)] over the
+// spectrum's mass-sorted peaks (first 512), the most intense peak wins (ties -> later peak, `>=`);
+// ppm += I*|mz - m|*2e6/(mz + m); b ions first (k asc, z asc), then y ions; longest consecutive ladder with the
+// reference's run bookkeeping; hyperscore = ln(f64(sb+1) * f64(sy+1)) + lnfact(mb) + lnfact(my), 255 if not
+// finite. One wave64 per spectrum group: peaks are staged in LDS once, lanes take candidates, the tolerance
+// window is found by binary search in LDS. The log-factorial (Stirling, lnfact(0) = 1) comes from an f64 LDS
+// table built once per block.
+// Ranking restates psm_table's order (engine.py:366-385): mb + my >= min_matched_peaks, hyperscore desc,
+// row asc (then candidate index, the stable-sort tie order).
+#include "vh_common.h"
+
+#define K2_WAVES 4
+#define K2_LNF 256
+
+__device__ __forceinline__ double lnfact_dev(int n) {   // kernels.cu:246-250
+  if (n == 0) return 1.0;
+  double x = (double)n;
+  return x * log(x) - x + 0.5 * log(x) + 0.5 * log(2.0 * 3.141592653589793 * x);
+}
+
+struct Run {
+  int start, length, last, longest;
+};
+
+__device__ __forceinline__ void run_matched(Run& r, int idx) {   // kernels.cu:252-258
+  if (r.last == idx) return;
+  if (r.start + r.length == idx) {
+    r.length += 1;
+    r.longest = r.longest > r.length ? r.longest : r.length;
+  } else {
+    r.start = idx;
+    r.length = 1;
+    r.longest = r.longest > 1 ? r.longest : 1;
+  }
+  r.last = idx;
+}
+
+struct K2Buf {
+  double* hs;
+  float *sb, *sy, *ppm;
+  int16_t *mb, *my, *lb, *ly;
+};
+
+__global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups, const int32_t* __restrict__ grp_spec,
+                                                 const int64_t* __restrict__ grp_start,
+                                                 const int32_t* __restrict__ grp_count, int stride,
+                                                 const int32_t* __restrict__ c_row, const int32_t* __restrict__ c_hyp,
+                                                 const int8_t* __restrict__ h_maxfz,
+                                                 const int64_t* __restrict__ pk_off,
+                                                 const float* __restrict__ pk_mass,
+                                                 const float* __restrict__ pk_int, K2Buf O) {
+  __shared__ float rm[52];
+  __shared__ double lnf[K2_LNF];
+  __shared__ float s_m[K2_WAVES][VH_K2_MAX_PEAKS];
+  __shared__ float s_i[K2_WAVES][VH_K2_MAX_PEAKS];
+  load_rm(rm, P);
+  for (int i = threadIdx.x; i < K2_LNF; i += blockDim.x) lnf[i] = lnfact_dev(i);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  const int g = blockIdx.x * K2_WAVES + w;
+  if (g >= n_groups) return;
+  const int s = grp_spec ? grp_spec[g] : g;
+  const int64_t c0 = grp_start ? grp_start[g] : (int64_t)g * stride;
+  const int nc = grp_count[g];
+  const int64_t q0 = pk_off[s];
+  int np = (int)(pk_off[s + 1] - q0);
+  np = np < VH_K2_MAX_PEAKS ? np : VH_K2_MAX_PEAKS;
+  float* sm = s_m[w];
+  float* si = s_i[w];
+  for (int j = lane; j < np; j += 64) {
+    sm[j] = pk_mass[q0 + j];
+    si[j] = pk_int[q0 + j];
+  }
+  wave_sync();
+  for (int64_t c = c0 + lane; c < c0 + nc; c += 64) {
+    const int r = c_row[c];
+    const int maxfz = h_maxfz[c_hyp[c]];
+    const uint8_t* sq = X.residues + X.seq_off[r];
+    const int L = X.length[r];
+    const uint64_t mk = X.mask[r];
+    const float nt = (mk >> 63) ? P.nterm : 0.f;
+    float sb = 0.f, sy = 0.f, ppm = 0.f;
+    int mb = 0, my = 0;
+    Run rb = {0, 0, 0, 0}, ry = {0, 0, 0, 0};
+    for (int kind = 0; kind < 2; ++kind) {
+      float cum = kind == 0 ? nt : X.mass32[r] - nt;
+      for (int k = 0; k < L - 1; ++k) {
+        const float rmv = rm[((mk >> k) & 1ULL) * 26 + (sq[k] - 'A')];
+        cum = kind == 0 ? cum + rmv : cum - rmv;
+        for (int z = 1; z < maxfz; ++z) {
+          const float mz = cum / (float)z;
+          const float a = tol_bound(mz, P.tol_lo, P.tol_da);
+          const float bd = tol_bound(mz, P.tol_hi, P.tol_da);
+          int lo = 0, hi = np;
+          while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (sm[mid] < a) lo = mid + 1; else hi = mid;
+          }
+          int best = -1;
+          float mx = 0.f;
+          for (int j = lo; j < np && sm[j] <= bd; ++j) {
+            const float v = si[j];
+            if (v >= mx) { mx = v; best = j; }
+          }
+          if (best >= 0) {
+            const float pmass = sm[best], pint = si[best];
+            const float num = (pint * fabsf(mz - pmass)) * 2.0e6f;
+            ppm = ppm + num / (mz + pmass);
+            if (kind == 0) { mb += 1; sb = sb + pint; run_matched(rb, k); }
+            else { my += 1; sy = sy + pint; run_matched(ry, k); }
+          }
+        }
+      }
+    }
+    const double i2 = (double)(sb + 1.0f) * (double)(sy + 1.0f);
+    double hs = log(i2) + (mb < K2_LNF ? lnf[mb] : lnfact_dev(mb)) + (my < K2_LNF ? lnf[my] : lnfact_dev(my));
+    if (!isfinite(hs)) hs = 255.0;
+    O.hs[c] = hs;
+    O.sb[c] = sb;
+    O.sy[c] = sy;
+    O.ppm[c] = ppm / (sb + sy);
+    O.mb[c] = (int16_t)mb;
+    O.my[c] = (int16_t)my;
+    O.lb[c] = (int16_t)rb.longest;
+    O.ly[c] = (int16_t)ry.longest;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ ranking
+struct RKey {
+  double hs;
+  int row;
+  long long idx;   // -1 = none
+};
+
+// a before b in (hyperscore desc, row asc, index asc)
+__device__ __forceinline__ bool rk_before(const RKey& a, const RKey& b) {
+  if (a.idx < 0) return false;
+  if (b.idx < 0) return true;
+  if (a.hs != b.hs) return a.hs > b.hs;
+  if (a.row != b.row) return a.row < b.row;
+  return a.idx < b.idx;
+}
+
+__global__ __launch_bounds__(256) void rank_kernel(DevP P, int n_groups, const int64_t* __restrict__ grp_start,
+                                                   const int32_t* __restrict__ grp_count, int stride,
+                                                   const int32_t* __restrict__ c_row, const double* __restrict__ hs,
+                                                   const int16_t* __restrict__ mb, const int16_t* __restrict__ my,
+                                                   int report, int64_t* o_sel, double* o_next) {
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  const int g = blockIdx.x * (blockDim.x >> 6) + w;
+  if (g >= n_groups) return;
+  const int64_t c0 = grp_start ? grp_start[g] : (int64_t)g * stride;
+  const int nc = grp_count[g];
+  RKey prev = {0.0, 0, -2};   // idx -2: nothing taken yet
+  for (int i = 0; i <= report; ++i) {
+    RKey best = {0.0, 0, -1};
+    for (int j = lane; j < nc; j += 64) {
+      int64_t c = c0 + j;
+      if ((int)mb[c] + (int)my[c] < P.min_matched_peaks) continue;
+      RKey k = {hs[c], c_row[c], (long long)c};
+      if (prev.idx != -2 && !rk_before(prev, k)) continue;   // only candidates after the previous pick
+      if (rk_before(k, best)) best = k;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      RKey x;
+      x.hs = __shfl_xor(best.hs, o);
+      x.row = __shfl_xor(best.row, o);
+      x.idx = __shfl_xor(best.idx, o);
+      if (rk_before(x, best)) best = x;
+    }
+    if (lane == 0) {
+      if (i < report) o_sel[(int64_t)g * report + i] = best.idx;
+      if (i > 0) o_next[(int64_t)g * report + i - 1] = best.idx >= 0 ? best.hs : 0.0;
+    }
+    if (best.idx < 0) {
+      if (lane == 0)
+        for (int k = i + 1; k <= report; ++k) {
+          if (k < report) o_sel[(int64_t)g * report + k] = -1;
+          o_next[(int64_t)g * report + k - 1] = 0.0;
+        }
+      break;
+    }
+    prev = best;
+  }
+}
+
+extern "C" {
+
+int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* grp_spec,
+          const int64_t* grp_start, const int32_t* grp_count, int32_t stride, const int32_t* c_row,
+          const int32_t* c_hyp, const int8_t* h_maxfz, const float* c_shift, const int64_t* pk_off,
+          const float* pk_mass, const float* pk_int, double* o_hs, float* o_sb, float* o_sy, float* o_ppm,
+          int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, void* stream) {
+  if (!p || !ix || n_groups < 0 || !grp_count) return VH_E_ARG;
+  if (!grp_start && stride <= 0) return VH_E_ARG;
+  if (c_shift) return VH_E_ARG;   // shifted scoring: separate entry in a later ABI version
+  if (n_groups == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  K2Buf O = {o_hs, o_sb, o_sy, o_ppm, o_mb, o_my, o_lb, o_ly};
+  unsigned g = (unsigned)((n_groups + K2_WAVES - 1) / K2_WAVES);
+  k2_kernel<<<g, 64 * K2_WAVES, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_groups, grp_spec, grp_start,
+                                         grp_count, stride, c_row, c_hyp, h_maxfz, pk_off, pk_mass, pk_int, O);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_rank(const VhParams* p, int32_t n_groups, const int64_t* grp_start, const int32_t* grp_count, int32_t stride,
+            const int32_t* c_row, const double* hs, const int16_t* mb, const int16_t* my, int32_t report,
+            int64_t* o_sel, double* o_next, void* stream) {
+  if (!p || n_groups < 0 || !grp_count || report < 1 || !o_sel || !o_next) return VH_E_ARG;
+  if (!grp_start && stride <= 0) return VH_E_ARG;
+  if (n_groups == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  unsigned g = (unsigned)((n_groups + 3) / 4);
+  rank_kernel<<<g, 256, 0, st>>>(vh_make_devp(p), n_groups, grp_start, grp_count, stride, c_row, hs, mb, my, report,
+                                 o_sel, o_next);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // extern "C"

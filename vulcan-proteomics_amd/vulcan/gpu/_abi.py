@@ -1,0 +1,94 @@
+"""ctypes mirror of include/vulcan_hip.h (VhParams, VhIndexView) and conversion from SearchParams.
+
+Torch-free. The struct layouts are checked against the C side with vh_abi_sizeof (tests).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .host import SAGE_MONO, SearchParams
+
+F32, I32, I64, U32, P = C.c_float, C.c_int32, C.c_int64, C.c_uint32, C.c_void_p
+
+
+class VhParams(C.Structure):
+    _fields_ = [("mono", F32 * 26), ("stat", F32 * 26), ("var", F32 * 26), ("nterm", F32), ("tol_da", I32),
+                ("tol_lo", F32), ("tol_hi", F32), ("fmin", F32), ("fmax", F32), ("min_ion_index", I32),
+                ("prec_lo", F32), ("prec_hi", F32), ("iso_lo", I32), ("iso_hi", I32), ("z_lo", I32), ("z_hi", I32),
+                ("max_fragment_charge", I32), ("max_peaks", I32), ("min_peaks", I32), ("min_matched_peaks", I32),
+                ("topk", I32), ("k1_threshold", I32), ("pair_closure", I32), ("tile", I32), ("reserved", I32 * 8)]
+
+
+class VhIndexView(C.Structure):
+    _fields_ = [("n_rows", I64), ("seq_off", P), ("length", P), ("mask", P), ("mass32", P), ("residues", P),
+                ("partner", P), ("bucket_log2", I32), ("bin_shift", I32), ("bin_base", U32), ("n_bins", I32),
+                ("n_buckets", I64), ("offsets", P), ("entries", P), ("n_postings", I64)]
+
+
+def mod_arrays(static_mods: dict, var_mods: dict):
+    """f32 static/variable tables and the N-term delta as the reference sets them (engine.py:71-82)."""
+    stat = np.zeros(26, np.float32)
+    var = np.zeros(26, np.float32)
+    nterm = 0.0
+    for r, m in static_mods.items():
+        stat[ord(r) - 65] = np.float32(m)
+    for r, m in var_mods.items():
+        if r == "^":
+            nterm = float(np.float32(m))
+            continue
+        var[ord(r) - 65] = np.float32(m)
+    return stat, var, nterm
+
+
+def make_params(p: SearchParams, static_mods: dict, var_mods: dict) -> VhParams:
+    if p.fragment_unit not in ("ppm", "da"):
+        raise ValueError("fragment_unit must be 'ppm' or 'da'")
+    if p.peak_mode != "global":
+        raise ValueError("only the global peak view is implemented")
+    stat, var, nterm = mod_arrays(static_mods, var_mods)
+    v = VhParams()
+    v.mono[:] = SAGE_MONO.tolist()
+    v.stat[:] = stat.tolist()
+    v.var[:] = var.tolist()
+    v.nterm = nterm
+    v.tol_da = int(p.fragment_unit == "da")
+    v.tol_lo, v.tol_hi = float(p.fragment_ppm[0]), float(p.fragment_ppm[1])
+    v.fmin, v.fmax = float(p.fragment_min_mz), float(p.fragment_max_mz)
+    v.min_ion_index = int(p.min_ion_index)
+    v.prec_lo, v.prec_hi = float(p.precursor_ppm[0]), float(p.precursor_ppm[1])
+    v.iso_lo, v.iso_hi = int(p.isotope_errors[0]), int(p.isotope_errors[1])
+    v.z_lo, v.z_hi = int(p.precursor_charge[0]), int(p.precursor_charge[1])
+    v.max_fragment_charge = int(p.max_fragment_charge or 0)
+    v.max_peaks, v.min_peaks = int(p.max_peaks), int(p.min_peaks)
+    v.min_matched_peaks = int(p.min_matched_peaks)
+    v.topk = int(p.topk)
+    v.k1_threshold = int(p.k1_threshold)
+    v.pair_closure = int(bool(p.pair_closure))
+    v.tile = int(p.tile)
+    return v
+
+
+def max_hyp_per_spectrum(p: SearchParams) -> int:
+    niso = p.isotope_errors[1] - p.isotope_errors[0] + 1
+    return niso * max(1, p.precursor_charge[1] - p.precursor_charge[0] + 1)
+
+
+def f32_bits(x: float) -> int:
+    return int(np.array([x], np.float32).view(np.uint32)[0])
+
+
+def index_geometry(p: SearchParams, n_rows: int):
+    """(bucket_log2, bin_shift, bin_base, n_bins, n_buckets) of the fragment index for `n_rows` rows."""
+    c = int(p.chunk_size)
+    if c <= 0 or c & (c - 1) or c > 2 ** 16:
+        raise ValueError("chunk_size must be a power of two <= 65536")
+    lb = c.bit_length() - 1
+    s = p.bin_shift()
+    base = f32_bits(p.fragment_min_mz) >> s
+    n_bins = (f32_bits(p.fragment_max_mz) >> s) - base + 1
+    n_buckets = max(1, (n_rows + c - 1) // c)
+    if n_buckets * n_bins >= 2 ** 32:
+        raise ValueError("bucket x bin directory exceeds 2^32 entries; raise chunk_size or bin_ppm")
+    return lb, s, base, n_bins, n_buckets

@@ -1,0 +1,81 @@
+"""ctypes binding of libvulcan_hip.so (csrc/*.hip, C ABI in include/vulcan_hip.h).
+
+Same surface as the reference's vulcan/gpu/_lib.py:22-76: `lib()` (re)builds the library when the recorded
+source hash differs and loads it with ctypes, `ptr()` checks device + contiguity, `stream()` is torch's
+current stream (a hipStream_t on ROCm), `check()` raises RuntimeError on a nonzero return.
+There is no CPU fallback: without the library or a ROCm device every call fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from .. import _build
+from ._abi import VhIndexView, VhParams
+
+P = C.c_void_p
+I32, I64, SZ = C.c_int32, C.c_int64, C.c_size_t
+PP = C.POINTER(VhParams)
+PIX = C.POINTER(VhIndexView)
+
+_SIGS = {
+    "vh_version": ([], I32),
+    "vh_abi_sizeof": ([I32], SZ),
+    "vh_scan_workspace_bytes": ([I64], SZ),
+    "vh_copy_u32": ([P, P, I64, P], I32),
+    "vh_frag_count": ([PP, PIX, P, P, SZ, P], I32),
+    "vh_index_workspace_bytes": ([I64, I64, I32], SZ),
+    "vh_index_build": ([PP, PIX, P, P, P, P, SZ, P], I32),
+    "vh_k0": ([PP, I32, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
+    "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
+    "vh_k1": ([PP, PIX, I32, P, P, P, P, P, P, I64, P, P, P, P, P, P, P], I32),
+    "vh_k1_topk": ([PP, PIX, I32, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P], I32),
+    "vh_k1_brute": ([PP, PIX, I64, P, P, P, P, P, P, P, P], I32),
+    "vh_k2": ([PP, PIX, I32, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
+}
+
+ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "workspace too small", -4: "size limit exceeded"}
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load the library without touching the GPU (symbol/ABI checks on CPU-only hosts)."""
+    L = C.CDLL(path or _build.build_hip(force=False))
+    for name, (args, res) in _SIGS.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not torch.cuda.is_available():
+        raise RuntimeError("libvulcan_hip needs a ROCm device (torch.cuda.is_available() is False)")
+    torch.cuda.init()
+    _lib = load()
+    if _lib.vh_abi_sizeof(0) != C.sizeof(VhParams) or _lib.vh_abi_sizeof(1) != C.sizeof(VhIndexView):
+        raise RuntimeError("VhParams / VhIndexView layout differs between Python and libvulcan_hip.so")
+    return _lib
+
+
+def ptr(t: torch.Tensor | None):
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous(), "kernels need contiguous device tensors"
+    return t.data_ptr()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def check(code: int, what: str):
+    if code != 0:
+        msg = ERRORS.get(code, f"HIP error {code}")
+        raise RuntimeError(f"{what}: {msg} ({code})")

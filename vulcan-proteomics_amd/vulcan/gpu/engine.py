@@ -1,0 +1,603 @@
+"""Closed search on the MI355X: K0 (peak view) -> hypotheses -> K1 (fragment index, top-K, pair closure) -> K2
+(exact rescoring) -> ranking. Call surface of the reference's vulcan/gpu/engine.py (GpuIndex, k0_preprocess,
+run_k1, select_topk, run_k2, lnfact, psm_table, search, window_candidates, k1_brute, search_exhaustive, Stats,
+Timer) so callers switch without changes.
+
+Every compute step is a HIP kernel of libvulcan_hip.so; torch tensors are only device buffers. The fused
+device pipeline (`DeviceSearch`) runs K0 -> rank 1 with no host synchronisation and no allocation per call, so
+it can be captured in a hipGraph. Nothing here reads labels: candidates are rows of the content-sorted table.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import resource
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import pyarrow as pa
+import torch
+
+from . import _lib as K
+from ._abi import VhIndexView, index_geometry, make_params, max_hyp_per_spectrum
+from .host import NEUTRON, PROTON, SAGE_MONO, SearchParams, Spectra, make_hypotheses, pre_hypotheses  # noqa: F401
+
+ROW_BITS = 28   # peptidoform rows < 2^28 (engine.py:35 of the reference)
+
+
+def _u8(t):
+    return t.view(torch.uint8) if t.dtype != torch.uint8 else t
+
+
+@dataclass
+class Slice:
+    """One resident posting directory (the whole index is one slice here; 288 GB HBM holds it)."""
+    chunk0: int
+    chunk1: int
+    offsets: torch.Tensor   # int32 bit pattern of u32, n_buckets * n_bins + 1
+    entries: torch.Tensor   # int32 bit pattern of u32 postings
+
+
+@dataclass
+class Stats:
+    h2d_bytes: int = 0
+    gpu_ms: dict = field(default_factory=dict)
+    wall_s: dict = field(default_factory=dict)
+    counts: dict = field(default_factory=dict)
+
+
+class GpuIndex:
+    """Device-resident peptidoform table + fragment index (reference engine.py:59-153).
+
+    Buckets of `chunk_size` consecutive rows (precursor-mass intervals) x f32-bit-pattern fragment bins; the
+    keys are sorted on the device by an LSD radix sort (vh_index_build)."""
+
+    def __init__(self, l1, params: SearchParams, static_mods: dict, var_mods: dict, device="cuda"):
+        t0 = time.perf_counter()
+        self.p = params
+        self.dev = torch.device(device)
+        L = K.lib()
+        self.vp = make_params(params, static_mods, var_mods)
+        if params.fragment_min_mz <= 0:
+            raise ValueError("fragment_min_mz must be positive")
+        self.n = len(l1)
+        d = self.dev
+        self.h2d = 0
+
+        def up(a, dtype=None):
+            t = torch.from_numpy(np.ascontiguousarray(a)).to(d)
+            self.h2d += t.numel() * t.element_size()
+            return t
+        self.mass32_np = np.ascontiguousarray(l1.mass32, np.float32)
+        self.seq_off = up(l1.seq_off.astype(np.int64))
+        self.length = up(l1.length.astype(np.uint8))
+        self.mask = up(l1.mask.view(np.int64))
+        self.mass32 = up(self.mass32_np)
+        self.residues = up(l1.residues.astype(np.uint8))
+        self.partner = up(l1.partner.astype(np.int64))
+        if self.n >= 2 ** ROW_BITS:
+            raise ValueError("more than 2^28 peptidoform rows")
+        lb, s, base, n_bins, n_buckets = index_geometry(params, self.n)
+        self.bucket_log2, self.bin_shift, self.bin_base, self.n_bins, self.n_buckets = lb, s, base, n_bins, n_buckets
+        self.view = VhIndexView()
+        v = self.view
+        v.n_rows = self.n
+        v.seq_off, v.length, v.mask = K.ptr(self.seq_off), K.ptr(self.length), K.ptr(self.mask)
+        v.mass32, v.residues, v.partner = K.ptr(self.mass32), K.ptr(self.residues), K.ptr(self.partner)
+        v.bucket_log2, v.bin_shift, v.bin_base, v.n_bins, v.n_buckets = lb, s, base, n_bins, n_buckets
+        st = K.stream()
+        self.post_off = torch.empty(self.n + 1, dtype=torch.int64, device=d)
+        ws = torch.empty(int(L.vh_scan_workspace_bytes(self.n)), dtype=torch.uint8, device=d)
+        K.check(L.vh_frag_count(C.byref(self.vp), C.byref(v), K.ptr(self.post_off), K.ptr(ws), ws.numel(), st),
+                "frag_count")
+        self.n_postings = int(self.post_off[-1].item())
+        if self.n_postings >= 2 ** 32 or self.n_postings > params.max_slice_entries:
+            raise ValueError("more than 2^32 - 1 postings")
+        v.n_postings = self.n_postings
+        self.offsets = torch.empty(n_buckets * n_bins + 1, dtype=torch.int32, device=d)
+        self.entries = torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d)
+        wsb = int(L.vh_index_workspace_bytes(self.n_postings, n_buckets, n_bins))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=d)
+        K.check(L.vh_index_build(C.byref(self.vp), C.byref(v), K.ptr(self.post_off), K.ptr(self.offsets),
+                                 K.ptr(self.entries), K.ptr(ws), wsb, st), "index_build")
+        v.offsets, v.entries = K.ptr(self.offsets), K.ptr(self.entries)
+        torch.cuda.synchronize()
+        del ws
+        self.slices = [Slice(0, n_buckets, self.offsets, self.entries)]
+        self.inv_logw = None
+        self.build_s = time.perf_counter() - t0
+        self.device_bytes = sum(t.numel() * t.element_size() for t in
+                                [self.seq_off, self.length, self.mask, self.mass32, self.residues, self.partner,
+                                 self.offsets, self.entries])
+
+    def params_for(self, p: SearchParams):
+        """VhParams for a SearchParams that shares this index's residue tables (topk / tile / tolerance may
+        differ; the fragment range and min_ion_index must match the index)."""
+        if (p.fragment_min_mz, p.fragment_max_mz, p.min_ion_index) != (self.p.fragment_min_mz,
+                                                                       self.p.fragment_max_mz, self.p.min_ion_index):
+            raise ValueError("fragment range / min_ion_index differ from the index")
+        v = make_params(p, {}, {})
+        for f in ("mono", "stat", "var"):
+            getattr(v, f)[:] = getattr(self.vp, f)[:]
+        v.nterm = self.vp.nterm
+        return v
+
+
+# ------------------------------------------------------------------------------------------------ spectra
+def upload_spectra(sp: Spectra, dev, stats: Stats | None = None) -> dict:
+    """Host run -> device buffers (raw peaks CSR, precursor m/z f64, charge i8)."""
+    d = {"mz": torch.from_numpy(np.ascontiguousarray(sp.mz, np.float32)).to(dev),
+         "int": torch.from_numpy(np.ascontiguousarray(sp.intensity, np.float32)).to(dev),
+         "off": torch.from_numpy(np.ascontiguousarray(sp.offsets, np.int64)).to(dev),
+         "pmz": torch.from_numpy(np.ascontiguousarray(sp.precursor_mz, np.float64)).to(dev),
+         "z": torch.from_numpy(np.ascontiguousarray(sp.charge, np.int8)).to(dev),
+         "n": len(sp)}
+    if stats is not None:
+        stats.h2d_bytes += sum(v.numel() * v.element_size() for k, v in d.items() if k != "n")
+    return d
+
+
+def _k0_dev(vp, raw, p: SearchParams, dev, out=None):
+    L = K.lib()
+    n = raw["n"]
+    cap = max(n * p.max_peaks, 1)
+    o = out or {"off": torch.empty(n + 1, dtype=torch.int64, device=dev),
+                "mass": torch.empty(cap, dtype=torch.float32, device=dev),
+                "int": torch.empty(cap, dtype=torch.float32, device=dev),
+                "tic": torch.empty(max(n, 1), dtype=torch.float32, device=dev),
+                "valid_d": torch.empty(max(n, 1), dtype=torch.uint8, device=dev),
+                "ws": torch.empty(int(L.vh_scan_workspace_bytes(max(n, 1))), dtype=torch.uint8, device=dev)}
+    K.check(L.vh_k0(C.byref(vp), n, K.ptr(raw["mz"]), K.ptr(raw["int"]), K.ptr(raw["off"]), K.ptr(o["off"]),
+                    K.ptr(o["mass"]), K.ptr(o["int"]), K.ptr(o["tic"]), K.ptr(o["valid_d"]), cap, K.ptr(o["ws"]),
+                    o["ws"].numel(), K.stream()), "k0")
+    return o
+
+
+def k0_preprocess(sp: Spectra, p: SearchParams, dev, stats: Stats, gi: GpuIndex | None = None):
+    """K0 on the device (reference engine.py:171-210). Returns device peaks (CSR) + host validity mask."""
+    raw = sp if isinstance(sp, dict) else upload_spectra(sp, dev, stats)
+    vp = gi.vp if gi is not None else make_params(p, {}, {})
+    o = _k0_dev(vp, raw, p, dev)
+    n = raw["n"]
+    o["valid"] = o["valid_d"][:n].cpu().numpy().astype(bool)
+    o["raw"] = raw
+    return o
+
+
+def device_hypotheses(gi: GpuIndex, raw: dict, peaks: dict, p: SearchParams, vp=None, out=None):
+    """Precursor hypotheses on the device (vh_hypotheses; host.py:64-91 semantics). Device dict; `hyp_off`
+    groups them by spectrum."""
+    L = K.lib()
+    dev = gi.dev
+    n = raw["n"]
+    cap = max(n * max_hyp_per_spectrum(p), 1)
+    o = out or {"hyp_off": torch.empty(n + 1, dtype=torch.int64, device=dev),
+                "spec": torch.empty(cap, dtype=torch.int32, device=dev),
+                "z": torch.empty(cap, dtype=torch.int8, device=dev),
+                "iso": torch.empty(cap, dtype=torch.int8, device=dev),
+                "pm": torch.empty(cap, dtype=torch.float32, device=dev),
+                "M": torch.empty(cap, dtype=torch.float32, device=dev),
+                "lo": torch.empty(cap, dtype=torch.int64, device=dev),
+                "hi": torch.empty(cap, dtype=torch.int64, device=dev),
+                "maxfz": torch.empty(cap, dtype=torch.int8, device=dev),
+                "ws": torch.empty(int(L.vh_scan_workspace_bytes(max(n, 1))), dtype=torch.uint8, device=dev)}
+    K.check(L.vh_hypotheses(C.byref(vp or gi.vp), n, K.ptr(raw["pmz"]), K.ptr(raw["z"]), K.ptr(peaks["valid_d"]),
+                            K.ptr(gi.mass32), gi.n, K.ptr(o["hyp_off"]), K.ptr(o["spec"]), K.ptr(o["z"]),
+                            K.ptr(o["iso"]), K.ptr(o["pm"]), K.ptr(o["M"]), K.ptr(o["lo"]), K.ptr(o["hi"]),
+                            K.ptr(o["maxfz"]), cap, K.ptr(o["ws"]), o["ws"].numel(), K.stream()), "hypotheses")
+    return o
+
+
+def hyp_to_host(hd: dict) -> dict:
+    n = int(hd["hyp_off"][-1].item())
+    out = {k: hd[k][:n].cpu().numpy() for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz")}
+    out["hyp_off"] = hd["hyp_off"].cpu().numpy()
+    return out
+
+
+def _hyp_device(gi, hyp, dev):
+    """Accept a host hypothesis dict (reference layout) or a device one; return device tensors + hyp_off."""
+    if isinstance(hyp.get("spec"), torch.Tensor):
+        return hyp
+    s = np.asarray(hyp["spec"])
+    n_spec = int(s.max()) + 1 if len(s) else 0
+    off = np.zeros(n_spec + 1, np.int64)
+    if len(s):
+        if np.any(np.diff(s) < 0):
+            raise ValueError("hypotheses must be grouped by spectrum (make_hypotheses order)")
+        np.cumsum(np.bincount(s, minlength=n_spec), out=off[1:])
+    t = {k: torch.from_numpy(np.ascontiguousarray(hyp[k])).to(dev) for k in ("spec", "maxfz", "lo", "hi")}
+    t["lo"], t["hi"] = t["lo"].to(torch.int64), t["hi"].to(torch.int64)
+    t["spec"] = t["spec"].to(torch.int32)
+    t["maxfz"] = t["maxfz"].to(torch.int8)
+    t["hyp_off"] = torch.from_numpy(off).to(dev)
+    return t
+
+
+class Timer:
+    """Per-phase GPU events + wall clock (reference engine.py:232-247)."""
+
+    def __init__(self, stats: Stats, name: str):
+        self.stats, self.name = stats, name
+
+    def __enter__(self):
+        self.e0 = torch.cuda.Event(enable_timing=True)
+        self.e1 = torch.cuda.Event(enable_timing=True)
+        self.e0.record()
+        self.t0 = time.perf_counter()
+        return self
+
+    def __exit__(self, *a):
+        self.e1.record()
+        self.e1.synchronize()
+        self.stats.gpu_ms[self.name] = self.stats.gpu_ms.get(self.name, 0.0) + self.e0.elapsed_time(self.e1)
+        self.stats.wall_s[self.name] = self.stats.wall_s.get(self.name, 0.0) + time.perf_counter() - self.t0
+
+
+def run_k1(gi: GpuIndex, hyp, peaks, p: SearchParams, stats: Stats):
+    """K1 append form (reference engine.py:250-281): (o_h, o_r, o_c, scored, matched, d_spec, d_maxfz)."""
+    L = K.lib()
+    dev = gi.dev
+    hd = _hyp_device(gi, hyp, dev)
+    n_spec = hd["hyp_off"].numel() - 1
+    n_h = int(hd["hyp_off"][-1].item())
+    n_window = int((hd["hi"][:n_h] - hd["lo"][:n_h]).clamp(min=0).sum().item()) if n_h else 0
+    cap = max(1, n_window)
+    scored = torch.zeros(max(n_h, 1), dtype=torch.int32, device=dev)
+    matched = torch.zeros(max(n_h, 1), dtype=torch.int64, device=dev)
+    n_out = torch.zeros(1, dtype=torch.int64, device=dev)
+    o_h = torch.empty(cap, dtype=torch.int32, device=dev)
+    o_r = torch.empty(cap, dtype=torch.int32, device=dev)
+    o_c = torch.empty(cap, dtype=torch.int32, device=dev)
+    vp = gi.params_for(p)
+    n_pk = peaks["off"].numel() - 1
+    if n_pk < n_spec:
+        raise ValueError("hypotheses reference spectra beyond the peak view")
+    K.check(L.vh_k1(C.byref(vp), C.byref(gi.view), n_spec, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
+                    K.ptr(hd["hyp_off"]), K.ptr(hd["maxfz"]), K.ptr(hd["lo"]), K.ptr(hd["hi"]), cap, K.ptr(n_out),
+                    K.ptr(o_h), K.ptr(o_r), K.ptr(o_c), K.ptr(scored), K.ptr(matched), K.stream()), "k1")
+    n = int(n_out.item())
+    assert n <= cap
+    stats.counts["k1_window_candidates"] = stats.counts.get("k1_window_candidates", 0) + n_window
+    stats.counts["k1_appended"] = stats.counts.get("k1_appended", 0) + n
+    return o_h[:n], o_r[:n], o_c[:n], scored[:n_h], matched[:n_h], hd["spec"][:n_h], hd["maxfz"][:n_h]
+
+
+def select_topk(o_h, o_r, o_c, d_spec, gi: GpuIndex, p: SearchParams):
+    """Top-K per spectrum + pair closure over an appended K1 list (reference engine.py:284-312), as torch
+    device ops for callers that hold an appended list; `search` uses the fused vh_k1_topk kernel instead.
+    Identical keys (same row and count reached from two hypotheses) order the lower hypothesis first."""
+    dev = gi.dev
+    spec = d_spec[o_h.long()].long()
+    key = (spec << 43) | ((32767 - o_c.long().clamp(max=32767)) << ROW_BITS) | ((2 ** ROW_BITS - 1) - o_r.long())
+    o = torch.argsort(o_h.long(), stable=True)
+    o = o[torch.argsort(key[o], stable=True)]
+    ss = spec[o]
+    first = torch.searchsorted(ss, ss, side="left")
+    rank = torch.arange(len(ss), device=dev) - first
+    sel = o[rank < p.topk]
+    h, r, c = o_h[sel].long(), o_r[sel].long(), o_c[sel].long()
+    if p.pair_closure:
+        pr = gi.partner[r]
+        h = torch.cat([h, h])
+        r = torch.cat([r, pr])
+        c = torch.cat([c, torch.full_like(c, -1)])
+    sp_ = d_spec[h].long()
+    k = (sp_ << ROW_BITS) | r
+    uk, inv = torch.unique(k, sorted=True, return_inverse=True)
+    hh = torch.full((len(uk),), 2 ** 62, dtype=torch.int64, device=dev).scatter_reduce_(0, inv, h, reduce="amin")
+    cc = torch.full((len(uk),), -1, dtype=torch.int64, device=dev)
+    cc.scatter_reduce_(0, inv, c, reduce="amax")
+    rr = uk & (2 ** ROW_BITS - 1)
+    return hh, rr, cc
+
+
+def _k2_out(n, dev):
+    out = {"hyperscore": torch.empty(max(n, 1), dtype=torch.float64, device=dev)}
+    for k in ("summed_b", "summed_y", "ppm"):
+        out[k] = torch.empty(max(n, 1), dtype=torch.float32, device=dev)
+    for k in ("matched_b", "matched_y", "longest_b", "longest_y"):
+        out[k] = torch.empty(max(n, 1), dtype=torch.int16, device=dev)
+    return out
+
+
+def _k2_call(gi, vp, n_groups, grp_spec, grp_start, grp_count, stride, row, hyp, maxfz, peaks, out):
+    L = K.lib()
+    K.check(L.vh_k2(C.byref(vp), C.byref(gi.view), n_groups, K.ptr(grp_spec), K.ptr(grp_start), K.ptr(grp_count),
+                    stride, K.ptr(row), K.ptr(hyp), K.ptr(maxfz), None, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
+                    K.ptr(peaks["int"]), K.ptr(out["hyperscore"]), K.ptr(out["summed_b"]), K.ptr(out["summed_y"]),
+                    K.ptr(out["ppm"]), K.ptr(out["matched_b"]), K.ptr(out["matched_y"]), K.ptr(out["longest_b"]),
+                    K.ptr(out["longest_y"]), K.stream()), "k2")
+
+
+def run_k2(gi: GpuIndex, h, r, d_spec, d_maxfz, peaks, p: SearchParams, shift: torch.Tensor | None = None):
+    """K2 over candidates sorted by spectrum (reference engine.py:315-351). Returns dict of device tensors."""
+    if shift is not None:
+        raise NotImplementedError("shifted K2 scoring is not part of this ABI version")
+    dev = gi.dev
+    spec = d_spec[h].long()
+    n = len(r)
+    out = _k2_out(n, dev)
+    if n:
+        us, cnts = torch.unique_consecutive(spec, return_counts=True)
+        start = torch.zeros(len(us), dtype=torch.int64, device=dev)
+        start[1:] = torch.cumsum(cnts, 0)[:-1]
+        _k2_call(gi, gi.params_for(p), len(us), us.to(torch.int32).contiguous(), start,
+                 cnts.to(torch.int32).contiguous(), 0, r.to(torch.int32).contiguous(), h.to(torch.int32).contiguous(),
+                 d_maxfz.contiguous(), peaks, out)
+    out = {k: v[:n] for k, v in out.items()}
+    out["spec"] = spec
+    return out
+
+
+def lnfact(n):
+    """Stirling log-factorial with lnfact(0) = 1.0 (reference engine.py:354-358)."""
+    n = np.asarray(n, np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        v = n * np.log(n) - n + 0.5 * np.log(n) + 0.5 * np.log(2 * np.pi * n)
+    return np.where(n == 0, 1.0, v)
+
+
+def rank_groups(gi, p: SearchParams, n_groups, grp_start, grp_count, stride, row, hs, mb, my, report):
+    """vh_rank: per group the top `report` candidate indices and the next hyperscore (device)."""
+    L = K.lib()
+    dev = gi.dev
+    sel = torch.empty(max(n_groups * report, 1), dtype=torch.int64, device=dev)
+    nxt = torch.empty(max(n_groups * report, 1), dtype=torch.float64, device=dev)
+    if n_groups:
+        K.check(L.vh_rank(C.byref(gi.params_for(p)), n_groups, K.ptr(grp_start), K.ptr(grp_count), stride,
+                          K.ptr(row), K.ptr(hs), K.ptr(mb), K.ptr(my), report, K.ptr(sel), K.ptr(nxt), K.stream()),
+                "rank")
+    return sel[:n_groups * report], nxt[:n_groups * report]
+
+
+def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched, peaks, p: SearchParams,
+              report: int | None = None) -> pa.Table:
+    """Rank candidates per spectrum (vh_rank: hyperscore desc, row asc) and build the PSM table with the
+    reference's columns (engine.py:361-432, PSM_SCHEMA.md section 1)."""
+    report = p.report_psms if report is None else report
+    dev = gi.dev
+    spec = k2["spec"]
+    n = len(spec)
+    if n:
+        us, cnts = torch.unique_consecutive(spec, return_counts=True)
+        start = torch.zeros(len(us), dtype=torch.int64, device=dev)
+        start[1:] = torch.cumsum(cnts, 0)[:-1]
+        row32 = r.to(torch.int32).contiguous()
+        sel, nxt = rank_groups(gi, p, len(us), start, cnts.to(torch.int32).contiguous(), 0, row32,
+                               k2["hyperscore"].contiguous(), k2["matched_b"].contiguous(),
+                               k2["matched_y"].contiguous(), report)
+        sel = sel.view(-1, report)
+        nxt = nxt.view(-1, report)
+        ok = sel >= 0
+        idx = sel[ok]
+        rk = (torch.nonzero(ok)[:, 1] + 1).cpu().numpy()
+        best_idx = sel[:, 0].clamp(min=0)
+        best = torch.repeat_interleave(k2["hyperscore"][best_idx], ok.sum(1))
+        dn_next = nxt[ok]
+    else:
+        idx = torch.zeros(0, dtype=torch.int64, device=dev)
+        rk = np.zeros(0, np.int64)
+        best = torch.zeros(0, dtype=torch.float64, device=dev)
+        dn_next = torch.zeros(0, dtype=torch.float64, device=dev)
+    out = {k: v[idx].cpu().numpy() for k, v in k2.items() if k != "spec"}
+    s = spec[idx].cpu().numpy()
+    hsk = out["hyperscore"]
+    dn = hsk - dn_next.cpu().numpy()
+    db = best.cpu().numpy() - hsk
+    hh = h[idx].cpu().numpy()
+    rows = r[idx].cpu().numpy()
+    k1 = k1c[idx].cpu().numpy()
+    n_spec = len(sp)
+    hyp_h = hyp if not isinstance(hyp.get("spec"), torch.Tensor) else hyp_to_host(hyp)
+    n_h = len(hyp_h["spec"])
+    sc = np.zeros(n_spec, np.int64)
+    mt = np.zeros(n_spec, np.int64)
+    np.add.at(sc, hyp_h["spec"], scored[:n_h].cpu().numpy().astype(np.int64))
+    np.add.at(mt, hyp_h["spec"], matched[:n_h].cpu().numpy().astype(np.int64))
+    tic = peaks["tic"][:n_spec].cpu().numpy()
+    mbk, myk = out["matched_b"].astype(np.int64), out["matched_y"].astype(np.int64)
+    kk = mbk + myk
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        lam = mt[s] / sc[s]
+        pois = lam ** kk * np.exp(-lam) / np.exp(lnfact(kk))
+        pois = np.where(np.isinf(pois), 1e-325, pois)
+        log_pois = np.log10(pois)
+    z = hyp_h["z"][hh].astype(np.int64)
+    iso = hyp_h["iso"][hh].astype(np.int64)
+    pm = hyp_h["pm"][hh]
+    calc = l1.mass32[rows]
+    ie = (iso.astype(np.float32) * NEUTRON).astype(np.float32)
+    dm = (np.abs(pm - calc - ie) * np.float32(2e6) / (pm - ie + calc)).astype(np.float32)
+    plen = l1.length[rows].astype(np.int64)
+    cols = {
+        "run": pa.array([sp.run] * len(s)),
+        "spec_idx": s.astype(np.int64), "scan": sp.scan[s], "rt": sp.rt[s],
+        "rank": rk.astype(np.int32), "pf_row": rows.astype(np.int64), "pf_id": l1.pf_id[rows],
+        "charge": z.astype(np.int8), "isotope_error": iso.astype(np.int8),
+        "expmass": pm.astype(np.float32), "calcmass": calc.astype(np.float32), "delta_mass_ppm": dm,
+        "hyperscore": hsk, "delta_next": dn, "delta_best": db,
+        "matched_b": mbk.astype(np.int16), "matched_y": myk.astype(np.int16), "matched_peaks": kk.astype(np.int16),
+        "summed_intensity": (out["summed_b"] + out["summed_y"]).astype(np.float32),
+        "matched_intensity_pct": (100.0 * (out["summed_b"] + out["summed_y"]) / tic[s]).astype(np.float32),
+        "average_ppm": out["ppm"], "longest_b": out["longest_b"], "longest_y": out["longest_y"],
+        "longest_y_pct": (out["longest_y"] / plen).astype(np.float32),
+        "peptide_len": plen.astype(np.int16), "missed_cleavages": l1.missed[rows], "n_var": l1.n_var[rows],
+        "k1_count": k1.astype(np.int32), "scored_candidates": sc[s].astype(np.int32), "poisson": log_pois,
+    }
+    return pa.table(cols)
+
+
+class DeviceSearch:
+    """The fused closed-search step on preallocated device buffers: vh_k0 -> vh_hypotheses -> vh_k1_topk ->
+    vh_k2 -> vh_rank (report 1). `run(raw)` enqueues the five launches on the current stream and returns the
+    per-spectrum rank-1 result; no host synchronisation, no allocation, hipGraph-capturable."""
+
+    def __init__(self, gi: GpuIndex, p: SearchParams, n_spec: int, report: int = 1):
+        self.gi, self.p, self.n_spec, self.report = gi, p, n_spec, report
+        self.vp = gi.params_for(p)
+        dev = gi.dev
+        L = K.lib()
+        n = max(n_spec, 1)
+        self.stride = 2 * p.topk if p.pair_closure else p.topk
+        maxh = max_hyp_per_spectrum(p)
+        cap_h = n * maxh
+        cap_c = n * self.stride
+        ws = int(L.vh_scan_workspace_bytes(n))
+        e = lambda m, dt: torch.empty(m, dtype=dt, device=dev)  # noqa: E731
+        self.peaks = {"off": e(n + 1, torch.int64), "mass": e(n * p.max_peaks, torch.float32),
+                      "int": e(n * p.max_peaks, torch.float32), "tic": e(n, torch.float32),
+                      "valid_d": e(n, torch.uint8), "ws": e(ws, torch.uint8)}
+        self.hyp = {"hyp_off": e(n + 1, torch.int64), "spec": e(cap_h, torch.int32), "z": e(cap_h, torch.int8),
+                    "iso": e(cap_h, torch.int8), "pm": e(cap_h, torch.float32), "M": e(cap_h, torch.float32),
+                    "lo": e(cap_h, torch.int64), "hi": e(cap_h, torch.int64), "maxfz": e(cap_h, torch.int8),
+                    "ws": e(ws, torch.uint8)}
+        self.n_cand = e(n, torch.int32)
+        self.c_row, self.c_hyp, self.c_cnt = e(cap_c, torch.int32), e(cap_c, torch.int32), e(cap_c, torch.int32)
+        self.scored, self.matched = e(cap_h, torch.int32), e(cap_h, torch.int64)
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.k2 = _k2_out(cap_c, dev)
+        self.sel = e(n * report, torch.int64)
+        self.nxt = e(n * report, torch.float64)
+
+    def run(self, raw: dict, ev: dict | None = None):
+        """ev (optional): stage name -> list, receives a (start, end) pair of torch.cuda.Event per stage,
+        recorded on the launch stream (timing of the individual kernels)."""
+        L = K.lib()
+        gi, vp, p = self.gi, self.vp, self.p
+        n = raw["n"]
+        assert n <= self.n_spec
+        st = K.stream()
+        pk, hy = self.peaks, self.hyp
+
+        def mark(name):
+            if ev is None:
+                return None
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            ev.setdefault(name, []).append(e)
+            return e
+        mark("k0")
+        K.check(L.vh_k0(C.byref(vp), n, K.ptr(raw["mz"]), K.ptr(raw["int"]), K.ptr(raw["off"]), K.ptr(pk["off"]),
+                        K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(pk["tic"]), K.ptr(pk["valid_d"]),
+                        pk["mass"].numel(), K.ptr(pk["ws"]), pk["ws"].numel(), st), "k0")
+        mark("k0")
+        mark("hyp")
+        K.check(L.vh_hypotheses(C.byref(vp), n, K.ptr(raw["pmz"]), K.ptr(raw["z"]), K.ptr(pk["valid_d"]),
+                                K.ptr(gi.mass32), gi.n, K.ptr(hy["hyp_off"]), K.ptr(hy["spec"]), K.ptr(hy["z"]),
+                                K.ptr(hy["iso"]), K.ptr(hy["pm"]), K.ptr(hy["M"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
+                                K.ptr(hy["maxfz"]), hy["spec"].numel(), K.ptr(hy["ws"]), hy["ws"].numel(), st),
+                "hypotheses")
+        mark("hyp")
+        mark("k1_topk")
+        K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.view), n, K.ptr(pk["off"]), K.ptr(pk["mass"]),
+                             K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride,
+                             K.ptr(self.n_cand), K.ptr(self.c_row), K.ptr(self.c_hyp), K.ptr(self.c_cnt),
+                             K.ptr(self.scored), K.ptr(self.matched), K.ptr(self.status), st), "k1_topk")
+        mark("k1_topk")
+        mark("k2")
+        _k2_call(gi, vp, n, None, None, self.n_cand, self.stride, self.c_row, self.c_hyp, hy["maxfz"], pk, self.k2)
+        mark("k2")
+        mark("rank")
+        K.check(L.vh_rank(C.byref(vp), n, None, K.ptr(self.n_cand), self.stride, K.ptr(self.c_row),
+                          K.ptr(self.k2["hyperscore"]), K.ptr(self.k2["matched_b"]), K.ptr(self.k2["matched_y"]),
+                          self.report, K.ptr(self.sel), K.ptr(self.nxt), st), "rank")
+        mark("rank")
+        return {"sel": self.sel[:n * self.report], "next": self.nxt[:n * self.report], "n_cand": self.n_cand[:n]}
+
+    def candidates(self, n: int):
+        """Flatten the per-spectrum candidate lists into (h, r, c) sorted by (spectrum, row) (device)."""
+        cnt = self.n_cand[:n].long()
+        spec = torch.repeat_interleave(torch.arange(n, device=self.gi.dev), cnt)
+        pos = torch.arange(len(spec), device=self.gi.dev) - torch.repeat_interleave(torch.cumsum(cnt, 0) - cnt, cnt)
+        flat = spec * self.stride + pos
+        return self.c_hyp[flat].long(), self.c_row[flat].long(), self.c_cnt[flat].long(), flat
+
+
+def search(sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int | None = None, pre_hyp=None):
+    """Full closed search of one run (reference engine.py:435-464). Returns (PSM table, stats, debug)."""
+    st = Stats()
+    report = p.report_psms if report is None else report
+    ru0 = resource.getrusage(resource.RUSAGE_SELF)
+    t0 = time.perf_counter()
+    dev = gi.dev
+    raw = upload_spectra(sp, dev, st)
+    ds = DeviceSearch(gi, p, len(sp), report=1)
+    with Timer(st, "search_device"):
+        ds.run(raw)
+    n = len(sp)
+    with Timer(st, "features"):
+        h, r, c, flat = ds.candidates(n)
+        k2 = {k: v[flat] for k, v in ds.k2.items()}
+        k2["spec"] = torch.repeat_interleave(torch.arange(n, device=dev), ds.n_cand[:n].long())
+        hyp = hyp_to_host(ds.hyp)
+        tbl = psm_table(sp, gi, l1, hyp, h, r, c, k2, ds.scored, ds.matched, ds.peaks, p, report)
+    if int(ds.status.item()) & 1:
+        raise RuntimeError("k1_topk: a spectrum has more hypotheses than one wave handles")
+    ru1 = resource.getrusage(resource.RUSAGE_SELF)
+    st.wall_s["total"] = time.perf_counter() - t0
+    valid = ds.peaks["valid_d"][:n].cpu().numpy().astype(bool)
+    st.counts.update(n_spectra=n, n_searched=int(valid.sum()), n_hypotheses=len(hyp["spec"]),
+                     n_rescored=int(len(r)), n_psms=tbl.num_rows)
+    st.counts["cpu_core_s"] = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
+    peaks = dict(ds.peaks)
+    peaks["valid"] = valid
+    return tbl, st, {"peaks": peaks, "hyp": hyp, "h": h, "r": r, "k1c": c, "k2": k2, "device": ds}
+
+
+# ------------------------------------------------------------------------------------------------ references
+def window_candidates(hyp, dev, spec_subset: np.ndarray | None = None):
+    """Every (hypothesis, row) in the precursor windows, sorted by (spectrum, row) (engine.py:468-480)."""
+    if isinstance(hyp.get("spec"), torch.Tensor):
+        hyp = hyp_to_host(hyp)
+    lo, hi = np.asarray(hyp["lo"]), np.asarray(hyp["hi"])
+    hsel = np.arange(len(lo))
+    if spec_subset is not None:
+        hsel = hsel[np.isin(hyp["spec"], spec_subset)]
+    n = np.maximum(hi - lo, 0)[hsel]
+    hh = np.repeat(hsel, n)
+    rr = np.repeat(lo[hsel], n) + (np.arange(n.sum()) - np.repeat(np.cumsum(n) - n, n))
+    s = np.asarray(hyp["spec"])[hh]
+    o = np.lexsort((rr, s))
+    return torch.from_numpy(hh[o].astype(np.int64)).to(dev), torch.from_numpy(rr[o].astype(np.int64)).to(dev)
+
+
+def k1_brute(gi: GpuIndex, hyp, peaks, h: torch.Tensor, r: torch.Tensor, p: SearchParams) -> torch.Tensor:
+    """Brute-force K1 counts for (hyp, row) pairs without the index (vh_k1_brute, kernels.cu:195-232)."""
+    L = K.lib()
+    dev = gi.dev
+    hd = _hyp_device(gi, hyp, dev)
+    out = torch.empty(max(len(h), 1), dtype=torch.int32, device=dev)
+    hh = h.to(torch.int32).contiguous()
+    rr = r.to(torch.int32).contiguous()
+    if len(h):
+        K.check(L.vh_k1_brute(C.byref(gi.params_for(p)), C.byref(gi.view), len(h), K.ptr(hh), K.ptr(rr),
+                              K.ptr(hd["spec"]), K.ptr(hd["maxfz"]), K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
+                              K.ptr(out), K.stream()), "k1_brute")
+    return out[:len(h)]
+
+
+def search_exhaustive(sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int | None = None,
+                      spec_subset: np.ndarray | None = None, batch: int = 20_000_000):
+    """K2 over EVERY candidate in the precursor windows (no K1 shortlist): the recall reference
+    (engine.py:500-523)."""
+    st = Stats()
+    dev = gi.dev
+    peaks = k0_preprocess(sp, p, dev, st, gi)
+    hd = device_hypotheses(gi, peaks["raw"], peaks, p)
+    hyp = hyp_to_host(hd)
+    o_h, o_r, o_c, scored, matched, d_spec, d_maxfz = run_k1(gi, hd, peaks, p, st)
+    h, r = window_candidates(hyp, dev, spec_subset)
+    spec = d_spec[h].long()
+    parts, i, n = [], 0, len(h)
+    while i < n:
+        j = min(n, i + batch)
+        if j < n:
+            j = int(torch.searchsorted(spec, spec[j - 1], side="right"))
+        parts.append(run_k2(gi, h[i:j], r[i:j], d_spec, d_maxfz, peaks, p))
+        i = j
+    k2 = {k: torch.cat([q[k] for q in parts]) for k in parts[0]} if parts else run_k2(gi, h, r, d_spec, d_maxfz,
+                                                                                       peaks, p)
+    c = torch.full((len(h),), -1, dtype=torch.int64, device=dev)
+    tbl = psm_table(sp, gi, l1, hyp, h, r, c, k2, scored, matched, peaks, p, report)
+    return tbl, st, {"peaks": peaks, "hyp": hyp, "h": h, "r": r, "k2": k2}

@@ -1,0 +1,206 @@
+"""L1 peptidoform-table builder and loader (the input of the search path).
+
+Restates vulcan/index/build.py:78-248 and :424-450 of the reference:
+  1. digestion (Sage enzyme semantics: cleave after `cleave_at` unless followed by `restrict`, up to
+     `missed_cleavages` internal sites, length filter), unique I/L-collapsed peptides (lib.rs:89-150);
+  2. decoys (vulcan.index.decoys, paired with their forward peptide);
+  3. variable-mod expansion, ids and Sage f32 masses (libvulcan_l1.so, lib.rs:283-399);
+  4. content sort by (mass32, pf_id) (build.py:168) and the symmetric `partner` row (build.py:171-172).
+The index side is label-free; labels are kept apart (labels.arrow) and only used for reporting.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+import pyarrow as pa
+
+from . import _core
+from . import decoys as D
+
+INDEX_VERSION = "vulcan-mi355x-l1-0.1"
+
+
+def default_params() -> dict:
+    """Reference defaults (build.py:52-63): trypsin KR/P, 2 missed, len 7-50, 500-5000 Da, CAM static,
+    Met-ox variable (<= 2)."""
+    return {
+        "enzyme": {"cleave_at": "KR", "restrict": "P", "c_terminal": True, "missed_cleavages": 2,
+                   "min_len": 7, "max_len": 50},
+        "static_mods": {"C": 57.021464},
+        "variable_mods": {"M": 15.9949},
+        "max_variable_mods": 2,
+        "variable_mod_caps": None,
+        "peptide_min_mass": 500.0, "peptide_max_mass": 5000.0,
+        "decoy_seed": D.DECOY_SEED, "id_seed": 0,
+    }
+
+
+def digest(proteins, enzyme: dict):
+    """Unique I/L-collapsed tryptic peptides of `proteins` (sorted) and their minimal missed-cleavage count."""
+    cut = np.zeros(256, bool)
+    for c in enzyme["cleave_at"]:
+        cut[ord(c)] = True
+    restrict = ord(enzyme["restrict"]) if enzyme.get("restrict") else -1
+    mc, lo, hi = enzyme["missed_cleavages"], enzyme["min_len"], enzyme["max_len"]
+    best: dict[bytes, int] = {}
+    for prot in proteins:
+        a = np.frombuffer(prot.encode() if isinstance(prot, str) else prot, np.uint8)
+        n = len(a)
+        if n == 0:
+            continue
+        site = cut[a[:-1]] & (a[1:] != restrict)
+        b = np.concatenate([[0], np.nonzero(site)[0] + 1, [n]])
+        raw = a.tobytes().replace(b"I", b"L")
+        nb = len(b) - 1
+        for m in range(mc + 1):
+            if nb - m <= 0:
+                break
+            s0 = b[:nb - m]
+            s1 = b[m + 1:]
+            ln = s1 - s0
+            ok = np.nonzero((ln >= lo) & (ln <= hi))[0]
+            for i in ok:
+                k = raw[s0[i]:s1[i]]
+                v = best.get(k)
+                if v is None or m < v:
+                    best[k] = m
+    keys = sorted(best)
+    return [k.decode() for k in keys], np.array([best[k] for k in keys], np.uint8)
+
+
+class L1Index:
+    """Index side of an L1 table (label-free), reference build.py:424-450."""
+
+    def __init__(self, path: str | None = None, arrays: dict | None = None):
+        self.path = path
+        if path is not None:
+            with pa.OSFile(os.path.join(path, "peptidoforms.arrow"), "rb") as src:
+                t = pa.ipc.open_file(src).read_all()
+            arrays = {k: t.column(k).to_numpy() for k in t.column_names}
+            arrays["residues"] = np.load(os.path.join(path, "residues.npy"))
+            with open(os.path.join(path, "meta.json")) as fh:
+                self.meta = json.load(fh)
+        else:
+            self.meta = arrays.pop("meta", {})
+        self.pf_id = np.ascontiguousarray(arrays["pf_id"], np.uint64)
+        self.mass32 = np.ascontiguousarray(arrays["mass32"], np.float32)
+        self.seq_off = np.ascontiguousarray(arrays["seq_off"], np.int64)
+        self.length = np.ascontiguousarray(arrays["length"], np.uint8)
+        self.mask = np.ascontiguousarray(arrays["mask"], np.uint64)
+        self.n_var = np.ascontiguousarray(arrays["n_var"], np.uint8)
+        self.missed = np.ascontiguousarray(arrays["missed"], np.uint8)
+        self.partner = np.ascontiguousarray(arrays["partner"], np.int64)
+        self.partners = [self.partner]
+        self.residues = np.ascontiguousarray(arrays["residues"], np.uint8)
+
+    def __len__(self):
+        return len(self.pf_id)
+
+    def sequence(self, i: int) -> str:
+        o = int(self.seq_off[i])
+        return self.residues[o:o + int(self.length[i])].tobytes().decode()
+
+    def save(self, path: str, labels: dict | None = None):
+        os.makedirs(path, exist_ok=True)
+        tbl = pa.table({"pf_id": pa.array(self.pf_id, pa.uint64()), "mass32": pa.array(self.mass32, pa.float32()),
+                        "seq_off": pa.array(self.seq_off, pa.int64()), "length": pa.array(self.length, pa.uint8()),
+                        "mask": pa.array(self.mask, pa.uint64()), "n_var": pa.array(self.n_var, pa.uint8()),
+                        "missed": pa.array(self.missed, pa.uint8()), "partner": pa.array(self.partner, pa.int64())})
+        _write(tbl, os.path.join(path, "peptidoforms.arrow"))
+        np.save(os.path.join(path, "residues.npy"), self.residues)
+        if labels is not None:
+            _write(pa.table({k: pa.array(v) for k, v in labels.items()}), os.path.join(path, "labels.arrow"))
+        with open(os.path.join(path, "meta.json"), "w") as fh:
+            json.dump(self.meta, fh, indent=1, default=str)
+
+
+def _write(table: pa.Table, path: str) -> None:
+    with pa.OSFile(path + ".partial", "wb") as sink, pa.ipc.new_file(sink, table.schema) as w:
+        w.write_table(table)
+    os.replace(path + ".partial", path)
+
+
+def read_labels(path: str) -> dict:
+    with pa.OSFile(os.path.join(path, "labels.arrow"), "rb") as src:
+        t = pa.ipc.open_file(src).read_all()
+    return {k: t.column(k).to_numpy() for k in t.column_names}
+
+
+def build_tables(proteins, params: dict | None = None, shuffle_rows_seed: int | None = None,
+                 swap_labels: bool = False):
+    """Build the L1 table in memory. Returns (L1Index, labels dict). shuffle_rows_seed / swap_labels exist only
+    for the label-blindness test (reference build.py:80-83): the index side must come out bit-identical."""
+    p = default_params()
+    p.update(params or {})
+    t0 = time.perf_counter()
+    e = p["enzyme"]
+    targets, t_missed = digest(proteins, e)
+    t_missed_of = dict(zip(targets, t_missed.tolist()))
+    dec, n_shuf = D.make_decoys(targets, p["decoy_seed"])
+    fw = [t for t in targets if t in dec]
+    dc = [dec[t][0] for t in fw]
+    perms = [dec[t][1] for t in fw]
+    order = np.arange(len(fw))
+    if shuffle_rows_seed is not None:
+        order = np.random.default_rng(shuffle_rows_seed).permutation(len(fw))
+    a_seq = [fw[i] for i in order]
+    b_seq = [dc[i] for i in order]
+    pr = []
+    for i in order:
+        q = perms[i]
+        if q is None:
+            n = len(fw[i])
+            q = np.array([n - 2 - j for j in range(n - 1)] + [n - 1], np.uint8)
+        pr.append(q)
+    if swap_labels:
+        a_seq, b_seq = b_seq, a_seq
+        pr = [np.argsort(q).astype(np.uint8) for q in pr]
+    r = _core.expand_pairs(a_seq, pr, p["static_mods"], p["variable_mods"], p["max_variable_mods"],
+                           p["peptide_min_mass"], p["peptide_max_mass"], p["id_seed"], p.get("variable_mod_caps"))
+    n_pairs = len(r["seq"])
+    seq_i = r["seq"].astype(np.int64)
+    lens = np.array([len(s) for s in a_seq], np.int64)
+    a_off = np.zeros(len(a_seq) + 1, np.int64)
+    np.cumsum(lens, out=a_off[1:])
+    residues = np.concatenate([np.frombuffer("".join(a_seq).encode(), np.uint8),
+                               np.frombuffer("".join(b_seq).encode(), np.uint8)])
+    pf_id = np.concatenate([r["id_f"], r["id_d"]])
+    mass32 = np.concatenate([r["mass32_f"], r["mass32_d"]])
+    seq_off = np.concatenate([a_off[seq_i], a_off[-1] + a_off[seq_i]])
+    length = np.concatenate([lens[seq_i], lens[seq_i]]).astype(np.uint8)
+    mask = np.concatenate([r["mask_f"], r["mask_d"]])
+    n_var = np.concatenate([r["n_var"], r["n_var"]])
+    fwd_of = [y if swap_labels else x for x, y in zip(a_seq, b_seq)]
+    missed_seq = np.array([t_missed_of[f] for f in fwd_of], np.uint8)
+    missed = np.concatenate([missed_seq[seq_i], missed_seq[seq_i]])
+    if len(np.unique(pf_id)) != len(pf_id):
+        raise RuntimeError("peptidoform id collision")
+    srt = np.lexsort((pf_id, mass32))
+    inv = np.empty_like(srt)
+    inv[srt] = np.arange(len(srt))
+    partner_unsorted = np.concatenate([np.arange(n_pairs) + n_pairs, np.arange(n_pairs)])
+    partner = inv[partner_unsorted][srt]
+    lens_s = length[srt].astype(np.int64)
+    new_off = np.zeros(len(srt) + 1, np.int64)
+    np.cumsum(lens_s, out=new_off[1:])
+    gather = np.repeat(seq_off[srt] - new_off[:-1], lens_s) + np.arange(new_off[-1])
+    is_dec_side = np.concatenate([np.zeros(n_pairs, bool), np.ones(n_pairs, bool)])
+    is_decoy = (~is_dec_side if swap_labels else is_dec_side)[srt]
+    meta = {"index_version": INDEX_VERSION, "params": p, "n_targets": len(targets), "n_forward": len(fw),
+            "n_decoy_shuffled": n_shuf, "n_peptidoforms": int(len(pf_id)), "build_s": time.perf_counter() - t0}
+    l1 = L1Index(arrays={"pf_id": pf_id[srt], "mass32": mass32[srt], "seq_off": new_off[:-1],
+                         "length": length[srt], "mask": mask[srt], "n_var": n_var[srt], "missed": missed[srt],
+                         "partner": partner, "residues": residues[gather], "meta": meta})
+    labels = {"pf_id": l1.pf_id, "is_decoy": is_decoy}
+    return l1, labels
+
+
+def build(proteins, out_dir: str, params: dict | None = None, shuffle_rows_seed: int | None = None,
+          swap_labels: bool = False) -> dict:
+    """Build and write peptidoforms.arrow, residues.npy, labels.arrow, meta.json (reference file layout)."""
+    l1, labels = build_tables(proteins, params, shuffle_rows_seed, swap_labels)
+    l1.save(out_dir, labels)
+    return l1.meta
