@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 3
+#define VH_ABI_VERSION 4
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -61,11 +61,11 @@ typedef struct VhParams {
 
 /* Device view of the peptidoform table (vulcan/index/build.py:13-21) and the fragment index.
  * Fragment index: rows are cut into buckets of 2^bucket_log2 consecutive rows (the table is sorted by
- * precursor mass, so a bucket is a precursor-mass interval); inside a bucket, postings are grouped by
- * fragment bin = (f32 bit pattern of the fragment mass >> bin_shift) - bin_base and, inside a bin, ordered
- * by row. offsets[bucket * n_bins + bin] .. offsets[... + 1] delimit a (bucket, bin) posting list.
- * A posting is (row - bucket start) << bin_shift | (f32 bits & (2^bin_shift - 1)), i.e. it holds the exact
- * f32 fragment mass, so index lookups compare exactly what Sage compares (no lossy quantisation). */
+ * precursor mass, so a bucket is a precursor-mass interval). Inside a bucket the postings are sorted by the
+ * IEEE-754 bit pattern of the fragment mass (order-preserving for positive floats, so the exact f32 value is
+ * the key: no lossy quantisation), ties by row. post_key[i] = f32 bits of the fragment, post_row[i] = row -
+ * bucket start. The directory offsets[bucket * n_bins + bin] = first posting of the bucket whose key has
+ * (key >> bin_shift) - bin_base >= bin; offsets has n_buckets * n_bins + 1 entries. */
 typedef struct VhIndexView {
   int64_t n_rows;
   const int64_t* seq_off;
@@ -80,7 +80,8 @@ typedef struct VhIndexView {
   int32_t n_bins;
   int64_t n_buckets;
   const uint32_t* offsets;
-  const uint32_t* entries;
+  const uint32_t* post_key;
+  const uint16_t* post_row;
   int64_t n_postings;
 } VhIndexView;
 
@@ -100,11 +101,12 @@ int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, v
                   void* stream);
 /* Scratch bytes of vh_index_build for n_postings keys. */
 size_t vh_index_workspace_bytes(int64_t n_postings, int64_t n_buckets, int32_t n_bins);
-/* Emit 64-bit keys ((bucket * n_bins + bin) << 32 | posting), LSD radix sort them on the device, write
- * ix->entries (n_postings u32) and ix->offsets (n_buckets * n_bins + 1 u32). Replaces vk_frag_emit
- * (kernels.cu:73-101, 401-411) and the torch.sort / bincount / cumsum CSR assembly of engine.py:129-148. */
+/* Emit 64-bit keys (bucket << 42 | (f32 bits - bits(fmin)) << 16 | row in bucket), LSD radix sort them on the
+ * device, write post_key / post_row (n_postings each) and the directory offsets (n_buckets * n_bins + 1).
+ * Replaces vk_frag_emit (kernels.cu:73-101, 401-411) and the torch.sort / bincount / cumsum CSR assembly of
+ * engine.py:129-148. Needs bucket_log2 <= 16 and bits(fmax) - bits(fmin) < 2^26. */
 int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
-                   uint32_t* entries, void* ws, size_t ws_bytes, void* stream);
+                   uint32_t* post_key, uint16_t* post_row, void* ws, size_t ws_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------------- K0 */
 /* Peak view (engine.py:171-210): keep fmin <= m/z <= fmax (not NaN), mass = m/z - PROTON (f32), top

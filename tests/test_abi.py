@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     from vulcan.gpu import _lib
     L = _lib.load()
     for name in declared_symbols():
-        assert hasattr(L, name), name
+        assert hasattr(L._cdll, name), name
     assert set(declared_symbols()) <= set(_lib._SIGS), "binding lacks a signature"
 
 
@@ -54,3 +54,42 @@ def test_argument_errors_are_reported_without_a_gpu():
     vp.max_peaks = 4096
     assert L.vh_k0(C.byref(vp), 1, None, None, C.c_void_p(8), C.c_void_p(8), None, None, C.c_void_p(8),
                    C.c_void_p(8), 10, None, 0, None) == -4
+
+
+def _declarations():
+    """name -> list of parameter type strings, parsed from the header."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"^(?:int|size_t)\s+(vh_\w+)\(([^)]*)\);", text, flags=re.M):
+        params = [x.strip() for x in m.group(2).replace("\n", " ").split(",") if x.strip()]
+        out[m.group(1)] = [] if params == ["void"] else params
+    return out
+
+
+def test_binding_argtypes_match_header_arity_and_kind():
+    """Every ctypes signature has the header's parameter count, and pointer/integer/float kinds line up (a
+    mismatch silently corrupts trailing stack arguments such as the stream handle)."""
+    from vulcan.gpu import _lib
+    decl = _declarations()
+    assert set(decl) == set(_lib._SIGS)
+    for name, params in decl.items():
+        args = _lib._SIGS[name][0]
+        assert len(args) == len(params), (name, len(args), len(params))
+        for a, prm in zip(args, params):
+            is_ptr = "*" in prm
+            if is_ptr:
+                assert a is C.c_void_p or hasattr(a, "_type_") and not isinstance(a._type_, str), (name, prm, a)
+            elif prm.startswith(("int64_t", "size_t")):
+                assert a in (C.c_int64, C.c_size_t), (name, prm, a)
+            elif prm.startswith(("int32_t", "int ")):
+                assert a is C.c_int32, (name, prm, a)
+
+
+def test_binding_rejects_wrong_argument_count():
+    import pytest
+
+    from vulcan.gpu import _lib
+    L = _lib.load()
+    with pytest.raises(TypeError):
+        L.vh_copy_u32(None, None, 0, None, None)
+    assert L.vh_copy_u32(None, None, 0, None) == 0

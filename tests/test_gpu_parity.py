@@ -55,10 +55,11 @@ def _np(t):
 
 # ------------------------------------------------------------------------------------------------ index
 def _postings_numpy(l1, vp, gi):
-    """Posting keys of every row by f32 accumulation (np.add/subtract.accumulate are sequential)."""
+    """(bucket, f32 bits, row in bucket) of every posting, by f32 accumulation in emission order
+    (np.add/subtract.accumulate are sequential)."""
     rm0 = np.array([np.float32(vp.mono[i]) + np.float32(vp.stat[i]) for i in range(26)], np.float32)
     rm1 = np.array([np.float32(vp.mono[i]) + np.float32(vp.var[i]) for i in range(26)], np.float32)
-    keys = []
+    out = []
     mii, fmin, fmax = vp.min_ion_index, np.float32(vp.fmin), np.float32(vp.fmax)
     for i in range(len(l1)):
         L = int(l1.length[i])
@@ -67,21 +68,17 @@ def _postings_numpy(l1, vp, gi):
         rm = np.where(bits[:L - 1] == 1, rm1[s[:L - 1]], rm0[s[:L - 1]]).astype(np.float32)
         b = np.add.accumulate(rm, dtype=np.float32)
         y = np.subtract.accumulate(np.concatenate([[l1.mass32[i]], rm]).astype(np.float32), dtype=np.float32)[1:]
-        k = np.arange(L - 1)
-        for f, ok in ((b, k + 1 > mii), (y, L - 1 - k > mii)):
-            for j in range(L - 1):
-                if ok[j] and fmin <= f[j] <= fmax:
-                    fb = int(np.array([f[j]], np.float32).view(np.uint32)[0])
-                    bucket = i >> gi.bucket_log2
-                    local = i & ((1 << gi.bucket_log2) - 1)
-                    lin = bucket * gi.n_bins + ((fb >> gi.bin_shift) - gi.bin_base)
-                    keys.append((lin << 32) | (local << gi.bin_shift) | (fb & ((1 << gi.bin_shift) - 1)))
-    return np.array(keys, np.uint64)
+        for j in range(L - 1):
+            for f, ok in ((b[j], j + 1 > mii), (y[j], L - 1 - j > mii)):
+                if ok and fmin <= f <= fmax:
+                    fb = int(np.array([f], np.float32).view(np.uint32)[0])
+                    out.append((i >> gi.bucket_log2, fb, i & ((1 << gi.bucket_log2) - 1)))
+    return np.array(out, np.int64).reshape(-1, 3)
 
 
 def test_index_build_matches_numpy(env0):
-    """Device emission + LSD radix sort + directory == numpy emission + stable sort on the (bucket, bin) part,
-    on the first 3,000 rows (a sub-table keeps the Python loop short)."""
+    """Device emission + LSD radix sort + directory == numpy emission + stable sort by (bucket, f32 bits), on the
+    first 3,000 rows (a sub-table keeps the Python loop short)."""
     from vulcan.gpu import engine as E
     from vulcan.index.build import L1Index
     l1 = env0.l1
@@ -92,13 +89,14 @@ def test_index_build_matches_numpy(env0):
     p = env0.p
     p2 = type(p)(**{**p.to_dict(), "chunk_size": 512})
     gi = E.GpuIndex(sub, p2, STATIC, {})
-    keys = _postings_numpy(sub, gi.vp, gi)
-    assert gi.n_postings == len(keys)
-    o = np.argsort(keys >> np.uint64(32), kind="stable")
-    ks = keys[o]
-    assert np.array_equal(_np(gi.entries).view(np.uint32)[:len(ks)], (ks & np.uint64(0xFFFFFFFF)).astype(np.uint32))
-    hi = (ks >> np.uint64(32)).astype(np.int64)
-    want = np.searchsorted(hi, np.arange(gi.n_buckets * gi.n_bins + 1), side="left")
+    post = _postings_numpy(sub, gi.vp, gi)
+    assert gi.n_postings == len(post)
+    o = np.lexsort((post[:, 1], post[:, 0]))              # stable: equal (bucket, key) keep emission order
+    ps = post[o]
+    assert np.array_equal(_np(gi.post_key).view(np.uint32)[:len(ps)].astype(np.int64), ps[:, 1])
+    assert np.array_equal(_np(gi.post_row).view(np.uint16)[:len(ps)].astype(np.int64), ps[:, 2])
+    d = ps[:, 0] * gi.n_bins + ((ps[:, 1] >> gi.bin_shift) - gi.bin_base)
+    want = np.searchsorted(d, np.arange(gi.n_buckets * gi.n_bins + 1), side="left")
     assert np.array_equal(_np(gi.offsets).view(np.uint32).astype(np.int64), want)
 
 

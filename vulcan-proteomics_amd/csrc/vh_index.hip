@@ -1,12 +1,13 @@
-// Fragment inverted index on the device: per-row fragment count, key emission, LSD radix sort, CSR directory.
+// Fragment inverted index on the device: per-row fragment count, key emission, LSD radix sort, directory.
 //
 // Replaces frag_count_kernel / frag_emit_kernel (vulcan/gpu/kernels.cu:53-101) and the torch.sort + bincount +
-// cumsum assembly of GpuIndex.__init__ (vulcan/gpu/engine.py:101-148). Key = (bucket * n_bins + bin) << 32 |
-// posting, where bucket = row >> bucket_log2 (rows are sorted by precursor mass), bin = f32 bits >> bin_shift
-// (minus the bin of fragment_min_mz) and posting = local row << bin_shift | low f32 bits. The f32 bit pattern
-// of a positive float is order-preserving, so every posting keeps the exact fragment mass.
-// Only the high 32 bits are sorted (stable LSD, 8-bit digits): inside a (bucket, bin) list the postings keep
-// their emission order, i.e. rows ascending, which K1 binary-searches.
+// cumsum assembly of GpuIndex.__init__ (vulcan/gpu/engine.py:101-148). Emitted key =
+// bucket << (16 + nd) | (f32 bits - bits(fmin)) << 16 | row in bucket, with bucket = row >> bucket_log2 (rows are
+// sorted by precursor mass) and nd = bits of the f32-bit range of [fmin, fmax]. The f32 bit pattern of a positive
+// float is order-preserving, so the sorted postings of a bucket are in exact fragment-mass order. Only bits
+// [16, 16 + nd + bucket bits) are sorted (stable LSD, 8-bit digits): equal masses keep emission (row) order.
+#include <string.h>
+
 #include "vh_common.h"
 
 // ------------------------------------------------------------------------------------------------ count / emit
@@ -33,14 +34,17 @@ __global__ __launch_bounds__(256) void frag_count_kernel(DevP P, DevIdx X, int64
   post_off[i + 1] = c;
 }
 
-__device__ __forceinline__ uint64_t posting_key(float f, uint64_t bucket, uint32_t local, const DevIdx& X) {
-  uint32_t bits = fbits(f);
-  uint64_t bin = (uint64_t)((bits >> X.bin_shift) - X.bin_base);
-  uint32_t low = bits & ((1u << X.bin_shift) - 1u);
-  return ((bucket * (uint64_t)X.n_bins + bin) << 32) | (uint64_t)((local << X.bin_shift) | low);
+struct KeyGeom {
+  uint32_t fbase;   // f32 bits of fragment_min_mz
+  int nd;           // bits of (bits(fmax) - bits(fmin))
+};
+
+__device__ __forceinline__ uint64_t posting_key(float f, uint64_t bucket, uint32_t local, const KeyGeom& G) {
+  return (bucket << (16 + G.nd)) | ((uint64_t)(fbits(f) - G.fbase) << 16) | (uint64_t)local;
 }
 
-__global__ __launch_bounds__(256) void frag_emit_kernel(DevP P, DevIdx X, const int64_t* post_off, uint64_t* keys) {
+__global__ __launch_bounds__(256) void frag_emit_kernel(DevP P, DevIdx X, KeyGeom G, const int64_t* post_off,
+                                                       uint64_t* keys) {
   __shared__ float rm[52];
   load_rm(rm, P);
   __syncthreads();
@@ -58,8 +62,8 @@ __global__ __launch_bounds__(256) void frag_emit_kernel(DevP P, DevIdx X, const 
     float r = rm[((mk >> k) & 1ULL) * 26 + (s[k] - 'A')];
     b = b + r;
     y = y - r;
-    if (k + 1 > P.min_ion_index && b >= P.fmin && b <= P.fmax) keys[o++] = posting_key(b, bucket, local, X);
-    if (L - 1 - k > P.min_ion_index && y >= P.fmin && y <= P.fmax) keys[o++] = posting_key(y, bucket, local, X);
+    if (k + 1 > P.min_ion_index && b >= P.fmin && b <= P.fmax) keys[o++] = posting_key(b, bucket, local, G);
+    if (L - 1 - k > P.min_ion_index && y >= P.fmin && y <= P.fmax) keys[o++] = posting_key(y, bucket, local, G);
   }
 }
 
@@ -148,8 +152,15 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
 }
 
 // ------------------------------------------------------------------------------------------------ directory
-__global__ __launch_bounds__(256) void directory_kernel(const uint64_t* __restrict__ keys, int64_t n,
-                                                        uint32_t* offsets, uint32_t* entries, int64_t n_dir) {
+__device__ __forceinline__ int64_t dir_of(uint64_t key, const KeyGeom& G, const DevIdx& X) {
+  uint64_t bucket = key >> (16 + G.nd);
+  uint32_t fb = (uint32_t)((key >> 16) & ((1ULL << G.nd) - 1ULL)) + G.fbase;
+  return (int64_t)bucket * X.n_bins + (int64_t)((fb >> X.bin_shift) - X.bin_base);
+}
+
+__global__ __launch_bounds__(256) void directory_kernel(const uint64_t* __restrict__ keys, int64_t n, DevIdx X,
+                                                        KeyGeom G, uint32_t* offsets, uint32_t* post_key,
+                                                        uint16_t* post_row, int64_t n_dir) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n == 0) {
     for (int64_t b = i; b <= n_dir; b += (int64_t)gridDim.x * blockDim.x) offsets[b] = 0;
@@ -157,12 +168,13 @@ __global__ __launch_bounds__(256) void directory_kernel(const uint64_t* __restri
   }
   if (i >= n) return;
   uint64_t kk = keys[i];
-  int64_t hi = (int64_t)(kk >> 32);
-  int64_t prev = i ? (int64_t)(keys[i - 1] >> 32) : -1;
-  for (int64_t b = prev + 1; b <= hi; ++b) offsets[b] = (uint32_t)i;
-  entries[i] = (uint32_t)kk;
+  int64_t d = dir_of(kk, G, X);
+  int64_t prev = i ? dir_of(keys[i - 1], G, X) : -1;
+  for (int64_t b = prev + 1; b <= d; ++b) offsets[b] = (uint32_t)i;
+  post_key[i] = (uint32_t)((kk >> 16) & ((1ULL << G.nd) - 1ULL)) + G.fbase;
+  post_row[i] = (uint16_t)(kk & 0xFFFFULL);
   if (i == n - 1)
-    for (int64_t b = hi + 1; b <= n_dir; ++b) offsets[b] = (uint32_t)n;
+    for (int64_t b = d + 1; b <= n_dir; ++b) offsets[b] = (uint32_t)n;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -194,15 +206,27 @@ int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, v
 }
 
 int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
-                   uint32_t* entries, void* ws, size_t ws_bytes, void* stream) {
-  if (!p || !ix || !post_off || !offsets || (ix->n_postings && !entries)) return VH_E_ARG;
-  if (ix->bin_shift < 4 || ix->bin_shift > 12 || ix->bucket_log2 < 0 || ix->bucket_log2 + ix->bin_shift > 32)
-    return VH_E_ARG;
+                   uint32_t* post_key, uint16_t* post_row, void* ws, size_t ws_bytes, void* stream) {
+  if (!p || !ix || !post_off || !offsets || (ix->n_postings && (!post_key || !post_row))) return VH_E_ARG;
+  if (ix->bin_shift < 4 || ix->bin_shift > 16 || ix->bucket_log2 < 0 || ix->bucket_log2 > 16) return VH_E_ARG;
+  if (!(p->fmin > 0.f) || !(p->fmax >= p->fmin)) return VH_E_ARG;
   if (ix->n_postings < 0 || ix->n_postings >= (1LL << 32)) return VH_E_LIMIT;
   int64_t n_dir = ix->n_buckets * (int64_t)ix->n_bins;
   if (n_dir <= 0 || n_dir >= (1LL << 32)) return VH_E_LIMIT;
   if (ix->n_buckets < ((ix->n_rows + (1LL << ix->bucket_log2) - 1) >> ix->bucket_log2)) return VH_E_ARG;
   if (ws_bytes < vh_index_workspace_bytes(ix->n_postings, ix->n_buckets, ix->n_bins)) return VH_E_WORKSPACE;
+  uint32_t fb0, fb1;
+  memcpy(&fb0, &p->fmin, 4);
+  memcpy(&fb1, &p->fmax, 4);
+  KeyGeom G;
+  G.fbase = fb0;
+  G.nd = 1;
+  while (G.nd < 32 && ((uint64_t)(fb1 - fb0) >> G.nd) != 0) ++G.nd;
+  int nbk = 0;
+  while (nbk < 32 && ((uint64_t)(ix->n_buckets - 1) >> nbk) != 0) ++nbk;
+  if (G.nd > 26 || 16 + G.nd + nbk > 64) return VH_E_LIMIT;
+  if ((fb0 >> ix->bin_shift) != ix->bin_base || (int64_t)((fb1 >> ix->bin_shift) - ix->bin_base) >= ix->n_bins)
+    return VH_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   DevP P = vh_make_devp(p);
   DevIdx X = vh_make_devidx(ix);
@@ -217,14 +241,12 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
   void* sws = w + 2 * kb + align256(hist_n * 4);
   size_t sws_bytes = ws_bytes - (2 * kb + align256(hist_n * 4));
   unsigned grid = (unsigned)((ix->n_rows + 255) / 256);
-  if (ix->n_rows) frag_emit_kernel<<<grid, 256, 0, st>>>(P, X, post_off, ka);
+  if (ix->n_rows) frag_emit_kernel<<<grid, 256, 0, st>>>(P, X, G, post_off, ka);
   VH_CHECK_LAUNCH();
-  int nbits = 0;
-  while (nbits < 32 && ((n_dir - 1) >> nbits) != 0) ++nbits;
   uint64_t* src = ka;
   uint64_t* dst = kb2;
   if (n > 0) {
-    for (int shift = 32; shift < 32 + nbits; shift += 8) {
+    for (int shift = 16; shift < 16 + G.nd + nbk; shift += 8) {
       rs_upsweep_kernel<<<(unsigned)nt, RS_THREADS, 0, st>>>(src, n, shift, hist, nt);
       int rc = vh_scan_exclusive_u32(hist, (int64_t)hist_n, sws, sws_bytes, st);
       if (rc) return rc;
@@ -236,7 +258,7 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
     }
   }
   unsigned g2 = (unsigned)((n > 0 ? n : 1024) + 255) / 256;
-  directory_kernel<<<g2, 256, 0, st>>>(src, n, offsets, entries, n_dir);
+  directory_kernel<<<g2, 256, 0, st>>>(src, n, X, G, offsets, post_key, post_row, n_dir);
   VH_CHECK_LAUNCH();
   return 0;
 }

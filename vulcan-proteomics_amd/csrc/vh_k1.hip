@@ -8,18 +8,26 @@
 //   two hypotheses: lower hypothesis first), + partner rows (count -1), dedupe by row (lowest hypothesis,
 //   highest count), row order.
 // Design: one wave64 per spectrum. All (hypothesis, row) pairs of the spectrum's windows get a slot in an LDS
-// u32 histogram (tiled when they exceed `tile` slots). Each lane takes peaks; for each (peak, z) the exact f32
-// key range [bits(lo bound), bits(hi bound)] selects the bins of every bucket a window touches; inside a bin the
-// postings are row-ordered, so a binary search finds the window's first row, the scan stops at its last row,
-// and matching postings increment their slot with an LDS atomic. The scan of the histogram feeds a wave-private
-// top-K buffer (ballot-compacted inserts, pruned by the current K-th key, bitonic compaction when full).
-// Closure and dedupe are one more bitonic sort. Nothing leaves the wave but the final candidate list.
+// u32 histogram (tiled when they exceed `tile` slots). A tile's segments (hypothesis, row range) are grouped
+// by index bucket. Each lane takes peaks; for each (peak, z) the exact f32 key range [bits(lo bound),
+// bits(hi bound)] is looked up once per bucket: one directory read gives the first posting of the range's first
+// bin, the key-ordered postings are scanned until the key passes the high bound, and postings whose row falls in
+// a segment of that bucket increment their slot with an LDS atomic. The scan of the histogram feeds a
+// wave-private top-K buffer (ballot-compacted inserts, pruned by the current K-th key, bitonic compaction when
+// full). Closure and dedupe are one more bitonic sort. Nothing leaves the wave but the final candidate list.
 #include "vh_common.h"
 
-#define K1_MAXSEG 64
+#define K1_MAXSEG 32
+#define K1_MAXBK 32
 
-struct Seg {
-  int hl, r0, r1, base, mfz, pad0, pad1, pad2;
+struct Seg {   // one (hypothesis, row range) piece of a tile: slots [base, base + r1 - r0)
+  int r0, r1, base, hl_mfz;   // hl_mfz = local hypothesis | maxfz << 8
+};
+
+struct BkEnt {   // a bucket touched by a tile: its segments (bit mask), their row hull and max maxfz
+  int bk, zmax, rlo, rhi;
+  uint32_t mask;
+  int pad0, pad1, pad2;
 };
 
 __device__ void bitonic_sort_u64(uint64_t* a, int n, bool descending) {
@@ -81,7 +89,8 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   unsigned char* wb = smem + (size_t)w * O.wave_bytes;
   uint32_t* hist = (uint32_t*)wb;
   Seg* segs = (Seg*)(wb + O.hist_bytes);
-  uint32_t* hsc = (uint32_t*)(segs + K1_MAXSEG);
+  BkEnt* bks = (BkEnt*)(segs + K1_MAXSEG);
+  uint32_t* hsc = (uint32_t*)(bks + K1_MAXBK);
   unsigned long long* hmt = (unsigned long long*)(hsc + 64);
   uint64_t* buf = (uint64_t*)(hmt + 64);
 
@@ -104,7 +113,6 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   const int np = (int)(pk_off[s + 1] - p0);
   const int T = P.tile;
   const int S = X.bin_shift, LB = X.bucket_log2;
-  const uint32_t lowmask = (1u << S) - 1u;
   const int base = (int)X.bin_base, n_bins = X.n_bins;
   const int B = 1 << LB;
   const int K = P.topk;
@@ -115,8 +123,10 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   int hc = 0;
   int rc = __shfl(my_lo, 0);
   for (;;) {
-    // ---- plan one tile of (hypothesis, row-range) segments (wave-uniform)
-    int used = 0, nseg = 0, zmax = 0;
+    // ---- plan one tile of (hypothesis, row-range) segments (wave-uniform); lane j holds bucket entry j
+    int used = 0, nseg = 0, zmax = 0, nbk = 0;
+    int my_bk = -1, my_bz = 0, my_rlo = 0x7fffffff, my_rhi = 0;
+    uint32_t my_mask = 0;
     while (hc < nh && used < T && nseg < K1_MAXSEG) {
       int hi_c = __shfl(my_hi, hc);
       if (rc >= hi_c) {
@@ -125,12 +135,28 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
         continue;
       }
       int take = hi_c - rc < T - used ? hi_c - rc : T - used;
+      // bucket entries this piece may add must fit: cap its span to the free entries
+      int free_bk = K1_MAXBK - nbk;
+      if (free_bk <= 0) break;
+      int cap_rows = (free_bk << LB) - (rc & (B - 1));
+      take = take < cap_rows ? take : cap_rows;
       int mf = __shfl(my_mfz, hc);
       if (lane == 0) {
         Seg g;
-        g.hl = hc; g.r0 = rc; g.r1 = rc + take; g.base = used; g.mfz = mf;
-        g.pad0 = g.pad1 = g.pad2 = 0;
+        g.r0 = rc; g.r1 = rc + take; g.base = used; g.hl_mfz = hc | (mf << 8);
         segs[nseg] = g;
+      }
+      for (int bk = rc >> LB; bk <= (rc + take - 1) >> LB; ++bk) {
+        uint64_t hit = ballot(lane < nbk && my_bk == bk);
+        int j = hit ? (int)__builtin_ctzll(hit) : nbk;
+        if (!hit) ++nbk;
+        if (lane == j) {
+          my_bk = bk;
+          my_mask |= 1u << nseg;
+          my_bz = mf > my_bz ? mf : my_bz;
+          my_rlo = rc < my_rlo ? rc : my_rlo;
+          my_rhi = rc + take > my_rhi ? rc + take : my_rhi;
+        }
       }
       zmax = mf > zmax ? mf : zmax;
       ++nseg;
@@ -138,6 +164,12 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       rc += take;
     }
     if (nseg == 0) break;
+    if (lane < nbk) {
+      BkEnt e;
+      e.bk = my_bk; e.zmax = my_bz; e.rlo = my_rlo; e.rhi = my_rhi; e.mask = my_mask;
+      e.pad0 = e.pad1 = e.pad2 = 0;
+      bks[lane] = e;
+    }
     for (int j = lane; j < used; j += 64) hist[j] = 0;
     wave_sync();
     // ---- accumulate shared-peak counts
@@ -155,33 +187,23 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
         bl = bl < 0 ? 0 : bl;
         bh = bh > n_bins - 1 ? n_bins - 1 : bh;
         if (bl > bh) continue;
-        for (int q = 0; q < nseg; ++q) {
-          const Seg g = segs[q];
-          if (z >= g.mfz) continue;
-          const int bk0 = g.r0 >> LB, bk1 = (g.r1 - 1) >> LB;
-          for (int bk = bk0; bk <= bk1; ++bk) {
-            const int rb = bk << LB;
-            const uint32_t lo_l = (uint32_t)(g.r0 - rb > 0 ? g.r0 - rb : 0);
-            const uint32_t hi_l = (uint32_t)(g.r1 - rb < B ? g.r1 - rb : B);
-            const uint32_t* off = X.offsets + (size_t)bk * (size_t)n_bins;
-            uint32_t e1 = off[bl];
-            for (int bin = bl; bin <= bh; ++bin) {
-              const uint32_t e0 = e1;
-              e1 = off[bin + 1];
-              if (e0 == e1) continue;
-              uint32_t L = e0, R = e1;
-              while (L < R) {
-                uint32_t M = (L + R) >> 1;
-                if ((X.entries[M] >> S) < lo_l) L = M + 1; else R = M;
-              }
-              const uint32_t hikey = (uint32_t)(bin + base) << S;
-              for (uint32_t e = L; e < e1; ++e) {
-                const uint32_t v = X.entries[e];
-                const uint32_t loc = v >> S;
-                if (loc >= hi_l) break;
-                const uint32_t key = hikey | (v & lowmask);
-                if (key >= ka && key <= kb) atomicAdd(&hist[g.base + rb + (int)loc - g.r0], 1u);
-              }
+        for (int j = 0; j < nbk; ++j) {
+          const BkEnt E = bks[j];
+          if (z >= E.zmax) continue;
+          const uint32_t* off = X.offsets + (size_t)E.bk * (size_t)n_bins;
+          const uint32_t e0 = off[bl], e1 = off[bh + 1];
+          const int rb = E.bk << LB;
+          for (uint32_t e = e0; e < e1; ++e) {
+            const uint32_t key = X.post_key[e];
+            if (key > kb) break;
+            const int row = rb + (int)X.post_row[e];
+            if (key < ka || row < E.rlo || row >= E.rhi) continue;
+            uint32_t msk = E.mask;
+            while (msk) {
+              const int q = __builtin_ctz(msk);
+              msk &= msk - 1u;
+              const Seg g = segs[q];
+              if (z < (g.hl_mfz >> 8) && row >= g.r0 && row < g.r1) atomicAdd(&hist[g.base + row - g.r0], 1u);
             }
           }
         }
@@ -191,6 +213,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
     // ---- histogram scan: per-hypothesis statistics, append or top-K insert
     for (int q = 0; q < nseg; ++q) {
       const Seg g = segs[q];
+      const int g_hl = g.hl_mfz & 0xFF;
       const int len = g.r1 - g.r0;
       uint32_t sc = 0;
       unsigned long long mt = 0;
@@ -210,7 +233,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
             if (flag) {
               unsigned long long pos = bp + (unsigned long long)__popcll(bal & lanemask_lt());
               if ((int64_t)pos < O.cap) {
-                O.out_hyp[pos] = (int32_t)(h0 + g.hl);
+                O.out_hyp[pos] = (int32_t)(h0 + g_hl);
                 O.out_row[pos] = g.r0 + j;
                 O.out_cnt[pos] = (int32_t)c;
               }
@@ -221,7 +244,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
           if (flag) {
             uint32_t cc = c < 32767u ? c : 32767u;
             clamped |= c > 32767u;
-            key = ((uint64_t)cc << 48) | ((uint64_t)(uint32_t)(g.r0 + j) << 20) | (uint64_t)(0xFFFFF - g.hl);
+            key = ((uint64_t)cc << 48) | ((uint64_t)(uint32_t)(g.r0 + j) << 20) | (uint64_t)(0xFFFFF - g_hl);
           }
           bool ins = flag && key > thresh;
           uint64_t bal = ballot(ins);
@@ -242,8 +265,8 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       sc = wave_sum(sc);
       mt = wave_sum(mt);
       if (lane == 0) {
-        hsc[g.hl] += sc;
-        hmt[g.hl] += mt;
+        hsc[g_hl] += sc;
+        hmt[g_hl] += mt;
       }
     }
     wave_sync();
@@ -367,10 +390,10 @@ static int pow2_at_least(int x) {
 }
 
 static int k1_check(const VhParams* p, const VhIndexView* ix) {
-  if (!p || !ix || !ix->offsets || !ix->entries) return VH_E_ARG;
+  if (!p || !ix || !ix->offsets || (ix->n_postings && (!ix->post_key || !ix->post_row))) return VH_E_ARG;
   if (p->tile < 64 || p->tile > 8192 || (p->tile & (p->tile - 1))) return VH_E_ARG;
   if (ix->n_rows >= (1LL << 28)) return VH_E_LIMIT;
-  if (ix->bin_shift < 4 || ix->bin_shift > 12 || ix->bucket_log2 + ix->bin_shift > 32) return VH_E_ARG;
+  if (ix->bin_shift < 4 || ix->bin_shift > 16 || ix->bucket_log2 < 0 || ix->bucket_log2 > 16) return VH_E_ARG;
   return 0;
 }
 
@@ -381,7 +404,8 @@ static void k1_geometry(const VhParams* p, int mode, K1Out& O, int& waves) {
   int hb = p->tile * 4;
   int cb = O.clos_n * 12;
   O.hist_bytes = ((hb > cb ? hb : cb) + 15) & ~15;
-  O.wave_bytes = O.hist_bytes + K1_MAXSEG * (int)sizeof(Seg) + 64 * 4 + 64 * 8 + O.capb * 8;
+  O.wave_bytes = O.hist_bytes + K1_MAXSEG * (int)sizeof(Seg) + K1_MAXBK * (int)sizeof(BkEnt) + 64 * 4 + 64 * 8 +
+                 O.capb * 8;
   waves = 65536 / O.wave_bytes;
   waves = waves < 1 ? 1 : (waves > 4 ? 4 : waves);
 }

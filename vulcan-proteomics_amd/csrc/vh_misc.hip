@@ -47,7 +47,8 @@ DevIdx vh_make_devidx(const VhIndexView* ix) {
   d.n_bins = ix->n_bins;
   d.n_buckets = ix->n_buckets;
   d.offsets = ix->offsets;
-  d.entries = ix->entries;
+  d.post_key = ix->post_key;
+  d.post_row = ix->post_row;
   return d;
 }
 
@@ -191,6 +192,7 @@ size_t vh_scan_workspace_bytes(int64_t n) { return scan_ws(n, sizeof(int64_t)); 
 
 int vh_copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, void* stream) {
   if (n < 0 || (n && (!src || !dst))) return VH_E_ARG;
+  if (n == 0) return 0;
   if (((uintptr_t)src | (uintptr_t)dst) & 15) return VH_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   int64_t n4 = n / 4;

@@ -24,7 +24,7 @@ class VhParams(C.Structure):
 class VhIndexView(C.Structure):
     _fields_ = [("n_rows", I64), ("seq_off", P), ("length", P), ("mask", P), ("mass32", P), ("residues", P),
                 ("partner", P), ("bucket_log2", I32), ("bin_shift", I32), ("bin_base", U32), ("n_bins", I32),
-                ("n_buckets", I64), ("offsets", P), ("entries", P), ("n_postings", I64)]
+                ("n_buckets", I64), ("offsets", P), ("post_key", P), ("post_row", P), ("n_postings", I64)]
 
 
 def mod_arrays(static_mods: dict, var_mods: dict):
@@ -84,6 +84,10 @@ def index_geometry(p: SearchParams, n_rows: int):
     c = int(p.chunk_size)
     if c <= 0 or c & (c - 1) or c > 2 ** 16:
         raise ValueError("chunk_size must be a power of two <= 65536")
+    if not 0 < p.fragment_min_mz <= p.fragment_max_mz:
+        raise ValueError("need 0 < fragment_min_mz <= fragment_max_mz")
+    if f32_bits(p.fragment_max_mz) - f32_bits(p.fragment_min_mz) >= 2 ** 26:
+        raise ValueError("fragment_max_mz / fragment_min_mz spans more than 8 binary octaves")
     lb = c.bit_length() - 1
     s = p.bin_shift()
     base = f32_bits(p.fragment_min_mz) >> s

@@ -26,7 +26,7 @@ _SIGS = {
     "vh_copy_u32": ([P, P, I64, P], I32),
     "vh_frag_count": ([PP, PIX, P, P, SZ, P], I32),
     "vh_index_workspace_bytes": ([I64, I64, I32], SZ),
-    "vh_index_build": ([PP, PIX, P, P, P, P, SZ, P], I32),
+    "vh_index_build": ([PP, PIX, P, P, P, P, P, SZ, P], I32),
     "vh_k0": ([PP, I32, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_k1": ([PP, PIX, I32, P, P, P, P, P, P, I64, P, P, P, P, P, P, P], I32),
@@ -41,14 +41,31 @@ ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "workspace to
 _lib = None
 
 
+class _Strict:
+    """ctypes accepts surplus arguments silently (they would land in the wrong stack slots, e.g. a truncated
+    stream handle); every call here must pass exactly the declared parameters."""
+
+    def __init__(self, cdll):
+        self._cdll = cdll
+        for name, (args, res) in _SIGS.items():
+            f = getattr(cdll, name)
+            f.argtypes = args
+            f.restype = res
+            setattr(self, name, self._wrap(name, f, len(args)))
+
+    @staticmethod
+    def _wrap(name, f, n):
+        def call(*a):
+            if len(a) != n:
+                raise TypeError(f"{name} takes {n} arguments, {len(a)} given")
+            return f(*a)
+        call.__name__ = name
+        return call
+
+
 def load(path: str | None = None):
     """Load the library without touching the GPU (symbol/ABI checks on CPU-only hosts)."""
-    L = C.CDLL(path or _build.build_hip(force=False))
-    for name, (args, res) in _SIGS.items():
-        f = getattr(L, name)
-        f.argtypes = args
-        f.restype = res
-    return L
+    return _Strict(C.CDLL(path or _build.build_hip(force=False)))
 
 
 def lib():

@@ -34,8 +34,9 @@ class Slice:
     """One resident posting directory (the whole index is one slice here; 288 GB HBM holds it)."""
     chunk0: int
     chunk1: int
-    offsets: torch.Tensor   # int32 bit pattern of u32, n_buckets * n_bins + 1
-    entries: torch.Tensor   # int32 bit pattern of u32 postings
+    offsets: torch.Tensor    # int32 bit pattern of u32, n_buckets * n_bins + 1
+    post_key: torch.Tensor   # int32 bit pattern of the f32 fragment masses (sorted inside each bucket)
+    post_row: torch.Tensor   # int16 bit pattern of the row inside the bucket
 
 
 @dataclass
@@ -49,8 +50,9 @@ class Stats:
 class GpuIndex:
     """Device-resident peptidoform table + fragment index (reference engine.py:59-153).
 
-    Buckets of `chunk_size` consecutive rows (precursor-mass intervals) x f32-bit-pattern fragment bins; the
-    keys are sorted on the device by an LSD radix sort (vh_index_build)."""
+    Buckets of `chunk_size` consecutive rows (precursor-mass intervals); inside a bucket the postings are
+    sorted by the exact f32 fragment mass (its bit pattern) by a device LSD radix sort (vh_index_build), with a
+    directory of f32-bit-pattern bins for the first posting of a mass range."""
 
     def __init__(self, l1, params: SearchParams, static_mods: dict, var_mods: dict, device="cuda"):
         t0 = time.perf_counter()
@@ -95,20 +97,21 @@ class GpuIndex:
             raise ValueError("more than 2^32 - 1 postings")
         v.n_postings = self.n_postings
         self.offsets = torch.empty(n_buckets * n_bins + 1, dtype=torch.int32, device=d)
-        self.entries = torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d)
+        self.post_key = torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d)
+        self.post_row = torch.empty(max(self.n_postings, 1), dtype=torch.int16, device=d)
         wsb = int(L.vh_index_workspace_bytes(self.n_postings, n_buckets, n_bins))
         ws = torch.empty(wsb, dtype=torch.uint8, device=d)
         K.check(L.vh_index_build(C.byref(self.vp), C.byref(v), K.ptr(self.post_off), K.ptr(self.offsets),
-                                 K.ptr(self.entries), K.ptr(ws), wsb, st), "index_build")
-        v.offsets, v.entries = K.ptr(self.offsets), K.ptr(self.entries)
+                                 K.ptr(self.post_key), K.ptr(self.post_row), K.ptr(ws), wsb, st), "index_build")
+        v.offsets, v.post_key, v.post_row = K.ptr(self.offsets), K.ptr(self.post_key), K.ptr(self.post_row)
         torch.cuda.synchronize()
         del ws
-        self.slices = [Slice(0, n_buckets, self.offsets, self.entries)]
+        self.slices = [Slice(0, n_buckets, self.offsets, self.post_key, self.post_row)]
         self.inv_logw = None
         self.build_s = time.perf_counter() - t0
         self.device_bytes = sum(t.numel() * t.element_size() for t in
                                 [self.seq_off, self.length, self.mask, self.mass32, self.residues, self.partner,
-                                 self.offsets, self.entries])
+                                 self.offsets, self.post_key, self.post_row])
 
     def params_for(self, p: SearchParams):
         """VhParams for a SearchParams that shares this index's residue tables (topk / tile / tolerance may
