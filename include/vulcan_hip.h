@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 4
+#define VH_ABI_VERSION 5
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -125,13 +125,23 @@ int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz,
                   int8_t* h_z, int8_t* h_iso, float* h_pm, float* h_M, int64_t* h_lo, int64_t* h_hi,
                   int8_t* h_maxfz, int64_t hyp_capacity, void* ws, size_t ws_bytes, void* stream);
 
+/* Order of the spectra for the K1/K2 launches: ascending first-hypothesis window row (i.e. precursor mass;
+ * spectra without hypotheses last), by a device radix sort. Waves then visit the index in mass order and the
+ * kernels map contiguous ranges of this order onto each XCD, so an XCD's L2 serves a narrow band of buckets.
+ * No reference counterpart (the reference launches work items in hypothesis order, engine.py:213-229). */
+size_t vh_spectrum_order_workspace_bytes(int32_t n_spec);
+int vh_spectrum_order(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo, int32_t* order, void* ws,
+                      size_t ws_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------------------- K1 */
 /* Candidate generation, append form (run_k1, engine.py:250-281; k1_kernel, kernels.cu:112-190): for every
  * hypothesis h and row r of its window, count(h, r) = #(indexed fragment of r, peak, z < maxfz) with
  * tol_bound(peak*z, lo) <= fragment <= tol_bound(peak*z, hi); append (h, r, count) when count >= threshold
  * (unordered; *n_out may exceed cap, then only cap entries are written); hyp_scored[h] = #rows with
- * count > 0, hyp_matched[h] = sum of counts (written, not accumulated). One wave64 per spectrum. */
-int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off, const float* pk_mass,
+ * count > 0, hyp_matched[h] = sum of counts (written, not accumulated). One wave64 per spectrum; `order`
+ * (NULL = 0..n_spec-1) is the processing order of the spectra (vh_spectrum_order). */
+int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order, const int64_t* pk_off,
+          const float* pk_mass,
           const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo, const int64_t* h_hi, int64_t cap,
           unsigned long long* n_out, int32_t* out_hyp, int32_t* out_row, int32_t* out_cnt, uint32_t* hyp_scored,
           unsigned long long* hyp_matched, void* stream);
@@ -141,8 +151,8 @@ int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_
  * deduplicated by row (lowest hypothesis, highest count), written in row order at s*cand_stride.
  * n_cand[s] = number of candidates (<= cand_stride). status[0] is set nonzero if a spectrum exceeded a
  * limit (its list is then empty). */
-int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off,
-               const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo,
+int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
+               const int64_t* pk_off, const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo,
                const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp,
                int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
                void* stream);
@@ -161,8 +171,9 @@ int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32
  * Groups of candidates share a spectrum: group g has spectrum grp_spec[g] (NULL: g), first candidate
  * grp_start[g] (NULL: g*stride) and grp_count[g] candidates. maxfz of a candidate = h_maxfz[c_hyp[c]].
  * c_shift (NULL = closed search) = observed - candidate mass: shifted + unshifted ions with the delta placed
- * on the best residue (vk_k2_shift, kernels.cu:291-363, 461-472). One wave64 per group. */
-int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* grp_spec,
+ * on the best residue (vk_k2_shift, kernels.cu:291-363, 461-472). One wave64 per group; `order` (NULL =
+ * 0..n_groups-1) is the processing order of the groups. */
+int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* order, const int32_t* grp_spec,
           const int64_t* grp_start, const int32_t* grp_count, int32_t stride, const int32_t* c_row,
           const int32_t* c_hyp, const int8_t* h_maxfz, const float* c_shift, const int64_t* pk_off,
           const float* pk_mass, const float* pk_int, double* o_hs, float* o_sb, float* o_sy, float* o_ppm,

@@ -320,3 +320,19 @@ def test_label_blindness_gpu():
         gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
         outs.append(E.search(sp, gi, l1, p, report=5)[0])
     assert outs[0].equals(outs[1])
+
+
+def test_spectrum_order_sorted_permutation(env1):
+    """vh_spectrum_order: a permutation of the spectra by first-hypothesis window row (spectra without
+    hypotheses last)."""
+    ds, _ = _topk_device(env1)
+    n = len(env1.d["sp"])
+    order = _np(ds.order)[:n]
+    assert np.array_equal(np.sort(order), np.arange(n))
+    off = env1.hyp["hyp_off"]
+    key = np.where(off[1:] > off[:-1], env1.hyp["lo"][np.minimum(off[:-1], max(len(env1.hyp["lo"]) - 1, 0))],
+                   2 ** 28 - 1)
+    k = key[order]
+    assert np.all(np.diff(k) >= 0)
+    same = np.diff(k) == 0
+    assert np.all(np.diff(order)[same] > 0)          # stable: equal keys keep spectrum order

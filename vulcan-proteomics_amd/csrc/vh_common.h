@@ -47,6 +47,21 @@ DevIdx vh_make_devidx(const VhIndexView* ix);
 int vh_scan_inclusive_i64(int64_t* a, int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
 int vh_scan_exclusive_u32(uint32_t* a, int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
 size_t vh_scan_ws_u32(int64_t n);
+size_t vh_radix_ws_bytes(int64_t n);
+int vh_radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bit_lo, int bit_hi, void* ws, size_t ws_bytes,
+                      hipStream_t st, uint64_t** result);
+
+// Blocks are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8). With gridDim.x = 8*q, position
+// (b % 8) * q + b / 8 gives each XCD a contiguous range of work items; when the items are spectra in
+// precursor-mass order, each XCD's L2 then serves a narrow band of index buckets and table rows.
+#define VH_XCDS 8
+__device__ __forceinline__ int xcd_band_position() {
+  const int q = gridDim.x / VH_XCDS;
+  return (blockIdx.x % VH_XCDS) * q + blockIdx.x / VH_XCDS;
+}
+static inline unsigned xcd_band_grid(int64_t n_blocks) {
+  return (unsigned)(((n_blocks + VH_XCDS - 1) / VH_XCDS) * VH_XCDS);
+}
 
 #define VH_CHECK_LAUNCH() do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return (int)e_; } while (0)
 

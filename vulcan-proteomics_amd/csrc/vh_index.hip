@@ -181,15 +181,46 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int64_t n_tiles_of(int64_t n) { return (n + RS_TILE - 1) / RS_TILE; }
 
+size_t vh_radix_ws_bytes(int64_t n) {
+  int64_t nt = n_tiles_of(n > 0 ? n : 1);
+  size_t hist = (size_t)256 * nt;
+  return align256(hist * 4) + align256(vh_scan_ws_u32(hist));
+}
+
+// Stable LSD radix sort of n u64 keys on bits [bit_lo, bit_hi) (8-bit digits), ping-ponging between a and b.
+// *result receives the buffer that holds the sorted keys.
+int vh_radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bit_lo, int bit_hi, void* ws, size_t ws_bytes,
+                      hipStream_t st, uint64_t** result) {
+  *result = a;
+  if (n <= 0) return 0;
+  if (ws_bytes < vh_radix_ws_bytes(n)) return VH_E_WORKSPACE;
+  int64_t nt = n_tiles_of(n);
+  size_t hist_n = (size_t)256 * nt;
+  uint32_t* hist = (uint32_t*)ws;
+  void* sws = (char*)ws + align256(hist_n * 4);
+  size_t sws_bytes = ws_bytes - align256(hist_n * 4);
+  uint64_t* src = a;
+  uint64_t* dst = b;
+  for (int shift = bit_lo; shift < bit_hi; shift += 8) {
+    rs_upsweep_kernel<<<(unsigned)nt, RS_THREADS, 0, st>>>(src, n, shift, hist, nt);
+    int rc = vh_scan_exclusive_u32(hist, (int64_t)hist_n, sws, sws_bytes, st);
+    if (rc) return rc;
+    rs_downsweep_kernel<<<(unsigned)nt, RS_THREADS, 0, st>>>(src, dst, n, shift, hist, nt);
+    VH_CHECK_LAUNCH();
+    uint64_t* t = src;
+    src = dst;
+    dst = t;
+  }
+  *result = src;
+  return 0;
+}
+
 extern "C" {
 
 size_t vh_index_workspace_bytes(int64_t n_postings, int64_t n_buckets, int32_t n_bins) {
   (void)n_buckets;
   (void)n_bins;
-  int64_t nt = n_tiles_of(n_postings > 0 ? n_postings : 1);
-  size_t hist = (size_t)256 * nt;
-  return 2 * align256((size_t)(n_postings > 0 ? n_postings : 1) * 8) + align256(hist * 4) +
-         align256(vh_scan_ws_u32(hist));
+  return 2 * align256((size_t)(n_postings > 0 ? n_postings : 1) * 8) + vh_radix_ws_bytes(n_postings);
 }
 
 int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, void* ws, size_t ws_bytes,
@@ -235,28 +266,12 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
   size_t kb = align256((size_t)(n > 0 ? n : 1) * 8);
   uint64_t* ka = (uint64_t*)w;
   uint64_t* kb2 = (uint64_t*)(w + kb);
-  int64_t nt = n_tiles_of(n > 0 ? n : 1);
-  size_t hist_n = (size_t)256 * nt;
-  uint32_t* hist = (uint32_t*)(w + 2 * kb);
-  void* sws = w + 2 * kb + align256(hist_n * 4);
-  size_t sws_bytes = ws_bytes - (2 * kb + align256(hist_n * 4));
   unsigned grid = (unsigned)((ix->n_rows + 255) / 256);
   if (ix->n_rows) frag_emit_kernel<<<grid, 256, 0, st>>>(P, X, G, post_off, ka);
   VH_CHECK_LAUNCH();
   uint64_t* src = ka;
-  uint64_t* dst = kb2;
-  if (n > 0) {
-    for (int shift = 16; shift < 16 + G.nd + nbk; shift += 8) {
-      rs_upsweep_kernel<<<(unsigned)nt, RS_THREADS, 0, st>>>(src, n, shift, hist, nt);
-      int rc = vh_scan_exclusive_u32(hist, (int64_t)hist_n, sws, sws_bytes, st);
-      if (rc) return rc;
-      rs_downsweep_kernel<<<(unsigned)nt, RS_THREADS, 0, st>>>(src, dst, n, shift, hist, nt);
-      VH_CHECK_LAUNCH();
-      uint64_t* t = src;
-      src = dst;
-      dst = t;
-    }
-  }
+  int rc = vh_radix_sort_u64(ka, kb2, n, 16, 16 + G.nd + nbk, w + 2 * kb, ws_bytes - 2 * kb, st, &src);
+  if (rc) return rc;
   unsigned g2 = (unsigned)((n > 0 ? n : 1024) + 255) / 256;
   directory_kernel<<<g2, 256, 0, st>>>(src, n, X, G, offsets, post_key, post_row, n_dir);
   VH_CHECK_LAUNCH();

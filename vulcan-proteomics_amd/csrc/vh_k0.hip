@@ -3,16 +3,19 @@
 // K0 restates k0_preprocess (vulcan/gpu/engine.py:171-210, global view): keep fmin <= m/z <= fmax (not NaN),
 // mass = m/z - PROTON, the top max_peaks peaks by (intensity desc with NaN first, mass desc, input order),
 // then mass ascending (ties keep that order); TIC = f64 sum; valid = kept >= min_peaks.
-// One wave64 per spectrum: in-range peaks are compacted into LDS (ballot prefix), each lane ranks its peaks by
-// counting the peaks that precede them (broadcast LDS reads), the kept peaks are placed in rank order, and a
-// second counting pass gives each its mass-ordered slot. Spectra with more than K0_LDS in-range peaks rank
-// straight from global memory.
+// One wave64 per spectrum: in-range peaks are compacted (ballot prefix, order preserved) and held in registers,
+// up to 8 per lane. The selection order is the 64-bit key (intensity key << 32 | mass bits) descending, ties
+// by input position, where the intensity key maps NaN above every number (torch's descending sort puts NaN
+// first) and -0 onto +0, and the mass key is the order-preserving u32 image of the float. A peak's rank = number
+// of peaks before it; candidates are broadcast with v_readlane (no LDS traffic in the O(n^2) passes). The kept
+// peaks (rank < max_peaks) then get their mass-ordered slot as #(kept with (mass bits, rank) smaller). Spectra
+// with more than 512 in-range peaks rank straight from global memory.
 // Hypotheses restate make_hypotheses (vulcan/gpu/host.py:64-91) with the same f32 operation order.
 #include "vh_common.h"
 
 #define K0_WAVES 4
 #define K0_LDS 512       // in-range peaks staged per wave
-#define K0_SEL 512       // max_peaks limit of the device path
+#define K0_SEL 512       // max_peaks limit of the device path (kept peaks are staged in the same LDS arrays)
 
 __device__ __forceinline__ bool inrange(float m, const DevP& P) { return !isnan(m) && m >= P.fmin && m <= P.fmax; }
 __device__ __forceinline__ bool gt_nan_first(float a, float b) { return (isnan(a) && !isnan(b)) || a > b; }
@@ -36,19 +39,35 @@ __global__ __launch_bounds__(256) void k0_count_kernel(DevP P, int n_spec, const
   if (lane_id() == 0) pk_off[s + 1] = c < P.max_peaks ? c : P.max_peaks;
 }
 
+__device__ __forceinline__ uint32_t int_key(float x) {
+  if (isnan(x)) return 0xFFFFFFFFu;
+  uint32_t b = __float_as_uint(x == 0.f ? 0.f : x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// order-preserving u32 image of a finite float (and its inverse), for the mass keys
+__device__ __forceinline__ uint32_t ord_key(float x) {
+  uint32_t b = __float_as_uint(x == 0.f ? 0.f : x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord_inv(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+__device__ __forceinline__ uint32_t rdl(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+
+#define K0_REG 8   // 8 x 64 = 512 in-range peaks in registers
+
 __global__ __launch_bounds__(256) void k0_emit_kernel(DevP P, int n_spec, const float* __restrict__ mz,
                                                       const float* __restrict__ it, const int64_t* __restrict__ raw_off,
                                                       const int64_t* __restrict__ pk_off, float* pk_mass, float* pk_int,
                                                       float* tic, uint8_t* valid) {
   __shared__ float s_m[K0_WAVES][K0_LDS], s_i[K0_WAVES][K0_LDS];
-  __shared__ float s_sm[K0_WAVES][K0_SEL], s_si[K0_WAVES][K0_SEL];
   const int w = threadIdx.x >> 6, lane = lane_id();
   int s = blockIdx.x * K0_WAVES + w;
   if (s >= n_spec) return;
   float* lm = s_m[w];
   float* li = s_i[w];
-  float* sm = s_sm[w];
-  float* si = s_si[w];
   const int64_t base = raw_off[s];
   const int n = (int)(raw_off[s + 1] - base);
   // compact in-range peaks (order preserved)
@@ -64,15 +83,70 @@ __global__ __launch_bounds__(256) void k0_emit_kernel(DevP P, int n_spec, const 
   }
   wave_sync();
   const int K = n_in < P.max_peaks ? n_in : P.max_peaks;
+  const int64_t o = pk_off[s];
+  double t = 0.0;
   if (n_in <= K0_LDS) {
-    for (int i = lane; i < n_in; i += 64) {
-      float mi = lm[i], ii = li[i];
-      int r = 0;
-      for (int j = 0; j < n_in; ++j) r += sel_before(li[j], lm[j], j, ii, mi, i);
-      if (r < K) { sm[r] = mi; si[r] = ii; }
+    const int nch = (n_in + 63) >> 6;
+    uint32_t mb[K0_REG], ik[K0_REG], rk[K0_REG];
+    float fi[K0_REG];
+#pragma unroll
+    for (int c = 0; c < K0_REG; ++c) {
+      int d = c * 64 + lane;
+      bool v = d < n_in;
+      float m = v ? lm[d] : 0.f;
+      fi[c] = v ? li[d] : 0.f;
+      mb[c] = ord_key(m);
+      ik[c] = int_key(fi[c]);
+      rk[c] = 0;
+    }
+    // rank = #(j before i): key_j > key_i, or equal keys and j < i
+#pragma unroll
+    for (int cj = 0; cj < K0_REG; ++cj) {
+      if (cj < nch) {
+        const int lim = n_in - cj * 64 < 64 ? n_in - cj * 64 : 64;
+        for (int l = 0; l < lim; ++l) {
+          const uint64_t kj = ((uint64_t)rdl(ik[cj], l) << 32) | rdl(mb[cj], l);
+          const int j = cj * 64 + l;
+#pragma unroll
+          for (int ci = 0; ci < K0_REG; ++ci) {
+            if (ci < nch) {
+              const uint64_t ki = ((uint64_t)ik[ci] << 32) | mb[ci];
+              rk[ci] += (kj > ki) || (kj == ki && j < ci * 64 + lane);
+            }
+          }
+        }
+      }
+    }
+    // slot among the kept peaks: (mass bits, rank) ascending
+    uint32_t ps[K0_REG];
+#pragma unroll
+    for (int c = 0; c < K0_REG; ++c) ps[c] = 0;
+#pragma unroll
+    for (int cj = 0; cj < K0_REG; ++cj) {
+      if (cj < nch) {
+        const int lim = n_in - cj * 64 < 64 ? n_in - cj * 64 : 64;
+        for (int l = 0; l < lim; ++l) {
+          const uint32_t rj = rdl(rk[cj], l);
+          if (rj >= (uint32_t)K) continue;   // uniform: only kept peaks count
+          const uint64_t kj = ((uint64_t)rdl(mb[cj], l) << 32) | rj;
+#pragma unroll
+          for (int ci = 0; ci < K0_REG; ++ci)
+            if (ci < nch) ps[ci] += kj < (((uint64_t)mb[ci] << 32) | rk[ci]);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < K0_REG; ++c) {
+      if (c < nch && c * 64 + lane < n_in && rk[c] < (uint32_t)K) {
+        pk_mass[o + ps[c]] = ord_inv(mb[c]);
+        pk_int[o + ps[c]] = fi[c];
+        t += (double)fi[c];
+      }
     }
   } else {
-    // rank over the raw peaks from global memory (indices in raw order preserve the input order)
+    // more than K0_LDS in-range peaks: rank over the raw peaks from global memory, kept ones staged by rank
+    float* sm = lm;
+    float* si = li;
     for (int i = lane; i < n; i += 64) {
       float mzi = mz[base + i];
       if (!inrange(mzi, P)) continue;
@@ -84,21 +158,18 @@ __global__ __launch_bounds__(256) void k0_emit_kernel(DevP P, int n_spec, const 
       }
       if (r < K) { sm[r] = mi; si[r] = ii; }
     }
-  }
-  wave_sync();
-  // mass ascending, ties keep the selection order
-  const int64_t o = pk_off[s];
-  double t = 0.0;
-  for (int i = lane; i < K; i += 64) {
-    float mi = sm[i];
-    int pos = 0;
-    for (int j = 0; j < K; ++j) {
-      float mj = sm[j];
-      pos += (mj < mi) || (mj == mi && j < i);
+    wave_sync();
+    for (int i = lane; i < K; i += 64) {
+      float mi = sm[i];
+      int pos = 0;
+      for (int j = 0; j < K; ++j) {
+        float mj = sm[j];
+        pos += (mj < mi) || (mj == mi && j < i);
+      }
+      pk_mass[o + pos] = mi;
+      pk_int[o + pos] = si[i];
+      t += (double)si[i];
     }
-    pk_mass[o + pos] = mi;
-    pk_int[o + pos] = si[i];
-    t += (double)si[i];
   }
   t = wave_sum(t);
   if (lane == 0) {
@@ -175,7 +246,50 @@ __global__ __launch_bounds__(256) void hyp_emit_kernel(DevP P, int n_spec, const
   }
 }
 
+// ------------------------------------------------------------------------------------------------ spectrum order
+__global__ __launch_bounds__(256) void order_keys_kernel(int n_spec, const int64_t* __restrict__ hyp_off,
+                                                         const int64_t* __restrict__ h_lo, uint64_t* keys) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_spec) return;
+  int64_t h0 = hyp_off[s];
+  uint64_t lo = hyp_off[s + 1] > h0 ? (uint64_t)h_lo[h0] : 0x0FFFFFFFULL;
+  keys[s] = (lo << 32) | (uint64_t)(uint32_t)s;
+}
+
+__global__ __launch_bounds__(256) void order_out_kernel(int n_spec, const uint64_t* __restrict__ keys,
+                                                        int32_t* order) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_spec) order[i] = (int32_t)(uint32_t)keys[i];
+}
+
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 extern "C" {
+
+size_t vh_spectrum_order_workspace_bytes(int32_t n_spec) {
+  size_t n = (size_t)(n_spec > 0 ? n_spec : 1);
+  return 2 * a256(n * 8) + vh_radix_ws_bytes((int64_t)n);
+}
+
+int vh_spectrum_order(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo, int32_t* order, void* ws,
+                      size_t ws_bytes, void* stream) {
+  if (n_spec < 0 || (n_spec && (!hyp_off || !h_lo || !order || !ws))) return VH_E_ARG;
+  if (n_spec == 0) return 0;
+  if (ws_bytes < vh_spectrum_order_workspace_bytes(n_spec)) return VH_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  size_t kb = a256((size_t)n_spec * 8);
+  uint64_t* a = (uint64_t*)ws;
+  uint64_t* b = (uint64_t*)((char*)ws + kb);
+  unsigned g = (unsigned)((n_spec + 255) / 256);
+  order_keys_kernel<<<g, 256, 0, st>>>(n_spec, hyp_off, h_lo, a);
+  VH_CHECK_LAUNCH();
+  uint64_t* res = a;
+  int rc = vh_radix_sort_u64(a, b, n_spec, 32, 60, (char*)ws + 2 * kb, ws_bytes - 2 * kb, st, &res);
+  if (rc) return rc;
+  order_out_kernel<<<g, 256, 0, st>>>(n_spec, res, order);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
 
 int vh_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* intensity, const int64_t* raw_off,
           int64_t* pk_off, float* pk_mass, float* pk_int, float* tic, uint8_t* valid, int64_t pk_capacity,

@@ -76,7 +76,8 @@ struct K1Out {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, const int64_t* __restrict__ pk_off,
+__global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, const int32_t* __restrict__ order,
+                                                 const int64_t* __restrict__ pk_off,
                                                  const float* __restrict__ pk_mass,
                                                  const int64_t* __restrict__ hyp_off,
                                                  const int8_t* __restrict__ h_maxfz,
@@ -84,8 +85,9 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
                                                  const int64_t* __restrict__ h_hi, K1Out O) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = threadIdx.x >> 6, lane = lane_id();
-  const int s = blockIdx.x * (blockDim.x >> 6) + w;
-  if (s >= n_spec) return;   // a whole wave leaves; no block-level barrier is used below
+  const int wv = xcd_band_position() * (blockDim.x >> 6) + w;
+  if (wv >= n_spec) return;   // a whole wave leaves; no block-level barrier is used below
+  const int s = order ? order[wv] : wv;
   unsigned char* wb = smem + (size_t)w * O.wave_bytes;
   uint32_t* hist = (uint32_t*)wb;
   Seg* segs = (Seg*)(wb + O.hist_bytes);
@@ -412,7 +414,8 @@ static void k1_geometry(const VhParams* p, int mode, K1Out& O, int& waves) {
 
 extern "C" {
 
-int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off, const float* pk_mass,
+int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order, const int64_t* pk_off,
+          const float* pk_mass,
           const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo, const int64_t* h_hi, int64_t cap,
           unsigned long long* n_out, int32_t* out_hyp, int32_t* out_row, int32_t* out_cnt, uint32_t* hyp_scored,
           unsigned long long* hyp_matched, void* stream) {
@@ -433,16 +436,16 @@ int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_
   int waves;
   k1_geometry(p, 0, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
-  unsigned g = (unsigned)((n_spec + waves - 1) / waves);
+  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves);
   k1_kernel<0><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
-                                                                      pk_off, pk_mass, hyp_off, h_maxfz, h_lo, h_hi,
-                                                                      O);
+                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_lo,
+                                                                      h_hi, O);
   VH_CHECK_LAUNCH();
   return 0;
 }
 
-int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off,
-               const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo,
+int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
+               const int64_t* pk_off, const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo,
                const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp,
                int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
                void* stream) {
@@ -466,10 +469,10 @@ int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const i
   int waves;
   k1_geometry(p, 1, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
-  unsigned g = (unsigned)((n_spec + waves - 1) / waves);
+  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves);
   k1_kernel<1><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
-                                                                      pk_off, pk_mass, hyp_off, h_maxfz, h_lo, h_hi,
-                                                                      O);
+                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_lo,
+                                                                      h_hi, O);
   VH_CHECK_LAUNCH();
   return 0;
 }

@@ -44,7 +44,8 @@ struct K2Buf {
   int16_t *mb, *my, *lb, *ly;
 };
 
-__global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups, const int32_t* __restrict__ grp_spec,
+__global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups, const int32_t* __restrict__ order,
+                                                 const int32_t* __restrict__ grp_spec,
                                                  const int64_t* __restrict__ grp_start,
                                                  const int32_t* __restrict__ grp_count, int stride,
                                                  const int32_t* __restrict__ c_row, const int32_t* __restrict__ c_hyp,
@@ -60,8 +61,9 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
   for (int i = threadIdx.x; i < K2_LNF; i += blockDim.x) lnf[i] = lnfact_dev(i);
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = lane_id();
-  const int g = blockIdx.x * K2_WAVES + w;
-  if (g >= n_groups) return;
+  const int wv = xcd_band_position() * K2_WAVES + w;
+  if (wv >= n_groups) return;
+  const int g = order ? order[wv] : wv;
   const int s = grp_spec ? grp_spec[g] : g;
   const int64_t c0 = grp_start ? grp_start[g] : (int64_t)g * stride;
   const int nc = grp_count[g];
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(256) void rank_kernel(DevP P, int n_groups, const i
 
 extern "C" {
 
-int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* grp_spec,
+int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* order, const int32_t* grp_spec,
           const int64_t* grp_start, const int32_t* grp_count, int32_t stride, const int32_t* c_row,
           const int32_t* c_hyp, const int8_t* h_maxfz, const float* c_shift, const int64_t* pk_off,
           const float* pk_mass, const float* pk_int, double* o_hs, float* o_sb, float* o_sy, float* o_ppm,
@@ -202,8 +204,8 @@ int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int3
   if (n_groups == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   K2Buf O = {o_hs, o_sb, o_sy, o_ppm, o_mb, o_my, o_lb, o_ly};
-  unsigned g = (unsigned)((n_groups + K2_WAVES - 1) / K2_WAVES);
-  k2_kernel<<<g, 64 * K2_WAVES, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_groups, grp_spec, grp_start,
+  unsigned g = xcd_band_grid((n_groups + K2_WAVES - 1) / K2_WAVES);
+  k2_kernel<<<g, 64 * K2_WAVES, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_groups, order, grp_spec, grp_start,
                                          grp_count, stride, c_row, c_hyp, h_maxfz, pk_off, pk_mass, pk_int, O);
   VH_CHECK_LAUNCH();
   return 0;

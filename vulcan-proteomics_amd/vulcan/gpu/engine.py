@@ -256,7 +256,7 @@ def run_k1(gi: GpuIndex, hyp, peaks, p: SearchParams, stats: Stats):
     n_pk = peaks["off"].numel() - 1
     if n_pk < n_spec:
         raise ValueError("hypotheses reference spectra beyond the peak view")
-    K.check(L.vh_k1(C.byref(vp), C.byref(gi.view), n_spec, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
+    K.check(L.vh_k1(C.byref(vp), C.byref(gi.view), n_spec, None, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
                     K.ptr(hd["hyp_off"]), K.ptr(hd["maxfz"]), K.ptr(hd["lo"]), K.ptr(hd["hi"]), cap, K.ptr(n_out),
                     K.ptr(o_h), K.ptr(o_r), K.ptr(o_c), K.ptr(scored), K.ptr(matched), K.stream()), "k1")
     n = int(n_out.item())
@@ -304,9 +304,10 @@ def _k2_out(n, dev):
     return out
 
 
-def _k2_call(gi, vp, n_groups, grp_spec, grp_start, grp_count, stride, row, hyp, maxfz, peaks, out):
+def _k2_call(gi, vp, n_groups, grp_spec, grp_start, grp_count, stride, row, hyp, maxfz, peaks, out, order=None):
     L = K.lib()
-    K.check(L.vh_k2(C.byref(vp), C.byref(gi.view), n_groups, K.ptr(grp_spec), K.ptr(grp_start), K.ptr(grp_count),
+    K.check(L.vh_k2(C.byref(vp), C.byref(gi.view), n_groups, K.ptr(order), K.ptr(grp_spec), K.ptr(grp_start),
+                    K.ptr(grp_count),
                     stride, K.ptr(row), K.ptr(hyp), K.ptr(maxfz), None, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
                     K.ptr(peaks["int"]), K.ptr(out["hyperscore"]), K.ptr(out["summed_b"]), K.ptr(out["summed_y"]),
                     K.ptr(out["ppm"]), K.ptr(out["matched_b"]), K.ptr(out["matched_y"]), K.ptr(out["longest_b"]),
@@ -432,9 +433,9 @@ def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched
 
 
 class DeviceSearch:
-    """The fused closed-search step on preallocated device buffers: vh_k0 -> vh_hypotheses -> vh_k1_topk ->
-    vh_k2 -> vh_rank (report 1). `run(raw)` enqueues the five launches on the current stream and returns the
-    per-spectrum rank-1 result; no host synchronisation, no allocation, hipGraph-capturable."""
+    """The fused closed-search step on preallocated device buffers: vh_k0 -> vh_hypotheses -> vh_spectrum_order
+    -> vh_k1_topk -> vh_k2 -> vh_rank (report 1). `run(raw)` enqueues the launches on the current stream and
+    returns the per-spectrum rank-1 result; no host synchronisation, no allocation, hipGraph-capturable."""
 
     def __init__(self, gi: GpuIndex, p: SearchParams, n_spec: int, report: int = 1):
         self.gi, self.p, self.n_spec, self.report = gi, p, n_spec, report
@@ -455,6 +456,8 @@ class DeviceSearch:
                     "iso": e(cap_h, torch.int8), "pm": e(cap_h, torch.float32), "M": e(cap_h, torch.float32),
                     "lo": e(cap_h, torch.int64), "hi": e(cap_h, torch.int64), "maxfz": e(cap_h, torch.int8),
                     "ws": e(ws, torch.uint8)}
+        self.order = e(n, torch.int32)
+        self.order_ws = e(int(L.vh_spectrum_order_workspace_bytes(n)), torch.uint8)
         self.n_cand = e(n, torch.int32)
         self.c_row, self.c_hyp, self.c_cnt = e(cap_c, torch.int32), e(cap_c, torch.int32), e(cap_c, torch.int32)
         self.scored, self.matched = e(cap_h, torch.int32), e(cap_h, torch.int64)
@@ -492,14 +495,20 @@ class DeviceSearch:
                                 K.ptr(hy["maxfz"]), hy["spec"].numel(), K.ptr(hy["ws"]), hy["ws"].numel(), st),
                 "hypotheses")
         mark("hyp")
+        mark("order")
+        K.check(L.vh_spectrum_order(n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(self.order),
+                                    K.ptr(self.order_ws), self.order_ws.numel(), st), "spectrum_order")
+        mark("order")
         mark("k1_topk")
-        K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.view), n, K.ptr(pk["off"]), K.ptr(pk["mass"]),
-                             K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride,
-                             K.ptr(self.n_cand), K.ptr(self.c_row), K.ptr(self.c_hyp), K.ptr(self.c_cnt),
-                             K.ptr(self.scored), K.ptr(self.matched), K.ptr(self.status), st), "k1_topk")
+        K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.view), n, K.ptr(self.order), K.ptr(pk["off"]),
+                             K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["lo"]),
+                             K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand), K.ptr(self.c_row), K.ptr(self.c_hyp),
+                             K.ptr(self.c_cnt), K.ptr(self.scored), K.ptr(self.matched), K.ptr(self.status), st),
+                "k1_topk")
         mark("k1_topk")
         mark("k2")
-        _k2_call(gi, vp, n, None, None, self.n_cand, self.stride, self.c_row, self.c_hyp, hy["maxfz"], pk, self.k2)
+        _k2_call(gi, vp, n, None, None, self.n_cand, self.stride, self.c_row, self.c_hyp, hy["maxfz"], pk, self.k2,
+                 order=self.order)
         mark("k2")
         mark("rank")
         K.check(L.vh_rank(C.byref(vp), n, None, K.ptr(self.n_cand), self.stride, K.ptr(self.c_row),
