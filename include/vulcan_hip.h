@@ -55,8 +55,9 @@ typedef struct VhParams {
   int32_t topk;               /* K1 top-K per spectrum (engine.py:291-296) */
   int32_t k1_threshold;       /* K1 append threshold (kernels.cu:171) */
   int32_t pair_closure;       /* add partner rows (engine.py:297-301) */
-  int32_t tile;               /* K1 LDS histogram slots per wave: 256..4096, power of two */
-  int32_t reserved[8];
+  int32_t tile;               /* K1 LDS histogram slots per wave: 64..8192, power of two */
+  int32_t xcd_group;          /* blocks per XCD per scheduling group (0: plain block order), see vh_common.h */
+  int32_t reserved[7];
 } VhParams;
 
 /* Device view of the peptidoform table (vulcan/index/build.py:13-21) and the fragment index.

@@ -22,7 +22,7 @@ struct DevP {
   int min_ion_index;
   float prec_lo, prec_hi;
   int iso_lo, iso_hi, z_lo, z_hi, max_fragment_charge;
-  int max_peaks, min_peaks, min_matched_peaks, topk, k1_threshold, pair_closure, tile;
+  int max_peaks, min_peaks, min_matched_peaks, topk, k1_threshold, pair_closure, tile, xcd_group;
 };
 
 struct DevIdx {
@@ -51,16 +51,23 @@ size_t vh_radix_ws_bytes(int64_t n);
 int vh_radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bit_lo, int bit_hi, void* ws, size_t ws_bytes,
                       hipStream_t st, uint64_t** result);
 
-// Blocks are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8). With gridDim.x = 8*q, position
-// (b % 8) * q + b / 8 gives each XCD a contiguous range of work items; when the items are spectra in
-// precursor-mass order, each XCD's L2 then serves a narrow band of index buckets and table rows.
+// Blocks are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8). Work items (spectra in precursor-mass
+// order) are handed out in groups of 8*c blocks: inside a group, XCD x takes the contiguous sub-range
+// [x*c, (x+1)*c) of the group's positions. Each XCD's L2 then serves a narrow band of index buckets and table
+// rows, while all XCDs move through the mass range together (no XCD is left with only the heaviest spectra).
+// c = 0 keeps the plain block order.
 #define VH_XCDS 8
-__device__ __forceinline__ int xcd_band_position() {
-  const int q = gridDim.x / VH_XCDS;
-  return (blockIdx.x % VH_XCDS) * q + blockIdx.x / VH_XCDS;
+__device__ __forceinline__ int xcd_band_position(int c) {
+  if (c <= 0) return blockIdx.x;
+  const int G = VH_XCDS * c;
+  const int b = blockIdx.x;
+  const int j = b % G;
+  return (b / G) * G + (j % VH_XCDS) * c + j / VH_XCDS;
 }
-static inline unsigned xcd_band_grid(int64_t n_blocks) {
-  return (unsigned)(((n_blocks + VH_XCDS - 1) / VH_XCDS) * VH_XCDS);
+static inline unsigned xcd_band_grid(int64_t n_blocks, int c) {
+  if (c <= 0) return (unsigned)n_blocks;
+  const int64_t G = (int64_t)VH_XCDS * c;
+  return (unsigned)(((n_blocks + G - 1) / G) * G);
 }
 
 #define VH_CHECK_LAUNCH() do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return (int)e_; } while (0)

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
                                                  const int64_t* __restrict__ h_hi, K1Out O) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = threadIdx.x >> 6, lane = lane_id();
-  const int wv = xcd_band_position() * (blockDim.x >> 6) + w;
+  const int wv = xcd_band_position(P.xcd_group) * (blockDim.x >> 6) + w;
   if (wv >= n_spec) return;   // a whole wave leaves; no block-level barrier is used below
   const int s = order ? order[wv] : wv;
   unsigned char* wb = smem + (size_t)w * O.wave_bytes;
@@ -189,23 +189,38 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
         bl = bl < 0 ? 0 : bl;
         bh = bh > n_bins - 1 ? n_bins - 1 : bh;
         if (bl > bh) continue;
-        for (int j = 0; j < nbk; ++j) {
-          const BkEnt E = bks[j];
-          if (z >= E.zmax) continue;
-          const uint32_t* off = X.offsets + (size_t)E.bk * (size_t)n_bins;
-          const uint32_t e0 = off[bl], e1 = off[bh + 1];
-          const int rb = E.bk << LB;
-          for (uint32_t e = e0; e < e1; ++e) {
-            const uint32_t key = X.post_key[e];
-            if (key > kb) break;
-            const int row = rb + (int)X.post_row[e];
-            if (key < ka || row < E.rlo || row >= E.rhi) continue;
-            uint32_t msk = E.mask;
-            while (msk) {
-              const int q = __builtin_ctz(msk);
-              msk &= msk - 1u;
-              const Seg g = segs[q];
-              if (z < (g.hl_mfz >> 8) && row >= g.r0 && row < g.r1) atomicAdd(&hist[g.base + row - g.r0], 1u);
+        // directory pairs of up to 4 buckets are loaded together (independent loads in flight), then scanned
+        for (int j0 = 0; j0 < nbk; j0 += 4) {
+          uint32_t e0[4], e1[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            e0[u] = e1[u] = 0;
+            if (j0 + u < nbk) {
+              const int bk = bks[j0 + u].bk;
+              if (z < bks[j0 + u].zmax) {
+                const uint32_t* off = X.offsets + (size_t)bk * (size_t)n_bins;
+                e0[u] = off[bl];
+                e1[u] = off[bh + 1];
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (e0[u] == e1[u]) continue;
+            const BkEnt E = bks[j0 + u];
+            const int rb = E.bk << LB;
+            for (uint32_t e = e0[u]; e < e1[u]; ++e) {
+              const uint32_t key = X.post_key[e];
+              if (key > kb) break;
+              const int row = rb + (int)X.post_row[e];
+              if (key < ka || row < E.rlo || row >= E.rhi) continue;
+              uint32_t msk = E.mask;
+              while (msk) {
+                const int q = __builtin_ctz(msk);
+                msk &= msk - 1u;
+                const Seg g = segs[q];
+                if (z < (g.hl_mfz >> 8) && row >= g.r0 && row < g.r1) atomicAdd(&hist[g.base + row - g.r0], 1u);
+              }
             }
           }
         }
@@ -436,7 +451,7 @@ int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_
   int waves;
   k1_geometry(p, 0, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
-  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves);
+  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
   k1_kernel<0><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
                                                                       order, pk_off, pk_mass, hyp_off, h_maxfz, h_lo,
                                                                       h_hi, O);
@@ -469,7 +484,7 @@ int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const i
   int waves;
   k1_geometry(p, 1, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
-  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves);
+  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
   k1_kernel<1><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
                                                                       order, pk_off, pk_mass, hyp_off, h_maxfz, h_lo,
                                                                       h_hi, O);

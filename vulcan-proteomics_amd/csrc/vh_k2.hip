@@ -5,15 +5,17 @@
 // spectrum's mass-sorted peaks (first 512), the most intense peak wins (ties -> later peak, `>=`);
 // ppm += I*|mz - m|*2e6/(mz + m); b ions first (k asc, z asc), then y ions; longest consecutive ladder with the
 // reference's run bookkeeping; hyperscore = ln(f64(sb+1) * f64(sy+1)) + lnfact(mb) + lnfact(my), 255 if not
-// finite. One wave64 per spectrum group: peaks are staged in LDS once, lanes take candidates, the tolerance
-// window is found by binary search in LDS. The log-factorial (Stirling, lnfact(0) = 1) comes from an f64 LDS
-// table built once per block.
+// finite. One wave64 per spectrum group: peaks are staged in LDS once with a coarse lookup table (cell c of
+// width w holds the first peak whose cell index is >= c; the cell index is a monotone function of the mass, so
+// the table entry never passes the first peak >= the window start and a short forward skip finds it), lanes take
+// candidates. The log-factorial (Stirling, lnfact(0) = 1) comes from an f64 LDS table built once per block.
 // Ranking restates psm_table's order (engine.py:366-385): mb + my >= min_matched_peaks, hyperscore desc,
 // row asc (then candidate index, the stable-sort tie order).
 #include "vh_common.h"
 
 #define K2_WAVES 4
 #define K2_LNF 256
+#define K2_CELLS 1024   // coarse peak lookup table per spectrum (u16 first-peak index per mass cell)
 
 __device__ __forceinline__ double lnfact_dev(int n) {   // kernels.cu:246-250
   if (n == 0) return 1.0;
@@ -57,11 +59,12 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
   __shared__ double lnf[K2_LNF];
   __shared__ float s_m[K2_WAVES][VH_K2_MAX_PEAKS];
   __shared__ float s_i[K2_WAVES][VH_K2_MAX_PEAKS];
+  __shared__ uint16_t s_t[K2_WAVES][K2_CELLS];
   load_rm(rm, P);
   for (int i = threadIdx.x; i < K2_LNF; i += blockDim.x) lnf[i] = lnfact_dev(i);
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = lane_id();
-  const int wv = xcd_band_position() * K2_WAVES + w;
+  const int wv = xcd_band_position(P.xcd_group) * K2_WAVES + w;
   if (wv >= n_groups) return;
   const int g = order ? order[wv] : wv;
   const int s = grp_spec ? grp_spec[g] : g;
@@ -72,9 +75,28 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
   np = np < VH_K2_MAX_PEAKS ? np : VH_K2_MAX_PEAKS;
   float* sm = s_m[w];
   float* si = s_i[w];
+  uint16_t* tb = s_t[w];
   for (int j = lane; j < np; j += 64) {
     sm[j] = pk_mass[q0 + j];
     si[j] = pk_int[q0 + j];
+  }
+  wave_sync();
+  // coarse table: cell(x) = floor((x - base) * inv_w), clamped to [0, K2_CELLS - 1]
+  const float base = np ? sm[0] : 0.f;
+  const float span = np ? sm[np - 1] - base : 0.f;
+  const float inv_w = span > 0.f ? (float)(K2_CELLS - 2) / span : 1.f;
+  for (int j = lane; j <= np; j += 64) {
+    int cj = K2_CELLS;   // sentinel past the last cell for j == np
+    if (j < np) {
+      float f = floorf((sm[j] - base) * inv_w);
+      cj = f < 0.f ? 0 : (f > (float)(K2_CELLS - 1) ? K2_CELLS - 1 : (int)f);
+    }
+    int cp = -1;
+    if (j > 0) {
+      float f = floorf((sm[j - 1] - base) * inv_w);
+      cp = f < 0.f ? 0 : (f > (float)(K2_CELLS - 1) ? K2_CELLS - 1 : (int)f);
+    }
+    for (int c = cp + 1; c <= cj && c < K2_CELLS; ++c) tb[c] = (uint16_t)j;
   }
   wave_sync();
   for (int64_t c = c0 + lane; c < c0 + nc; c += 64) {
@@ -96,11 +118,9 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
           const float mz = cum / (float)z;
           const float a = tol_bound(mz, P.tol_lo, P.tol_da);
           const float bd = tol_bound(mz, P.tol_hi, P.tol_da);
-          int lo = 0, hi = np;
-          while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (sm[mid] < a) lo = mid + 1; else hi = mid;
-          }
+          float fc = floorf((a - base) * inv_w);
+          int lo = np ? (int)tb[fc < 0.f ? 0 : (fc > (float)(K2_CELLS - 1) ? K2_CELLS - 1 : (int)fc)] : 0;
+          while (lo < np && sm[lo] < a) ++lo;
           int best = -1;
           float mx = 0.f;
           for (int j = lo; j < np && sm[j] <= bd; ++j) {
@@ -204,7 +224,7 @@ int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int3
   if (n_groups == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   K2Buf O = {o_hs, o_sb, o_sy, o_ppm, o_mb, o_my, o_lb, o_ly};
-  unsigned g = xcd_band_grid((n_groups + K2_WAVES - 1) / K2_WAVES);
+  unsigned g = xcd_band_grid((n_groups + K2_WAVES - 1) / K2_WAVES, p->xcd_group);
   k2_kernel<<<g, 64 * K2_WAVES, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_groups, order, grp_spec, grp_start,
                                          grp_count, stride, c_row, c_hyp, h_maxfz, pk_off, pk_mass, pk_int, O);
   VH_CHECK_LAUNCH();

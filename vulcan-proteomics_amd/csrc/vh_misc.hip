@@ -29,6 +29,7 @@ DevP vh_make_devp(const VhParams* p) {
   d.k1_threshold = p->k1_threshold;
   d.pair_closure = p->pair_closure;
   d.tile = p->tile;
+  d.xcd_group = p->xcd_group;
   return d;
 }
 

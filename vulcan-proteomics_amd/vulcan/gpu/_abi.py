@@ -18,7 +18,8 @@ class VhParams(C.Structure):
                 ("tol_lo", F32), ("tol_hi", F32), ("fmin", F32), ("fmax", F32), ("min_ion_index", I32),
                 ("prec_lo", F32), ("prec_hi", F32), ("iso_lo", I32), ("iso_hi", I32), ("z_lo", I32), ("z_hi", I32),
                 ("max_fragment_charge", I32), ("max_peaks", I32), ("min_peaks", I32), ("min_matched_peaks", I32),
-                ("topk", I32), ("k1_threshold", I32), ("pair_closure", I32), ("tile", I32), ("reserved", I32 * 8)]
+                ("topk", I32), ("k1_threshold", I32), ("pair_closure", I32), ("tile", I32), ("xcd_group", I32),
+                ("reserved", I32 * 7)]
 
 
 class VhIndexView(C.Structure):
@@ -67,6 +68,7 @@ def make_params(p: SearchParams, static_mods: dict, var_mods: dict) -> VhParams:
     v.k1_threshold = int(p.k1_threshold)
     v.pair_closure = int(bool(p.pair_closure))
     v.tile = int(p.tile)
+    v.xcd_group = int(p.xcd_group)
     return v
 
 

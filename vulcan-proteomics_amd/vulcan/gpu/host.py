@@ -51,6 +51,7 @@ class SearchParams:
     tile: int = 1024                        # K1 LDS histogram slots per wave (rows per tile)
     k2_threads: int = 128                   # kept for signature compatibility (K2 runs one wave64 per spectrum)
     max_slice_entries: int = 2 ** 32 - 1    # postings per index (u32 offsets)
+    xcd_group: int = 32                     # K1/K2 blocks per XCD per scheduling group (0 = plain block order)
 
     def to_dict(self):
         return asdict(self)
