@@ -81,6 +81,9 @@ def f32_bits(x: float) -> int:
     return int(np.array([x], np.float32).view(np.uint32)[0])
 
 
+DIRECTORY_MAX = 2 ** 30     # directory entries (4 GiB of u32 offsets)
+
+
 def index_geometry(p: SearchParams, n_rows: int):
     """(bucket_log2, bin_shift, bin_base, n_bins, n_buckets) of the fragment index for `n_rows` rows."""
     c = int(p.chunk_size)
@@ -90,10 +93,13 @@ def index_geometry(p: SearchParams, n_rows: int):
         raise ValueError("need 0 < fragment_min_mz <= fragment_max_mz")
     if f32_bits(p.fragment_max_mz) - f32_bits(p.fragment_min_mz) >= 2 ** 26:
         raise ValueError("fragment_max_mz / fragment_min_mz spans more than 8 binary octaves")
-    lb = c.bit_length() - 1
     s = p.bin_shift()
     base = f32_bits(p.fragment_min_mz) >> s
     n_bins = (f32_bits(p.fragment_max_mz) >> s) - base + 1
+    # a larger table gets larger buckets so that the directory stays under DIRECTORY_MAX entries
+    while c < 2 ** 16 and max(1, (n_rows + c - 1) // c) * n_bins > DIRECTORY_MAX:
+        c *= 2
+    lb = c.bit_length() - 1
     n_buckets = max(1, (n_rows + c - 1) // c)
     if n_buckets * n_bins >= 2 ** 32:
         raise ValueError("bucket x bin directory exceeds 2^32 entries; raise chunk_size or bin_ppm")

@@ -46,9 +46,9 @@ class SearchParams:
     window_th: float = 100.0
     window_k: int = 15
     # index layout
-    chunk_size: int = 8192                  # rows per precursor-mass bucket (power of two, <= 2^16)
-    bin_ppm: float = 20.0                   # fragment bin width; rounded down to a power-of-two f32 bit bin
-    tile: int = 1024                        # K1 LDS histogram slots per wave (rows per tile)
+    chunk_size: int = 512                   # rows per precursor-mass bucket (power of two, <= 2^16)
+    bin_ppm: float = 80.0                   # directory bin width; rounded down to a power-of-two f32 bit bin
+    tile: int = 512                         # K1 LDS histogram slots per wave (rows per tile)
     k2_threads: int = 128                   # kept for signature compatibility (K2 runs one wave64 per spectrum)
     max_slice_entries: int = 2 ** 32 - 1    # postings per index (u32 offsets)
     xcd_group: int = 32                     # K1/K2 blocks per XCD per scheduling group (0 = plain block order)
