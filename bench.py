@@ -102,6 +102,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the rank-1 comparison with the CPU sample")
+    ap.add_argument("--chunk", type=int, default=None, help="override rows per index bucket")
+    ap.add_argument("--bin-ppm", type=float, default=None, help="override directory bin width")
+    ap.add_argument("--tile", type=int, default=None, help="override K1 LDS tile")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run the u32 copy kernel on this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -134,10 +137,19 @@ def main():
     n = len(sp)
     log(f"spectra: {n} ({time.perf_counter() - t0:.1f} s)")
     p = w.search_params()
+    modes = {}
+    if w.open_da is not None:
+        from vulcan.ptm.search import ModeParams, mode_search_params
+        mp = ModeParams(open_da=w.open_da)
+        p = mode_search_params(p, mp)
+        modes = {"open_da": mp.open_da, "k1_threshold_open": mp.k1_threshold_open}
+    for k, v in (("chunk_size", a.chunk), ("bin_ppm", a.bin_ppm), ("tile", a.tile)):
+        if v is not None:
+            setattr(p, k, v)
     gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
     log(f"index: {gi.n_postings} postings, {gi.n_buckets} buckets x {gi.n_bins} bins, build {gi.build_s:.3f} s")
     raw = E.upload_spectra(sp, dev)
-    ds = E.DeviceSearch(gi, p, n)
+    ds = E.DeviceSearch(gi, p, n, **modes)
     for _ in range(a.warmup):
         ds.run(raw)
     torch.cuda.synchronize()
@@ -163,6 +175,8 @@ def main():
     recovery = float(np.mean(best[sig] == truth[sig]))
     ab = algorithmic_bytes(ds, gi, n, l1)
     dom = max(("k1_topk", "k2"), key=lambda k: stage_ms[k])
+    geometry = {"chunk_size": 1 << gi.bucket_log2, "bin_shift": gi.bin_shift, "n_bins": gi.n_bins, "tile": p.tile,
+                "xcd_group": p.xcd_group}
     achieved = ab[dom] / (stage_ms[dom] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"{dom}_cfg{a.cfg}_traffic.json")
@@ -177,12 +191,14 @@ def main():
                       "topk": p.topk, "pair_closure": True, "precursor_ppm": list(p.precursor_ppm),
                       "fragment_da": list(p.fragment_ppm), "max_fragment_charge": p.max_fragment_charge,
                       "parallelism": "single-gpu"},
-           "stage_ms": stage_ms, "index_build_s": gi.build_s,
+           "stage_ms": stage_ms, "index_build_s": gi.build_s, "index_geometry": geometry,
            "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                         "algorithmic_bytes_per_launch": ab[dom]},
            "counts": ab, "top1_truth_recovery": recovery}
-    if not a.no_cpu:
+    if modes:
+        out["config"]["open_da"] = list(modes["open_da"])
+    if not a.no_cpu and not modes:
         log("cpu baseline ...")
         cb, res, ncpu = cpu_baseline(ds.vp, l1, sp, a.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 5
+#define VH_ABI_VERSION 6
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -31,6 +31,10 @@ extern "C" {
 #define VH_E_LIMIT (-4)      /* a documented size limit is exceeded (rows, bins, hypotheses per spectrum) */
 
 #define VH_MAX_HYP_PER_SPEC 64   /* hypotheses of one spectrum handled by one K1 wave */
+#define VH_MAX_OFFSETS 8         /* mass-offset hypotheses per (spectrum, charge) */
+#define VH_MODE_CLOSED 0         /* hypothesis kinds (vulcan/ptm/search.py:35) */
+#define VH_MODE_OFFSET 1
+#define VH_MODE_OPEN 2
 #define VH_K2_MAX_PEAKS 512      /* K2 looks at the first 512 mass-sorted peaks (kernels.cu:260, 278) */
 #define VH_MAX_PEPTIDE_LEN 63    /* residues per peptidoform (mask bits 0..62; bit 63 = N-term mod) */
 
@@ -57,7 +61,15 @@ typedef struct VhParams {
   int32_t pair_closure;       /* add partner rows (engine.py:297-301) */
   int32_t tile;               /* K1 LDS histogram slots per wave: 64..8192, power of two */
   int32_t xcd_group;          /* blocks per XCD per scheduling group (0: plain block order), see vh_common.h */
-  int32_t reserved[7];
+  /* search modes (ModeParams, vulcan/ptm/search.py:38-45): per (spectrum, charge) at isotope 0, one hypothesis
+   * per mass offset (window M - offset +- precursor ppm) and one open hypothesis (calc mass in
+   * [M - open_hi, M - open_lo]); they follow the closed hypotheses of the spectrum in that order. */
+  int32_t n_offsets;
+  float offsets[VH_MAX_OFFSETS];
+  int32_t open_search;
+  float open_lo, open_hi;
+  int32_t k1_threshold_open;  /* K1 append threshold of open hypotheses */
+  int32_t reserved[8];
 } VhParams;
 
 /* Device view of the peptidoform table (vulcan/index/build.py:13-21) and the fragment index.
@@ -120,11 +132,15 @@ int vh_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* inten
 /* ---------------------------------------------------------------------------------------------- hypotheses */
 /* Precursor hypotheses (host.py:64-91) on the device: order (spectrum, charge, isotope), f32 arithmetic
  * mz = pmz - PROTON, pm = mz*z, M = pm - iso*NEUTRON, window = M + M*ppm/1e6, rows = searchsorted left/right
- * on mass32; maxfz = max(min(z, max_fragment_charge+1), 2). hyp_off (n_spec+1) groups them by spectrum. */
+ * on mass32; maxfz = max(min(z, max_fragment_charge+1), 2). hyp_off (n_spec+1) groups them by spectrum.
+ * With n_offsets / open_search set, each spectrum's closed hypotheses are followed by its mass-offset and open
+ * hypotheses (make_mode_hypotheses, vulcan/ptm/search.py:48-77; h_mode = VH_MODE_*, h_off = offset or NaN for
+ * open, M of the isotope-0 hypothesis). h_mode / h_off may be NULL for closed-only parameters. */
 int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz, const int8_t* charge,
                   const uint8_t* valid, const float* mass32, int64_t n_rows, int64_t* hyp_off, int32_t* h_spec,
                   int8_t* h_z, int8_t* h_iso, float* h_pm, float* h_M, int64_t* h_lo, int64_t* h_hi,
-                  int8_t* h_maxfz, int64_t hyp_capacity, void* ws, size_t ws_bytes, void* stream);
+                  int8_t* h_maxfz, uint8_t* h_mode, float* h_off, int64_t hyp_capacity, void* ws, size_t ws_bytes,
+                  void* stream);
 
 /* Order of the spectra for the K1/K2 launches: ascending first-hypothesis window row (i.e. precursor mass;
  * spectra without hypotheses last), by a device radix sort. Waves then visit the index in mass order and the
@@ -140,10 +156,11 @@ int vh_spectrum_order(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_l
  * tol_bound(peak*z, lo) <= fragment <= tol_bound(peak*z, hi); append (h, r, count) when count >= threshold
  * (unordered; *n_out may exceed cap, then only cap entries are written); hyp_scored[h] = #rows with
  * count > 0, hyp_matched[h] = sum of counts (written, not accumulated). One wave64 per spectrum; `order`
- * (NULL = 0..n_spec-1) is the processing order of the spectra (vh_spectrum_order). */
+ * (NULL = 0..n_spec-1) is the processing order of the spectra (vh_spectrum_order); h_mode (NULL = all closed)
+ * selects k1_threshold_open for open hypotheses. */
 int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order, const int64_t* pk_off,
-          const float* pk_mass,
-          const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo, const int64_t* h_hi, int64_t cap,
+          const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const uint8_t* h_mode,
+          const int64_t* h_lo, const int64_t* h_hi, int64_t cap,
           unsigned long long* n_out, int32_t* out_hyp, int32_t* out_row, int32_t* out_cnt, uint32_t* hyp_scored,
           unsigned long long* hyp_matched, void* stream);
 
@@ -153,8 +170,8 @@ int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_
  * n_cand[s] = number of candidates (<= cand_stride). status[0] is set nonzero if a spectrum exceeded a
  * limit (its list is then empty). */
 int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
-               const int64_t* pk_off, const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo,
-               const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp,
+               const int64_t* pk_off, const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz,
+               const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp,
                int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
                void* stream);
 
@@ -171,14 +188,22 @@ int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32
  * ladders; hyperscore = ln((sb+1)(sy+1)) + lnfact(mb) + lnfact(my) (255 if not finite).
  * Groups of candidates share a spectrum: group g has spectrum grp_spec[g] (NULL: g), first candidate
  * grp_start[g] (NULL: g*stride) and grp_count[g] candidates. maxfz of a candidate = h_maxfz[c_hyp[c]].
- * c_shift (NULL = closed search) = observed - candidate mass: shifted + unshifted ions with the delta placed
- * on the best residue (vk_k2_shift, kernels.cu:291-363, 461-472). One wave64 per group; `order` (NULL =
+ * c_shift (NULL = closed search) = observed - candidate mass: a candidate with a nonzero shift scores shifted +
+ * unshifted ions and the delta is placed on the residue that maximises the hyperscore (vk_k2_shift,
+ * kernels.cu:291-363, 461-472; peptides up to 63 residues). One wave64 per group; `order` (NULL =
  * 0..n_groups-1) is the processing order of the groups. */
 int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* order, const int32_t* grp_spec,
           const int64_t* grp_start, const int32_t* grp_count, int32_t stride, const int32_t* c_row,
           const int32_t* c_hyp, const int8_t* h_maxfz, const float* c_shift, const int64_t* pk_off,
           const float* pk_mass, const float* pk_int, double* o_hs, float* o_sb, float* o_sy, float* o_ppm,
           int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, void* stream);
+
+/* Per-candidate precursor delta for vh_k2's shifted scoring: c_shift[c] = h_M[h] - mass32[row] when h is an
+ * offset or open hypothesis, 0 for a closed one (search_modes, vulcan/ptm/search.py:187-189). Groups as in
+ * vh_k2. */
+int vh_candidate_shift(int32_t n_groups, const int64_t* grp_start, const int32_t* grp_count, int32_t stride,
+                       const int32_t* c_row, const int32_t* c_hyp, const uint8_t* h_mode, const float* h_M,
+                       const float* mass32, float* c_shift, void* stream);
 
 /* ---------------------------------------------------------------------------------------------- ranking */
 /* PSM ranking (psm_table, engine.py:366-385): per group, candidates with mb + my >= min_matched_peaks

@@ -40,10 +40,10 @@ def lib():
         "or_abi_sizeof": ([I32], SZ),
         "or_lnfact": ([I32], F64),
         "or_k0": ([PP, I32, P, P, P, P, P, P, P, P, I64], I32),
-        "or_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, I64], I32),
+        "or_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, I64], I32),
         "or_k1_counts": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, P], I32),
         "or_k1_brute": ([PP, PIX, I64, P, P, P, P, P, P, P], I32),
-        "or_topk": ([PP, P, I32, P, P, P, P, P, I32, P, P, P, P], I32),
+        "or_topk": ([PP, P, I32, P, P, P, P, P, P, I32, P, P, P, P], I32),
         "or_k2": ([PP, PIX, I32, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
         "or_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P], I32),
         "or_search": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, P], I32),
@@ -111,16 +111,17 @@ def hypotheses(vp, sp, valid, mass32, maxh: int):
     cap = max(n * maxh, 1)
     o = {"hyp_off": np.zeros(n + 1, np.int64), "spec": np.zeros(cap, np.int32), "z": np.zeros(cap, np.int8),
          "iso": np.zeros(cap, np.int8), "pm": np.zeros(cap, np.float32), "M": np.zeros(cap, np.float32),
-         "lo": np.zeros(cap, np.int64), "hi": np.zeros(cap, np.int64), "maxfz": np.zeros(cap, np.int8)}
+         "lo": np.zeros(cap, np.int64), "hi": np.zeros(cap, np.int64), "maxfz": np.zeros(cap, np.int8),
+         "mode": np.zeros(cap, np.uint8), "off": np.zeros(cap, np.float32)}
     pmz = np.ascontiguousarray(sp.precursor_mz, np.float64)
     z = np.ascontiguousarray(sp.charge, np.int8)
     va = np.ascontiguousarray(valid, np.uint8)
     m32 = np.ascontiguousarray(mass32, np.float32)
     _chk(L.or_hypotheses(C.byref(vp), n, _p(pmz), _p(z), _p(va), _p(m32), len(m32), _p(o["hyp_off"]), _p(o["spec"]),
                          _p(o["z"]), _p(o["iso"]), _p(o["pm"]), _p(o["M"]), _p(o["lo"]), _p(o["hi"]),
-                         _p(o["maxfz"]), cap), "hypotheses")
+                         _p(o["maxfz"]), _p(o["mode"]), _p(o["off"]), cap), "hypotheses")
     nh = int(o["hyp_off"][-1])
-    for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz"):
+    for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz", "mode", "off"):
         o[k] = o[k][:nh]
     return o
 
@@ -171,14 +172,17 @@ def topk(vp, hix: HostIndex, hyp, k1, stride: int):
     n = max(n_spec, 1)
     o = {"n_cand": np.zeros(n, np.int32), "row": np.zeros(n * stride, np.int32),
          "hyp": np.zeros(n * stride, np.int32), "cnt": np.zeros(n * stride, np.int32)}
-    _chk(L.or_topk(C.byref(vp), _p(hix.partner), n_spec, _p(hyp["hyp_off"]), _p(np.ascontiguousarray(hyp["lo"])),
+    mode = np.ascontiguousarray(hyp["mode"], np.uint8) if "mode" in hyp else None
+    _chk(L.or_topk(C.byref(vp), _p(hix.partner), n_spec, _p(hyp["hyp_off"]), _p(mode),
+                   _p(np.ascontiguousarray(hyp["lo"])),
                    _p(np.ascontiguousarray(hyp["hi"])), _p(k1["win_off"]), _p(np.ascontiguousarray(k1["counts"])),
                    stride, _p(o["n_cand"]), _p(o["row"]), _p(o["hyp"]), _p(o["cnt"])), "topk")
     o["n_cand"] = o["n_cand"][:n_spec]
     return o
 
 
-def k2(vp, hix: HostIndex, peaks, row, hyp_idx, maxfz, grp_spec=None, grp_start=None, grp_count=None, stride=0):
+def k2(vp, hix: HostIndex, peaks, row, hyp_idx, maxfz, grp_spec=None, grp_start=None, grp_count=None, stride=0,
+       shift=None):
     """K2 outputs for candidates grouped by spectrum (same grouping arguments as vh_k2)."""
     L = lib()
     row = np.ascontiguousarray(row, np.int32)
@@ -192,7 +196,9 @@ def k2(vp, hix: HostIndex, peaks, row, hyp_idx, maxfz, grp_spec=None, grp_start=
     gs = None if grp_spec is None else np.ascontiguousarray(grp_spec, np.int32)
     g0 = None if grp_start is None else np.ascontiguousarray(grp_start, np.int64)
     _chk(L.or_k2(C.byref(vp), C.byref(hix.view), len(gc), _p(gs), _p(g0), _p(gc), stride, _p(row), _p(hyp_idx),
-                 _p(np.ascontiguousarray(maxfz, np.int8)), None, _p(peaks["off"]), _p(peaks["mass"]), _p(peaks["int"]),
+                 _p(np.ascontiguousarray(maxfz, np.int8)),
+                 None if shift is None else _p(np.ascontiguousarray(shift, np.float32)), _p(peaks["off"]),
+                 _p(peaks["mass"]), _p(peaks["int"]),
                  *[_p(o[k]) for k in ("hyperscore", "summed_b", "summed_y", "ppm", "matched_b", "matched_y",
                                       "longest_b", "longest_y")]), "k2")
     return {k: v[:n] for k, v in o.items()}

@@ -91,7 +91,8 @@ int n_hyp_of(const VhParams& p, double pmz, int8_t charge, bool valid) {
   if (!valid || !std::isfinite(pmz)) return 0;
   int niso = p.iso_hi - p.iso_lo + 1;
   int nz = charge > 0 ? 1 : (p.z_hi - p.z_lo + 1);
-  return niso * nz;
+  int extra = (p.iso_lo <= 0 && p.iso_hi >= 0) ? nz * (p.n_offsets + (p.open_search ? 1 : 0)) : 0;
+  return niso * nz + extra;
 }
 
 int8_t maxfz_of(const VhParams& p, int z) {
@@ -99,9 +100,11 @@ int8_t maxfz_of(const VhParams& p, int z) {
   return (int8_t)std::max(m, 2);
 }
 
+// closed hypotheses (host.py:64-91), then per mass offset and for the open window one hypothesis per charge at
+// isotope 0 (make_mode_hypotheses, vulcan/ptm/search.py:48-77), all of one spectrum together
 void hyp_one(const VhParams& p, int32_t s, double pmz, int8_t charge, const float* mass32, int64_t n_rows,
              int64_t o, int32_t* h_spec, int8_t* h_z, int8_t* h_iso, float* h_pm, float* h_M, int64_t* h_lo,
-             int64_t* h_hi, int8_t* h_maxfz) {
+             int64_t* h_hi, int8_t* h_maxfz, uint8_t* h_mode, float* h_off) {
   int zlo = charge > 0 ? charge : p.z_lo, zhi = charge > 0 ? charge : p.z_hi;
   float mz = (float)pmz - PROTON;
   for (int z = zlo; z <= zhi; ++z) {
@@ -119,6 +122,35 @@ void hyp_one(const VhParams& p, int32_t s, double pmz, int8_t charge, const floa
       h_lo[o] = std::lower_bound(mass32, mass32 + n_rows, lo) - mass32;
       h_hi[o] = std::upper_bound(mass32, mass32 + n_rows, hi) - mass32;
       h_maxfz[o] = maxfz_of(p, z);
+      if (h_mode) { h_mode[o] = VH_MODE_CLOSED; h_off[o] = 0.f; }
+    }
+  }
+  if (!(p.iso_lo <= 0 && p.iso_hi >= 0)) return;
+  int nmode = p.n_offsets + (p.open_search ? 1 : 0);
+  for (int q = 0; q < nmode; ++q) {
+    bool open = q == p.n_offsets;
+    for (int z = zlo; z <= zhi; ++z, ++o) {
+      float pm = mz * (float)z;
+      float M0 = pm - 0.0f * NEUTRON;
+      float lo, hi;
+      if (!open) {
+        float Mo = M0 - p.offsets[q];
+        lo = Mo + (Mo * p.prec_lo) / 1000000.0f;
+        hi = Mo + (Mo * p.prec_hi) / 1000000.0f;
+      } else {
+        lo = M0 - p.open_hi;
+        hi = M0 - p.open_lo;
+      }
+      h_spec[o] = s;
+      h_z[o] = (int8_t)z;
+      h_iso[o] = 0;
+      h_pm[o] = pm;
+      h_M[o] = M0;
+      h_lo[o] = std::lower_bound(mass32, mass32 + n_rows, lo) - mass32;
+      h_hi[o] = std::upper_bound(mass32, mass32 + n_rows, hi) - mass32;
+      h_maxfz[o] = maxfz_of(p, z);
+      h_mode[o] = open ? VH_MODE_OPEN : VH_MODE_OFFSET;
+      h_off[o] = open ? NAN : p.offsets[q];
     }
   }
 }
@@ -393,7 +425,8 @@ int or_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* it, c
 int or_hypotheses(const VhParams* p, int32_t n_spec, const double* pmz, const int8_t* charge, const uint8_t* valid,
                   const float* mass32, int64_t n_rows, int64_t* hyp_off, int32_t* h_spec, int8_t* h_z,
                   int8_t* h_iso, float* h_pm, float* h_M, int64_t* h_lo, int64_t* h_hi, int8_t* h_maxfz,
-                  int64_t cap) {
+                  uint8_t* h_mode, float* h_off, int64_t cap) {
+  if ((p->n_offsets || p->open_search) && (!h_mode || !h_off)) return VH_E_ARG;
   hyp_off[0] = 0;
   for (int32_t s = 0; s < n_spec; ++s) hyp_off[s + 1] = hyp_off[s] + n_hyp_of(*p, pmz[s], charge[s], valid[s]);
   if (hyp_off[n_spec] > cap) return VH_E_CAPACITY;
@@ -401,7 +434,7 @@ int or_hypotheses(const VhParams* p, int32_t n_spec, const double* pmz, const in
   for (int32_t s = 0; s < n_spec; ++s)
     if (hyp_off[s + 1] > hyp_off[s])
       hyp_one(*p, s, pmz[s], charge[s], mass32, n_rows, hyp_off[s], h_spec, h_z, h_iso, h_pm, h_M, h_lo, h_hi,
-              h_maxfz);
+              h_maxfz, h_mode, h_off);
   return 0;
 }
 
@@ -449,20 +482,22 @@ int or_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32
 }
 
 // ------------------------------------------------------------------------------------------------ top-K
-int or_topk(const VhParams* p, const int64_t* partner, int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo,
-            const int64_t* h_hi, const int64_t* win_off, const int32_t* counts, int32_t stride, int32_t* n_cand,
-            int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt) {
+int or_topk(const VhParams* p, const int64_t* partner, int32_t n_spec, const int64_t* hyp_off, const uint8_t* h_mode,
+            const int64_t* h_lo, const int64_t* h_hi, const int64_t* win_off, const int32_t* counts, int32_t stride,
+            int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt) {
   int need = p->pair_closure ? 2 * p->topk : p->topk;
   if (stride < need) return VH_E_CAPACITY;
 #pragma omp parallel for schedule(dynamic, 64)
   for (int32_t s = 0; s < n_spec; ++s) {
     std::vector<Ent> e;
     int64_t h0 = hyp_off[s];
-    for (int64_t h = h0; h < hyp_off[s + 1]; ++h)
+    for (int64_t h = h0; h < hyp_off[s + 1]; ++h) {
+      int thr = (h_mode && h_mode[h] == VH_MODE_OPEN) ? p->k1_threshold_open : p->k1_threshold;
       for (int64_t r = h_lo[h]; r < h_hi[h]; ++r) {
         int k = counts[win_off[h] + (r - h_lo[h])];
-        if (k >= p->k1_threshold) e.push_back({r, (int32_t)(h - h0), k});
+        if (k >= thr) e.push_back({r, (int32_t)(h - h0), k});
       }
+    }
     std::vector<Ent> out(2 * e.size() + 1);
     int n = topk_one(*p, partner, e, out.data());
     n_cand[s] = n;
@@ -523,6 +558,7 @@ int or_rank(const VhParams* p, int32_t n_groups, const int64_t* grp_start, const
 int or_search(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const float* mz, const float* it,
               const int64_t* raw_off, const double* pmz, const int8_t* charge, int32_t* o_row, double* o_hs,
               double* o_dn, int32_t* o_ncand, int64_t* stats) {
+  if (p->n_offsets || p->open_search) return VH_E_ARG;   // closed search only (modes: component functions)
   Ctx c(p);
   int64_t st_resc = 0, st_win = 0, st_match = 0;
   int niso = p->iso_hi - p->iso_lo + 1;
@@ -544,7 +580,7 @@ int or_search(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const fl
       int nh = n_hyp_of(*p, pmz[s], charge[s], np >= p->min_peaks);
       if (!nh) continue;
       hyp_one(*p, s, pmz[s], charge[s], ix->mass32, ix->n_rows, 0, hs_spec.data(), hz.data(), hiso.data(),
-              hpm.data(), hM.data(), hlo.data(), hhi.data(), hmfz.data());
+              hpm.data(), hM.data(), hlo.data(), hhi.data(), hmfz.data(), nullptr, nullptr);
       int zmax = 2;
       for (int h = 0; h < nh; ++h) zmax = std::max(zmax, (int)hmfz[h]);
       pb.build(c, pm.data(), np, zmax);

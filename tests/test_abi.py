@@ -47,8 +47,8 @@ def test_argument_errors_are_reported_without_a_gpu():
     L = _lib.load()
     vp = make_params(SearchParams(), {}, {})
     ix = VhIndexView()
-    assert L.vh_k1_topk(C.byref(vp), C.byref(ix), 1, None, None, None, None, None, None, None, 2, None, None, None,
-                        None, None, None, None, None) == -1
+    assert L.vh_k1_topk(C.byref(vp), C.byref(ix), 1, None, None, None, None, None, None, None, None, 2, None, None,
+                        None, None, None, None, None, None) == -1
     assert L.vh_k2(C.byref(vp), C.byref(ix), 1, None, None, None, None, 0, None, None, None, None, None, None, None,
                    None, None, None, None, None, None, None, None, None) == -1
     assert L.vh_spectrum_order(3, None, None, None, None, 0, None) == -1

@@ -336,3 +336,93 @@ def test_spectrum_order_sorted_permutation(env1):
     assert np.all(np.diff(k) >= 0)
     same = np.diff(k) == 0
     assert np.all(np.diff(order)[same] > 0)          # stable: equal keys keep spectrum order
+
+
+# ------------------------------------------------------------------------------------------------ search modes
+@pytest.fixture(scope="module")
+def env_modes():
+    """300-protein cfg1-semantics table, spectra with 40% precursor-only shifts (the open-search workload's
+    generator), open window (-150, 500) Da and two mass offsets."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu._abi import make_params, max_hyp_per_spectrum, set_modes
+    from vulcan.index import synth
+    from vulcan.index.build import build_tables
+    from vulcan.ptm.search import ModeParams, mode_search_params
+    print("[gpu] building modes env", flush=True)
+    w1 = synth.WORKLOADS[1]
+    l1, lab = build_tables(synth.proteome(300, 103), w1.index_params())
+    sp, truth, meta = synth.spectra(l1, lab, 3, n=300, seed=33, index_params=w1.index_params())
+    mp = ModeParams(offsets=(15.994915, 42.010565), open_da=(-150.0, 500.0), open_tile=2048)
+    p = mode_search_params(w1.search_params(), mp)
+    p.topk = 20
+    gi = E.GpuIndex(l1, p, STATIC, w1.var_mods)
+    vp = set_modes(make_params(p, STATIC, w1.var_mods), mp.offsets, mp.open_da)
+    O = _O()
+    hix = O.HostIndex(l1)
+    pk = O.k0(vp, sp)
+    hyp = O.hypotheses(vp, sp, pk["valid"], l1.mass32, max_hyp_per_spectrum(p, 2, True))
+    k1 = O.k1_counts(vp, hix, pk, hyp)
+    return {"l1": l1, "sp": sp, "truth": truth, "shift": meta["shift"], "mp": mp, "p": p, "gi": gi, "vp": vp,
+            "hix": hix, "pk": pk, "hyp": hyp, "k1": k1, "var": w1.var_mods}
+
+
+def test_modes_device_pipeline_equals_oracle(env_modes):
+    """Mode hypotheses, K1 top-K over closed + offset + open windows, shifted K2 and rank 1 == the oracle."""
+    from vulcan.gpu import engine as E
+    e = env_modes
+    O = _O()
+    ds = E.DeviceSearch(e["gi"], e["p"], len(e["sp"]), offsets=e["mp"].offsets, open_da=e["mp"].open_da)
+    ds.run(E.upload_spectra(e["sp"], e["gi"].dev))
+    torch.cuda.synchronize()
+    assert int(ds.status.item()) == 0
+    hd = E.hyp_to_host(ds.hyp)
+    for k in ("hyp_off", "spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz", "mode"):
+        assert np.array_equal(hd[k], e["hyp"][k]), k
+    assert np.array_equal(hd["off"], e["hyp"]["off"], equal_nan=True)
+    tk = O.topk(e["vp"], e["hix"], e["hyp"], e["k1"], ds.stride)
+    n = len(e["sp"])
+    nc = _np(ds.n_cand)[:n]
+    assert np.array_equal(nc, tk["n_cand"])
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    rows = _np(ds.c_row)[:n * ds.stride][m]
+    hyps = _np(ds.c_hyp)[:n * ds.stride][m]
+    assert np.array_equal(rows, tk["row"][m]) and np.array_equal(hyps, tk["hyp"][m])
+    assert np.array_equal(_np(ds.c_cnt)[:n * ds.stride][m], tk["cnt"][m])
+    hm, hM = e["hyp"]["mode"], e["hyp"]["M"]
+    shift = np.where(hm[hyps] == 0, np.float32(0), (hM[hyps] - e["l1"].mass32[rows]).astype(np.float32))
+    assert np.array_equal(_np(ds.c_shift)[:n * ds.stride][m], shift.astype(np.float32))
+    start = np.concatenate([[0], np.cumsum(nc)[:-1]])
+    ref = O.k2(e["vp"], e["hix"], e["pk"], rows, hyps, e["hyp"]["maxfz"], grp_start=start, grp_count=nc,
+               shift=shift)
+    assert (shift != 0).sum() > 100
+    for k in ("matched_b", "matched_y", "longest_b", "longest_y", "summed_b", "summed_y"):
+        assert np.array_equal(_np(ds.k2[k])[:n * ds.stride][m], ref[k]), k
+    assert np.array_equal(_np(ds.k2["ppm"])[:n * ds.stride][m], ref["ppm"], equal_nan=True)
+    assert np.allclose(_np(ds.k2["hyperscore"])[:n * ds.stride][m], ref["hyperscore"], rtol=HS_RTOL, atol=0)
+
+
+def test_search_modes_recovers_shifted_spectra_and_is_label_blind(env_modes):
+    """Shifted spectra are found through offset / open hypotheses (reference test_ptm_modes.py:114-117 bar:
+    > 50% of rank-1 PSMs of shifted spectra come from mode > 0), and role swap + pair permutation give identical
+    tables (test_ptm_modes.py:94-117)."""
+    import pyarrow as pa
+
+    from vulcan.gpu import engine as E
+    from vulcan.index import synth
+    from vulcan.index.build import build_tables
+    from vulcan.ptm.search import search_modes
+    e = env_modes
+    tbl, st = search_modes(e["sp"], e["gi"], e["l1"], e["p"], e["mp"], report=5)
+    mode = tbl.column("mode").to_numpy()
+    r1 = tbl.column("rank").to_numpy() == 1
+    spec = tbl.column("spec_idx").to_numpy()
+    shifted = set(np.flatnonzero(np.abs(e["shift"]) > 1.0).tolist())
+    sel = r1 & np.isin(spec, list(shifted))
+    assert sel.sum() > 20 and (mode[sel] > 0).mean() > 0.5
+    w1 = synth.WORKLOADS[1]
+    b, _ = build_tables(synth.proteome(300, 103), w1.index_params(), shuffle_rows_seed=5, swap_labels=True)
+    gib = E.GpuIndex(b, e["p"], STATIC, e["var"])
+    tb, _ = search_modes(e["sp"], gib, b, e["p"], e["mp"], report=5)
+    ho = lambda t: t.set_column(t.column_names.index("hyp_offset"), "hyp_offset",  # noqa: E731
+                                pa.array(t.column("hyp_offset").to_numpy().view(np.uint32)))
+    assert ho(tbl).equals(ho(tb))

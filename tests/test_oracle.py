@@ -356,3 +356,95 @@ def test_builder_invariants(cfg0):
     assert all("I" not in s for s in seqs)
     i = 5
     assert sorted(l1.sequence(i)) == sorted(l1.sequence(int(l1.partner[i])))
+
+
+# ------------------------------------------------------------------------------------------------ search modes
+def test_mode_hypotheses_match_host_restatement(cfg1_small):
+    """Closed, mass-offset and open hypotheses (ptm/search.py:48-77): the C++ oracle and the host numpy
+    restatement agree field by field, and every spectrum's hypotheses keep the order closed, offsets, open."""
+    from vulcan.gpu._abi import make_params, max_hyp_per_spectrum, set_modes
+    from vulcan.ptm.search import ModeParams, make_mode_hypotheses
+    O = _O()
+    d = cfg1_small
+    p = d["p"]
+    mp = ModeParams(offsets=(15.994915, 42.010565), open_da=(-150.0, 500.0))
+    vp = set_modes(make_params(p, d["static"], d["var"]), mp.offsets, mp.open_da)
+    pk = O.k0(vp, d["sp"])
+    h = O.hypotheses(vp, d["sp"], pk["valid"], d["l1"].mass32, max_hyp_per_spectrum(p, 2, True))
+    ref = make_mode_hypotheses(d["sp"], pk["valid"].astype(bool), d["l1"].mass32, p, mp)
+    for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz", "mode"):
+        assert np.array_equal(h[k], ref[k]), k
+    assert np.array_equal(h["off"], ref["off"], equal_nan=True)
+    s = h["spec"]
+    for i in range(len(s) - 1):
+        if s[i] == s[i + 1]:
+            assert h["mode"][i] <= h["mode"][i + 1]
+    open_ = h["mode"] == 2
+    assert open_.any() and np.all(h["hi"][open_] - h["lo"][open_] > 100)
+
+
+def test_k2_shift_localizes_the_delta():
+    """PEPTIDEK with +79.966331 on residue 3 (T): unshifted b0-b2 and y3-y6, shifted b3-b6 and y0-y2 are all
+    present. Unshifted scoring misses half of them; shifted scoring places the delta at residue 3 and matches
+    all 7 b and 7 y ions, so hyperscore = ln(71^2) + 2 lnfact(7)."""
+    from vulcan.gpu.host import SearchParams
+    O = _O()
+    l1, _ = _tiny_index(["PEPTIDEK"])
+    r = _row_of(l1, "PEPTIDEK")
+    b, y, M = _f64_ladder("PEPTIDEK")
+    d = 79.966331
+    P = 1.0072764
+    bs = b + np.where(np.arange(7) >= 3, d, 0.0)
+    ys = y + np.where(np.arange(7) < 3, d, 0.0)
+    mz = np.sort(np.concatenate([bs, ys]) + P)
+    sp = _spectra([(mz, np.full(len(mz), 10.0))], [(M + d + 2 * P) / 2], [2])
+    p = SearchParams(fragment_unit="da", fragment_ppm=(-0.02, 0.02), min_peaks=1, fragment_min_mz=50.0)
+    vp = _vp(p)
+    pk = O.k0(vp, sp)
+    hix = O.HostIndex(l1)
+    shift = np.float32(np.float32(M + d) - l1.mass32[r])
+    closed = O.k2(vp, hix, pk, [r], [0], np.array([2], np.int8), grp_count=[1], grp_start=[0])
+    sh = O.k2(vp, hix, pk, [r], [0], np.array([2], np.int8), grp_count=[1], grp_start=[0], shift=[shift])
+    assert closed["matched_b"][0] + closed["matched_y"][0] < 14
+    assert sh["matched_b"][0] == 7 and sh["matched_y"][0] == 7
+    assert sh["summed_b"][0] == 70.0 and sh["summed_y"][0] == 70.0
+    want = math.log(71.0 * 71.0) + 2 * (7 * math.log(7) - 7 + 0.5 * math.log(7) + 0.5 * math.log(2 * math.pi * 7))
+    assert abs(sh["hyperscore"][0] - want) < 1e-12
+    zero = O.k2(vp, hix, pk, [r], [0], np.array([2], np.int8), grp_count=[1], grp_start=[0], shift=[0.0])
+    for k in closed:                                   # shift 0 is the closed path exactly
+        assert np.array_equal(zero[k], closed[k], equal_nan=True), k
+
+
+def test_open_search_oracle_finds_shifted_spectra():
+    """The oracle's mode pipeline (hypotheses -> K1 counts -> top-K -> shifted K2 -> rank 1) finds spectra whose
+    precursor carries an unexplained delta through offset / open hypotheses (> 50% of them at rank 1,
+    the reference's bar in test_ptm_modes.py:114-117)."""
+    from vulcan.gpu._abi import make_params, max_hyp_per_spectrum, set_modes
+    from vulcan.index import synth
+    from vulcan.index.build import build_tables
+    O = _O()
+    w1 = synth.WORKLOADS[1]
+    l1, lab = build_tables(synth.proteome(200, 104), w1.index_params())
+    sp, truth, meta = synth.spectra(l1, lab, 3, n=150, seed=34, index_params=w1.index_params())
+    p = w1.search_params()
+    p.topk = 20
+    vp = set_modes(make_params(p, STATIC, w1.var_mods), (15.994915,), (-150.0, 500.0))
+    hix = O.HostIndex(l1)
+    pk = O.k0(vp, sp)
+    hyp = O.hypotheses(vp, sp, pk["valid"], l1.mass32, max_hyp_per_spectrum(p, 1, True))
+    k1 = O.k1_counts(vp, hix, pk, hyp)
+    stride = 2 * p.topk
+    tk = O.topk(vp, hix, hyp, k1, stride)
+    n = len(sp)
+    m = (np.arange(stride)[None, :] < tk["n_cand"][:, None]).ravel()
+    rows, hyps = tk["row"][m], tk["hyp"][m]
+    shift = np.where(hyp["mode"][hyps] == 0, np.float32(0),
+                     (hyp["M"][hyps] - l1.mass32[rows]).astype(np.float32)).astype(np.float32)
+    start = np.concatenate([[0], np.cumsum(tk["n_cand"])[:-1]])
+    k2 = O.k2(vp, hix, pk, rows, hyps, hyp["maxfz"], grp_start=start, grp_count=tk["n_cand"], shift=shift)
+    sel, _ = O.rank(vp, start, tk["n_cand"], rows, k2["hyperscore"], k2["matched_b"], k2["matched_y"], 1)
+    shifted = np.flatnonzero(np.abs(meta["shift"]) > 1.0)
+    got = [hyp["mode"][hyps[sel[s]]] > 0 for s in shifted if sel[s] >= 0]
+    assert len(got) > 20 and np.mean(got) > 0.5
+    hit = [rows[sel[s]] == truth[s] for s in shifted if sel[s] >= 0]
+    assert np.mean(hit) > 0.5

@@ -23,6 +23,11 @@ struct DevP {
   float prec_lo, prec_hi;
   int iso_lo, iso_hi, z_lo, z_hi, max_fragment_charge;
   int max_peaks, min_peaks, min_matched_peaks, topk, k1_threshold, pair_closure, tile, xcd_group;
+  int n_offsets;
+  float offsets[VH_MAX_OFFSETS];
+  int open_search;
+  float open_lo, open_hi;
+  int k1_threshold_open;
 };
 
 struct DevIdx {

@@ -183,7 +183,8 @@ __device__ __forceinline__ int n_hyp_of(const DevP& P, double pmz, int8_t z, uin
   if (!ok || !isfinite(pmz)) return 0;
   int niso = P.iso_hi - P.iso_lo + 1;
   int nz = z > 0 ? 1 : (P.z_hi - P.z_lo + 1);
-  return niso * nz;
+  int extra = (P.iso_lo <= 0 && P.iso_hi >= 0) ? nz * (P.n_offsets + (P.open_search ? 1 : 0)) : 0;
+  return niso * nz + extra;
 }
 
 __global__ __launch_bounds__(256) void hyp_count_kernel(DevP P, int n_spec, const double* __restrict__ pmz,
@@ -218,7 +219,8 @@ __global__ __launch_bounds__(256) void hyp_emit_kernel(DevP P, int n_spec, const
                                                        const int64_t* __restrict__ hyp_off,
                                                        const float* __restrict__ mass32, int64_t n_rows,
                                                        int32_t* h_spec, int8_t* h_z, int8_t* h_iso, float* h_pm,
-                                                       float* h_M, int64_t* h_lo, int64_t* h_hi, int8_t* h_maxfz) {
+                                                       float* h_M, int64_t* h_lo, int64_t* h_hi, int8_t* h_maxfz,
+                                                       uint8_t* h_mode, float* h_off) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_spec) return;
   int64_t o = hyp_off[s];
@@ -242,6 +244,37 @@ __global__ __launch_bounds__(256) void hyp_emit_kernel(DevP P, int n_spec, const
       h_lo[o] = lower_bound_f(mass32, n_rows, lo);
       h_hi[o] = upper_bound_f(mass32, n_rows, hi);
       h_maxfz[o] = (int8_t)(mf > 2 ? mf : 2);
+      if (h_mode) { h_mode[o] = VH_MODE_CLOSED; h_off[o] = 0.f; }
+    }
+  }
+  if (!(P.iso_lo <= 0 && P.iso_hi >= 0)) return;
+  // mass-offset then open hypotheses of the isotope-0 base (vulcan/ptm/search.py:56-76)
+  const int nmode = P.n_offsets + (P.open_search ? 1 : 0);
+  for (int q = 0; q < nmode; ++q) {
+    const bool open = q == P.n_offsets;
+    for (int z = zlo; z <= zhi; ++z, ++o) {
+      float pm = mz * (float)z;
+      float M0 = pm;   // isotope 0: pm - 0 * NEUTRON
+      float lo, hi;
+      if (!open) {
+        float Mo = M0 - P.offsets[q];
+        lo = Mo + (Mo * P.prec_lo) / 1000000.0f;
+        hi = Mo + (Mo * P.prec_hi) / 1000000.0f;
+      } else {
+        lo = M0 - P.open_hi;
+        hi = M0 - P.open_lo;
+      }
+      int mf = P.max_fragment_charge ? (z < P.max_fragment_charge + 1 ? z : P.max_fragment_charge + 1) : z;
+      h_spec[o] = s;
+      h_z[o] = (int8_t)z;
+      h_iso[o] = 0;
+      h_pm[o] = pm;
+      h_M[o] = M0;
+      h_lo[o] = lower_bound_f(mass32, n_rows, lo);
+      h_hi[o] = upper_bound_f(mass32, n_rows, hi);
+      h_maxfz[o] = (int8_t)(mf > 2 ? mf : 2);
+      h_mode[o] = open ? VH_MODE_OPEN : VH_MODE_OFFSET;
+      h_off[o] = open ? __int_as_float(0x7fc00000) : P.offsets[q];
     }
   }
 }
@@ -314,11 +347,15 @@ int vh_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* inten
 int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz, const int8_t* charge,
                   const uint8_t* valid, const float* mass32, int64_t n_rows, int64_t* hyp_off, int32_t* h_spec,
                   int8_t* h_z, int8_t* h_iso, float* h_pm, float* h_M, int64_t* h_lo, int64_t* h_hi,
-                  int8_t* h_maxfz, int64_t hyp_capacity, void* ws, size_t ws_bytes, void* stream) {
+                  int8_t* h_maxfz, uint8_t* h_mode, float* h_off, int64_t hyp_capacity, void* ws, size_t ws_bytes,
+                  void* stream) {
   if (!p || n_spec < 0 || !hyp_off) return VH_E_ARG;
   int niso = p->iso_hi - p->iso_lo + 1, nz = p->z_hi - p->z_lo + 1;
   if (niso < 1 || nz < 1) return VH_E_ARG;
-  int maxh = niso * nz > niso ? niso * nz : niso;
+  if (p->n_offsets < 0 || p->n_offsets > VH_MAX_OFFSETS) return VH_E_ARG;
+  const bool modes = p->n_offsets > 0 || p->open_search;
+  if (modes && (!h_mode || !h_off)) return VH_E_ARG;
+  int maxh = niso * nz + nz * (p->n_offsets + (p->open_search ? 1 : 0));
   if (maxh > VH_MAX_HYP_PER_SPEC) return VH_E_LIMIT;
   if (hyp_capacity < (int64_t)n_spec * maxh) return VH_E_CAPACITY;
   if (n_spec == 0) return 0;
@@ -330,7 +367,7 @@ int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz,
   int rc = vh_scan_inclusive_i64(hyp_off + 1, n_spec, ws, ws_bytes, st);
   if (rc) return rc;
   hyp_emit_kernel<<<g, 256, 0, st>>>(P, n_spec, precursor_mz, charge, hyp_off, mass32, n_rows, h_spec, h_z, h_iso,
-                                     h_pm, h_M, h_lo, h_hi, h_maxfz);
+                                     h_pm, h_M, h_lo, h_hi, h_maxfz, h_mode, h_off);
   VH_CHECK_LAUNCH();
   return 0;
 }

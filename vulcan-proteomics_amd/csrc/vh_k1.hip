@@ -21,7 +21,7 @@
 #define K1_MAXBK 32
 
 struct Seg {   // one (hypothesis, row range) piece of a tile: slots [base, base + r1 - r0)
-  int r0, r1, base, hl_mfz;   // hl_mfz = local hypothesis | maxfz << 8
+  int r0, r1, base, hl_mfz;   // hl_mfz = local hypothesis | maxfz << 8 | open hypothesis << 16
 };
 
 struct BkEnt {   // a bucket touched by a tile: its segments (bit mask), their row hull and max maxfz
@@ -81,6 +81,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
                                                  const float* __restrict__ pk_mass,
                                                  const int64_t* __restrict__ hyp_off,
                                                  const int8_t* __restrict__ h_maxfz,
+                                                 const uint8_t* __restrict__ h_mode,
                                                  const int64_t* __restrict__ h_lo,
                                                  const int64_t* __restrict__ h_hi, K1Out O) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -103,11 +104,11 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
     if (nh > 64 && lane == 0) atomicOr(O.status, 1);
     return;
   }
-  int my_lo = 0, my_hi = 0, my_mfz = 0;
+  int my_lo = 0, my_hi = 0, my_mfz = 0;   // my_mfz also carries the open-hypothesis flag at bit 8
   if (lane < nh) {
     my_lo = (int)h_lo[h0 + lane];
     my_hi = (int)h_hi[h0 + lane];
-    my_mfz = h_maxfz[h0 + lane];
+    my_mfz = h_maxfz[h0 + lane] | ((h_mode && h_mode[h0 + lane] == VH_MODE_OPEN) ? 256 : 0);
   }
   hsc[lane] = 0;
   hmt[lane] = 0;
@@ -142,10 +143,11 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       if (free_bk <= 0) break;
       int cap_rows = (free_bk << LB) - (rc & (B - 1));
       take = take < cap_rows ? take : cap_rows;
-      int mf = __shfl(my_mfz, hc);
+      const int mfo = __shfl(my_mfz, hc);
+      const int mf = mfo & 0xFF;
       if (lane == 0) {
         Seg g;
-        g.r0 = rc; g.r1 = rc + take; g.base = used; g.hl_mfz = hc | (mf << 8);
+        g.r0 = rc; g.r1 = rc + take; g.base = used; g.hl_mfz = hc | (mfo << 8);
         segs[nseg] = g;
       }
       for (int bk = rc >> LB; bk <= (rc + take - 1) >> LB; ++bk) {
@@ -219,7 +221,8 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
                 const int q = __builtin_ctz(msk);
                 msk &= msk - 1u;
                 const Seg g = segs[q];
-                if (z < (g.hl_mfz >> 8) && row >= g.r0 && row < g.r1) atomicAdd(&hist[g.base + row - g.r0], 1u);
+                if (z < ((g.hl_mfz >> 8) & 0xFF) && row >= g.r0 && row < g.r1)
+                  atomicAdd(&hist[g.base + row - g.r0], 1u);
               }
             }
           }
@@ -231,6 +234,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
     for (int q = 0; q < nseg; ++q) {
       const Seg g = segs[q];
       const int g_hl = g.hl_mfz & 0xFF;
+      const int thr = (g.hl_mfz >> 16) & 1 ? P.k1_threshold_open : P.k1_threshold;
       const int len = g.r1 - g.r0;
       uint32_t sc = 0;
       unsigned long long mt = 0;
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
         const uint32_t c = act ? hist[g.base + j] : 0u;
         sc += c > 0u;
         mt += c;
-        const bool flag = act && (int)c >= P.k1_threshold;
+        const bool flag = act && (int)c >= thr;
         if (MODE == 0) {
           uint64_t bal = ballot(flag);
           if (bal) {
@@ -430,8 +434,8 @@ static void k1_geometry(const VhParams* p, int mode, K1Out& O, int& waves) {
 extern "C" {
 
 int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order, const int64_t* pk_off,
-          const float* pk_mass,
-          const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo, const int64_t* h_hi, int64_t cap,
+          const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const uint8_t* h_mode,
+          const int64_t* h_lo, const int64_t* h_hi, int64_t cap,
           unsigned long long* n_out, int32_t* out_hyp, int32_t* out_row, int32_t* out_cnt, uint32_t* hyp_scored,
           unsigned long long* hyp_matched, void* stream) {
   int rc = k1_check(p, ix);
@@ -453,16 +457,16 @@ int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
   unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
   k1_kernel<0><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
-                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_lo,
-                                                                      h_hi, O);
+                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode,
+                                                                      h_lo, h_hi, O);
   VH_CHECK_LAUNCH();
   return 0;
 }
 
 int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
-               const int64_t* pk_off, const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo,
-               const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp,
-               int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
+               const int64_t* pk_off, const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz,
+               const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi, int32_t cand_stride,
+               int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
                void* stream) {
   int rc = k1_check(p, ix);
   if (rc) return rc;
@@ -486,8 +490,8 @@ int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const i
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
   unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
   k1_kernel<1><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
-                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_lo,
-                                                                      h_hi, O);
+                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode,
+                                                                      h_lo, h_hi, O);
   VH_CHECK_LAUNCH();
   return 0;
 }

@@ -46,12 +46,64 @@ struct K2Buf {
   int16_t *mb, *my, *lb, *ly;
 };
 
+// Mass-sorted peaks of one spectrum in LDS with the coarse lookup table.
+struct PeakView {
+  const float* sm;
+  const float* si;
+  const uint16_t* tb;
+  int np;
+  float base, inv_w;
+
+  __device__ __forceinline__ int first_at_least(float a) const {
+    float fc = floorf((a - base) * inv_w);
+    int lo = np ? (int)tb[fc < 0.f ? 0 : (fc > (float)(K2_CELLS - 1) ? K2_CELLS - 1 : (int)fc)] : 0;
+    while (lo < np && sm[lo] < a) ++lo;
+    return lo;
+  }
+};
+
+// One tolerance window at m/z `mz`: the most intense peak (ties -> later peak) updates the running (mx, best)
+// shared across the unshifted/shifted windows and the window's own (mx_a, best_a) (kernels.cu:312-329).
+__device__ __forceinline__ void window_scan(const PeakView& V, const DevP& P, float mz, float& mx, int& best,
+                                            float& best_mz, float& mx_a, int& best_a) {
+  const float a = tol_bound(mz, P.tol_lo, P.tol_da);
+  const float bd = tol_bound(mz, P.tol_hi, P.tol_da);
+  for (int j = V.first_at_least(a); j < V.np && V.sm[j] <= bd; ++j) {
+    const float v = V.si[j];
+    if (v >= mx) { mx = v; best = j; best_mz = mz; }
+    if (v >= mx_a) { mx_a = v; best_a = j; }
+  }
+}
+
+// Per-ion unshifted (u) and shifted (s) matched intensity and counts over the fragment charges (the iu/nu/is/ns
+// arrays of kernels.cu:300-305, one ion at a time).
+__device__ __forceinline__ void ion_alt_sums(const PeakView& V, const DevP& P, float cum, float shift, int maxfz,
+                                             float& iu, int& nu, float& is, int& ns) {
+  iu = 0.f; is = 0.f; nu = 0; ns = 0;
+  for (int z = 1; z < maxfz; ++z) {
+    float mx = 0.f, bmz = 0.f;
+    int best = -1;
+    for (int alt = 0; alt < 2; ++alt) {
+      float mx_a = 0.f;
+      int best_a = -1;
+      const float base = alt == 0 ? cum : cum + shift;
+      window_scan(V, P, base / (float)z, mx, best, bmz, mx_a, best_a);
+      if (best_a >= 0) {
+        if (alt == 0) { iu = iu + V.si[best_a]; nu += 1; }
+        else { is = is + V.si[best_a]; ns += 1; }
+      }
+    }
+  }
+}
+
+template <bool SHIFT>
 __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups, const int32_t* __restrict__ order,
                                                  const int32_t* __restrict__ grp_spec,
                                                  const int64_t* __restrict__ grp_start,
                                                  const int32_t* __restrict__ grp_count, int stride,
                                                  const int32_t* __restrict__ c_row, const int32_t* __restrict__ c_hyp,
                                                  const int8_t* __restrict__ h_maxfz,
+                                                 const float* __restrict__ c_shift,
                                                  const int64_t* __restrict__ pk_off,
                                                  const float* __restrict__ pk_mass,
                                                  const float* __restrict__ pk_int, K2Buf O) {
@@ -99,6 +151,7 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
     for (int c = cp + 1; c <= cj && c < K2_CELLS; ++c) tb[c] = (uint16_t)j;
   }
   wave_sync();
+  const PeakView V = {sm, si, tb, np, base, inv_w};
   for (int64_t c = c0 + lane; c < c0 + nc; c += 64) {
     const int r = c_row[c];
     const int maxfz = h_maxfz[c_hyp[c]];
@@ -106,26 +159,32 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
     const int L = X.length[r];
     const uint64_t mk = X.mask[r];
     const float nt = (mk >> 63) ? P.nterm : 0.f;
+    const float shift = SHIFT ? c_shift[c] : 0.f;
+    const int n_alt = (SHIFT && shift != 0.f) ? 2 : 1;
     float sb = 0.f, sy = 0.f, ppm = 0.f;
     int mb = 0, my = 0;
     Run rb = {0, 0, 0, 0}, ry = {0, 0, 0, 0};
+    float bs_init = 0.f, yu_init = 0.f;   // sum_k is[0][k] and sum_k iu[1][k] (k ascending)
+    int nbs_init = 0, nyu_init = 0;
     for (int kind = 0; kind < 2; ++kind) {
       float cum = kind == 0 ? nt : X.mass32[r] - nt;
       for (int k = 0; k < L - 1; ++k) {
         const float rmv = rm[((mk >> k) & 1ULL) * 26 + (sq[k] - 'A')];
         cum = kind == 0 ? cum + rmv : cum - rmv;
+        float iu = 0.f, is = 0.f;
+        int nu = 0, ns = 0;
         for (int z = 1; z < maxfz; ++z) {
-          const float mz = cum / (float)z;
-          const float a = tol_bound(mz, P.tol_lo, P.tol_da);
-          const float bd = tol_bound(mz, P.tol_hi, P.tol_da);
-          float fc = floorf((a - base) * inv_w);
-          int lo = np ? (int)tb[fc < 0.f ? 0 : (fc > (float)(K2_CELLS - 1) ? K2_CELLS - 1 : (int)fc)] : 0;
-          while (lo < np && sm[lo] < a) ++lo;
           int best = -1;
-          float mx = 0.f;
-          for (int j = lo; j < np && sm[j] <= bd; ++j) {
-            const float v = si[j];
-            if (v >= mx) { mx = v; best = j; }
+          float mx = 0.f, mz = 0.f;
+          for (int alt = 0; alt < n_alt; ++alt) {
+            float mx_a = 0.f;
+            int best_a = -1;
+            const float b = alt == 0 ? cum : cum + shift;
+            window_scan(V, P, b / (float)z, mx, best, mz, mx_a, best_a);
+            if (SHIFT && n_alt == 2 && best_a >= 0) {
+              if (alt == 0) { iu = iu + si[best_a]; nu += 1; }
+              else { is = is + si[best_a]; ns += 1; }
+            }
           }
           if (best >= 0) {
             const float pmass = sm[best], pint = si[best];
@@ -135,6 +194,37 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
             else { my += 1; sy = sy + pint; run_matched(ry, k); }
           }
         }
+        if (SHIFT && n_alt == 2) {
+          if (kind == 0) { bs_init = bs_init + is; nbs_init += ns; }
+          else { yu_init = yu_init + iu; nyu_init += nu; }
+        }
+      }
+    }
+    if (SHIFT && n_alt == 2) {
+      // site st in [0, L): b_k carries the delta iff k >= st, y_k iff k < st (kernels.cu:344-363); the per-ion
+      // sums of ion st-1 are recomputed from the same f32 ladder instead of being stored
+      float b_u = 0.f, y_s = 0.f, b_s = bs_init, y_u = yu_init;
+      int nb_u = 0, ny_s = 0, nb_s = nbs_init, ny_u = nyu_init;
+      float bcum = nt, ycum = X.mass32[r] - nt;
+      double best_hs = -1e300;
+      for (int st = 0; st < L; ++st) {
+        if (st > 0) {
+          const int k = st - 1;
+          const float rmv = rm[((mk >> k) & 1ULL) * 26 + (sq[k] - 'A')];
+          bcum = bcum + rmv;
+          ycum = ycum - rmv;
+          float iu0, is0, iu1, is1;
+          int nu0, ns0, nu1, ns1;
+          ion_alt_sums(V, P, bcum, shift, maxfz, iu0, nu0, is0, ns0);
+          ion_alt_sums(V, P, ycum, shift, maxfz, iu1, nu1, is1, ns1);
+          b_u = b_u + iu0; nb_u += nu0; b_s = b_s - is0; nb_s -= ns0;
+          y_s = y_s + is1; ny_s += ns1; y_u = y_u - iu1; ny_u -= nu1;
+        }
+        const float sb2 = b_u + b_s, sy2 = y_u + y_s;
+        const int mb2 = nb_u + nb_s, my2 = ny_u + ny_s;
+        const double h2 = log((double)(fmaxf(sb2, 0.f) + 1.0f) * (double)(fmaxf(sy2, 0.f) + 1.0f)) +
+                          (mb2 < K2_LNF ? lnf[mb2] : lnfact_dev(mb2)) + (my2 < K2_LNF ? lnf[my2] : lnfact_dev(my2));
+        if (h2 > best_hs) { best_hs = h2; mb = mb2; my = my2; sb = fmaxf(sb2, 0.f); sy = fmaxf(sy2, 0.f); }
       }
     }
     const double i2 = (double)(sb + 1.0f) * (double)(sy + 1.0f);
@@ -148,6 +238,25 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
     O.my[c] = (int16_t)my;
     O.lb[c] = (int16_t)rb.longest;
     O.ly[c] = (int16_t)ry.longest;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ shifts
+// c_shift = M(h) - mass32(row) for offset / open hypotheses, 0 for closed ones (vulcan/ptm/search.py:187-189)
+__global__ __launch_bounds__(64) void cand_shift_kernel(int n_groups, const int64_t* __restrict__ grp_start,
+                                                        const int32_t* __restrict__ grp_count, int stride,
+                                                        const int32_t* __restrict__ c_row,
+                                                        const int32_t* __restrict__ c_hyp,
+                                                        const uint8_t* __restrict__ h_mode,
+                                                        const float* __restrict__ h_M,
+                                                        const float* __restrict__ mass32, float* c_shift) {
+  const int g = blockIdx.x;
+  const int64_t c0 = grp_start ? grp_start[g] : (int64_t)g * stride;
+  const int nc = grp_count[g];
+  for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+    const int64_t c = c0 + j;
+    const int h = c_hyp[c];
+    c_shift[c] = h_mode[h] == VH_MODE_CLOSED ? 0.f : h_M[h] - mass32[c_row[c]];
   }
 }
 
@@ -220,13 +329,30 @@ int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int3
           int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, void* stream) {
   if (!p || !ix || n_groups < 0 || !grp_count) return VH_E_ARG;
   if (!grp_start && stride <= 0) return VH_E_ARG;
-  if (c_shift) return VH_E_ARG;   // shifted scoring: separate entry in a later ABI version
   if (n_groups == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   K2Buf O = {o_hs, o_sb, o_sy, o_ppm, o_mb, o_my, o_lb, o_ly};
   unsigned g = xcd_band_grid((n_groups + K2_WAVES - 1) / K2_WAVES, p->xcd_group);
-  k2_kernel<<<g, 64 * K2_WAVES, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_groups, order, grp_spec, grp_start,
-                                         grp_count, stride, c_row, c_hyp, h_maxfz, pk_off, pk_mass, pk_int, O);
+  if (c_shift)
+    k2_kernel<true><<<g, 64 * K2_WAVES, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_groups, order, grp_spec,
+                                                 grp_start, grp_count, stride, c_row, c_hyp, h_maxfz, c_shift,
+                                                 pk_off, pk_mass, pk_int, O);
+  else
+    k2_kernel<false><<<g, 64 * K2_WAVES, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_groups, order, grp_spec,
+                                                  grp_start, grp_count, stride, c_row, c_hyp, h_maxfz, nullptr,
+                                                  pk_off, pk_mass, pk_int, O);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_candidate_shift(int32_t n_groups, const int64_t* grp_start, const int32_t* grp_count, int32_t stride,
+                       const int32_t* c_row, const int32_t* c_hyp, const uint8_t* h_mode, const float* h_M,
+                       const float* mass32, float* c_shift, void* stream) {
+  if (n_groups < 0 || !grp_count || (!grp_start && stride <= 0)) return VH_E_ARG;
+  if (n_groups == 0) return 0;
+  if (!c_row || !c_hyp || !h_mode || !h_M || !mass32 || !c_shift) return VH_E_ARG;
+  cand_shift_kernel<<<(unsigned)n_groups, 64, 0, (hipStream_t)stream>>>(n_groups, grp_start, grp_count, stride, c_row,
+                                                                         c_hyp, h_mode, h_M, mass32, c_shift);
   VH_CHECK_LAUNCH();
   return 0;
 }

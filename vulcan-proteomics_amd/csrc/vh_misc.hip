@@ -30,6 +30,12 @@ DevP vh_make_devp(const VhParams* p) {
   d.pair_closure = p->pair_closure;
   d.tile = p->tile;
   d.xcd_group = p->xcd_group;
+  d.n_offsets = p->n_offsets;
+  for (int i = 0; i < VH_MAX_OFFSETS; ++i) d.offsets[i] = p->offsets[i];
+  d.open_search = p->open_search;
+  d.open_lo = p->open_lo;
+  d.open_hi = p->open_hi;
+  d.k1_threshold_open = p->k1_threshold_open;
   return d;
 }
 

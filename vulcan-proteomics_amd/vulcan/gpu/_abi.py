@@ -19,7 +19,8 @@ class VhParams(C.Structure):
                 ("prec_lo", F32), ("prec_hi", F32), ("iso_lo", I32), ("iso_hi", I32), ("z_lo", I32), ("z_hi", I32),
                 ("max_fragment_charge", I32), ("max_peaks", I32), ("min_peaks", I32), ("min_matched_peaks", I32),
                 ("topk", I32), ("k1_threshold", I32), ("pair_closure", I32), ("tile", I32), ("xcd_group", I32),
-                ("reserved", I32 * 7)]
+                ("n_offsets", I32), ("offsets", F32 * 8), ("open_search", I32), ("open_lo", F32), ("open_hi", F32),
+                ("k1_threshold_open", I32), ("reserved", I32 * 8)]
 
 
 class VhIndexView(C.Structure):
@@ -69,12 +70,28 @@ def make_params(p: SearchParams, static_mods: dict, var_mods: dict) -> VhParams:
     v.pair_closure = int(bool(p.pair_closure))
     v.tile = int(p.tile)
     v.xcd_group = int(p.xcd_group)
+    v.k1_threshold_open = int(p.k1_threshold)
     return v
 
 
-def max_hyp_per_spectrum(p: SearchParams) -> int:
+def set_modes(v: VhParams, offsets=(), open_da=None, k1_threshold_open: int = 1) -> VhParams:
+    """Mass-offset / open hypotheses (ModeParams, vulcan/ptm/search.py:38-45) into a VhParams."""
+    if len(offsets) > 8:
+        raise ValueError("at most 8 mass offsets")
+    v.n_offsets = len(offsets)
+    for i, o in enumerate(offsets):
+        v.offsets[i] = float(np.float32(o))
+    v.open_search = int(open_da is not None)
+    if open_da is not None:
+        v.open_lo, v.open_hi = float(np.float32(open_da[0])), float(np.float32(open_da[1]))
+    v.k1_threshold_open = int(k1_threshold_open)
+    return v
+
+
+def max_hyp_per_spectrum(p: SearchParams, n_offsets: int = 0, open_search: bool = False) -> int:
     niso = p.isotope_errors[1] - p.isotope_errors[0] + 1
-    return niso * max(1, p.precursor_charge[1] - p.precursor_charge[0] + 1)
+    nz = max(1, p.precursor_charge[1] - p.precursor_charge[0] + 1)
+    return niso * nz + nz * (n_offsets + int(bool(open_search)))
 
 
 def f32_bits(x: float) -> int:

@@ -28,13 +28,14 @@ _SIGS = {
     "vh_index_workspace_bytes": ([I64, I64, I32], SZ),
     "vh_index_build": ([PP, PIX, P, P, P, P, P, SZ, P], I32),
     "vh_k0": ([PP, I32, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
-    "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
+    "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_spectrum_order_workspace_bytes": ([I32], SZ),
     "vh_spectrum_order": ([I32, P, P, P, P, SZ, P], I32),
-    "vh_k1": ([PP, PIX, I32, P, P, P, P, P, P, P, I64, P, P, P, P, P, P, P], I32),
-    "vh_k1_topk": ([PP, PIX, I32, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P], I32),
+    "vh_k1": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I64, P, P, P, P, P, P, P], I32),
+    "vh_k1_topk": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P], I32),
     "vh_k1_brute": ([PP, PIX, I64, P, P, P, P, P, P, P, P], I32),
     "vh_k2": ([PP, PIX, I32, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_candidate_shift": ([I32, P, P, I32, P, P, P, P, P, P, P], I32),
     "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
 }
 

@@ -19,7 +19,7 @@ import pyarrow as pa
 import torch
 
 from . import _lib as K
-from ._abi import VhIndexView, index_geometry, make_params, max_hyp_per_spectrum
+from ._abi import VhIndexView, index_geometry, make_params, max_hyp_per_spectrum, set_modes
 from .host import NEUTRON, PROTON, SAGE_MONO, SearchParams, Spectra, make_hypotheses, pre_hypotheses  # noqa: F401
 
 ROW_BITS = 28   # peptidoform rows < 2^28 (engine.py:35 of the reference)
@@ -173,8 +173,11 @@ def device_hypotheses(gi: GpuIndex, raw: dict, peaks: dict, p: SearchParams, vp=
     L = K.lib()
     dev = gi.dev
     n = raw["n"]
-    cap = max(n * max_hyp_per_spectrum(p), 1)
+    vp = vp or gi.vp
+    cap = max(n * max_hyp_per_spectrum(p, vp.n_offsets, vp.open_search), 1)
     o = out or {"hyp_off": torch.empty(n + 1, dtype=torch.int64, device=dev),
+                "mode": torch.empty(cap, dtype=torch.uint8, device=dev),
+                "off": torch.empty(cap, dtype=torch.float32, device=dev),
                 "spec": torch.empty(cap, dtype=torch.int32, device=dev),
                 "z": torch.empty(cap, dtype=torch.int8, device=dev),
                 "iso": torch.empty(cap, dtype=torch.int8, device=dev),
@@ -184,16 +187,18 @@ def device_hypotheses(gi: GpuIndex, raw: dict, peaks: dict, p: SearchParams, vp=
                 "hi": torch.empty(cap, dtype=torch.int64, device=dev),
                 "maxfz": torch.empty(cap, dtype=torch.int8, device=dev),
                 "ws": torch.empty(int(L.vh_scan_workspace_bytes(max(n, 1))), dtype=torch.uint8, device=dev)}
-    K.check(L.vh_hypotheses(C.byref(vp or gi.vp), n, K.ptr(raw["pmz"]), K.ptr(raw["z"]), K.ptr(peaks["valid_d"]),
+    K.check(L.vh_hypotheses(C.byref(vp), n, K.ptr(raw["pmz"]), K.ptr(raw["z"]), K.ptr(peaks["valid_d"]),
                             K.ptr(gi.mass32), gi.n, K.ptr(o["hyp_off"]), K.ptr(o["spec"]), K.ptr(o["z"]),
                             K.ptr(o["iso"]), K.ptr(o["pm"]), K.ptr(o["M"]), K.ptr(o["lo"]), K.ptr(o["hi"]),
-                            K.ptr(o["maxfz"]), cap, K.ptr(o["ws"]), o["ws"].numel(), K.stream()), "hypotheses")
+                            K.ptr(o["maxfz"]), K.ptr(o["mode"]), K.ptr(o["off"]), cap, K.ptr(o["ws"]), o["ws"].numel(),
+                            K.stream()), "hypotheses")
     return o
 
 
 def hyp_to_host(hd: dict) -> dict:
     n = int(hd["hyp_off"][-1].item())
-    out = {k: hd[k][:n].cpu().numpy() for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz")}
+    out = {k: hd[k][:n].cpu().numpy() for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz", "mode", "off")
+           if k in hd}
     out["hyp_off"] = hd["hyp_off"].cpu().numpy()
     return out
 
@@ -209,7 +214,8 @@ def _hyp_device(gi, hyp, dev):
         if np.any(np.diff(s) < 0):
             raise ValueError("hypotheses must be grouped by spectrum (make_hypotheses order)")
         np.cumsum(np.bincount(s, minlength=n_spec), out=off[1:])
-    t = {k: torch.from_numpy(np.ascontiguousarray(hyp[k])).to(dev) for k in ("spec", "maxfz", "lo", "hi")}
+    t = {k: torch.from_numpy(np.ascontiguousarray(hyp[k])).to(dev) for k in ("spec", "maxfz", "lo", "hi", "mode")
+         if k in hyp}
     t["lo"], t["hi"] = t["lo"].to(torch.int64), t["hi"].to(torch.int64)
     t["spec"] = t["spec"].to(torch.int32)
     t["maxfz"] = t["maxfz"].to(torch.int8)
@@ -237,7 +243,7 @@ class Timer:
         self.stats.wall_s[self.name] = self.stats.wall_s.get(self.name, 0.0) + time.perf_counter() - self.t0
 
 
-def run_k1(gi: GpuIndex, hyp, peaks, p: SearchParams, stats: Stats):
+def run_k1(gi: GpuIndex, hyp, peaks, p: SearchParams, stats: Stats, k1_threshold_open: int | None = None):
     """K1 append form (reference engine.py:250-281): (o_h, o_r, o_c, scored, matched, d_spec, d_maxfz)."""
     L = K.lib()
     dev = gi.dev
@@ -256,8 +262,11 @@ def run_k1(gi: GpuIndex, hyp, peaks, p: SearchParams, stats: Stats):
     n_pk = peaks["off"].numel() - 1
     if n_pk < n_spec:
         raise ValueError("hypotheses reference spectra beyond the peak view")
+    if "mode" in hd:
+        vp.k1_threshold_open = int(p.k1_threshold if k1_threshold_open is None else k1_threshold_open)
     K.check(L.vh_k1(C.byref(vp), C.byref(gi.view), n_spec, None, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
-                    K.ptr(hd["hyp_off"]), K.ptr(hd["maxfz"]), K.ptr(hd["lo"]), K.ptr(hd["hi"]), cap, K.ptr(n_out),
+                    K.ptr(hd["hyp_off"]), K.ptr(hd["maxfz"]), K.ptr(hd.get("mode")), K.ptr(hd["lo"]),
+                    K.ptr(hd["hi"]), cap, K.ptr(n_out),
                     K.ptr(o_h), K.ptr(o_r), K.ptr(o_c), K.ptr(scored), K.ptr(matched), K.stream()), "k1")
     n = int(n_out.item())
     assert n <= cap
@@ -304,20 +313,21 @@ def _k2_out(n, dev):
     return out
 
 
-def _k2_call(gi, vp, n_groups, grp_spec, grp_start, grp_count, stride, row, hyp, maxfz, peaks, out, order=None):
+def _k2_call(gi, vp, n_groups, grp_spec, grp_start, grp_count, stride, row, hyp, maxfz, peaks, out, order=None,
+             shift=None):
     L = K.lib()
     K.check(L.vh_k2(C.byref(vp), C.byref(gi.view), n_groups, K.ptr(order), K.ptr(grp_spec), K.ptr(grp_start),
                     K.ptr(grp_count),
-                    stride, K.ptr(row), K.ptr(hyp), K.ptr(maxfz), None, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
+                    stride, K.ptr(row), K.ptr(hyp), K.ptr(maxfz), K.ptr(shift), K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
                     K.ptr(peaks["int"]), K.ptr(out["hyperscore"]), K.ptr(out["summed_b"]), K.ptr(out["summed_y"]),
                     K.ptr(out["ppm"]), K.ptr(out["matched_b"]), K.ptr(out["matched_y"]), K.ptr(out["longest_b"]),
                     K.ptr(out["longest_y"]), K.stream()), "k2")
 
 
 def run_k2(gi: GpuIndex, h, r, d_spec, d_maxfz, peaks, p: SearchParams, shift: torch.Tensor | None = None):
-    """K2 over candidates sorted by spectrum (reference engine.py:315-351). Returns dict of device tensors."""
-    if shift is not None:
-        raise NotImplementedError("shifted K2 scoring is not part of this ABI version")
+    """K2 over candidates sorted by spectrum (reference engine.py:315-351). shift (optional, f32 per candidate):
+    observed - candidate mass; nonzero entries get shifted + unshifted ions with the delta localized on the best
+    residue (open / mass-offset modes). Returns dict of device tensors."""
     dev = gi.dev
     spec = d_spec[h].long()
     n = len(r)
@@ -326,9 +336,10 @@ def run_k2(gi: GpuIndex, h, r, d_spec, d_maxfz, peaks, p: SearchParams, shift: t
         us, cnts = torch.unique_consecutive(spec, return_counts=True)
         start = torch.zeros(len(us), dtype=torch.int64, device=dev)
         start[1:] = torch.cumsum(cnts, 0)[:-1]
+        sh = None if shift is None else shift.to(torch.float32).contiguous()
         _k2_call(gi, gi.params_for(p), len(us), us.to(torch.int32).contiguous(), start,
                  cnts.to(torch.int32).contiguous(), 0, r.to(torch.int32).contiguous(), h.to(torch.int32).contiguous(),
-                 d_maxfz.contiguous(), peaks, out)
+                 d_maxfz.contiguous(), peaks, out, shift=sh)
     out = {k: v[:n] for k, v in out.items()}
     out["spec"] = spec
     return out
@@ -356,7 +367,7 @@ def rank_groups(gi, p: SearchParams, n_groups, grp_start, grp_count, stride, row
 
 
 def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched, peaks, p: SearchParams,
-              report: int | None = None) -> pa.Table:
+              report: int | None = None, return_hyp: bool = False):
     """Rank candidates per spectrum (vh_rank: hyperscore desc, row asc) and build the PSM table with the
     reference's columns (engine.py:361-432, PSM_SCHEMA.md section 1)."""
     report = p.report_psms if report is None else report
@@ -429,7 +440,8 @@ def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched
         "peptide_len": plen.astype(np.int16), "missed_cleavages": l1.missed[rows], "n_var": l1.n_var[rows],
         "k1_count": k1.astype(np.int32), "scored_candidates": sc[s].astype(np.int32), "poisson": log_pois,
     }
-    return pa.table(cols)
+    tbl = pa.table(cols)
+    return (tbl, hh) if return_hyp else tbl
 
 
 class DeviceSearch:
@@ -437,14 +449,18 @@ class DeviceSearch:
     -> vh_k1_topk -> vh_k2 -> vh_rank (report 1). `run(raw)` enqueues the launches on the current stream and
     returns the per-spectrum rank-1 result; no host synchronisation, no allocation, hipGraph-capturable."""
 
-    def __init__(self, gi: GpuIndex, p: SearchParams, n_spec: int, report: int = 1):
+    def __init__(self, gi: GpuIndex, p: SearchParams, n_spec: int, report: int = 1, offsets=(), open_da=None,
+                 k1_threshold_open: int = 1):
         self.gi, self.p, self.n_spec, self.report = gi, p, n_spec, report
         self.vp = gi.params_for(p)
+        self.modes = bool(len(offsets) or open_da is not None)
+        if self.modes:
+            set_modes(self.vp, offsets, open_da, k1_threshold_open)
         dev = gi.dev
         L = K.lib()
         n = max(n_spec, 1)
         self.stride = 2 * p.topk if p.pair_closure else p.topk
-        maxh = max_hyp_per_spectrum(p)
+        maxh = max_hyp_per_spectrum(p, len(offsets), open_da is not None)
         cap_h = n * maxh
         cap_c = n * self.stride
         ws = int(L.vh_scan_workspace_bytes(n))
@@ -455,7 +471,8 @@ class DeviceSearch:
         self.hyp = {"hyp_off": e(n + 1, torch.int64), "spec": e(cap_h, torch.int32), "z": e(cap_h, torch.int8),
                     "iso": e(cap_h, torch.int8), "pm": e(cap_h, torch.float32), "M": e(cap_h, torch.float32),
                     "lo": e(cap_h, torch.int64), "hi": e(cap_h, torch.int64), "maxfz": e(cap_h, torch.int8),
-                    "ws": e(ws, torch.uint8)}
+                    "mode": e(cap_h, torch.uint8), "off": e(cap_h, torch.float32), "ws": e(ws, torch.uint8)}
+        self.c_shift = e(cap_c, torch.float32) if self.modes else None
         self.order = e(n, torch.int32)
         self.order_ws = e(int(L.vh_spectrum_order_workspace_bytes(n)), torch.uint8)
         self.n_cand = e(n, torch.int32)
@@ -492,8 +509,8 @@ class DeviceSearch:
         K.check(L.vh_hypotheses(C.byref(vp), n, K.ptr(raw["pmz"]), K.ptr(raw["z"]), K.ptr(pk["valid_d"]),
                                 K.ptr(gi.mass32), gi.n, K.ptr(hy["hyp_off"]), K.ptr(hy["spec"]), K.ptr(hy["z"]),
                                 K.ptr(hy["iso"]), K.ptr(hy["pm"]), K.ptr(hy["M"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
-                                K.ptr(hy["maxfz"]), hy["spec"].numel(), K.ptr(hy["ws"]), hy["ws"].numel(), st),
-                "hypotheses")
+                                K.ptr(hy["maxfz"]), K.ptr(hy["mode"]), K.ptr(hy["off"]), hy["spec"].numel(),
+                                K.ptr(hy["ws"]), hy["ws"].numel(), st), "hypotheses")
         mark("hyp")
         mark("order")
         K.check(L.vh_spectrum_order(n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(self.order),
@@ -501,14 +518,18 @@ class DeviceSearch:
         mark("order")
         mark("k1_topk")
         K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.view), n, K.ptr(self.order), K.ptr(pk["off"]),
-                             K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["lo"]),
-                             K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand), K.ptr(self.c_row), K.ptr(self.c_hyp),
+                             K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["mode"]),
+                             K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand), K.ptr(self.c_row), K.ptr(self.c_hyp),
                              K.ptr(self.c_cnt), K.ptr(self.scored), K.ptr(self.matched), K.ptr(self.status), st),
                 "k1_topk")
         mark("k1_topk")
         mark("k2")
+        if self.modes:
+            K.check(L.vh_candidate_shift(n, None, K.ptr(self.n_cand), self.stride, K.ptr(self.c_row),
+                                         K.ptr(self.c_hyp), K.ptr(hy["mode"]), K.ptr(hy["M"]), K.ptr(gi.mass32),
+                                         K.ptr(self.c_shift), st), "candidate_shift")
         _k2_call(gi, vp, n, None, None, self.n_cand, self.stride, self.c_row, self.c_hyp, hy["maxfz"], pk, self.k2,
-                 order=self.order)
+                 order=self.order, shift=self.c_shift)
         mark("k2")
         mark("rank")
         K.check(L.vh_rank(C.byref(vp), n, None, K.ptr(self.n_cand), self.stride, K.ptr(self.c_row),
@@ -526,6 +547,21 @@ class DeviceSearch:
         return self.c_hyp[flat].long(), self.c_row[flat].long(), self.c_cnt[flat].long(), flat
 
 
+def device_table(ds: DeviceSearch, sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int):
+    """PSM table from a DeviceSearch that has run on `sp` (candidates flattened in (spectrum, row) order).
+    Returns (table, per-PSM hypothesis index, debug dict)."""
+    n = len(sp)
+    dev = gi.dev
+    h, r, c, flat = ds.candidates(n)
+    k2 = {k: v[flat] for k, v in ds.k2.items()}
+    k2["spec"] = torch.repeat_interleave(torch.arange(n, device=dev), ds.n_cand[:n].long())
+    hyp = hyp_to_host(ds.hyp)
+    tbl, hh = psm_table(sp, gi, l1, hyp, h, r, c, k2, ds.scored, ds.matched, ds.peaks, p, report, return_hyp=True)
+    if int(ds.status.item()) & 1:
+        raise RuntimeError("k1_topk: a spectrum has more hypotheses than one wave handles")
+    return tbl, hh, {"h": h, "r": r, "k1c": c, "k2": k2, "hyp": hyp}
+
+
 def search(sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int | None = None, pre_hyp=None):
     """Full closed search of one run (reference engine.py:435-464). Returns (PSM table, stats, debug)."""
     st = Stats()
@@ -539,22 +575,17 @@ def search(sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int | None = 
         ds.run(raw)
     n = len(sp)
     with Timer(st, "features"):
-        h, r, c, flat = ds.candidates(n)
-        k2 = {k: v[flat] for k, v in ds.k2.items()}
-        k2["spec"] = torch.repeat_interleave(torch.arange(n, device=dev), ds.n_cand[:n].long())
-        hyp = hyp_to_host(ds.hyp)
-        tbl = psm_table(sp, gi, l1, hyp, h, r, c, k2, ds.scored, ds.matched, ds.peaks, p, report)
-    if int(ds.status.item()) & 1:
-        raise RuntimeError("k1_topk: a spectrum has more hypotheses than one wave handles")
+        tbl, _, dbg = device_table(ds, sp, gi, l1, p, report)
     ru1 = resource.getrusage(resource.RUSAGE_SELF)
     st.wall_s["total"] = time.perf_counter() - t0
     valid = ds.peaks["valid_d"][:n].cpu().numpy().astype(bool)
-    st.counts.update(n_spectra=n, n_searched=int(valid.sum()), n_hypotheses=len(hyp["spec"]),
-                     n_rescored=int(len(r)), n_psms=tbl.num_rows)
+    st.counts.update(n_spectra=n, n_searched=int(valid.sum()), n_hypotheses=len(dbg["hyp"]["spec"]),
+                     n_rescored=int(len(dbg["r"])), n_psms=tbl.num_rows)
     st.counts["cpu_core_s"] = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
     peaks = dict(ds.peaks)
     peaks["valid"] = valid
-    return tbl, st, {"peaks": peaks, "hyp": hyp, "h": h, "r": r, "k1c": c, "k2": k2, "device": ds}
+    dbg.update(peaks=peaks, device=ds)
+    return tbl, st, dbg
 
 
 # ------------------------------------------------------------------------------------------------ references
