@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 6
+#define VH_ABI_VERSION 7
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -37,6 +37,8 @@ extern "C" {
 #define VH_MODE_OPEN 2
 #define VH_K2_MAX_PEAKS 512      /* K2 looks at the first 512 mass-sorted peaks (kernels.cu:260, 278) */
 #define VH_MAX_PEPTIDE_LEN 63    /* residues per peptidoform (mask bits 0..62; bit 63 = N-term mod) */
+#define VH_MLP_N_PARAMS 68353    /* prefilter MLP 8 -> 256 -> 256 -> 1: W1 [256][8], b1, W2 [256][256], b2, w3, b3 */
+#define VH_MLP_BLOB_BYTES 281728 /* packed bf16 model (vh_mlp_pack) */
 
 /* Search parameters. Reference: SearchParams (vulcan/gpu/host.py:15-44) plus the constant-memory state of
  * kernels.cu:19-38 (FragParams, c_nterm, c_tol_da). */
@@ -114,8 +116,10 @@ int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, v
                   void* stream);
 /* Scratch bytes of vh_index_build for n_postings keys. */
 size_t vh_index_workspace_bytes(int64_t n_postings, int64_t n_buckets, int32_t n_bins);
-/* Emit 64-bit keys (bucket << 42 | (f32 bits - bits(fmin)) << 16 | row in bucket), LSD radix sort them on the
- * device, write post_key / post_row (n_postings each) and the directory offsets (n_buckets * n_bins + 1).
+/* Emit 64-bit keys (bucket << (16 + nd) | (f32 bits - bits(fmin)) << 16 | row in bucket, nd = bit width of
+ * bits(fmax) - bits(fmin)), LSD radix sort them on the device over bits [16, 16 + nd + bucket bits) (stable,
+ * so ties keep row order), write post_key / post_row (n_postings each) and the directory offsets
+ * (n_buckets * n_bins + 1).
  * Replaces vk_frag_emit (kernels.cu:73-101, 401-411) and the torch.sort / bincount / cumsum CSR assembly of
  * engine.py:129-148. Needs bucket_log2 <= 16 and bits(fmax) - bits(fmin) < 2^26. */
 int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
@@ -175,6 +179,22 @@ int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const i
                int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
                void* stream);
 
+/* Index top-K with per-candidate features for the learned prefilter (no reference counterpart; the top-K set and
+ * order are those of vh_k1_topk without closure, engine.py:284-312 at pair_closure off). Per spectrum s: the
+ * topk best (count, row) entries of its hypotheses, deduplicated by row (lowest hypothesis, highest count) and
+ * written in row order at s*cand_stride; besides row / hypothesis / count, c_iq[c] = 14-bit pseudo-log2 of
+ * (1 + the sum of the quantised intensities of the peak matches counted for that entry) = e << 8 | the 8 bits
+ * below the leading one of 1 + sum, where e is the leading bit position; a peak's quantised intensity is
+ * trunc(I * 2^(16 - x)) with x the frexp exponent of the spectrum's largest finite peak intensity (0 for
+ * non-finite, negative or denormal I). s_maxcnt[s] = the largest count of the spectrum's entries (0 if none).
+ * Among duplicates of a row the entry with the larger count (ties: lower hypothesis) gives the intensity. */
+int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
+                   const int64_t* pk_off, const float* pk_mass, const float* pk_int, const int64_t* hyp_off,
+                   const int8_t* h_maxfz, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
+                   int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt,
+                   uint16_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored, unsigned long long* hyp_matched,
+                   int32_t* status, void* stream);
+
 /* Brute-force counts for given (hypothesis, row) pairs without the index (k1_brute_kernel,
  * kernels.cu:195-232): the definition K1 must equal. */
 int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32_t* c_hyp, const int32_t* c_row,
@@ -212,6 +232,58 @@ int vh_candidate_shift(int32_t n_groups, const int64_t* grp_start, const int32_t
 int vh_rank(const VhParams* p, int32_t n_groups, const int64_t* grp_start, const int32_t* grp_count,
             int32_t stride, const int32_t* c_row, const double* hs, const int16_t* mb, const int16_t* my,
             int32_t report, int64_t* o_sel, double* o_next, void* stream);
+
+/* ---------------------------------------------------------------------------------------------- learned prefilter */
+/* No reference counterpart (the reference's learned models, vulcan/rescore/fast.py:198-254 and static.py:25-82,
+ * rescore PSMs after K2). An MLP s = w3 . relu(W2 relu(W1 x + b1) + b2) + b3 over the 8 normalised features
+ * x_j = (f_j - mu_j) * isd_j of an index top-K candidate chooses which candidates reach K2. Weights live as an
+ * f32 parameter vector theta[VH_MLP_N_PARAMS] (W1 row-major [out][in] at 0, b1 at 2048, W2 [out][in] at 2304,
+ * b2 at 67840, w3 at 68096, b3 at 68352); the kernels read a packed copy (bf16 W1/W2, f32 biases). Arithmetic:
+ * x, W1, relu(layer 1) and W2 are rounded to bf16 (nearest even), products accumulate in f32 (MFMA), biases,
+ * relu(layer 2) and layer 3 are f32. */
+
+/* Pack theta + feature normalisation (mu[8], isd[8]) into blob (VH_MLP_BLOB_BYTES, device). */
+int vh_mlp_pack(const float* theta, const float* mu, const float* isd, void* blob, void* stream);
+
+/* Forward over a feature matrix x [n][8] (raw features): s[n]. With a1 (bf16 bits [n][256], relu of layer 1)
+ * and a2 (f32 [n][256], relu of layer 2) the activations are stored for vh_mlp_backward (both or neither). */
+int vh_mlp_forward(const void* blob, int64_t n, const float* x, float* s, uint16_t* a1, float* a2, void* stream);
+
+/* Gradient of L = (1/n) sum_r (s_r - y_r)^2 with respect to theta, from the activations of vh_mlp_forward on the
+ * same x: grad[VH_MLP_N_PARAMS]. dZ2 is rounded to bf16 for the two layer-2 products (MFMA, f32 accumulation);
+ * the layer-1 and bias gradients are f32 sums in a fixed order (deterministic). */
+size_t vh_mlp_workspace_bytes(int64_t n);
+int vh_mlp_backward(const void* blob, int64_t n, const float* x, const uint16_t* a1, const float* a2, const float* s,
+                    const float* y, float* grad, void* ws, size_t ws_bytes, void* stream);
+
+/* Adam step on theta (f32): m1 = b1 m1 + (1-b1) g; m2 = b2 m2 + (1-b2) g^2;
+ * theta -= lr * (m1 c1) / (sqrt(m2 c2) + eps), with c1 = 1/(1-b1^t), c2 = 1/(1-b2^t) given by the caller. */
+int vh_mlp_adam(float* theta, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1,
+                float beta2, float eps, float c1, float c2, void* stream);
+
+/* Raw features of every candidate slot of vh_k1_features' lists (stride slots per spectrum): out[slot][8] =
+ * (count, count / s_maxcnt (0 if s_maxcnt is 0), c_iq / 256, ((h_M - mass32) / mass32) * 1e6,
+ * length, charge h_z, isotope h_iso, number of variable modifications = popcount(mask bits 0..62)), f32;
+ * zeros for empty slots. */
+int vh_prefilter_features(const VhIndexView* ix, int32_t n_spec, int32_t stride, const int32_t* n_cand,
+                          const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const uint16_t* c_iq,
+                          const int32_t* s_maxcnt, const int8_t* h_z, const int8_t* h_iso, const float* h_M,
+                          float* out, void* stream);
+
+/* MLP score of every non-empty candidate slot (features as vh_prefilter_features, computed in registers):
+ * score[slot]; empty slots are not written. */
+int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, int32_t stride,
+                       const int32_t* n_cand, const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt,
+                       const uint16_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z, const int8_t* h_iso,
+                       const float* h_M, float* score, void* stream);
+
+/* Per spectrum: the `keep` best slots by (score desc, NaN last; ties: earlier slot), then the reference's pair
+ * closure and dedupe (partner rows with count -1; by row: lowest hypothesis, highest count; engine.py:284-312),
+ * written in row order at s*out_stride. stride_in <= 1024, keep <= 512. */
+int vh_prefilter_select(const VhIndexView* ix, int32_t n_spec, int32_t stride_in, const int32_t* n_cand,
+                        const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const float* score,
+                        int32_t keep, int32_t pair_closure, int32_t out_stride, int32_t* out_n, int32_t* out_row,
+                        int32_t* out_hyp, int32_t* out_cnt, void* stream);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,9 @@ def lib():
         "or_k2": ([PP, PIX, I32, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
         "or_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P], I32),
         "or_search": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, P], I32),
+        "or_topk_feat": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P], I32),
+        "or_iq_log": ([C.c_uint64], C.c_uint32),
+        "or_iq_quant": ([C.c_float, C.c_float], C.c_uint32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -179,6 +182,33 @@ def topk(vp, hix: HostIndex, hyp, k1, stride: int):
                    stride, _p(o["n_cand"]), _p(o["row"]), _p(o["hyp"]), _p(o["cnt"])), "topk")
     o["n_cand"] = o["n_cand"][:n_spec]
     return o
+
+
+def topk_feat(vp, hix: HostIndex, peaks, hyp, stride: int):
+    """vh_k1_features restated: top-K lists (no closure) with count, intensity pseudo-log2 and best count."""
+    L = lib()
+    n_spec = len(hyp["hyp_off"]) - 1
+    n = max(n_spec, 1)
+    o = {"n_cand": np.zeros(n, np.int32), "row": np.zeros(n * stride, np.int32),
+         "hyp": np.zeros(n * stride, np.int32), "cnt": np.zeros(n * stride, np.int32),
+         "iq": np.zeros(n * stride, np.uint16), "maxcnt": np.zeros(n, np.int32)}
+    mode = np.ascontiguousarray(hyp["mode"], np.uint8) if "mode" in hyp else None
+    _chk(L.or_topk_feat(C.byref(vp), C.byref(hix.view), n_spec, _p(peaks["off"]), _p(peaks["mass"]),
+                        _p(peaks["int"]), _p(hyp["hyp_off"]), _p(np.ascontiguousarray(hyp["maxfz"])), _p(mode),
+                        _p(np.ascontiguousarray(hyp["lo"])), _p(np.ascontiguousarray(hyp["hi"])), stride,
+                        _p(o["n_cand"]), _p(o["row"]), _p(o["hyp"]), _p(o["cnt"]), _p(o["iq"]), _p(o["maxcnt"])),
+         "topk_feat")
+    o["n_cand"] = o["n_cand"][:n_spec]
+    o["maxcnt"] = o["maxcnt"][:n_spec]
+    return o
+
+
+def iq_log(s: int) -> int:
+    return int(lib().or_iq_log(s))
+
+
+def iq_quant(intensity: float, max_intensity: float) -> int:
+    return int(lib().or_iq_quant(intensity, max_intensity))
 
 
 def k2(vp, hix: HostIndex, peaks, row, hyp_idx, maxfz, grp_spec=None, grp_start=None, grp_count=None, stride=0,

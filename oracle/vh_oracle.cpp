@@ -9,6 +9,8 @@
 //   or_k1_counts    vulcan/gpu/kernels.cu:112-190        (k1_kernel counts, scored, matched) computed per row
 //                   from the fragment ladder with two binary searches per (fragment, charge) instead of an index
 //   or_topk         vulcan/gpu/engine.py:284-312         (select_topk + pair closure)
+//   or_topk_feat    vh_k1_features (no reference counterpart): the same top-K without closure, plus the
+//                   quantised matched-intensity sum of each entry, computed per row like or_k1_counts
 //   or_k2           vulcan/gpu/kernels.cu:237-376        (k2_kernel incl. the shifted branch; f64 hyperscore)
 //   or_rank         vulcan/gpu/engine.py:361-386         (psm_table ordering: spectrum, hyperscore desc, row asc)
 //   or_search       vulcan/gpu/engine.py:435-464         (the whole per-spectrum pipeline, OpenMP over spectra)
@@ -387,6 +389,61 @@ void rank_one(const VhParams& p, int64_t start, int n, const int32_t* c_row, con
   }
 }
 
+// ---- feature mode of K1 (vh_k1_features): integer intensity quantisation and its 14-bit pseudo-log2
+int iq_shift(float maxI) {
+  if (!(maxI >= 1.17549435e-38f) || maxI > 3.40282347e38f) return -1000;
+  int x;
+  frexpf(maxI, &x);
+  return 16 - x;
+}
+
+uint32_t iq_quant(float I, int e) {
+  if (e == -1000 || !(I >= 1.17549435e-38f)) return 0u;
+  const float v = ldexpf(I, e);
+  return v >= 1.f ? (uint32_t)v : 0u;
+}
+
+uint32_t iq_log(uint64_t sum) {
+  const uint64_t x = sum + 1ULL;
+  const int e = 63 - __builtin_clzll(x);
+  const uint32_t m = e >= 8 ? (uint32_t)((x >> (e - 8)) & 0xFFULL) : (uint32_t)((x << (8 - e)) & 0xFFULL);
+  return ((uint32_t)e << 8) | m;
+}
+
+// count and intensity sum of one row: the peaks matching fragment f at charge z are the contiguous index range
+// [lower_bound(hi bounds, f), upper_bound(lo bounds, f)) of the mass-sorted peaks
+void k1_feat_row(const Ctx& c, const VhIndexView& ix, int64_t r, int maxfz, const PeakBounds& pb,
+                 const std::vector<uint64_t>& qpre, int* cnt_out, uint64_t* isum_out) {
+  const VhParams& p = *c.p;
+  const uint8_t* sq = ix.residues + ix.seq_off[r];
+  int L = ix.length[r];
+  uint64_t mk = ix.mask[r];
+  float b = c.nterm_of(mk), y = ix.mass32[r] - c.nterm_of(mk);
+  int cnt = 0;
+  uint64_t isum = 0;
+  auto add = [&](float f) {
+    for (int z = 1; z < maxfz; ++z) {
+      const float* A = pb.a.data() + (size_t)(z - 1) * pb.np;
+      const float* Bv = pb.b.data() + (size_t)(z - 1) * pb.np;
+      int ua = (int)(std::upper_bound(A, A + pb.np, f) - A);
+      int lb = (int)(std::lower_bound(Bv, Bv + pb.np, f) - Bv);
+      if (ua > lb) {
+        cnt += ua - lb;
+        isum += qpre[ua] - qpre[lb];
+      }
+    }
+  };
+  for (int k = 0; k < L - 1; ++k) {
+    float rm = c.res(sq[k], k, mk);
+    b = b + rm;
+    y = y - rm;
+    if (k + 1 > p.min_ion_index && b >= p.fmin && b <= p.fmax) add(b);
+    if (L - 1 - k > p.min_ion_index && y >= p.fmin && y <= p.fmax) add(y);
+  }
+  *cnt_out = cnt;
+  *isum_out = isum;
+}
+
 }  // namespace
 
 extern "C" {
@@ -510,6 +567,80 @@ int or_topk(const VhParams* p, const int64_t* partner, int32_t n_spec, const int
   }
   return 0;
 }
+
+// vh_k1_features: per spectrum the top-K (count clamped at 32767 desc, row desc, lower hypothesis first) of the
+// entries with count >= threshold, deduplicated by row (lowest hypothesis; count and intensity of the entry with
+// the larger count, ties: lower hypothesis), row order; c_iq = iq_log(intensity sum), s_maxcnt = best count.
+int or_topk_feat(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off,
+                 const float* pk_mass, const float* pk_int, const int64_t* hyp_off, const int8_t* h_maxfz,
+                 const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi, int32_t stride, int32_t* n_cand,
+                 int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, uint16_t* c_iq, int32_t* s_maxcnt) {
+  if (stride < p->topk) return VH_E_CAPACITY;
+  Ctx c(p);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int32_t s = 0; s < n_spec; ++s) {
+    const int64_t h0 = hyp_off[s], h1 = hyp_off[s + 1];
+    n_cand[s] = 0;
+    s_maxcnt[s] = 0;
+    if (h0 == h1) continue;
+    const int np = (int)(pk_off[s + 1] - pk_off[s]);
+    const float* pm = pk_mass + pk_off[s];
+    const float* pi = pk_int + pk_off[s];
+    float mx = 0.f;
+    for (int q = 0; q < np; ++q) mx = fmaxf(mx, pi[q]);
+    const int e = iq_shift(mx);
+    std::vector<uint64_t> qpre(np + 1, 0);
+    for (int q = 0; q < np; ++q) qpre[q + 1] = qpre[q] + iq_quant(pi[q], e);
+    int zmax = 2;
+    for (int64_t h = h0; h < h1; ++h) zmax = std::max(zmax, (int)h_maxfz[h]);
+    PeakBounds pb;
+    pb.build(c, pm, np, zmax);
+    struct FE { int64_t row; int32_t hl; int32_t cnt; uint64_t isum; };
+    std::vector<FE> ents;
+    for (int64_t h = h0; h < h1; ++h) {
+      int thr = (h_mode && h_mode[h] == VH_MODE_OPEN) ? p->k1_threshold_open : p->k1_threshold;
+      for (int64_t r = h_lo[h]; r < h_hi[h]; ++r) {
+        int k;
+        uint64_t is;
+        k1_feat_row(c, *ix, r, h_maxfz[h], pb, qpre, &k, &is);
+        if (k >= thr) ents.push_back({r, (int32_t)(h - h0), std::min(k, 32767), is});
+      }
+    }
+    std::sort(ents.begin(), ents.end(), [](const FE& a, const FE& b) {
+      if (a.cnt != b.cnt) return a.cnt > b.cnt;
+      if (a.row != b.row) return a.row > b.row;
+      return a.hl < b.hl;
+    });
+    size_t k = std::min<size_t>(ents.size(), (size_t)std::max(p->topk, 0));
+    if (k) s_maxcnt[s] = ents[0].cnt;
+    std::vector<FE> sel(ents.begin(), ents.begin() + k);
+    std::sort(sel.begin(), sel.end(), [](const FE& a, const FE& b) {
+      if (a.row != b.row) return a.row < b.row;
+      return a.hl < b.hl;
+    });
+    int n = 0;
+    for (size_t i = 0; i < sel.size(); ++i) {
+      const int64_t o = (int64_t)s * stride + n;
+      if (n && c_row[o - 1] == (int32_t)sel[i].row) {
+        if (sel[i].cnt > c_cnt[o - 1]) {
+          c_cnt[o - 1] = sel[i].cnt;
+          c_iq[o - 1] = (uint16_t)iq_log(sel[i].isum);
+        }
+        continue;
+      }
+      c_row[o] = (int32_t)sel[i].row;
+      c_hyp[o] = (int32_t)(h0 + sel[i].hl);
+      c_cnt[o] = sel[i].cnt;
+      c_iq[o] = (uint16_t)iq_log(sel[i].isum);
+      ++n;
+    }
+    n_cand[s] = n;
+  }
+  return 0;
+}
+
+extern "C" uint32_t or_iq_log(uint64_t sum) { return iq_log(sum); }
+extern "C" uint32_t or_iq_quant(float I, float maxI) { return iq_quant(I, iq_shift(maxI)); }
 
 // ------------------------------------------------------------------------------------------------ K2
 int or_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* grp_spec,

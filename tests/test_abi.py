@@ -17,7 +17,9 @@ def declared_symbols():
 def test_header_declares_the_hot_path():
     names = declared_symbols()
     for need in ("vh_frag_count", "vh_index_build", "vh_k0", "vh_hypotheses", "vh_k1", "vh_k1_topk", "vh_k1_brute",
-                 "vh_k2", "vh_rank", "vh_version", "vh_abi_sizeof"):
+                 "vh_k2", "vh_rank", "vh_version", "vh_abi_sizeof", "vh_k1_features", "vh_mlp_pack",
+                 "vh_mlp_forward", "vh_mlp_backward", "vh_mlp_adam", "vh_prefilter_features", "vh_prefilter_score",
+                 "vh_prefilter_select"):
         assert need in names
 
 
@@ -52,6 +54,17 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert L.vh_k2(C.byref(vp), C.byref(ix), 1, None, None, None, None, 0, None, None, None, None, None, None, None,
                    None, None, None, None, None, None, None, None, None) == -1
     assert L.vh_spectrum_order(3, None, None, None, None, 0, None) == -1
+    assert L.vh_mlp_pack(None, None, None, None, None) == -1
+    assert L.vh_mlp_forward(None, 10, None, None, None, None, None) == -1
+    assert L.vh_mlp_forward(C.c_void_p(8), 10, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8), None, None) == -1
+    assert L.vh_mlp_backward(C.c_void_p(8), 100, *[C.c_void_p(8)] * 7, 16, None) == -3
+    assert L.vh_mlp_workspace_bytes(-1) == 0 and L.vh_mlp_workspace_bytes(1000) > 1000 * 256 * 6
+    assert L.vh_prefilter_select(C.byref(ix), 1, 2000, *[None] * 5, 16, 1, 32, *[None] * 4, None) == -1
+    assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 0, 1, 32, *[None] * 4, None) == -1
+    ixp = VhIndexView()
+    ixp.partner = 8
+    assert L.vh_prefilter_select(C.byref(ixp), 1, 200, *[None] * 5, 16, 1, 31, *[None] * 4, None) == -2
+    assert L.vh_k1_features(C.byref(vp), C.byref(ix), 1, *[None] * 9, 10, *[None] * 9, None) == -1
     vp.max_peaks = 4096
     assert L.vh_k0(C.byref(vp), 1, None, None, C.c_void_p(8), C.c_void_p(8), None, None, C.c_void_p(8),
                    C.c_void_p(8), 10, None, 0, None) == -4

@@ -1,0 +1,319 @@
+"""Learned prefilter on the GPU vs its oracle (oracle/prefilter.py, or_topk_feat): K1 feature lists and features
+bit-exact, MLP forward / gradient within f32 accumulation error of the rounding-emulating oracle and within the
+bf16 bound of the exact float64 definition, Adam and selection bit-exact, training lowers the loss, and the
+trained network keeps the plain rank-1 more often than the K1-count rule at the same K2 budget."""
+import numpy as np
+import pytest
+
+from conftest import STATIC
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("needs a ROCm GPU", allow_module_level=True)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _O():
+    from oracle import oracle as O
+    return O
+
+
+def _P():
+    from oracle import prefilter as P
+    return P
+
+
+class FEnv:
+    """Device front end (K0, hypotheses, order) + K1 feature lists on a table/spectra pair, and the oracle twin."""
+
+    def __init__(self, d, topk=200, n=None):
+        from vulcan.gpu import engine as E
+        from vulcan.gpu._abi import make_params, max_hyp_per_spectrum
+        from vulcan.rescore import prefilter as PF
+        self.d = d
+        sp = d["sp"] if n is None else _take(d["sp"], n)
+        self.sp = sp
+        self.p = type(d["p"])(**d["p"].to_dict())
+        self.l1 = d["l1"]
+        self.gi = E.GpuIndex(self.l1, self.p, d["static"], d["var"])
+        self.n = len(sp)
+        self.raw = E.upload_spectra(sp, self.gi.dev)
+        self.ds = E.DeviceSearch(self.gi, self.p, self.n)
+        self.ds.front(self.raw, self.n, E._marker(None))
+        self.vpf = PF._feature_params(self.ds.vp, topk)
+        self.fl = PF.FeatureLists(self.n, topk, self.gi.dev)
+        self.fl.k1(self.ds, self.vpf, self.n)
+        torch.cuda.synchronize()
+        self.topk = topk
+        O = _O()
+        vp = make_params(self.p, d["static"], d["var"])
+        vp.topk = topk
+        vp.pair_closure = 0
+        self.vp = vp
+        self.hix = O.HostIndex(self.l1)
+        self.pk = O.k0(vp, sp)
+        self.hyp = O.hypotheses(vp, sp, self.pk["valid"], self.l1.mass32, max_hyp_per_spectrum(self.p))
+
+
+def _take(sp, n):
+    from vulcan.gpu.host import Spectra
+    off = sp.offsets[: n + 1]
+    return Spectra(sp.run, sp.scan[:n], sp.index[:n], sp.rt[:n], sp.precursor_mz[:n], sp.charge[:n],
+                   sp.mz[: off[-1]], sp.intensity[: off[-1]], off.copy())
+
+
+@pytest.fixture(scope="module")
+def fenv0(cfg0):
+    print("[gpu] prefilter env cfg0", flush=True)
+    return FEnv(cfg0, topk=200)
+
+
+@pytest.fixture(scope="module")
+def fenv1(cfg1_small):
+    print("[gpu] prefilter env cfg1-small", flush=True)
+    return FEnv(cfg1_small, topk=64)
+
+
+@pytest.mark.parametrize("which", ["fenv0", "fenv1"])
+def test_k1_features_equal_oracle(which, request):
+    e = request.getfixturevalue(which)
+    O = _O()
+    tf = O.topk_feat(e.vp, e.hix, e.pk, e.hyp, e.topk)
+    fl = e.fl
+    nc = _np(fl.n_cand)[:e.n]
+    assert np.array_equal(nc, tf["n_cand"])
+    assert np.array_equal(_np(fl.maxcnt)[:e.n], tf["maxcnt"])
+    m = (np.arange(e.topk)[None, :] < nc[:, None]).ravel()
+    assert m.sum() > 5000
+    for dv, ov in ((fl.row, tf["row"]), (fl.hyp, tf["hyp"]), (fl.cnt, tf["cnt"])):
+        assert np.array_equal(_np(dv)[:e.n * e.topk][m], ov[:e.n * e.topk][m])
+    iq = _np(fl.iq).view(np.uint16)[:e.n * e.topk][m]
+    assert np.array_equal(iq, tf["iq"][:e.n * e.topk][m])
+    assert iq.max() > 0
+
+
+def test_k1_features_lists_equal_fused_topk_without_closure(fenv0):
+    """Feature mode (u64 LDS histogram + payload in the key) selects exactly what vh_k1_topk selects."""
+    from vulcan.gpu import engine as E
+    e = fenv0
+    p = type(e.p)(**{**e.p.to_dict(), "topk": e.topk, "pair_closure": False})
+    ds = E.DeviceSearch(e.gi, p, e.n)
+    ds.run(e.raw)
+    torch.cuda.synchronize()
+    m = (np.arange(e.topk)[None, :] < _np(ds.n_cand)[:e.n, None]).ravel()
+    assert np.array_equal(_np(ds.n_cand)[:e.n], _np(e.fl.n_cand)[:e.n])
+    for a, b in ((ds.c_row, e.fl.row), (ds.c_hyp, e.fl.hyp), (ds.c_cnt, e.fl.cnt)):
+        assert np.array_equal(_np(a)[:e.n * e.topk][m], _np(b)[:e.n * e.topk][m])
+
+
+def _features_oracle(e):
+    P = _P()
+    fl = e.fl
+    hy = e.ds.hyp
+    return P.features(e.l1.mass32, e.l1.length, e.l1.mask, _np(fl.n_cand)[:e.n], _np(fl.row), _np(fl.hyp),
+                      _np(fl.cnt), _np(fl.iq).view(np.uint16), _np(fl.maxcnt)[:e.n], _np(hy["z"]), _np(hy["iso"]),
+                      _np(hy["M"]), e.topk)
+
+
+def test_features_kernel_equals_oracle(fenv0):
+    e = fenv0
+    x = _np(e.fl.features(e.ds, e.n))
+    xo = _features_oracle(e)
+    assert x.shape == xo.shape
+    assert np.array_equal(x.view(np.uint32), xo.view(np.uint32))
+
+
+def _model(seed=0, scale=1.0):
+    from vulcan.rescore import prefilter as PF
+    m = PF.PrefilterModel.init(seed)
+    rng = np.random.default_rng(seed + 100)
+    m.theta[2048:2304] = rng.standard_normal(256) * 0.1          # b1
+    m.theta[67840:68096] = rng.standard_normal(256) * 0.1        # b2
+    m.theta *= np.float32(scale)
+    m.mu = rng.standard_normal(8).astype(np.float32)
+    m.isd = (0.2 + rng.random(8)).astype(np.float32)
+    return m
+
+
+def test_mlp_forward_equals_oracle():
+    from vulcan.rescore import prefilter as PF
+    P = _P()
+    dev = torch.device("cuda:0")
+    m = _model(1)
+    rng = np.random.default_rng(2)
+    for n in (1, 17, 64, 1000, 4099):
+        X = (rng.standard_normal((n, 8)) * 4).astype(np.float32)
+        blob = m.blob(dev)
+        s = _np(PF.forward(blob, torch.from_numpy(X).to(dev))).astype(np.float64)
+        em = P.forward(m.theta, m.mu, m.isd, X, exact=False)["s"]
+        ex = P.forward(m.theta, m.mu, m.isd, X, exact=True)["s"]
+        scale = np.max(np.abs(em)) + 1.0
+        assert np.max(np.abs(s - em)) <= 2e-3 * scale, n       # f32 accumulation order (+ rare bf16 flips)
+        assert np.max(np.abs(s - ex)) <= 5e-2 * scale, n       # bf16 operand rounding
+        assert np.corrcoef(s, ex)[0, 1] > 0.999 if n > 10 else True
+
+
+def test_mlp_forward_stores_the_backward_activations():
+    from vulcan.rescore import prefilter as PF
+    P = _P()
+    dev = torch.device("cuda:0")
+    m = _model(3)
+    X = (np.random.default_rng(4).standard_normal((300, 8)) * 3).astype(np.float32)
+    blob = m.blob(dev)
+    xt = torch.from_numpy(X).to(dev)
+    s, a1, a2 = PF.forward(blob, xt, store=True)
+    s2 = PF.forward(blob, xt)
+    assert torch.equal(s, s2)
+    F = P.forward(m.theta, m.mu, m.isd, X, exact=False)
+    a1f = (_np(a1).view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    assert np.mean(a1f == F["a1"].astype(np.float32)) > 0.995
+    assert np.max(np.abs(_np(a2) - F["a2"])) <= 2e-3 * (np.max(np.abs(F["a2"])) + 1)
+
+
+def _blocks(g):
+    P = _P()
+    return {k: g[o:o + int(np.prod(sh))] for k, (o, sh) in P.OFF.items()}
+
+
+def test_mlp_backward_equals_oracle_gradients():
+    from vulcan.rescore import prefilter as PF
+    P = _P()
+    dev = torch.device("cuda:0")
+    m = _model(5, scale=0.5)
+    rng = np.random.default_rng(6)
+    for n in (64, 777, 5000):
+        X = (rng.standard_normal((n, 8)) * 3).astype(np.float32)
+        y = (rng.standard_normal(n) * 5).astype(np.float32)
+        blob = m.blob(dev)
+        xt, yt = torch.from_numpy(X).to(dev), torch.from_numpy(y).to(dev)
+        s, a1, a2 = PF.forward(blob, xt, store=True)
+        g = _np(PF.backward(blob, xt, a1, a2, s, yt)).astype(np.float64)
+        ge = P.grad(m.theta, m.mu, m.isd, X, y, exact=False)
+        gx = P.grad(m.theta, m.mu, m.isd, X, y, exact=True)
+        for k in P.OFF:
+            a, b, c = _blocks(g)[k], _blocks(ge)[k], _blocks(gx)[k]
+            assert np.linalg.norm(a - b) <= 5e-3 * np.linalg.norm(b) + 1e-7, (n, k)
+            assert np.linalg.norm(a - c) <= 6e-2 * np.linalg.norm(c) + 1e-6, (n, k)
+
+
+def test_adam_step_bit_exact():
+    from vulcan.rescore import prefilter as PF
+    P = _P()
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(8)
+    th = rng.standard_normal(PF.N_PARAMS).astype(np.float32)
+    g = rng.standard_normal(PF.N_PARAMS).astype(np.float32)
+    m1 = (rng.standard_normal(PF.N_PARAMS) * 0.1).astype(np.float32)
+    m2 = rng.random(PF.N_PARAMS).astype(np.float32)
+    T = [torch.from_numpy(a.copy()).to(dev) for a in (th, g, m1, m2)]
+    PF.adam_step(T[0], T[1], T[2], T[3], 7, 1e-3, 0.9, 0.999, 1e-8)
+    c1 = np.float32(1.0 / (1.0 - 0.9 ** 7))
+    c2 = np.float32(1.0 / (1.0 - 0.999 ** 7))
+    to, mo, vo = P.adam(th, g, m1, m2, 1e-3, 0.9, 0.999, 1e-8, c1, c2)
+    assert np.array_equal(_np(T[0]), to)
+    assert np.array_equal(_np(T[2]), mo)
+    assert np.array_equal(_np(T[3]), vo)
+
+
+def test_score_kernel_equals_forward_on_materialised_features(fenv0):
+    from vulcan.rescore import prefilter as PF
+    e = fenv0
+    m = _model(9)
+    blob = m.blob(e.gi.dev)
+    e.fl.score_with(e.ds, blob, e.n)
+    x = e.fl.features(e.ds, e.n)
+    s = PF.forward(blob, x)
+    torch.cuda.synchronize()
+    nc = _np(e.fl.n_cand)[:e.n]
+    msk = (np.arange(e.topk)[None, :] < nc[:, None]).ravel()
+    assert np.array_equal(_np(e.fl.score)[:e.n * e.topk][msk], _np(s)[msk])
+
+
+def test_select_equals_oracle(fenv0):
+    from vulcan.gpu import _lib as K
+    import ctypes as C
+    P = _P()
+    e = fenv0
+    fl = e.fl
+    rng = np.random.default_rng(10)
+    sc = rng.standard_normal(e.n * e.topk).astype(np.float32)
+    sc[::97] = np.nan
+    sc[5::13] = sc[4::13][: len(sc[5::13])]          # ties
+    fl.score.copy_(torch.from_numpy(sc))
+    for keep, clos in ((16, True), (1, True), (16, False), (300, True)):
+        ostr = 2 * keep if clos else keep
+        dev = e.gi.dev
+        on = torch.empty(e.n, dtype=torch.int32, device=dev)
+        orow, ohyp, ocnt = (torch.empty(e.n * ostr, dtype=torch.int32, device=dev) for _ in range(3))
+        K.check(K.lib().vh_prefilter_select(C.byref(e.gi.view), e.n, e.topk, K.ptr(fl.n_cand), K.ptr(fl.row),
+                                            K.ptr(fl.hyp), K.ptr(fl.cnt), K.ptr(fl.score), keep, int(clos), ostr,
+                                            K.ptr(on), K.ptr(orow), K.ptr(ohyp), K.ptr(ocnt), K.stream()), "select")
+        o = P.select(_np(fl.n_cand)[:e.n], _np(fl.row), _np(fl.hyp), _np(fl.cnt), sc, e.topk, keep, clos,
+                     e.l1.partner, ostr)
+        assert np.array_equal(_np(on), o["n"]), keep
+        mm = (np.arange(ostr)[None, :] < o["n"][:, None]).ravel()
+        for dv, ov in ((orow, o["row"]), (ohyp, o["hyp"]), (ocnt, o["cnt"])):
+            assert np.array_equal(_np(dv)[mm], ov[mm]), keep
+
+
+# ------------------------------------------------------------------------------------------------ end to end
+@pytest.fixture(scope="module")
+def trained(cfg1_small):
+    """Train on the first 2,000 spectra of the cfg1-small set, evaluate on the other 1,000."""
+    from vulcan.gpu import engine as E
+    from vulcan.rescore import prefilter as PF
+    d = cfg1_small
+    p = type(d["p"])(**d["p"].to_dict())
+    gi = E.GpuIndex(d["l1"], p, d["static"], d["var"])
+    sp = d["sp"]
+    n_tr = 2000
+    tr = _take(sp, n_tr)
+    pp = PF.PrefilterParams(index_topk=100, keep=8, steps=600, batch=4096)
+    raw = E.upload_spectra(tr, gi.dev)
+    x, y, spec, hs = PF.training_data(gi, p, raw, pp)
+    m0 = PF.PrefilterModel.init(0)
+    print(f"[gpu] training on {x.shape[0]} candidates", flush=True)
+    m = PF.train(m0, x, y, pp, log_every=50)
+    return {"gi": gi, "p": p, "pp": pp, "model": m, "x": x, "y": y, "d": d, "n_tr": n_tr}
+
+
+def test_training_lowers_the_loss(trained):
+    h = [v for t, v in trained["model"].history if isinstance(t, int)]
+    assert len(h) >= 5 and h[-1] < 0.5 * h[0], h
+
+
+def test_learned_search_keeps_the_plain_rank1_better_than_the_count_rule(trained):
+    """Plain = index top-K + closure rescored; learned = MLP top-keep + closure; ablation = K1-count top-keep +
+    closure (the reference's own rule at K = keep): same K2 budget as learned."""
+    from vulcan.gpu import engine as E
+    from vulcan.rescore import prefilter as PF
+    t = trained
+    gi, p, pp = t["gi"], t["p"], t["pp"]
+    sp = t["d"]["sp"]
+    n0 = t["n_tr"]
+    from vulcan.gpu.host import Spectra
+    off = sp.offsets[n0:] - sp.offsets[n0]
+    ev = Spectra(sp.run, sp.scan[n0:], sp.index[n0:], sp.rt[n0:], sp.precursor_mz[n0:], sp.charge[n0:],
+                 sp.mz[sp.offsets[n0]:], sp.intensity[sp.offsets[n0]:], off)
+    n = len(ev)
+    raw = E.upload_spectra(ev, gi.dev)
+    plain = E.DeviceSearch(gi, type(p)(**{**p.to_dict(), "topk": pp.index_topk}), n)
+    plain.run(raw)
+    ls = PF.LearnedSearch(gi, p, n, t["model"], pp)
+    ls.run(raw)
+    abl = E.DeviceSearch(gi, type(p)(**{**p.to_dict(), "topk": pp.keep}), n)
+    abl.run(raw)
+    torch.cuda.synchronize()
+    r_plain, r_learn, r_abl = plain.rank1_rows(n), ls.rank1_rows(n), abl.rank1_rows(n)
+    a_learn = PF.agreement(r_plain, r_learn)
+    a_abl = PF.agreement(r_plain, r_abl)
+    print(f"[gpu] top-1 agreement learned {a_learn:.4f} vs count rule {a_abl:.4f}", flush=True)
+    assert a_learn > 0.9
+    assert a_learn >= a_abl
+    # the learned path rescored at most 2 * keep candidates per spectrum
+    assert int(_np(ls.n_cand)[:n].max()) <= 2 * pp.keep
+    assert (r_learn >= 0).sum() > 0.5 * n

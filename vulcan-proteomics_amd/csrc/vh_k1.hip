@@ -15,6 +15,11 @@
 // a segment of that bucket increment their slot with an LDS atomic. The scan of the histogram feeds a
 // wave-private top-K buffer (ballot-compacted inserts, pruned by the current K-th key, bitonic compaction when
 // full). Closure and dedupe are one more bitonic sort. Nothing leaves the wave but the final candidate list.
+//
+// Feature mode (MODE 2, the learned prefilter's input): the histogram slots are u64, count in the low 24 bits and
+// the sum of the matching peaks' quantised intensities in the high 40 bits (one LDS u64 atomic per match). The
+// winners carry a 14-bit pseudo-log2 of that sum in the low bits of their top-K key, below the ordering fields, so
+// the top-K set and order are exactly those of MODE 1; no closure.
 #include "vh_common.h"
 
 #define K1_MAXSEG 32
@@ -61,6 +66,30 @@ __device__ __forceinline__ int topk_compact(uint64_t* buf, int capb, int nbuf, i
   return n;
 }
 
+// Intensity quantisation of the feature mode: each peak's intensity scaled by the power of two that maps the
+// spectrum's largest finite intensity into [2^15, 2^16), truncated to an integer (non-finite, negative, denormal
+// and NaN intensities count 0). vh_k1_features documents the same rule.
+__device__ __forceinline__ int iq_shift(float maxI) {
+  if (!(maxI >= 1.17549435e-38f) || maxI > 3.40282347e38f) return -1000;
+  int x;
+  frexpf(maxI, &x);
+  return 16 - x;
+}
+
+__device__ __forceinline__ uint32_t iq_quant(float I, int e) {
+  if (e == -1000 || !(I >= 1.17549435e-38f)) return 0u;
+  const float v = ldexpf(I, e);
+  return v >= 1.f ? (uint32_t)v : 0u;
+}
+
+// 14-bit pseudo-log2 of 1 + sum: exponent << 8 | the 8 bits below the leading one
+__device__ __forceinline__ uint32_t iq_log(uint64_t sum) {
+  const uint64_t x = sum + 1ULL;
+  const int e = 63 - __builtin_clzll(x);
+  const uint32_t m = e >= 8 ? (uint32_t)((x >> (e - 8)) & 0xFFULL) : (uint32_t)((x << (8 - e)) & 0xFFULL);
+  return ((uint32_t)e << 8) | m;
+}
+
 struct K1Out {
   // append mode
   int64_t cap;
@@ -73,6 +102,10 @@ struct K1Out {
   unsigned long long* hyp_matched;
   int32_t* status;
   int wave_bytes, hist_bytes;
+  // feature mode
+  const float* pk_int;
+  uint16_t* c_iq;
+  int32_t* s_maxcnt;
 };
 
 template <int MODE>
@@ -91,6 +124,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   const int s = order ? order[wv] : wv;
   unsigned char* wb = smem + (size_t)w * O.wave_bytes;
   uint32_t* hist = (uint32_t*)wb;
+  unsigned long long* hist64 = (unsigned long long*)wb;   // feature mode
   Seg* segs = (Seg*)(wb + O.hist_bytes);
   BkEnt* bks = (BkEnt*)(segs + K1_MAXSEG);
   uint32_t* hsc = (uint32_t*)(bks + K1_MAXBK);
@@ -100,7 +134,8 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   const int64_t h0 = hyp_off[s];
   const int nh = (int)(hyp_off[s + 1] - h0);
   if (nh == 0 || nh > 64) {
-    if (MODE == 1 && lane == 0) O.n_cand[s] = 0;
+    if (MODE >= 1 && lane == 0) O.n_cand[s] = 0;
+    if (MODE == 2 && lane == 0) O.s_maxcnt[s] = 0;
     if (nh > 64 && lane == 0) atomicOr(O.status, 1);
     return;
   }
@@ -119,6 +154,13 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   const int base = (int)X.bin_base, n_bins = X.n_bins;
   const int B = 1 << LB;
   const int K = P.topk;
+  int qe = -1000;
+  if (MODE == 2) {
+    float mx = 0.f;
+    for (int p = lane; p < np; p += 64) mx = fmaxf(mx, O.pk_int[p0 + p]);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    qe = iq_shift(mx);
+  }
   int nbuf = 0;
   uint64_t thresh = 0;
   bool clamped = false;
@@ -174,11 +216,17 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       e.pad0 = e.pad1 = e.pad2 = 0;
       bks[lane] = e;
     }
-    for (int j = lane; j < used; j += 64) hist[j] = 0;
+    if (MODE == 2) {
+      for (int j = lane; j < used; j += 64) hist64[j] = 0ULL;
+    } else {
+      for (int j = lane; j < used; j += 64) hist[j] = 0;
+    }
     wave_sync();
     // ---- accumulate shared-peak counts
     for (int p = lane; p < np; p += 64) {
       const float pm = pk_mass[p0 + p];
+      const unsigned long long inc = MODE == 2 ? 1ULL | ((unsigned long long)iq_quant(O.pk_int[p0 + p], qe) << 24)
+                                               : 0ULL;
       for (int z = 1; z < zmax; ++z) {
         float m = pm * (float)z;
         float a = tol_bound(m, P.tol_lo, P.tol_da);
@@ -221,8 +269,12 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
                 const int q = __builtin_ctz(msk);
                 msk &= msk - 1u;
                 const Seg g = segs[q];
-                if (z < ((g.hl_mfz >> 8) & 0xFF) && row >= g.r0 && row < g.r1)
-                  atomicAdd(&hist[g.base + row - g.r0], 1u);
+                if (z < ((g.hl_mfz >> 8) & 0xFF) && row >= g.r0 && row < g.r1) {
+                  if (MODE == 2)
+                    atomicAdd(&hist64[g.base + row - g.r0], inc);
+                  else
+                    atomicAdd(&hist[g.base + row - g.r0], 1u);
+                }
               }
             }
           }
@@ -241,7 +293,14 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       for (int j0 = 0; j0 < len; j0 += 64) {
         const int j = j0 + lane;
         const bool act = j < len;
-        const uint32_t c = act ? hist[g.base + j] : 0u;
+        uint32_t c, qv = 0;
+        if (MODE == 2) {
+          const unsigned long long c64 = act ? hist64[g.base + j] : 0ULL;
+          c = (uint32_t)(c64 & 0xFFFFFFULL);
+          qv = iq_log(c64 >> 24);
+        } else {
+          c = act ? hist[g.base + j] : 0u;
+        }
         sc += c > 0u;
         mt += c;
         const bool flag = act && (int)c >= thr;
@@ -265,7 +324,8 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
           if (flag) {
             uint32_t cc = c < 32767u ? c : 32767u;
             clamped |= c > 32767u;
-            key = ((uint64_t)cc << 48) | ((uint64_t)(uint32_t)(g.r0 + j) << 20) | (uint64_t)(0xFFFFF - g_hl);
+            key = ((uint64_t)cc << 48) | ((uint64_t)(uint32_t)(g.r0 + j) << 20) | ((uint64_t)(63 - g_hl) << 14) |
+                  (uint64_t)qv;
           }
           bool ins = flag && key > thresh;
           uint64_t bal = ballot(ins);
@@ -303,21 +363,22 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
     if (lane == 0) atomicOr(O.status, 2);
   }
   const int ntop = topk_compact(buf, O.capb, nbuf, K, thresh);
+  if (MODE == 2 && lane == 0) O.s_maxcnt[s] = ntop ? (int32_t)(buf[0] >> 48) : 0;
   uint64_t* ck = (uint64_t*)wb;                      // reuses the histogram area
   int32_t* cv = (int32_t*)(ck + O.clos_n);
-  const int ncl = P.pair_closure ? 2 * ntop : ntop;
+  const int ncl = (MODE == 1 && P.pair_closure) ? 2 * ntop : ntop;
   for (int i = lane; i < O.clos_n; i += 64) {
     uint64_t v = ~0ULL;
     if (i < ntop) {
       uint64_t key = buf[i];
       uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
-      uint32_t hl = 0xFFFFFu - (uint32_t)(key & 0xFFFFFULL);
-      cv[i] = (int32_t)(key >> 48);
+      uint32_t hl = 63u - (uint32_t)((key >> 14) & 63ULL);
+      cv[i] = (int32_t)(key >> 48) | (MODE == 2 ? (int32_t)((key & 0x3FFFULL) << 16) : 0);
       v = ((uint64_t)row << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
     } else if (i < ncl) {
       uint64_t key = buf[i - ntop];
       uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
-      uint32_t hl = 0xFFFFFu - (uint32_t)(key & 0xFFFFFULL);
+      uint32_t hl = 63u - (uint32_t)((key >> 14) & 63ULL);
       uint32_t pr = (uint32_t)X.partner[row];
       cv[i] = -1;
       v = ((uint64_t)pr << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
@@ -342,7 +403,11 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
         cc = cv[v & 0xFFFFFFu];
         for (int j = i + 1; j < ncl && (uint32_t)(ck[j] >> 32) == row; ++j) {
           int32_t c2 = cv[ck[j] & 0xFFFFFFu];
-          cc = c2 > cc ? c2 : cc;
+          if (MODE == 2) {   // the duplicate with the larger count (ties: lower hypothesis) gives the payload
+            cc = (c2 & 0xFFFF) > (cc & 0xFFFF) ? c2 : cc;
+          } else {
+            cc = c2 > cc ? c2 : cc;
+          }
         }
       }
     }
@@ -351,7 +416,12 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       int pos = nout + __popcll(bal & lanemask_lt());
       O.c_row[ob + pos] = (int32_t)row;
       O.c_hyp[ob + pos] = (int32_t)(h0 + hl);
-      O.c_cnt[ob + pos] = cc;
+      if (MODE == 2) {
+        O.c_cnt[ob + pos] = cc & 0xFFFF;
+        O.c_iq[ob + pos] = (uint16_t)((uint32_t)cc >> 16);
+      } else {
+        O.c_cnt[ob + pos] = cc;
+      }
     }
     nout += __popcll(bal);
   }
@@ -422,7 +492,7 @@ static void k1_geometry(const VhParams* p, int mode, K1Out& O, int& waves) {
   int K = p->topk > 0 ? p->topk : 1;
   O.capb = mode ? pow2_at_least(K + 64) : 0;
   O.clos_n = mode ? pow2_at_least((2 * K) > 64 ? 2 * K : 64) : 0;
-  int hb = p->tile * 4;
+  int hb = p->tile * (mode == 2 ? 8 : 4);
   int cb = O.clos_n * 12;
   O.hist_bytes = ((hb > cb ? hb : cb) + 15) & ~15;
   O.wave_bytes = O.hist_bytes + K1_MAXSEG * (int)sizeof(Seg) + K1_MAXBK * (int)sizeof(BkEnt) + 64 * 4 + 64 * 8 +
@@ -490,6 +560,44 @@ int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const i
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
   unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
   k1_kernel<1><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
+                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode,
+                                                                      h_lo, h_hi, O);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
+                   const int64_t* pk_off, const float* pk_mass, const float* pk_int, const int64_t* hyp_off,
+                   const int8_t* h_maxfz, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
+                   int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt,
+                   uint16_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored, unsigned long long* hyp_matched,
+                   int32_t* status, void* stream) {
+  int rc = k1_check(p, ix);
+  if (rc) return rc;
+  if (n_spec < 0 || !n_cand || !c_row || !c_hyp || !c_cnt || !c_iq || !s_maxcnt || !hyp_scored || !hyp_matched ||
+      !status || !pk_int)
+    return VH_E_ARG;
+  if (p->topk < 1 || p->topk > 1024) return VH_E_ARG;
+  if (cand_stride < p->topk) return VH_E_CAPACITY;
+  if (n_spec == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  K1Out O = {};
+  O.cand_stride = cand_stride;
+  O.n_cand = n_cand;
+  O.c_row = c_row;
+  O.c_hyp = c_hyp;
+  O.c_cnt = c_cnt;
+  O.hyp_scored = hyp_scored;
+  O.hyp_matched = hyp_matched;
+  O.status = status;
+  O.pk_int = pk_int;
+  O.c_iq = c_iq;
+  O.s_maxcnt = s_maxcnt;
+  int waves;
+  k1_geometry(p, 2, O, waves);
+  if (O.wave_bytes > 65536) return VH_E_LIMIT;
+  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
+  k1_kernel<2><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
                                                                       order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode,
                                                                       h_lo, h_hi, O);
   VH_CHECK_LAUNCH();

@@ -15,7 +15,7 @@ from .. import _build
 from ._abi import VhIndexView, VhParams
 
 P = C.c_void_p
-I32, I64, SZ = C.c_int32, C.c_int64, C.c_size_t
+I32, I64, SZ, F32 = C.c_int32, C.c_int64, C.c_size_t, C.c_float
 PP = C.POINTER(VhParams)
 PIX = C.POINTER(VhIndexView)
 
@@ -37,6 +37,15 @@ _SIGS = {
     "vh_k2": ([PP, PIX, I32, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_candidate_shift": ([I32, P, P, I32, P, P, P, P, P, P, P], I32),
     "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
+    "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_mlp_pack": ([P, P, P, P, P], I32),
+    "vh_mlp_forward": ([P, I64, P, P, P, P, P], I32),
+    "vh_mlp_workspace_bytes": ([I64], SZ),
+    "vh_mlp_backward": ([P, I64, P, P, P, P, P, P, P, SZ, P], I32),
+    "vh_mlp_adam": ([P, P, P, P, I64, F32, F32, F32, F32, F32, F32, P], I32),
+    "vh_prefilter_features": ([PIX, I32, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_prefilter_score": ([PIX, P, I32, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_prefilter_select": ([PIX, I32, I32, P, P, P, P, P, I32, I32, I32, P, P, P, P, P], I32),
 }
 
 ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "workspace too small", -4: "size limit exceeded"}

@@ -444,6 +444,17 @@ def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched
     return (tbl, hh) if return_hyp else tbl
 
 
+def _marker(ev: dict | None):
+    def mark(name):
+        if ev is None:
+            return None
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        ev.setdefault(name, []).append(e)
+        return e
+    return mark
+
+
 class DeviceSearch:
     """The fused closed-search step on preallocated device buffers: vh_k0 -> vh_hypotheses -> vh_spectrum_order
     -> vh_k1_topk -> vh_k2 -> vh_rank (report 1). `run(raw)` enqueues the launches on the current stream and
@@ -486,20 +497,19 @@ class DeviceSearch:
     def run(self, raw: dict, ev: dict | None = None):
         """ev (optional): stage name -> list, receives a (start, end) pair of torch.cuda.Event per stage,
         recorded on the launch stream (timing of the individual kernels)."""
-        L = K.lib()
-        gi, vp, p = self.gi, self.vp, self.p
         n = raw["n"]
         assert n <= self.n_spec
+        mark = _marker(ev)
+        self.front(raw, n, mark)
+        self.candidate_stage(n, mark)
+        return self.back(n, mark)
+
+    def front(self, raw: dict, n: int, mark):
+        """K0 peak view, precursor hypotheses, processing order."""
+        L = K.lib()
+        gi, vp = self.gi, self.vp
         st = K.stream()
         pk, hy = self.peaks, self.hyp
-
-        def mark(name):
-            if ev is None:
-                return None
-            e = torch.cuda.Event(enable_timing=True)
-            e.record()
-            ev.setdefault(name, []).append(e)
-            return e
         mark("k0")
         K.check(L.vh_k0(C.byref(vp), n, K.ptr(raw["mz"]), K.ptr(raw["int"]), K.ptr(raw["off"]), K.ptr(pk["off"]),
                         K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(pk["tic"]), K.ptr(pk["valid_d"]),
@@ -516,13 +526,26 @@ class DeviceSearch:
         K.check(L.vh_spectrum_order(n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(self.order),
                                     K.ptr(self.order_ws), self.order_ws.numel(), st), "spectrum_order")
         mark("order")
+
+    def candidate_stage(self, n: int, mark):
+        """Index top-K + pair closure into the candidate lists (n_cand / c_row / c_hyp / c_cnt, self.stride)."""
+        L = K.lib()
+        gi, vp = self.gi, self.vp
+        pk, hy = self.peaks, self.hyp
         mark("k1_topk")
         K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.view), n, K.ptr(self.order), K.ptr(pk["off"]),
                              K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["mode"]),
-                             K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand), K.ptr(self.c_row), K.ptr(self.c_hyp),
-                             K.ptr(self.c_cnt), K.ptr(self.scored), K.ptr(self.matched), K.ptr(self.status), st),
-                "k1_topk")
+                             K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand), K.ptr(self.c_row),
+                             K.ptr(self.c_hyp), K.ptr(self.c_cnt), K.ptr(self.scored), K.ptr(self.matched),
+                             K.ptr(self.status), K.stream()), "k1_topk")
         mark("k1_topk")
+
+    def back(self, n: int, mark):
+        """Exact rescoring of the candidate lists and the rank-`report` selection."""
+        L = K.lib()
+        gi, vp = self.gi, self.vp
+        pk, hy = self.peaks, self.hyp
+        st = K.stream()
         mark("k2")
         if self.modes:
             K.check(L.vh_candidate_shift(n, None, K.ptr(self.n_cand), self.stride, K.ptr(self.c_row),
@@ -537,6 +560,12 @@ class DeviceSearch:
                           self.report, K.ptr(self.sel), K.ptr(self.nxt), st), "rank")
         mark("rank")
         return {"sel": self.sel[:n * self.report], "next": self.nxt[:n * self.report], "n_cand": self.n_cand[:n]}
+
+    def rank1_rows(self, n: int) -> np.ndarray:
+        """Host copy of the rank-1 row per spectrum (-1: no PSM)."""
+        sel = self.sel[:n * self.report].view(n, self.report)[:, 0].cpu().numpy()
+        rows = self.c_row[: n * self.stride].cpu().numpy()
+        return np.where(sel >= 0, rows[np.maximum(sel, 0)], -1)
 
     def candidates(self, n: int):
         """Flatten the per-spectrum candidate lists into (h, r, c) sorted by (spectrum, row) (device)."""
