@@ -32,6 +32,94 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _stages(ev):
+    import numpy as np
+    out = {}
+    for k, lst in ev.items():
+        pairs = list(zip(lst[0::2], lst[1::2]))
+        out[k] = float(np.mean([s.elapsed_time(e) for s, e in pairs]))
+    return out
+
+
+def time_steps(obj, raw, steps, warmup):
+    """Warm up, then time `steps` runs (wall clock between synchronisations) with per-stage HIP events."""
+    import torch
+    for _ in range(warmup):
+        obj.run(raw)
+    torch.cuda.synchronize()
+    ev: dict = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        obj.run(raw, ev)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, _stages(ev)
+
+
+MLP_FLOP_PER_CANDIDATE = 2 * (8 * 256 + 256 * 256 + 256)   # 135,680
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
+    """Plain (index top-K + closure -> K2) vs learned prefilter (index top-K -> MLP -> top keep + closure -> K2)
+    vs the K1-count rule at the same K2 budget (index top keep + closure), on the same spectra; the network is
+    trained first on separately seeded spectra of the same workload (labels from the plain K2)."""
+    import numpy as np
+    import torch
+
+    from vulcan.gpu import engine as E
+    from vulcan.index import synth
+    from vulcan.rescore import prefilter as PF
+    pp = PF.PrefilterParams(index_topk=p.topk, keep=a.keep, steps=a.train_steps)
+    t0 = time.perf_counter()
+    sp_tr, _, _ = synth.spectra(l1, lab, a.cfg, n=a.n_train, seed=900 + a.cfg)
+    raw_tr = E.upload_spectra(sp_tr, gi.dev)
+    x, y, _, _ = PF.training_data(gi, p, raw_tr, pp)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    log(f"prefilter training set: {x.shape[0]} candidates of {len(sp_tr)} spectra ({t_data:.1f} s)")
+    model = PF.train(PF.PrefilterModel.init(pp.seed), x, y, pp, log_every=max(pp.steps // 10, 1))
+    losses = [v for t, v in model.history if isinstance(t, int)]
+    train_s = [v for t, v in model.history if t == "train_s"][0]
+    log(f"trained {pp.steps} steps in {train_s:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    del raw_tr, x, y
+    q_plain = type(p)(**{**p.to_dict(), "topk": pp.index_topk, "pair_closure": True})
+    q_rule = type(p)(**{**p.to_dict(), "topk": pp.keep, "pair_closure": True})
+    out = {}
+    rows = {}
+    for name, mk in (("plain", lambda: E.DeviceSearch(gi, q_plain, n)),
+                     ("learned", lambda: PF.LearnedSearch(gi, p, n, model, pp)),
+                     ("count_rule", lambda: E.DeviceSearch(gi, q_rule, n))):
+        obj = mk()
+        dt, st = time_steps(obj, raw, a.steps, a.warmup)
+        if int(obj.status.item()) != 0:
+            raise RuntimeError(f"{name}: k1 status {int(obj.status.item())}")
+        rows[name] = obj.rank1_rows(n)
+        nc = obj.n_cand[:n].cpu().numpy()
+        out[name] = {"spectra_per_s": n / dt, "ms_per_step": dt * 1e3, "stage_ms": st,
+                     "rescored_per_spectrum": float(nc.mean())}
+        if name == "learned":
+            scored = int(obj.lists.n_cand[:n].sum().item())
+            mlp_s = st["mlp"] * 1e-3
+            fl = MLP_FLOP_PER_CANDIDATE * scored / mlp_s / 1e12
+            out[name]["mlp_roofline"] = {"bound": "mfma", "achieved": fl, "peak": MFMA_BF16_PEAK_TFLOPS,
+                                         "unit": "TFLOP/s", "frac": fl / MFMA_BF16_PEAK_TFLOPS,
+                                         "candidates_scored": scored,
+                                         "flop_per_candidate": MLP_FLOP_PER_CANDIDATE}
+            learned_obj = obj
+        else:
+            del obj
+        log(f"{name}: {n / dt:,.0f} spectra/s ({dt * 1e3:.2f} ms/step)")
+    ag = {k: float(np.mean(rows[k][rows["plain"] >= 0] == rows["plain"][rows["plain"] >= 0]))
+          for k in ("learned", "count_rule")}
+    res = {"index_topk": pp.index_topk, "keep": pp.keep, "pair_closure": True,
+           "top1_agreement": ag["learned"], "count_rule_top1_agreement": ag["count_rule"],
+           "speedup_vs_plain": out["learned"]["spectra_per_s"] / out["plain"]["spectra_per_s"],
+           "train": {"spectra": a.n_train, "steps": pp.steps, "batch": pp.batch, "train_s": train_s,
+                     "loss_first": losses[0], "loss_last": losses[-1], "labels": "K2 hyperscore - spectrum best"},
+           **out}
+    return res, learned_obj
+
+
 def algorithmic_bytes(ds, gi, n, l1):
     """Per-launch algorithmic HBM bytes of K1 (fused top-K) and K2 on this batch.
     K1: sum_h [4*matched_h + 8*peaks_s(h)*(maxfz_h - 1)] + 4*peaks_s + 12*n_cand_s + 29 per hypothesis
@@ -105,6 +193,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=None, help="override rows per index bucket")
     ap.add_argument("--bin-ppm", type=float, default=None, help="override directory bin width")
     ap.add_argument("--tile", type=int, default=None, help="override K1 LDS tile")
+    ap.add_argument("--prefilter", action="store_true",
+                    help="plain vs learned prefilter comparison (default for the phospho-prefilter workload)")
+    ap.add_argument("--keep", type=int, default=16, help="prefilter: candidates kept before closure")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--n-train", type=int, default=20000, help="prefilter: training spectra (own seed)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run the u32 copy kernel on this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -149,6 +242,24 @@ def main():
     gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
     log(f"index: {gi.n_postings} postings, {gi.n_buckets} buckets x {gi.n_bins} bins, build {gi.build_s:.3f} s")
     raw = E.upload_spectra(sp, dev)
+    if a.prefilter or a.cfg == 4:
+        res, ls = prefilter_leg(a, w, l1, lab, gi, p, raw, n)
+        L = res["learned"]
+        st = L["stage_ms"]
+        mr = L["mlp_roofline"]
+        out = {"metric": METRIC, "value": L["spectra_per_s"], "unit": "spectra/s", "n_gpus": 1, "steps": a.steps,
+               "warmup": a.warmup, "ms_per_step": L["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
+               "config": {"workload": w.name, "n_spectra": n, "n_peptidoforms": len(l1),
+                          "n_postings": gi.n_postings, "index_topk": res["index_topk"], "keep": res["keep"],
+                          "pair_closure": True, "precursor_ppm": list(p.precursor_ppm),
+                          "fragment_da": list(p.fragment_ppm), "max_fragment_charge": p.max_fragment_charge,
+                          "parallelism": "single-gpu"},
+               "stage_ms": st, "index_build_s": gi.build_s,
+               "roofline": dict(mr, kernel="prefilter_score", traffic=None),
+               "prefilter": res, "cpu_baseline": None}
+        print(json.dumps(out), flush=True)
+        return
     ds = E.DeviceSearch(gi, p, n, **modes)
     for _ in range(a.warmup):
         ds.run(raw)

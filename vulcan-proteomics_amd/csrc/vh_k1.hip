@@ -165,34 +165,48 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   uint64_t thresh = 0;
   bool clamped = false;
 
-  int hc = 0;
-  int rc = __shfl(my_lo, 0);
+  // Windows are scanned from the top row down (hypotheses by descending window start, rows descending): the
+  // top-K key breaks count ties by the larger row, so once the buffer is full an equal-count row met later has
+  // the smaller key and is pruned by the threshold instead of forcing another compaction. The selected set does
+  // not depend on the scan order.
+  int my_rank = 0;
+  for (int h2 = 0; h2 < nh; ++h2) {
+    const int lo2 = __shfl(my_lo, h2);
+    my_rank += (lo2 > my_lo || (lo2 == my_lo && h2 < lane)) ? 1 : 0;
+  }
+  int it = 0;
+  int hc = (int)__builtin_ctzll(ballot(lane < nh && my_rank == 0));
+  int rc = __shfl(my_hi, hc);
   for (;;) {
     // ---- plan one tile of (hypothesis, row-range) segments (wave-uniform); lane j holds bucket entry j
     int used = 0, nseg = 0, zmax = 0, nbk = 0;
     int my_bk = -1, my_bz = 0, my_rlo = 0x7fffffff, my_rhi = 0;
     uint32_t my_mask = 0;
-    while (hc < nh && used < T && nseg < K1_MAXSEG) {
-      int hi_c = __shfl(my_hi, hc);
-      if (rc >= hi_c) {
-        ++hc;
-        if (hc < nh) rc = __shfl(my_lo, hc);
+    while (it < nh && used < T && nseg < K1_MAXSEG) {
+      const int lo_c = __shfl(my_lo, hc);
+      if (rc <= lo_c) {
+        ++it;
+        if (it < nh) {
+          hc = (int)__builtin_ctzll(ballot(lane < nh && my_rank == it));
+          rc = __shfl(my_hi, hc);
+        }
         continue;
       }
-      int take = hi_c - rc < T - used ? hi_c - rc : T - used;
+      int take = rc - lo_c < T - used ? rc - lo_c : T - used;
       // bucket entries this piece may add must fit: cap its span to the free entries
       int free_bk = K1_MAXBK - nbk;
       if (free_bk <= 0) break;
-      int cap_rows = (free_bk << LB) - (rc & (B - 1));
+      const int cap_rows = (((rc - 1) & (B - 1)) + 1) + ((free_bk - 1) << LB);
       take = take < cap_rows ? take : cap_rows;
       const int mfo = __shfl(my_mfz, hc);
       const int mf = mfo & 0xFF;
+      const int r0 = rc - take;
       if (lane == 0) {
         Seg g;
-        g.r0 = rc; g.r1 = rc + take; g.base = used; g.hl_mfz = hc | (mfo << 8);
+        g.r0 = r0; g.r1 = rc; g.base = used; g.hl_mfz = hc | (mfo << 8);
         segs[nseg] = g;
       }
-      for (int bk = rc >> LB; bk <= (rc + take - 1) >> LB; ++bk) {
+      for (int bk = r0 >> LB; bk <= (rc - 1) >> LB; ++bk) {
         uint64_t hit = ballot(lane < nbk && my_bk == bk);
         int j = hit ? (int)__builtin_ctzll(hit) : nbk;
         if (!hit) ++nbk;
@@ -200,14 +214,14 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
           my_bk = bk;
           my_mask |= 1u << nseg;
           my_bz = mf > my_bz ? mf : my_bz;
-          my_rlo = rc < my_rlo ? rc : my_rlo;
-          my_rhi = rc + take > my_rhi ? rc + take : my_rhi;
+          my_rlo = r0 < my_rlo ? r0 : my_rlo;
+          my_rhi = rc > my_rhi ? rc : my_rhi;
         }
       }
       zmax = mf > zmax ? mf : zmax;
       ++nseg;
       used += take;
-      rc += take;
+      rc = r0;
     }
     if (nseg == 0) break;
     if (lane < nbk) {
@@ -291,8 +305,9 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       uint32_t sc = 0;
       unsigned long long mt = 0;
       for (int j0 = 0; j0 < len; j0 += 64) {
-        const int j = j0 + lane;
-        const bool act = j < len;
+        const int jd = j0 + lane;          // rows descending: slot j holds row g.r0 + j
+        const int j = len - 1 - jd;
+        const bool act = jd < len;
         uint32_t c, qv = 0;
         if (MODE == 2) {
           const unsigned long long c64 = act ? hist64[g.base + j] : 0ULL;
