@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 7
+#define VH_ABI_VERSION 8
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -76,11 +76,12 @@ typedef struct VhParams {
 
 /* Device view of the peptidoform table (vulcan/index/build.py:13-21) and the fragment index.
  * Fragment index: rows are cut into buckets of 2^bucket_log2 consecutive rows (the table is sorted by
- * precursor mass, so a bucket is a precursor-mass interval). Inside a bucket the postings are sorted by the
- * IEEE-754 bit pattern of the fragment mass (order-preserving for positive floats, so the exact f32 value is
- * the key: no lossy quantisation), ties by row. post_key[i] = f32 bits of the fragment, post_row[i] = row -
- * bucket start. The directory offsets[bucket * n_bins + bin] = first posting of the bucket whose key has
- * (key >> bin_shift) - bin_base >= bin; offsets has n_buckets * n_bins + 1 entries. */
+ * precursor mass, so a bucket is a precursor-mass interval). A fragment's key is the IEEE-754 bit pattern of
+ * its f32 mass (order-preserving for positive floats, so the exact value is the key: no lossy quantisation) and
+ * its bin is (key >> bin_shift) - bin_base. Postings are ordered bin-major: by (bin, bucket, key, row), so the
+ * postings of consecutive buckets in one bin are contiguous. postings[i] = (key & (2^bin_shift - 1)) << 16 |
+ * (row - bucket start) (the bin gives the key's high bits). The directory offsets[bin * n_buckets + bucket] =
+ * first posting of (bin, bucket); offsets has n_bins * n_buckets + 1 entries (the last = n_postings). */
 typedef struct VhIndexView {
   int64_t n_rows;
   const int64_t* seq_off;
@@ -95,8 +96,7 @@ typedef struct VhIndexView {
   int32_t n_bins;
   int64_t n_buckets;
   const uint32_t* offsets;
-  const uint32_t* post_key;
-  const uint16_t* post_row;
+  const uint32_t* postings;
   int64_t n_postings;
 } VhIndexView;
 
@@ -116,14 +116,13 @@ int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, v
                   void* stream);
 /* Scratch bytes of vh_index_build for n_postings keys. */
 size_t vh_index_workspace_bytes(int64_t n_postings, int64_t n_buckets, int32_t n_bins);
-/* Emit 64-bit keys (bucket << (16 + nd) | (f32 bits - bits(fmin)) << 16 | row in bucket, nd = bit width of
- * bits(fmax) - bits(fmin)), LSD radix sort them on the device over bits [16, 16 + nd + bucket bits) (stable,
- * so ties keep row order), write post_key / post_row (n_postings each) and the directory offsets
- * (n_buckets * n_bins + 1).
- * Replaces vk_frag_emit (kernels.cu:73-101, 401-411) and the torch.sort / bincount / cumsum CSR assembly of
- * engine.py:129-148. Needs bucket_log2 <= 16 and bits(fmax) - bits(fmin) < 2^26. */
+/* Emit 64-bit keys (bin << (16 + bin_shift + bucket bits) | bucket << (16 + bin_shift) | (f32 bits &
+ * (2^bin_shift - 1)) << 16 | row in bucket), LSD radix sort them on the device over bits [16, 64) that are used
+ * (stable, so equal keys keep row order), write postings (n_postings) and the directory offsets
+ * (n_bins * n_buckets + 1). Replaces vk_frag_emit (kernels.cu:73-101, 401-411) and the torch.sort / bincount /
+ * cumsum CSR assembly of engine.py:129-148. Needs bucket_log2 <= 16, bin_shift <= 16 and the key to fit 64 bits. */
 int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
-                   uint32_t* post_key, uint16_t* post_row, void* ws, size_t ws_bytes, void* stream);
+                   uint32_t* postings, void* ws, size_t ws_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------------- K0 */
 /* Peak view (engine.py:171-210): keep fmin <= m/z <= fmax (not NaN), mass = m/z - PROTON (f32), top

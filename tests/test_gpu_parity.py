@@ -77,7 +77,7 @@ def _postings_numpy(l1, vp, gi):
 
 
 def test_index_build_matches_numpy(env0):
-    """Device emission + LSD radix sort + directory == numpy emission + stable sort by (bucket, f32 bits), on the
+    """Device emission + LSD radix sort + directory == numpy emission + stable sort by (bin, bucket, f32 bits), on the
     first 3,000 rows (a sub-table keeps the Python loop short)."""
     from vulcan.gpu import engine as E
     from vulcan.index.build import L1Index
@@ -91,12 +91,14 @@ def test_index_build_matches_numpy(env0):
     gi = E.GpuIndex(sub, p2, STATIC, {})
     post = _postings_numpy(sub, gi.vp, gi)
     assert gi.n_postings == len(post)
-    o = np.lexsort((post[:, 1], post[:, 0]))              # stable: equal (bucket, key) keep emission order
-    ps = post[o]
-    assert np.array_equal(_np(gi.post_key).view(np.uint32)[:len(ps)].astype(np.int64), ps[:, 1])
-    assert np.array_equal(_np(gi.post_row).view(np.uint16)[:len(ps)].astype(np.int64), ps[:, 2])
-    d = ps[:, 0] * gi.n_bins + ((ps[:, 1] >> gi.bin_shift) - gi.bin_base)
-    want = np.searchsorted(d, np.arange(gi.n_buckets * gi.n_bins + 1), side="left")
+    S = gi.bin_shift
+    bins = (post[:, 1] >> S) - gi.bin_base
+    o = np.lexsort((post[:, 1], post[:, 0], bins))        # stable: equal (bin, bucket, key) keep emission order
+    ps, pb = post[o], bins[o]
+    want_post = ((ps[:, 1] & ((1 << S) - 1)) << 16) | ps[:, 2]
+    assert np.array_equal(_np(gi.postings).view(np.uint32)[:len(ps)].astype(np.int64), want_post)
+    d = pb * gi.n_buckets + ps[:, 0]
+    want = np.searchsorted(d, np.arange(gi.n_bins * gi.n_buckets + 1), side="left")
     assert np.array_equal(_np(gi.offsets).view(np.uint32).astype(np.int64), want)
 
 

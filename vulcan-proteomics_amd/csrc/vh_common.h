@@ -43,8 +43,7 @@ struct DevIdx {
   int n_bins;
   int64_t n_buckets;
   const uint32_t* offsets;
-  const uint32_t* post_key;
-  const uint16_t* post_row;
+  const uint32_t* postings;
 };
 
 DevP vh_make_devp(const VhParams* p);

@@ -1,11 +1,13 @@
 // Fragment inverted index on the device: per-row fragment count, key emission, LSD radix sort, directory.
 //
 // Replaces frag_count_kernel / frag_emit_kernel (vulcan/gpu/kernels.cu:53-101) and the torch.sort + bincount +
-// cumsum assembly of GpuIndex.__init__ (vulcan/gpu/engine.py:101-148). Emitted key =
-// bucket << (16 + nd) | (f32 bits - bits(fmin)) << 16 | row in bucket, with bucket = row >> bucket_log2 (rows are
-// sorted by precursor mass) and nd = bits of the f32-bit range of [fmin, fmax]. The f32 bit pattern of a positive
-// float is order-preserving, so the sorted postings of a bucket are in exact fragment-mass order. Only bits
-// [16, 16 + nd + bucket bits) are sorted (stable LSD, 8-bit digits): equal masses keep emission (row) order.
+// cumsum assembly of GpuIndex.__init__ (vulcan/gpu/engine.py:101-148). A fragment's key is its f32 bit pattern
+// (order-preserving for positive floats: exact mass order), its bin the key's high bits. Emitted sort key =
+// bin << (16 + S + nbk) | bucket << (16 + S) | (key & (2^S - 1)) << 16 | row in bucket (S = bin_shift, nbk =
+// bucket bits, bucket = row >> bucket_log2 with rows sorted by precursor mass). Bin-major order puts the postings
+// of consecutive buckets of one bin next to each other, so a K1 lookup over a window's run of buckets reads one
+// directory line and one contiguous posting range. Only the used bits above 16 are sorted (stable LSD, 8-bit
+// digits): equal keys keep emission (row) order.
 #include <string.h>
 
 #include "vh_common.h"
@@ -35,12 +37,16 @@ __global__ __launch_bounds__(256) void frag_count_kernel(DevP P, DevIdx X, int64
 }
 
 struct KeyGeom {
-  uint32_t fbase;   // f32 bits of fragment_min_mz
-  int nd;           // bits of (bits(fmax) - bits(fmin))
+  int S;      // bin_shift
+  int nbk;    // bucket bits
+  uint32_t bin_base;
 };
 
 __device__ __forceinline__ uint64_t posting_key(float f, uint64_t bucket, uint32_t local, const KeyGeom& G) {
-  return (bucket << (16 + G.nd)) | ((uint64_t)(fbits(f) - G.fbase) << 16) | (uint64_t)local;
+  const uint32_t k = fbits(f);
+  const uint64_t bin = (uint64_t)((k >> G.S) - G.bin_base);
+  return (bin << (16 + G.S + G.nbk)) | (bucket << (16 + G.S)) | ((uint64_t)(k & ((1u << G.S) - 1u)) << 16) |
+         (uint64_t)local;
 }
 
 __global__ __launch_bounds__(256) void frag_emit_kernel(DevP P, DevIdx X, KeyGeom G, const int64_t* post_off,
@@ -153,14 +159,14 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
 
 // ------------------------------------------------------------------------------------------------ directory
 __device__ __forceinline__ int64_t dir_of(uint64_t key, const KeyGeom& G, const DevIdx& X) {
-  uint64_t bucket = key >> (16 + G.nd);
-  uint32_t fb = (uint32_t)((key >> 16) & ((1ULL << G.nd) - 1ULL)) + G.fbase;
-  return (int64_t)bucket * X.n_bins + (int64_t)((fb >> X.bin_shift) - X.bin_base);
+  const int64_t bin = (int64_t)(key >> (16 + G.S + G.nbk));
+  const int64_t bucket = (int64_t)((key >> (16 + G.S)) & ((1ULL << G.nbk) - 1ULL));
+  return bin * X.n_buckets + bucket;
 }
 
 __global__ __launch_bounds__(256) void directory_kernel(const uint64_t* __restrict__ keys, int64_t n, DevIdx X,
-                                                        KeyGeom G, uint32_t* offsets, uint32_t* post_key,
-                                                        uint16_t* post_row, int64_t n_dir) {
+                                                        KeyGeom G, uint32_t* offsets, uint32_t* postings,
+                                                        int64_t n_dir) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n == 0) {
     for (int64_t b = i; b <= n_dir; b += (int64_t)gridDim.x * blockDim.x) offsets[b] = 0;
@@ -171,8 +177,7 @@ __global__ __launch_bounds__(256) void directory_kernel(const uint64_t* __restri
   int64_t d = dir_of(kk, G, X);
   int64_t prev = i ? dir_of(keys[i - 1], G, X) : -1;
   for (int64_t b = prev + 1; b <= d; ++b) offsets[b] = (uint32_t)i;
-  post_key[i] = (uint32_t)((kk >> 16) & ((1ULL << G.nd) - 1ULL)) + G.fbase;
-  post_row[i] = (uint16_t)(kk & 0xFFFFULL);
+  postings[i] = (uint32_t)(((kk >> 16) & ((1ULL << G.S) - 1ULL)) << 16) | (uint32_t)(kk & 0xFFFFULL);
   if (i == n - 1)
     for (int64_t b = d + 1; b <= n_dir; ++b) offsets[b] = (uint32_t)n;
 }
@@ -237,8 +242,8 @@ int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, v
 }
 
 int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
-                   uint32_t* post_key, uint16_t* post_row, void* ws, size_t ws_bytes, void* stream) {
-  if (!p || !ix || !post_off || !offsets || (ix->n_postings && (!post_key || !post_row))) return VH_E_ARG;
+                   uint32_t* postings, void* ws, size_t ws_bytes, void* stream) {
+  if (!p || !ix || !post_off || !offsets || (ix->n_postings && !postings)) return VH_E_ARG;
   if (ix->bin_shift < 4 || ix->bin_shift > 16 || ix->bucket_log2 < 0 || ix->bucket_log2 > 16) return VH_E_ARG;
   if (!(p->fmin > 0.f) || !(p->fmax >= p->fmin)) return VH_E_ARG;
   if (ix->n_postings < 0 || ix->n_postings >= (1LL << 32)) return VH_E_LIMIT;
@@ -250,12 +255,13 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
   memcpy(&fb0, &p->fmin, 4);
   memcpy(&fb1, &p->fmax, 4);
   KeyGeom G;
-  G.fbase = fb0;
-  G.nd = 1;
-  while (G.nd < 32 && ((uint64_t)(fb1 - fb0) >> G.nd) != 0) ++G.nd;
-  int nbk = 0;
-  while (nbk < 32 && ((uint64_t)(ix->n_buckets - 1) >> nbk) != 0) ++nbk;
-  if (G.nd > 26 || 16 + G.nd + nbk > 64) return VH_E_LIMIT;
+  G.S = ix->bin_shift;
+  G.bin_base = ix->bin_base;
+  G.nbk = 0;
+  while (G.nbk < 32 && ((uint64_t)(ix->n_buckets - 1) >> G.nbk) != 0) ++G.nbk;
+  int nbn = 1;
+  while (nbn < 32 && ((uint64_t)(ix->n_bins - 1) >> nbn) != 0) ++nbn;
+  if (16 + G.S + G.nbk + nbn > 64) return VH_E_LIMIT;
   if ((fb0 >> ix->bin_shift) != ix->bin_base || (int64_t)((fb1 >> ix->bin_shift) - ix->bin_base) >= ix->n_bins)
     return VH_E_ARG;
   hipStream_t st = (hipStream_t)stream;
@@ -270,10 +276,10 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
   if (ix->n_rows) frag_emit_kernel<<<grid, 256, 0, st>>>(P, X, G, post_off, ka);
   VH_CHECK_LAUNCH();
   uint64_t* src = ka;
-  int rc = vh_radix_sort_u64(ka, kb2, n, 16, 16 + G.nd + nbk, w + 2 * kb, ws_bytes - 2 * kb, st, &src);
+  int rc = vh_radix_sort_u64(ka, kb2, n, 16, 16 + G.S + G.nbk + nbn, w + 2 * kb, ws_bytes - 2 * kb, st, &src);
   if (rc) return rc;
   unsigned g2 = (unsigned)((n > 0 ? n : 1024) + 255) / 256;
-  directory_kernel<<<g2, 256, 0, st>>>(src, n, X, G, offsets, post_key, post_row, n_dir);
+  directory_kernel<<<g2, 256, 0, st>>>(src, n, X, G, offsets, postings, n_dir);
   VH_CHECK_LAUNCH();
   return 0;
 }

@@ -9,10 +9,11 @@
 //   highest count), row order.
 // Design: one wave64 per spectrum. All (hypothesis, row) pairs of the spectrum's windows get a slot in an LDS
 // u32 histogram (tiled when they exceed `tile` slots). A tile's segments (hypothesis, row range) are grouped
-// by index bucket. Each lane takes peaks; for each (peak, z) the exact f32 key range [bits(lo bound),
-// bits(hi bound)] is looked up once per bucket: one directory read gives the first posting of the range's first
-// bin, the key-ordered postings are scanned until the key passes the high bound, and postings whose row falls in
-// a segment of that bucket increment their slot with an LDS atomic. The scan of the histogram feeds a
+// by index bucket (ascending). Each lane takes peaks; for each (peak, z) the exact f32 key range [bits(lo bound),
+// bits(hi bound)] is looked up per bin it touches and per bucket of the tile: the bin-major directory holds the
+// entries of consecutive buckets side by side (one line for a group of four), each bucket's postings in that bin
+// are key-sorted and scanned until the key passes the high bound, and postings whose row falls in a segment of
+// that bucket increment their slot with an LDS atomic. The scan of the histogram feeds a
 // wave-private top-K buffer (ballot-compacted inserts, pruned by the current K-th key, bitonic compaction when
 // full). Closure and dedupe are one more bitonic sort. Nothing leaves the wave but the final candidate list.
 //
@@ -24,6 +25,7 @@
 
 #define K1_MAXSEG 32
 #define K1_MAXBK 32
+#define K1_PREF 4   // postings per bucket run loaded in the first batch
 
 struct Seg {   // one (hypothesis, row range) piece of a tile: slots [base, base + r1 - r0)
   int r0, r1, base, hl_mfz;   // hl_mfz = local hypothesis | maxfz << 8 | open hypothesis << 16
@@ -224,11 +226,15 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       rc = r0;
     }
     if (nseg == 0) break;
-    if (lane < nbk) {
-      BkEnt e;
-      e.bk = my_bk; e.zmax = my_bz; e.rlo = my_rlo; e.rhi = my_rhi; e.mask = my_mask;
-      e.pad0 = e.pad1 = e.pad2 = 0;
-      bks[lane] = e;
+    {   // bucket entries in ascending bucket order: groups of consecutive buckets share directory lines
+      int rank = 0;
+      for (int j = 0; j < nbk; ++j) rank += __shfl(my_bk, j) < my_bk ? 1 : 0;
+      if (lane < nbk) {
+        BkEnt e;
+        e.bk = my_bk; e.zmax = my_bz; e.rlo = my_rlo; e.rhi = my_rhi; e.mask = my_mask;
+        e.pad0 = e.pad1 = e.pad2 = 0;
+        bks[rank] = e;
+      }
     }
     if (MODE == 2) {
       for (int j = lane; j < used; j += 64) hist64[j] = 0ULL;
@@ -252,44 +258,61 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
         int bh = (int)(kb >> S) - base;
         bl = bl < 0 ? 0 : bl;
         bh = bh > n_bins - 1 ? n_bins - 1 : bh;
-        if (bl > bh) continue;
-        // directory pairs of up to 4 buckets are loaded together (independent loads in flight), then scanned
-        for (int j0 = 0; j0 < nbk; j0 += 4) {
-          uint32_t e0[4], e1[4];
+        for (int bn = bl; bn <= bh; ++bn) {
+          const uint32_t khi = (uint32_t)(bn + base) << S;
+          const uint32_t* dirb = X.offsets + (size_t)bn * (size_t)X.n_buckets;
+          // directory entries of up to 4 buckets (consecutive buckets: one line) are loaded together, then the
+          // buckets' key-sorted posting runs of this bin are scanned
+          for (int j0 = 0; j0 < nbk; j0 += 4) {
+            uint32_t e0[4], e1[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            e0[u] = e1[u] = 0;
-            if (j0 + u < nbk) {
-              const int bk = bks[j0 + u].bk;
-              if (z < bks[j0 + u].zmax) {
-                const uint32_t* off = X.offsets + (size_t)bk * (size_t)n_bins;
-                e0[u] = off[bl];
-                e1[u] = off[bh + 1];
+            for (int u = 0; u < 4; ++u) {
+              e0[u] = e1[u] = 0;
+              if (j0 + u < nbk && z < bks[j0 + u].zmax) {
+                const int bk = bks[j0 + u].bk;
+                e0[u] = dirb[bk];
+                e1[u] = dirb[bk + 1];
               }
             }
-          }
+            // the first K1_PREF postings of each bucket's run are loaded in one batch (independent loads), so a
+            // typical lookup costs two memory round trips; longer runs continue one posting at a time
+            uint32_t pw[4][K1_PREF];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (e0[u] == e1[u]) continue;
-            const BkEnt E = bks[j0 + u];
-            const int rb = E.bk << LB;
-            for (uint32_t e = e0[u]; e < e1[u]; ++e) {
-              const uint32_t key = X.post_key[e];
-              if (key > kb) break;
-              const int row = rb + (int)X.post_row[e];
-              if (key < ka || row < E.rlo || row >= E.rhi) continue;
-              uint32_t msk = E.mask;
-              while (msk) {
-                const int q = __builtin_ctz(msk);
-                msk &= msk - 1u;
-                const Seg g = segs[q];
-                if (z < ((g.hl_mfz >> 8) & 0xFF) && row >= g.r0 && row < g.r1) {
-                  if (MODE == 2)
-                    atomicAdd(&hist64[g.base + row - g.r0], inc);
-                  else
-                    atomicAdd(&hist[g.base + row - g.r0], 1u);
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+              for (int t = 0; t < K1_PREF; ++t) pw[u][t] = e0[u] + t < e1[u] ? X.postings[e0[u] + t] : 0u;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              if (e0[u] == e1[u]) continue;
+              const BkEnt E = bks[j0 + u];
+              const int rb = E.bk << LB;
+              // one posting: false once the key passes the high bound (the run is key-sorted)
+              auto visit = [&](uint32_t w) -> bool {
+                const uint32_t key = khi | (w >> 16);
+                if (key > kb) return false;
+                const int row = rb + (int)(w & 0xFFFFu);
+                if (key < ka || row < E.rlo || row >= E.rhi) return true;
+                uint32_t msk = E.mask;
+                while (msk) {
+                  const int q = __builtin_ctz(msk);
+                  msk &= msk - 1u;
+                  const Seg g = segs[q];
+                  if (z < ((g.hl_mfz >> 8) & 0xFF) && row >= g.r0 && row < g.r1) {
+                    if (MODE == 2)
+                      atomicAdd(&hist64[g.base + row - g.r0], inc);
+                    else
+                      atomicAdd(&hist[g.base + row - g.r0], 1u);
+                  }
                 }
-              }
+                return true;
+              };
+              bool more = true;
+#pragma unroll
+              for (int t = 0; t < K1_PREF; ++t)
+                if (more) more = e0[u] + t < e1[u] && visit(pw[u][t]);
+              if (more)
+                for (uint32_t e = e0[u] + K1_PREF; e < e1[u]; ++e)
+                  if (!visit(X.postings[e])) break;
             }
           }
         }
@@ -496,7 +519,7 @@ static int pow2_at_least(int x) {
 }
 
 static int k1_check(const VhParams* p, const VhIndexView* ix) {
-  if (!p || !ix || !ix->offsets || (ix->n_postings && (!ix->post_key || !ix->post_row))) return VH_E_ARG;
+  if (!p || !ix || !ix->offsets || (ix->n_postings && !ix->postings)) return VH_E_ARG;
   if (p->tile < 64 || p->tile > 8192 || (p->tile & (p->tile - 1))) return VH_E_ARG;
   if (ix->n_rows >= (1LL << 28)) return VH_E_LIMIT;
   if (ix->bin_shift < 4 || ix->bin_shift > 16 || ix->bucket_log2 < 0 || ix->bucket_log2 > 16) return VH_E_ARG;

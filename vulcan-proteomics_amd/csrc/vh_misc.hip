@@ -54,8 +54,7 @@ DevIdx vh_make_devidx(const VhIndexView* ix) {
   d.n_bins = ix->n_bins;
   d.n_buckets = ix->n_buckets;
   d.offsets = ix->offsets;
-  d.post_key = ix->post_key;
-  d.post_row = ix->post_row;
+  d.postings = ix->postings;
   return d;
 }
 

@@ -23,6 +23,7 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
 typedef __attribute__((ext_vector_type(8))) unsigned short u16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2_t;
 
 // parameter vector (f32 master weights): W1 [256][8] (out, in), b1, W2 [256][256] (out, in), b2, w3, b3
 #define TH_W1 0
@@ -55,6 +56,12 @@ __device__ __forceinline__ unsigned short f2bf(float f) {   // round to nearest 
   return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+
+// two f32 -> packed bf16 (lo = a), round to nearest even: one v_cvt_pk_bf16_f32 on gfx950 (same bits as f2bf)
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+  const bf16x2_t v = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(uint32_t, v);
+}
 
 __device__ __forceinline__ f32x4 mfma_bf16(u16x8 a, u16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
@@ -103,12 +110,13 @@ __global__ __launch_bounds__(256) void features_kernel(FeatIn F, int64_t n_slots
 __device__ __forceinline__ u16x8 feature_operand(const unsigned char* lds, const float* f, bool valid) {
   const float* mu = (const float*)(lds + BL_MU);
   const float* isd = (const float*)(lds + BL_ISD);
-  u16x8 x = {0, 0, 0, 0, 0, 0, 0, 0};
+  u32x4 x = {0u, 0u, 0u, 0u};
   if (valid) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = f2bf((f[j] - mu[j]) * isd[j]);
+    for (int j = 0; j < 4; ++j)
+      x[j] = pack_bf16x2((f[2 * j] - mu[2 * j]) * isd[2 * j], (f[2 * j + 1] - mu[2 * j + 1]) * isd[2 * j + 1]);
   }
-  return x;
+  return __builtin_bit_cast(u16x8, x);
 }
 
 // ------------------------------------------------------------------------------------------------ MLP tile
@@ -133,21 +141,19 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* lds, const u16x8 (
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
       const f32x4 acc = mfma_bf16(a, xb[nb], zero);
-      uint32_t hb[4];
+      const f32x4 bb = *(const f32x4*)(b1 + 16 * mb + 4 * g);
+      float v[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float v = acc[r] + b1[16 * mb + 4 * g + r];
-        v = v > 0.f ? v : 0.f;
-        hb[r] = f2bf(v);
-      }
-      h1[nb][mb >> 1][2 * (mb & 1)] = hb[0] | (hb[1] << 16);
-      h1[nb][mb >> 1][2 * (mb & 1) + 1] = hb[2] | (hb[3] << 16);
+      for (int r = 0; r < 4; ++r) v[r] = fmaxf(acc[r] + bb[r], 0.f);
+      const uint32_t w0 = pack_bf16x2(v[0], v[1]), w1 = pack_bf16x2(v[2], v[3]);
+      h1[nb][mb >> 1][2 * (mb & 1)] = w0;
+      h1[nb][mb >> 1][2 * (mb & 1) + 1] = w1;
       if (STORE) {
         const int64_t c = c0 + 16 * nb + cl;
         if (c < n) {
           uint32_t* d = (uint32_t*)(a1 + c * 256 + 16 * mb + 4 * g);
-          d[0] = hb[0] | (hb[1] << 16);
-          d[1] = hb[2] | (hb[3] << 16);
+          d[0] = w0;
+          d[1] = w1;
         }
       }
     }
@@ -170,16 +176,16 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* lds, const u16x8 (
         acc[nb] = mfma_bf16(a, __builtin_bit_cast(u16x8, hw), acc[nb]);
       }
     }
+    const f32x4 bb = *(const f32x4*)(b2 + 16 * mb2 + 4 * g);
+    const f32x4 ww = *(const f32x4*)(w3 + 16 * mb2 + 4 * g);
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
       float v4[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = 16 * mb2 + 4 * g + r;
-        float v = acc[nb][r] + b2[m];
-        v = v > 0.f ? v : 0.f;
+        const float v = fmaxf(acc[nb][r] + bb[r], 0.f);
         v4[r] = v;
-        part[nb] = part[nb] + w3[m] * v;
+        part[nb] = fmaf(ww[r], v, part[nb]);
       }
       if (STORE) {
         const int64_t c = c0 + 16 * nb + cl;

@@ -26,7 +26,7 @@ class VhParams(C.Structure):
 class VhIndexView(C.Structure):
     _fields_ = [("n_rows", I64), ("seq_off", P), ("length", P), ("mask", P), ("mass32", P), ("residues", P),
                 ("partner", P), ("bucket_log2", I32), ("bin_shift", I32), ("bin_base", U32), ("n_bins", I32),
-                ("n_buckets", I64), ("offsets", P), ("post_key", P), ("post_row", P), ("n_postings", I64)]
+                ("n_buckets", I64), ("offsets", P), ("postings", P), ("n_postings", I64)]
 
 
 def mod_arrays(static_mods: dict, var_mods: dict):

@@ -26,7 +26,7 @@ _SIGS = {
     "vh_copy_u32": ([P, P, I64, P], I32),
     "vh_frag_count": ([PP, PIX, P, P, SZ, P], I32),
     "vh_index_workspace_bytes": ([I64, I64, I32], SZ),
-    "vh_index_build": ([PP, PIX, P, P, P, P, P, SZ, P], I32),
+    "vh_index_build": ([PP, PIX, P, P, P, P, SZ, P], I32),
     "vh_k0": ([PP, I32, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_spectrum_order_workspace_bytes": ([I32], SZ),

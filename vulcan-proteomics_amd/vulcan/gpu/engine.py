@@ -34,9 +34,8 @@ class Slice:
     """One resident posting directory (the whole index is one slice here; 288 GB HBM holds it)."""
     chunk0: int
     chunk1: int
-    offsets: torch.Tensor    # int32 bit pattern of u32, n_buckets * n_bins + 1
-    post_key: torch.Tensor   # int32 bit pattern of the f32 fragment masses (sorted inside each bucket)
-    post_row: torch.Tensor   # int16 bit pattern of the row inside the bucket
+    offsets: torch.Tensor    # int32 bit pattern of u32, n_bins * n_buckets + 1 (bin-major)
+    postings: torch.Tensor   # int32 bit pattern of (f32 key & (2^bin_shift - 1)) << 16 | row in bucket
 
 
 @dataclass
@@ -97,21 +96,20 @@ class GpuIndex:
             raise ValueError("more than 2^32 - 1 postings")
         v.n_postings = self.n_postings
         self.offsets = torch.empty(n_buckets * n_bins + 1, dtype=torch.int32, device=d)
-        self.post_key = torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d)
-        self.post_row = torch.empty(max(self.n_postings, 1), dtype=torch.int16, device=d)
+        self.postings = torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d)
         wsb = int(L.vh_index_workspace_bytes(self.n_postings, n_buckets, n_bins))
         ws = torch.empty(wsb, dtype=torch.uint8, device=d)
         K.check(L.vh_index_build(C.byref(self.vp), C.byref(v), K.ptr(self.post_off), K.ptr(self.offsets),
-                                 K.ptr(self.post_key), K.ptr(self.post_row), K.ptr(ws), wsb, st), "index_build")
-        v.offsets, v.post_key, v.post_row = K.ptr(self.offsets), K.ptr(self.post_key), K.ptr(self.post_row)
+                                 K.ptr(self.postings), K.ptr(ws), wsb, st), "index_build")
+        v.offsets, v.postings = K.ptr(self.offsets), K.ptr(self.postings)
         torch.cuda.synchronize()
         del ws
-        self.slices = [Slice(0, n_buckets, self.offsets, self.post_key, self.post_row)]
+        self.slices = [Slice(0, n_buckets, self.offsets, self.postings)]
         self.inv_logw = None
         self.build_s = time.perf_counter() - t0
         self.device_bytes = sum(t.numel() * t.element_size() for t in
                                 [self.seq_off, self.length, self.mask, self.mass32, self.residues, self.partner,
-                                 self.offsets, self.post_key, self.post_row])
+                                 self.offsets, self.postings])
 
     def params_for(self, p: SearchParams):
         """VhParams for a SearchParams that shares this index's residue tables (topk / tile / tolerance may
