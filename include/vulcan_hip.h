@@ -79,8 +79,8 @@ typedef struct VhParams {
  * precursor mass, so a bucket is a precursor-mass interval). A fragment's key is the IEEE-754 bit pattern of
  * its f32 mass (order-preserving for positive floats, so the exact value is the key: no lossy quantisation) and
  * its bin is (key >> bin_shift) - bin_base. Postings are ordered bin-major: by (bin, bucket, key, row), so the
- * postings of consecutive buckets in one bin are contiguous. postings[i] = (key & (2^bin_shift - 1)) << 16 |
- * (row - bucket start) (the bin gives the key's high bits). The directory offsets[bin * n_buckets + bucket] =
+ * postings of consecutive buckets in one bin are contiguous. postings[i] = (key & (2^bin_shift - 1)) << 17 |
+ * ion << 16 | (row - bucket start), ion 0 = b, 1 = y (the bin gives the key's high bits). The directory offsets[bin * n_buckets + bucket] =
  * first posting of (bin, bucket); offsets has n_bins * n_buckets + 1 entries (the last = n_postings). */
 typedef struct VhIndexView {
   int64_t n_rows;
@@ -116,11 +116,11 @@ int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, v
                   void* stream);
 /* Scratch bytes of vh_index_build for n_postings keys. */
 size_t vh_index_workspace_bytes(int64_t n_postings, int64_t n_buckets, int32_t n_bins);
-/* Emit 64-bit keys (bin << (16 + bin_shift + bucket bits) | bucket << (16 + bin_shift) | (f32 bits &
- * (2^bin_shift - 1)) << 16 | row in bucket), LSD radix sort them on the device over bits [16, 64) that are used
- * (stable, so equal keys keep row order), write postings (n_postings) and the directory offsets
+/* Emit 64-bit keys (bin << (17 + bin_shift + bucket bits) | bucket << (17 + bin_shift) | (f32 bits &
+ * (2^bin_shift - 1)) << 17 | ion << 16 | row in bucket), LSD radix sort them on the device over the used bits
+ * from 17 up (stable, so equal keys keep emission order), write postings (n_postings) and the directory offsets
  * (n_bins * n_buckets + 1). Replaces vk_frag_emit (kernels.cu:73-101, 401-411) and the torch.sort / bincount /
- * cumsum CSR assembly of engine.py:129-148. Needs bucket_log2 <= 16, bin_shift <= 16 and the key to fit 64 bits. */
+ * cumsum CSR assembly of engine.py:129-148. Needs bucket_log2 <= 16, bin_shift <= 15 and the key to fit 64 bits. */
 int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
                    uint32_t* postings, void* ws, size_t ws_bytes, void* stream);
 
@@ -181,18 +181,19 @@ int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const i
 /* Index top-K with per-candidate features for the learned prefilter (no reference counterpart; the top-K set and
  * order are those of vh_k1_topk without closure, engine.py:284-312 at pair_closure off). Per spectrum s: the
  * topk best (count, row) entries of its hypotheses, deduplicated by row (lowest hypothesis, highest count) and
- * written in row order at s*cand_stride; besides row / hypothesis / count, c_iq[c] = 14-bit pseudo-log2 of
- * (1 + the sum of the quantised intensities of the peak matches counted for that entry) = e << 8 | the 8 bits
- * below the leading one of 1 + sum, where e is the leading bit position; a peak's quantised intensity is
- * trunc(I * 2^(16 - x)) with x the frexp exponent of the spectrum's largest finite peak intensity (0 for
- * non-finite, negative or denormal I). s_maxcnt[s] = the largest count of the spectrum's entries (0 if none).
- * Among duplicates of a row the entry with the larger count (ties: lower hypothesis) gives the intensity. */
+ * written in row order at s*cand_stride. Besides row / hypothesis / count (clamped at 32767): c_nb[c] = the
+ * count's b-ion part (clamped at 32767); c_iq[c] = iq(sum over its b-ion matches) << 16 | iq(sum over its y-ion
+ * matches), where a match adds the peak's quantised intensity trunc(I * 2^(16 - x)) (x = frexp exponent of the
+ * spectrum's largest finite peak intensity; 0 for non-finite, negative or denormal I) and iq(s) = e << 8 | the 8
+ * bits below the leading one of 1 + s (e = its leading bit position), a 14-bit pseudo-log2. s_maxcnt[s] = the
+ * largest count of the spectrum's entries (0 if none). Among duplicates of a row the entry with the larger count
+ * (ties: lower hypothesis) gives the features. */
 int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
                    const int64_t* pk_off, const float* pk_mass, const float* pk_int, const int64_t* hyp_off,
                    const int8_t* h_maxfz, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
                    int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt,
-                   uint16_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored, unsigned long long* hyp_matched,
-                   int32_t* status, void* stream);
+                   int16_t* c_nb, uint32_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored,
+                   unsigned long long* hyp_matched, int32_t* status, void* stream);
 
 /* Brute-force counts for given (hypothesis, row) pairs without the index (k1_brute_kernel,
  * kernels.cu:195-232): the definition K1 must equal. */
@@ -261,19 +262,19 @@ int vh_mlp_adam(float* theta, const float* grad, float* m1, float* m2, int64_t n
                 float beta2, float eps, float c1, float c2, void* stream);
 
 /* Raw features of every candidate slot of vh_k1_features' lists (stride slots per spectrum): out[slot][8] =
- * (count, count / s_maxcnt (0 if s_maxcnt is 0), c_iq / 256, ((h_M - mass32) / mass32) * 1e6,
- * length, charge h_z, isotope h_iso, number of variable modifications = popcount(mask bits 0..62)), f32;
+ * (count, count / s_maxcnt (0 if s_maxcnt is 0), b-ion count c_nb, b-ion intensity pseudo-log (c_iq >> 16) / 256,
+ * y-ion intensity pseudo-log (c_iq & 0xFFFF) / 256, ((h_M - mass32) / mass32) * 1e6, length, charge h_z), f32;
  * zeros for empty slots. */
 int vh_prefilter_features(const VhIndexView* ix, int32_t n_spec, int32_t stride, const int32_t* n_cand,
-                          const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const uint16_t* c_iq,
-                          const int32_t* s_maxcnt, const int8_t* h_z, const int8_t* h_iso, const float* h_M,
+                          const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const int16_t* c_nb,
+                          const uint32_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z, const float* h_M,
                           float* out, void* stream);
 
 /* MLP score of every non-empty candidate slot (features as vh_prefilter_features, computed in registers):
  * score[slot]; empty slots are not written. */
 int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, int32_t stride,
                        const int32_t* n_cand, const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt,
-                       const uint16_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z, const int8_t* h_iso,
+                       const int16_t* c_nb, const uint32_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z,
                        const float* h_M, float* score, void* stream);
 
 /* Per spectrum: the `keep` best slots by (score desc, NaN last; ties: earlier slot), then the reference's pair

@@ -47,7 +47,7 @@ def lib():
         "or_k2": ([PP, PIX, I32, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
         "or_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P], I32),
         "or_search": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, P], I32),
-        "or_topk_feat": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P], I32),
+        "or_topk_feat": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P], I32),
         "or_iq_log": ([C.c_uint64], C.c_uint32),
         "or_iq_quant": ([C.c_float, C.c_float], C.c_uint32),
     }
@@ -185,18 +185,21 @@ def topk(vp, hix: HostIndex, hyp, k1, stride: int):
 
 
 def topk_feat(vp, hix: HostIndex, peaks, hyp, stride: int):
-    """vh_k1_features restated: top-K lists (no closure) with count, intensity pseudo-log2 and best count."""
+    """vh_k1_features restated: top-K lists (no closure) with count, b-ion count, per-ion intensity
+    pseudo-log2s and the best count."""
     L = lib()
     n_spec = len(hyp["hyp_off"]) - 1
     n = max(n_spec, 1)
     o = {"n_cand": np.zeros(n, np.int32), "row": np.zeros(n * stride, np.int32),
          "hyp": np.zeros(n * stride, np.int32), "cnt": np.zeros(n * stride, np.int32),
-         "iq": np.zeros(n * stride, np.uint16), "maxcnt": np.zeros(n, np.int32)}
+         "nb": np.zeros(n * stride, np.int16), "iq": np.zeros(n * stride, np.uint32),
+         "maxcnt": np.zeros(n, np.int32)}
     mode = np.ascontiguousarray(hyp["mode"], np.uint8) if "mode" in hyp else None
     _chk(L.or_topk_feat(C.byref(vp), C.byref(hix.view), n_spec, _p(peaks["off"]), _p(peaks["mass"]),
                         _p(peaks["int"]), _p(hyp["hyp_off"]), _p(np.ascontiguousarray(hyp["maxfz"])), _p(mode),
                         _p(np.ascontiguousarray(hyp["lo"])), _p(np.ascontiguousarray(hyp["hi"])), stride,
-                        _p(o["n_cand"]), _p(o["row"]), _p(o["hyp"]), _p(o["cnt"]), _p(o["iq"]), _p(o["maxcnt"])),
+                        _p(o["n_cand"]), _p(o["row"]), _p(o["hyp"]), _p(o["cnt"]), _p(o["nb"]), _p(o["iq"]),
+                        _p(o["maxcnt"])),
          "topk_feat")
     o["n_cand"] = o["n_cand"][:n_spec]
     o["maxcnt"] = o["maxcnt"][:n_spec]

@@ -34,7 +34,7 @@ def unpack(theta):
     return {k: t[o:o + int(np.prod(sh))].reshape(sh) for k, (o, sh) in OFF.items()}
 
 
-def features(mass32, length, mask, n_cand, row, hyp, cnt, iq, maxcnt, h_z, h_iso, h_M, stride: int):
+def features(mass32, length, n_cand, row, hyp, cnt, nb, iq, maxcnt, h_z, h_M, stride: int):
     """f32 restatement of vh_prefilter_features: [n_spec * stride, 8], zeros in empty slots."""
     n_spec = len(n_cand)
     slots = n_spec * stride
@@ -52,13 +52,13 @@ def features(mass32, length, mask, n_cand, row, hyp, cnt, iq, maxcnt, h_z, h_iso
     f[:, 0] = c
     with np.errstate(divide="ignore", invalid="ignore"):
         f[:, 1] = np.where(mx > 0, c / np.where(mx > 0, mx, np.float32(1)), np.float32(0))
-    f[:, 2] = np.asarray(iq)[:slots][ok].astype(np.float32) * np.float32(0.00390625)
-    f[:, 3] = ((M - m32) / m32) * np.float32(1000000.0)
-    f[:, 4] = np.asarray(length)[r].astype(np.float32)
-    f[:, 5] = np.asarray(h_z)[h].astype(np.float32)
-    f[:, 6] = np.asarray(h_iso)[h].astype(np.float32)
-    mk = np.asarray(mask).view(np.uint64)[r] & np.uint64(0x7FFFFFFFFFFFFFFF)
-    f[:, 7] = np.bitwise_count(mk).astype(np.float32)
+    q = np.asarray(iq).view(np.uint32)[:slots][ok]
+    f[:, 2] = np.asarray(nb)[:slots][ok].astype(np.float32)
+    f[:, 3] = (q >> 16).astype(np.float32) * np.float32(0.00390625)
+    f[:, 4] = (q & 0xFFFF).astype(np.float32) * np.float32(0.00390625)
+    f[:, 5] = ((M - m32) / m32) * np.float32(1000000.0)
+    f[:, 6] = np.asarray(length)[r].astype(np.float32)
+    f[:, 7] = np.asarray(h_z)[h].astype(np.float32)
     out[ok] = f
     return out
 

@@ -410,26 +410,26 @@ uint32_t iq_log(uint64_t sum) {
   return ((uint32_t)e << 8) | m;
 }
 
-// count and intensity sum of one row: the peaks matching fragment f at charge z are the contiguous index range
-// [lower_bound(hi bounds, f), upper_bound(lo bounds, f)) of the mass-sorted peaks
+// per-ion (b = 0, y = 1) counts and intensity sums of one row: the peaks matching fragment f at charge z are the
+// contiguous index range [lower_bound(hi bounds, f), upper_bound(lo bounds, f)) of the mass-sorted peaks
 void k1_feat_row(const Ctx& c, const VhIndexView& ix, int64_t r, int maxfz, const PeakBounds& pb,
-                 const std::vector<uint64_t>& qpre, int* cnt_out, uint64_t* isum_out) {
+                 const std::vector<uint64_t>& qpre, int cnt_out[2], uint64_t isum_out[2]) {
   const VhParams& p = *c.p;
   const uint8_t* sq = ix.residues + ix.seq_off[r];
   int L = ix.length[r];
   uint64_t mk = ix.mask[r];
   float b = c.nterm_of(mk), y = ix.mass32[r] - c.nterm_of(mk);
-  int cnt = 0;
-  uint64_t isum = 0;
-  auto add = [&](float f) {
+  int cnt[2] = {0, 0};
+  uint64_t isum[2] = {0, 0};
+  auto add = [&](float f, int ion) {
     for (int z = 1; z < maxfz; ++z) {
       const float* A = pb.a.data() + (size_t)(z - 1) * pb.np;
       const float* Bv = pb.b.data() + (size_t)(z - 1) * pb.np;
       int ua = (int)(std::upper_bound(A, A + pb.np, f) - A);
       int lb = (int)(std::lower_bound(Bv, Bv + pb.np, f) - Bv);
       if (ua > lb) {
-        cnt += ua - lb;
-        isum += qpre[ua] - qpre[lb];
+        cnt[ion] += ua - lb;
+        isum[ion] += qpre[ua] - qpre[lb];
       }
     }
   };
@@ -437,11 +437,13 @@ void k1_feat_row(const Ctx& c, const VhIndexView& ix, int64_t r, int maxfz, cons
     float rm = c.res(sq[k], k, mk);
     b = b + rm;
     y = y - rm;
-    if (k + 1 > p.min_ion_index && b >= p.fmin && b <= p.fmax) add(b);
-    if (L - 1 - k > p.min_ion_index && y >= p.fmin && y <= p.fmax) add(y);
+    if (k + 1 > p.min_ion_index && b >= p.fmin && b <= p.fmax) add(b, 0);
+    if (L - 1 - k > p.min_ion_index && y >= p.fmin && y <= p.fmax) add(y, 1);
   }
-  *cnt_out = cnt;
-  *isum_out = isum;
+  for (int t = 0; t < 2; ++t) {
+    cnt_out[t] = cnt[t];
+    isum_out[t] = isum[t];
+  }
 }
 
 }  // namespace
@@ -569,12 +571,14 @@ int or_topk(const VhParams* p, const int64_t* partner, int32_t n_spec, const int
 }
 
 // vh_k1_features: per spectrum the top-K (count clamped at 32767 desc, row desc, lower hypothesis first) of the
-// entries with count >= threshold, deduplicated by row (lowest hypothesis; count and intensity of the entry with
-// the larger count, ties: lower hypothesis), row order; c_iq = iq_log(intensity sum), s_maxcnt = best count.
+// entries with count >= threshold, deduplicated by row (lowest hypothesis; features of the entry with the larger
+// count, ties: lower hypothesis), row order; c_nb = b-ion count, c_iq = iq_log(b sum) << 16 | iq_log(y sum),
+// s_maxcnt = best count.
 int or_topk_feat(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int64_t* pk_off,
                  const float* pk_mass, const float* pk_int, const int64_t* hyp_off, const int8_t* h_maxfz,
                  const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi, int32_t stride, int32_t* n_cand,
-                 int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, uint16_t* c_iq, int32_t* s_maxcnt) {
+                 int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, int16_t* c_nb, uint32_t* c_iq,
+                 int32_t* s_maxcnt) {
   if (stride < p->topk) return VH_E_CAPACITY;
   Ctx c(p);
 #pragma omp parallel for schedule(dynamic, 16)
@@ -595,15 +599,18 @@ int or_topk_feat(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const
     for (int64_t h = h0; h < h1; ++h) zmax = std::max(zmax, (int)h_maxfz[h]);
     PeakBounds pb;
     pb.build(c, pm, np, zmax);
-    struct FE { int64_t row; int32_t hl; int32_t cnt; uint64_t isum; };
+    struct FE { int64_t row; int32_t hl; int32_t cnt; int32_t nb; uint32_t iq; };
     std::vector<FE> ents;
     for (int64_t h = h0; h < h1; ++h) {
       int thr = (h_mode && h_mode[h] == VH_MODE_OPEN) ? p->k1_threshold_open : p->k1_threshold;
       for (int64_t r = h_lo[h]; r < h_hi[h]; ++r) {
-        int k;
-        uint64_t is;
-        k1_feat_row(c, *ix, r, h_maxfz[h], pb, qpre, &k, &is);
-        if (k >= thr) ents.push_back({r, (int32_t)(h - h0), std::min(k, 32767), is});
+        int k[2];
+        uint64_t is[2];
+        k1_feat_row(c, *ix, r, h_maxfz[h], pb, qpre, k, is);
+        const int kk = k[0] + k[1];
+        if (kk >= thr)
+          ents.push_back({r, (int32_t)(h - h0), std::min(kk, 32767), std::min(k[0], 32767),
+                          (iq_log(is[0]) << 16) | iq_log(is[1])});
       }
     }
     std::sort(ents.begin(), ents.end(), [](const FE& a, const FE& b) {
@@ -624,14 +631,16 @@ int or_topk_feat(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const
       if (n && c_row[o - 1] == (int32_t)sel[i].row) {
         if (sel[i].cnt > c_cnt[o - 1]) {
           c_cnt[o - 1] = sel[i].cnt;
-          c_iq[o - 1] = (uint16_t)iq_log(sel[i].isum);
+          c_nb[o - 1] = (int16_t)sel[i].nb;
+          c_iq[o - 1] = sel[i].iq;
         }
         continue;
       }
       c_row[o] = (int32_t)sel[i].row;
       c_hyp[o] = (int32_t)(h0 + sel[i].hl);
       c_cnt[o] = sel[i].cnt;
-      c_iq[o] = (uint16_t)iq_log(sel[i].isum);
+      c_nb[o] = (int16_t)sel[i].nb;
+      c_iq[o] = sel[i].iq;
       ++n;
     }
     n_cand[s] = n;

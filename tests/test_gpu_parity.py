@@ -55,8 +55,8 @@ def _np(t):
 
 # ------------------------------------------------------------------------------------------------ index
 def _postings_numpy(l1, vp, gi):
-    """(bucket, f32 bits, row in bucket) of every posting, by f32 accumulation in emission order
-    (np.add/subtract.accumulate are sequential)."""
+    """(bucket, f32 bits, row in bucket, ion 0 = b / 1 = y) of every posting, by f32 accumulation in emission
+    order (np.add/subtract.accumulate are sequential)."""
     rm0 = np.array([np.float32(vp.mono[i]) + np.float32(vp.stat[i]) for i in range(26)], np.float32)
     rm1 = np.array([np.float32(vp.mono[i]) + np.float32(vp.var[i]) for i in range(26)], np.float32)
     out = []
@@ -69,11 +69,11 @@ def _postings_numpy(l1, vp, gi):
         b = np.add.accumulate(rm, dtype=np.float32)
         y = np.subtract.accumulate(np.concatenate([[l1.mass32[i]], rm]).astype(np.float32), dtype=np.float32)[1:]
         for j in range(L - 1):
-            for f, ok in ((b[j], j + 1 > mii), (y[j], L - 1 - j > mii)):
+            for f, ok, ion in ((b[j], j + 1 > mii, 0), (y[j], L - 1 - j > mii, 1)):
                 if ok and fmin <= f <= fmax:
                     fb = int(np.array([f], np.float32).view(np.uint32)[0])
-                    out.append((i >> gi.bucket_log2, fb, i & ((1 << gi.bucket_log2) - 1)))
-    return np.array(out, np.int64).reshape(-1, 3)
+                    out.append((i >> gi.bucket_log2, fb, i & ((1 << gi.bucket_log2) - 1), ion))
+    return np.array(out, np.int64).reshape(-1, 4)
 
 
 def test_index_build_matches_numpy(env0):
@@ -95,7 +95,7 @@ def test_index_build_matches_numpy(env0):
     bins = (post[:, 1] >> S) - gi.bin_base
     o = np.lexsort((post[:, 1], post[:, 0], bins))        # stable: equal (bin, bucket, key) keep emission order
     ps, pb = post[o], bins[o]
-    want_post = ((ps[:, 1] & ((1 << S) - 1)) << 16) | ps[:, 2]
+    want_post = ((ps[:, 1] & ((1 << S) - 1)) << 17) | (ps[:, 3] << 16) | ps[:, 2]
     assert np.array_equal(_np(gi.postings).view(np.uint32)[:len(ps)].astype(np.int64), want_post)
     d = pb * gi.n_buckets + ps[:, 0]
     want = np.searchsorted(d, np.arange(gi.n_bins * gi.n_buckets + 1), side="left")

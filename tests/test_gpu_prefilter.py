@@ -91,9 +91,10 @@ def test_k1_features_equal_oracle(which, request):
     assert m.sum() > 5000
     for dv, ov in ((fl.row, tf["row"]), (fl.hyp, tf["hyp"]), (fl.cnt, tf["cnt"])):
         assert np.array_equal(_np(dv)[:e.n * e.topk][m], ov[:e.n * e.topk][m])
-    iq = _np(fl.iq).view(np.uint16)[:e.n * e.topk][m]
+    iq = _np(fl.iq).view(np.uint32)[:e.n * e.topk][m]
     assert np.array_equal(iq, tf["iq"][:e.n * e.topk][m])
-    assert iq.max() > 0
+    assert np.array_equal(_np(fl.nb)[:e.n * e.topk][m], tf["nb"][:e.n * e.topk][m])
+    assert (iq >> 16).max() > 0 and (iq & 0xFFFF).max() > 0
 
 
 def test_k1_features_lists_equal_fused_topk_without_closure(fenv0):
@@ -114,9 +115,9 @@ def _features_oracle(e):
     P = _P()
     fl = e.fl
     hy = e.ds.hyp
-    return P.features(e.l1.mass32, e.l1.length, e.l1.mask, _np(fl.n_cand)[:e.n], _np(fl.row), _np(fl.hyp),
-                      _np(fl.cnt), _np(fl.iq).view(np.uint16), _np(fl.maxcnt)[:e.n], _np(hy["z"]), _np(hy["iso"]),
-                      _np(hy["M"]), e.topk)
+    return P.features(e.l1.mass32, e.l1.length, _np(fl.n_cand)[:e.n], _np(fl.row), _np(fl.hyp), _np(fl.cnt),
+                      _np(fl.nb), _np(fl.iq).view(np.uint32), _np(fl.maxcnt)[:e.n], _np(hy["z"]), _np(hy["M"]),
+                      e.topk)
 
 
 def test_features_kernel_equals_oracle(fenv0):

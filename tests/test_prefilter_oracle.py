@@ -147,7 +147,7 @@ def test_topk_feat_lists_equal_the_plain_topk_without_closure(feat_env):
     assert np.array_equal(tf["maxcnt"], best)
 
 
-def _ladder(vp, l1, r):
+def _ladder(vp, l1, r, ion):
     rm0 = np.array([np.float32(vp.mono[i]) + np.float32(vp.stat[i]) for i in range(26)], np.float32)
     rm1 = np.array([np.float32(vp.mono[i]) + np.float32(vp.var[i]) for i in range(26)], np.float32)
     seq = l1.residues[l1.seq_off[r]:l1.seq_off[r] + l1.length[r]]
@@ -160,16 +160,16 @@ def _ladder(vp, l1, r):
         rm = (rm1 if (mk >> k) & 1 else rm0)[seq[k] - 65]
         b = np.float32(b + rm)
         y = np.float32(y - rm)
-        if k + 1 > vp.min_ion_index and vp.fmin <= b <= vp.fmax:
+        if ion == 0 and k + 1 > vp.min_ion_index and vp.fmin <= b <= vp.fmax:
             out.append(b)
-        if L - 1 - k > vp.min_ion_index and vp.fmin <= y <= vp.fmax:
+        if ion == 1 and L - 1 - k > vp.min_ion_index and vp.fmin <= y <= vp.fmax:
             out.append(y)
     return np.array(out, np.float32)
 
 
 def test_topk_feat_intensity_sums_by_brute_force(feat_env):
-    """Each entry's c_iq = iq_log(sum over matched (fragment, peak, z) of the peak's quantised intensity),
-    recomputed by brute force over all peaks (Da tolerance)."""
+    """Each entry's b-ion count and per-ion c_iq = iq_log(sum over matched (fragment, peak, z) of the peak's
+    quantised intensity), recomputed by brute force over all peaks (Da tolerance)."""
     O = _O()
     e = feat_env
     vp, pk, hyp, l1 = e["vp"], e["pk"], e["hyp"], e["l1"]
@@ -187,20 +187,22 @@ def test_topk_feat_intensity_sums_by_brute_force(feat_env):
         for j in range(tf["n_cand"][s]):
             o = s * 40 + j
             r, h = int(tf["row"][o]), int(tf["hyp"][o])
-            fr = _ladder(vp, l1, r)
-            cnt, tot = 0, 0
-            for z in range(1, int(hyp["maxfz"][h])):
-                m = (pm * np.float32(z)).astype(np.float32)
-                lo = (m + np.float32(vp.tol_lo)).astype(np.float32)
-                hi = (m + np.float32(vp.tol_hi)).astype(np.float32)
-                hit = (fr[:, None] >= lo[None, :]) & (fr[:, None] <= hi[None, :])
-                cnt += int(hit.sum())
-                tot += int((hit * q[None, :]).sum())
+            cnt, tot = [0, 0], [0, 0]
+            for ion in (0, 1):
+                fr = _ladder(vp, l1, r, ion)
+                for z in range(1, int(hyp["maxfz"][h])):
+                    m = (pm * np.float32(z)).astype(np.float32)
+                    lo = (m + np.float32(vp.tol_lo)).astype(np.float32)
+                    hi = (m + np.float32(vp.tol_hi)).astype(np.float32)
+                    hit = (fr[:, None] >= lo[None, :]) & (fr[:, None] <= hi[None, :])
+                    cnt[ion] += int(hit.sum())
+                    tot[ion] += int((hit * q[None, :]).sum())
             # a row in two hypotheses' windows keeps the lower hypothesis but the larger count (and its intensity)
-            if cnt != tf["cnt"][o]:
+            if sum(cnt) != tf["cnt"][o]:
                 skipped += 1
                 continue
-            assert tf["iq"][o] == O.iq_log(tot), (s, r)
+            assert tf["nb"][o] == cnt[0], (s, r)
+            assert tf["iq"][o] == (O.iq_log(tot[0]) << 16) | O.iq_log(tot[1]), (s, r)
             checked += 1
     assert checked > 200 and skipped <= checked // 20, (checked, skipped)
 
@@ -236,14 +238,14 @@ def test_features_restatement_known_values():
     mass32 = np.array([1000.0, 2000.0], np.float32)
     length = np.array([10, 12], np.uint8)
     mask = np.array([0b101, (1 << 63) | 1], np.uint64)
-    f = P.features(mass32, length, mask, n_cand=np.array([2, 0], np.int32), row=np.array([0, 1, 0, 0], np.int32),
+    f = P.features(mass32, length, n_cand=np.array([2, 0], np.int32), row=np.array([0, 1, 0, 0], np.int32),
                    hyp=np.array([0, 1, 0, 0], np.int32), cnt=np.array([6, 3, 0, 0], np.int32),
-                   iq=np.array([512, 1000, 0, 0], np.uint16), maxcnt=np.array([6, 0], np.int32),
-                   h_z=np.array([2, 3], np.int8), h_iso=np.array([0, 1], np.int8),
-                   h_M=np.array([1000.01, 2000.0], np.float32), stride=2)
+                   nb=np.array([4, 0, 0, 0], np.int16),
+                   iq=np.array([(512 << 16) | 256, 1000, 0, 0], np.uint32), maxcnt=np.array([6, 0], np.int32),
+                   h_z=np.array([2, 3], np.int8), h_M=np.array([1000.01, 2000.0], np.float32), stride=2)
     assert f.shape == (4, 8)
-    assert f[0].tolist()[:3] == [6.0, 1.0, 2.0]
-    assert f[0][3] == pytest.approx(10.0, rel=1e-3)
-    assert f[0][4:].tolist() == [10.0, 2.0, 0.0, 2.0]
-    assert f[1].tolist() == [3.0, 0.5, 1000 / 256, 0.0, 12.0, 3.0, 1.0, 1.0]
+    assert f[0].tolist()[:5] == [6.0, 1.0, 4.0, 2.0, 1.0]
+    assert f[0][5] == pytest.approx(10.0, rel=1e-3)
+    assert f[0][6:].tolist() == [10.0, 2.0]
+    assert f[1].tolist() == [3.0, 0.5, 0.0, 0.0, 1000 / 256, 0.0, 12.0, 3.0]
     assert not f[2:].any()

@@ -3,11 +3,11 @@
 // Replaces frag_count_kernel / frag_emit_kernel (vulcan/gpu/kernels.cu:53-101) and the torch.sort + bincount +
 // cumsum assembly of GpuIndex.__init__ (vulcan/gpu/engine.py:101-148). A fragment's key is its f32 bit pattern
 // (order-preserving for positive floats: exact mass order), its bin the key's high bits. Emitted sort key =
-// bin << (16 + S + nbk) | bucket << (16 + S) | (key & (2^S - 1)) << 16 | row in bucket (S = bin_shift, nbk =
-// bucket bits, bucket = row >> bucket_log2 with rows sorted by precursor mass). Bin-major order puts the postings
-// of consecutive buckets of one bin next to each other, so a K1 lookup over a window's run of buckets reads one
-// directory line and one contiguous posting range. Only the used bits above 16 are sorted (stable LSD, 8-bit
-// digits): equal keys keep emission (row) order.
+// bin << (17 + S + nbk) | bucket << (17 + S) | (key & (2^S - 1)) << 17 | ion << 16 | row in bucket (S =
+// bin_shift, nbk = bucket bits, bucket = row >> bucket_log2 with rows sorted by precursor mass, ion 0 = b,
+// 1 = y). Bin-major order puts the postings of consecutive buckets of one bin next to each other, so a K1 lookup
+// over a window's run of buckets reads one directory line and one contiguous posting range. Only the used bits
+// from 17 up are sorted (stable LSD, 8-bit digits): equal keys keep emission order (row, then ladder order).
 #include <string.h>
 
 #include "vh_common.h"
@@ -42,11 +42,12 @@ struct KeyGeom {
   uint32_t bin_base;
 };
 
-__device__ __forceinline__ uint64_t posting_key(float f, uint64_t bucket, uint32_t local, const KeyGeom& G) {
+__device__ __forceinline__ uint64_t posting_key(float f, uint64_t bucket, uint32_t local, uint32_t ion,
+                                                const KeyGeom& G) {
   const uint32_t k = fbits(f);
   const uint64_t bin = (uint64_t)((k >> G.S) - G.bin_base);
-  return (bin << (16 + G.S + G.nbk)) | (bucket << (16 + G.S)) | ((uint64_t)(k & ((1u << G.S) - 1u)) << 16) |
-         (uint64_t)local;
+  return (bin << (17 + G.S + G.nbk)) | (bucket << (17 + G.S)) | ((uint64_t)(k & ((1u << G.S) - 1u)) << 17) |
+         ((uint64_t)ion << 16) | (uint64_t)local;
 }
 
 __global__ __launch_bounds__(256) void frag_emit_kernel(DevP P, DevIdx X, KeyGeom G, const int64_t* post_off,
@@ -68,8 +69,8 @@ __global__ __launch_bounds__(256) void frag_emit_kernel(DevP P, DevIdx X, KeyGeo
     float r = rm[((mk >> k) & 1ULL) * 26 + (s[k] - 'A')];
     b = b + r;
     y = y - r;
-    if (k + 1 > P.min_ion_index && b >= P.fmin && b <= P.fmax) keys[o++] = posting_key(b, bucket, local, G);
-    if (L - 1 - k > P.min_ion_index && y >= P.fmin && y <= P.fmax) keys[o++] = posting_key(y, bucket, local, G);
+    if (k + 1 > P.min_ion_index && b >= P.fmin && b <= P.fmax) keys[o++] = posting_key(b, bucket, local, 0u, G);
+    if (L - 1 - k > P.min_ion_index && y >= P.fmin && y <= P.fmax) keys[o++] = posting_key(y, bucket, local, 1u, G);
   }
 }
 
@@ -159,8 +160,8 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
 
 // ------------------------------------------------------------------------------------------------ directory
 __device__ __forceinline__ int64_t dir_of(uint64_t key, const KeyGeom& G, const DevIdx& X) {
-  const int64_t bin = (int64_t)(key >> (16 + G.S + G.nbk));
-  const int64_t bucket = (int64_t)((key >> (16 + G.S)) & ((1ULL << G.nbk) - 1ULL));
+  const int64_t bin = (int64_t)(key >> (17 + G.S + G.nbk));
+  const int64_t bucket = (int64_t)((key >> (17 + G.S)) & ((1ULL << G.nbk) - 1ULL));
   return bin * X.n_buckets + bucket;
 }
 
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(256) void directory_kernel(const uint64_t* __restri
   int64_t d = dir_of(kk, G, X);
   int64_t prev = i ? dir_of(keys[i - 1], G, X) : -1;
   for (int64_t b = prev + 1; b <= d; ++b) offsets[b] = (uint32_t)i;
-  postings[i] = (uint32_t)(((kk >> 16) & ((1ULL << G.S) - 1ULL)) << 16) | (uint32_t)(kk & 0xFFFFULL);
+  postings[i] = (uint32_t)(((kk >> 17) & ((1ULL << G.S) - 1ULL)) << 17) | (uint32_t)(kk & 0x1FFFFULL);
   if (i == n - 1)
     for (int64_t b = d + 1; b <= n_dir; ++b) offsets[b] = (uint32_t)n;
 }
@@ -244,7 +245,7 @@ int vh_frag_count(const VhParams* p, const VhIndexView* ix, int64_t* post_off, v
 int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
                    uint32_t* postings, void* ws, size_t ws_bytes, void* stream) {
   if (!p || !ix || !post_off || !offsets || (ix->n_postings && !postings)) return VH_E_ARG;
-  if (ix->bin_shift < 4 || ix->bin_shift > 16 || ix->bucket_log2 < 0 || ix->bucket_log2 > 16) return VH_E_ARG;
+  if (ix->bin_shift < 4 || ix->bin_shift > 15 || ix->bucket_log2 < 0 || ix->bucket_log2 > 16) return VH_E_ARG;
   if (!(p->fmin > 0.f) || !(p->fmax >= p->fmin)) return VH_E_ARG;
   if (ix->n_postings < 0 || ix->n_postings >= (1LL << 32)) return VH_E_LIMIT;
   int64_t n_dir = ix->n_buckets * (int64_t)ix->n_bins;
@@ -261,7 +262,7 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
   while (G.nbk < 32 && ((uint64_t)(ix->n_buckets - 1) >> G.nbk) != 0) ++G.nbk;
   int nbn = 1;
   while (nbn < 32 && ((uint64_t)(ix->n_bins - 1) >> nbn) != 0) ++nbn;
-  if (16 + G.S + G.nbk + nbn > 64) return VH_E_LIMIT;
+  if (17 + G.S + G.nbk + nbn > 64) return VH_E_LIMIT;
   if ((fb0 >> ix->bin_shift) != ix->bin_base || (int64_t)((fb1 >> ix->bin_shift) - ix->bin_base) >= ix->n_bins)
     return VH_E_ARG;
   hipStream_t st = (hipStream_t)stream;
@@ -276,7 +277,7 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
   if (ix->n_rows) frag_emit_kernel<<<grid, 256, 0, st>>>(P, X, G, post_off, ka);
   VH_CHECK_LAUNCH();
   uint64_t* src = ka;
-  int rc = vh_radix_sort_u64(ka, kb2, n, 16, 16 + G.S + G.nbk + nbn, w + 2 * kb, ws_bytes - 2 * kb, st, &src);
+  int rc = vh_radix_sort_u64(ka, kb2, n, 17, 17 + G.S + G.nbk + nbn, w + 2 * kb, ws_bytes - 2 * kb, st, &src);
   if (rc) return rc;
   unsigned g2 = (unsigned)((n > 0 ? n : 1024) + 255) / 256;
   directory_kernel<<<g2, 256, 0, st>>>(src, n, X, G, offsets, postings, n_dir);

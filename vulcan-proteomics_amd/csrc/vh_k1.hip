@@ -17,10 +17,11 @@
 // wave-private top-K buffer (ballot-compacted inserts, pruned by the current K-th key, bitonic compaction when
 // full). Closure and dedupe are one more bitonic sort. Nothing leaves the wave but the final candidate list.
 //
-// Feature mode (MODE 2, the learned prefilter's input): the histogram slots are u64, count in the low 24 bits and
-// the sum of the matching peaks' quantised intensities in the high 40 bits (one LDS u64 atomic per match). The
-// winners carry a 14-bit pseudo-log2 of that sum in the low bits of their top-K key, below the ordering fields, so
-// the top-K set and order are exactly those of MODE 1; no closure.
+// Feature mode (MODE 2, the learned prefilter's input): each slot has two u64 LDS counters, one per ion type
+// (the posting's ion bit), count in the low 24 bits and the sum of the matching peaks' quantised intensities in
+// the high 40 bits (one u64 atomic per match). A winner's per-ion counts and intensity pseudo-logs travel in a
+// payload array beside the top-K key buffer (moved with the keys by the bitonic compaction); the keys themselves
+// are those of MODE 1, so the top-K set and order are exactly MODE 1's; no closure.
 #include "vh_common.h"
 
 #define K1_MAXSEG 32
@@ -57,11 +58,40 @@ __device__ void bitonic_sort_u64(uint64_t* a, int n, bool descending) {
   }
 }
 
-// sort the buffer descending, keep the best K; returns the new fill and updates the pruning threshold
-__device__ __forceinline__ int topk_compact(uint64_t* buf, int capb, int nbuf, int K, uint64_t& thresh) {
+// bitonic sort of keys a (descending) carrying the payloads b along
+__device__ void bitonic_sort_u64_pair(uint64_t* a, uint64_t* b, int n) {
+  const int lane = lane_id();
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < n; i += 64) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          uint64_t x = a[i], y = a[ixj];
+          bool up = (i & k) != 0;
+          if (up ? (x > y) : (x < y)) {
+            a[i] = y;
+            a[ixj] = x;
+            uint64_t t = b[i];
+            b[i] = b[ixj];
+            b[ixj] = t;
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+// sort the buffer descending, keep the best K; returns the new fill and updates the pruning threshold.
+// pay (feature mode): payload array moved with the keys
+__device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int capb, int nbuf, int K,
+                                            uint64_t& thresh) {
   for (int i = nbuf + lane_id(); i < capb; i += 64) buf[i] = 0ULL;
   wave_sync();
-  bitonic_sort_u64(buf, capb, true);
+  if (pay)
+    bitonic_sort_u64_pair(buf, pay, capb);
+  else
+    bitonic_sort_u64(buf, capb, true);
   int n = nbuf < K ? nbuf : K;
   thresh = n == K ? buf[K - 1] : 0ULL;
   wave_sync();
@@ -106,7 +136,8 @@ struct K1Out {
   int wave_bytes, hist_bytes;
   // feature mode
   const float* pk_int;
-  uint16_t* c_iq;
+  int16_t* c_nb;
+  uint32_t* c_iq;
   int32_t* s_maxcnt;
 };
 
@@ -132,6 +163,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   uint32_t* hsc = (uint32_t*)(bks + K1_MAXBK);
   unsigned long long* hmt = (unsigned long long*)(hsc + 64);
   uint64_t* buf = (uint64_t*)(hmt + 64);
+  uint64_t* pay = MODE == 2 ? buf + O.capb : nullptr;   // feature mode payloads beside the keys
 
   const int64_t h0 = hyp_off[s];
   const int nh = (int)(hyp_off[s + 1] - h0);
@@ -237,7 +269,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       }
     }
     if (MODE == 2) {
-      for (int j = lane; j < used; j += 64) hist64[j] = 0ULL;
+      for (int j = lane; j < 2 * used; j += 64) hist64[j] = 0ULL;
     } else {
       for (int j = lane; j < used; j += 64) hist[j] = 0;
     }
@@ -288,7 +320,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
               const int rb = E.bk << LB;
               // one posting: false once the key passes the high bound (the run is key-sorted)
               auto visit = [&](uint32_t w) -> bool {
-                const uint32_t key = khi | (w >> 16);
+                const uint32_t key = khi | (w >> 17);
                 if (key > kb) return false;
                 const int row = rb + (int)(w & 0xFFFFu);
                 if (key < ka || row < E.rlo || row >= E.rhi) return true;
@@ -299,7 +331,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
                   const Seg g = segs[q];
                   if (z < ((g.hl_mfz >> 8) & 0xFF) && row >= g.r0 && row < g.r1) {
                     if (MODE == 2)
-                      atomicAdd(&hist64[g.base + row - g.r0], inc);
+                      atomicAdd(&hist64[2 * (g.base + row - g.r0) + ((w >> 16) & 1u)], inc);
                     else
                       atomicAdd(&hist[g.base + row - g.r0], 1u);
                   }
@@ -331,11 +363,15 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
         const int jd = j0 + lane;          // rows descending: slot j holds row g.r0 + j
         const int j = len - 1 - jd;
         const bool act = jd < len;
-        uint32_t c, qv = 0;
+        uint32_t c;
+        uint64_t pl = 0;
         if (MODE == 2) {
-          const unsigned long long c64 = act ? hist64[g.base + j] : 0ULL;
-          c = (uint32_t)(c64 & 0xFFFFFFULL);
-          qv = iq_log(c64 >> 24);
+          const unsigned long long cb = act ? hist64[2 * (g.base + j)] : 0ULL;
+          const unsigned long long cy = act ? hist64[2 * (g.base + j) + 1] : 0ULL;
+          const uint32_t nb = (uint32_t)(cb & 0xFFFFFFULL);
+          c = nb + (uint32_t)(cy & 0xFFFFFFULL);
+          pl = ((uint64_t)(nb < 32767u ? nb : 32767u) << 28) | ((uint64_t)iq_log(cb >> 24) << 14) |
+               (uint64_t)iq_log(cy >> 24);
         } else {
           c = act ? hist[g.base + j] : 0u;
         }
@@ -362,20 +398,23 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
           if (flag) {
             uint32_t cc = c < 32767u ? c : 32767u;
             clamped |= c > 32767u;
-            key = ((uint64_t)cc << 48) | ((uint64_t)(uint32_t)(g.r0 + j) << 20) | ((uint64_t)(63 - g_hl) << 14) |
-                  (uint64_t)qv;
+            key = ((uint64_t)cc << 48) | ((uint64_t)(uint32_t)(g.r0 + j) << 20) | ((uint64_t)(63 - g_hl) << 14);
           }
           bool ins = flag && key > thresh;
           uint64_t bal = ballot(ins);
           int cnt = __popcll(bal);
           if (cnt) {
             if (nbuf + cnt > O.capb) {
-              nbuf = topk_compact(buf, O.capb, nbuf, K, thresh);
+              nbuf = topk_compact(buf, pay, O.capb, nbuf, K, thresh);
               ins = flag && key > thresh;
               bal = ballot(ins);
               cnt = __popcll(bal);
             }
-            if (ins) buf[nbuf + __popcll(bal & lanemask_lt())] = key;
+            if (ins) {
+              const int at = nbuf + __popcll(bal & lanemask_lt());
+              buf[at] = key;
+              if (MODE == 2) pay[at] = pl;
+            }
             nbuf += cnt;
             wave_sync();
           }
@@ -400,7 +439,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   if (ballot(clamped)) {
     if (lane == 0) atomicOr(O.status, 2);
   }
-  const int ntop = topk_compact(buf, O.capb, nbuf, K, thresh);
+  const int ntop = topk_compact(buf, pay, O.capb, nbuf, K, thresh);
   if (MODE == 2 && lane == 0) O.s_maxcnt[s] = ntop ? (int32_t)(buf[0] >> 48) : 0;
   uint64_t* ck = (uint64_t*)wb;                      // reuses the histogram area
   int32_t* cv = (int32_t*)(ck + O.clos_n);
@@ -411,7 +450,7 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       uint64_t key = buf[i];
       uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
       uint32_t hl = 63u - (uint32_t)((key >> 14) & 63ULL);
-      cv[i] = (int32_t)(key >> 48) | (MODE == 2 ? (int32_t)((key & 0x3FFFULL) << 16) : 0);
+      cv[i] = (int32_t)(key >> 48);
       v = ((uint64_t)row << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
     } else if (i < ncl) {
       uint64_t key = buf[i - ntop];
@@ -432,19 +471,21 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
     bool first = false;
     uint32_t row = 0, hl = 0;
     int32_t cc = -1;
+    uint32_t best = 0;
     if (i < ncl) {
       uint64_t v = ck[i];
       row = (uint32_t)(v >> 32);
       hl = (uint32_t)((v >> 24) & 0xFFu);
       first = i == 0 || (uint32_t)(ck[i - 1] >> 32) != row;
       if (first) {
-        cc = cv[v & 0xFFFFFFu];
+        best = (uint32_t)(v & 0xFFFFFFu);
+        cc = cv[best];
         for (int j = i + 1; j < ncl && (uint32_t)(ck[j] >> 32) == row; ++j) {
-          int32_t c2 = cv[ck[j] & 0xFFFFFFu];
-          if (MODE == 2) {   // the duplicate with the larger count (ties: lower hypothesis) gives the payload
-            cc = (c2 & 0xFFFF) > (cc & 0xFFFF) ? c2 : cc;
-          } else {
-            cc = c2 > cc ? c2 : cc;
+          const uint32_t q = (uint32_t)(ck[j] & 0xFFFFFFu);
+          // the duplicate with the larger count (ties: lower hypothesis) gives the feature payload
+          if (cv[q] > cc) {
+            cc = cv[q];
+            best = q;
           }
         }
       }
@@ -454,11 +495,11 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
       int pos = nout + __popcll(bal & lanemask_lt());
       O.c_row[ob + pos] = (int32_t)row;
       O.c_hyp[ob + pos] = (int32_t)(h0 + hl);
+      O.c_cnt[ob + pos] = cc;
       if (MODE == 2) {
-        O.c_cnt[ob + pos] = cc & 0xFFFF;
-        O.c_iq[ob + pos] = (uint16_t)((uint32_t)cc >> 16);
-      } else {
-        O.c_cnt[ob + pos] = cc;
+        const uint64_t pv = pay[best];
+        O.c_nb[ob + pos] = (int16_t)((pv >> 28) & 0x7FFFULL);
+        O.c_iq[ob + pos] = (uint32_t)(((pv >> 14) & 0x3FFFULL) << 16) | (uint32_t)(pv & 0x3FFFULL);
       }
     }
     nout += __popcll(bal);
@@ -530,11 +571,11 @@ static void k1_geometry(const VhParams* p, int mode, K1Out& O, int& waves) {
   int K = p->topk > 0 ? p->topk : 1;
   O.capb = mode ? pow2_at_least(K + 64) : 0;
   O.clos_n = mode ? pow2_at_least((2 * K) > 64 ? 2 * K : 64) : 0;
-  int hb = p->tile * (mode == 2 ? 8 : 4);
+  int hb = p->tile * (mode == 2 ? 16 : 4);
   int cb = O.clos_n * 12;
   O.hist_bytes = ((hb > cb ? hb : cb) + 15) & ~15;
   O.wave_bytes = O.hist_bytes + K1_MAXSEG * (int)sizeof(Seg) + K1_MAXBK * (int)sizeof(BkEnt) + 64 * 4 + 64 * 8 +
-                 O.capb * 8;
+                 O.capb * (mode == 2 ? 16 : 8);
   waves = 65536 / O.wave_bytes;
   waves = waves < 1 ? 1 : (waves > 4 ? 4 : waves);
 }
@@ -608,12 +649,12 @@ int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, con
                    const int64_t* pk_off, const float* pk_mass, const float* pk_int, const int64_t* hyp_off,
                    const int8_t* h_maxfz, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
                    int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt,
-                   uint16_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored, unsigned long long* hyp_matched,
-                   int32_t* status, void* stream) {
+                   int16_t* c_nb, uint32_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored,
+                   unsigned long long* hyp_matched, int32_t* status, void* stream) {
   int rc = k1_check(p, ix);
   if (rc) return rc;
-  if (n_spec < 0 || !n_cand || !c_row || !c_hyp || !c_cnt || !c_iq || !s_maxcnt || !hyp_scored || !hyp_matched ||
-      !status || !pk_int)
+  if (n_spec < 0 || !n_cand || !c_row || !c_hyp || !c_cnt || !c_nb || !c_iq || !s_maxcnt || !hyp_scored ||
+      !hyp_matched || !status || !pk_int)
     return VH_E_ARG;
   if (p->topk < 1 || p->topk > 1024) return VH_E_ARG;
   if (cand_stride < p->topk) return VH_E_CAPACITY;
@@ -629,6 +670,7 @@ int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, con
   O.hyp_matched = hyp_matched;
   O.status = status;
   O.pk_int = pk_int;
+  O.c_nb = c_nb;
   O.c_iq = c_iq;
   O.s_maxcnt = s_maxcnt;
   int waves;

@@ -72,8 +72,9 @@ __device__ __forceinline__ f32x4 mfma_bf16(u16x8 a, u16x8 b, f32x4 c) {
 struct FeatIn {
   int32_t stride;
   const int32_t *n_cand, *c_row, *c_hyp, *c_cnt, *s_maxcnt;
-  const uint16_t* c_iq;
-  const int8_t *h_z, *h_iso;
+  const int16_t* c_nb;
+  const uint32_t* c_iq;
+  const int8_t* h_z;
   const float *h_M, *mass32;
   const uint8_t* length;
   const uint64_t* mask;
@@ -86,14 +87,15 @@ __device__ __forceinline__ bool cand_features(const FeatIn& F, int64_t c, float*
   if (i >= F.n_cand[s]) return false;
   const int row = F.c_row[c], h = F.c_hyp[c], cnt = F.c_cnt[c], mx = F.s_maxcnt[s];
   const float m32 = F.mass32[row], M = F.h_M[h];
+  const uint32_t iq = F.c_iq[c];
   f[0] = (float)cnt;
   f[1] = mx > 0 ? (float)cnt / (float)mx : 0.f;
-  f[2] = (float)F.c_iq[c] * 0.00390625f;
-  f[3] = ((M - m32) / m32) * 1000000.0f;
-  f[4] = (float)F.length[row];
-  f[5] = (float)F.h_z[h];
-  f[6] = (float)F.h_iso[h];
-  f[7] = (float)__popcll(F.mask[row] & 0x7FFFFFFFFFFFFFFFULL);
+  f[2] = (float)F.c_nb[c];
+  f[3] = (float)(iq >> 16) * 0.00390625f;
+  f[4] = (float)(iq & 0xFFFFu) * 0.00390625f;
+  f[5] = ((M - m32) / m32) * 1000000.0f;
+  f[6] = (float)F.length[row];
+  f[7] = (float)F.h_z[h];
   return true;
 }
 
@@ -597,12 +599,13 @@ static int allow_big_lds() {
 }
 
 static FeatIn make_feat(const VhIndexView* ix, int32_t stride, const int32_t* n_cand, const int32_t* c_row,
-                        const int32_t* c_hyp, const int32_t* c_cnt, const uint16_t* c_iq, const int32_t* s_maxcnt,
-                        const int8_t* h_z, const int8_t* h_iso, const float* h_M) {
+                        const int32_t* c_hyp, const int32_t* c_cnt, const int16_t* c_nb, const uint32_t* c_iq,
+                        const int32_t* s_maxcnt, const int8_t* h_z, const float* h_M) {
   FeatIn F;
   F.stride = stride;
-  F.n_cand = n_cand; F.c_row = c_row; F.c_hyp = c_hyp; F.c_cnt = c_cnt; F.s_maxcnt = s_maxcnt; F.c_iq = c_iq;
-  F.h_z = h_z; F.h_iso = h_iso; F.h_M = h_M;
+  F.n_cand = n_cand; F.c_row = c_row; F.c_hyp = c_hyp; F.c_cnt = c_cnt; F.s_maxcnt = s_maxcnt;
+  F.c_nb = c_nb; F.c_iq = c_iq;
+  F.h_z = h_z; F.h_M = h_M;
   F.mass32 = ix->mass32; F.length = ix->length; F.mask = (const uint64_t*)ix->mask;
   return F;
 }
@@ -708,14 +711,14 @@ int vh_mlp_adam(float* theta, const float* grad, float* m1, float* m2, int64_t n
 }
 
 int vh_prefilter_features(const VhIndexView* ix, int32_t n_spec, int32_t stride, const int32_t* n_cand,
-                          const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const uint16_t* c_iq,
-                          const int32_t* s_maxcnt, const int8_t* h_z, const int8_t* h_iso, const float* h_M,
+                          const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const int16_t* c_nb,
+                          const uint32_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z, const float* h_M,
                           float* out, void* stream) {
   if (!ix || n_spec < 0 || stride < 1 || !out) return VH_E_ARG;
   if (n_spec == 0) return 0;
-  if (!n_cand || !c_row || !c_hyp || !c_cnt || !c_iq || !s_maxcnt || !h_z || !h_iso || !h_M) return VH_E_ARG;
+  if (!n_cand || !c_row || !c_hyp || !c_cnt || !c_nb || !c_iq || !s_maxcnt || !h_z || !h_M) return VH_E_ARG;
   const int64_t slots = (int64_t)n_spec * stride;
-  FeatIn F = make_feat(ix, stride, n_cand, c_row, c_hyp, c_cnt, c_iq, s_maxcnt, h_z, h_iso, h_M);
+  FeatIn F = make_feat(ix, stride, n_cand, c_row, c_hyp, c_cnt, c_nb, c_iq, s_maxcnt, h_z, h_M);
   features_kernel<<<(unsigned)((slots + 255) / 256), 256, 0, (hipStream_t)stream>>>(F, slots, out);
   VH_CHECK_LAUNCH();
   return 0;
@@ -723,13 +726,13 @@ int vh_prefilter_features(const VhIndexView* ix, int32_t n_spec, int32_t stride,
 
 int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, int32_t stride,
                        const int32_t* n_cand, const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt,
-                       const uint16_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z, const int8_t* h_iso,
+                       const int16_t* c_nb, const uint32_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z,
                        const float* h_M, float* score, void* stream) {
   if (!ix || !blob || n_spec < 0 || stride < 1 || !score) return VH_E_ARG;
   if (n_spec == 0) return 0;
-  if (!n_cand || !c_row || !c_hyp || !c_cnt || !c_iq || !s_maxcnt || !h_z || !h_iso || !h_M) return VH_E_ARG;
+  if (!n_cand || !c_row || !c_hyp || !c_cnt || !c_nb || !c_iq || !s_maxcnt || !h_z || !h_M) return VH_E_ARG;
   const int64_t slots = (int64_t)n_spec * stride;
-  FeatIn F = make_feat(ix, stride, n_cand, c_row, c_hyp, c_cnt, c_iq, s_maxcnt, h_z, h_iso, h_M);
+  FeatIn F = make_feat(ix, stride, n_cand, c_row, c_hyp, c_cnt, c_nb, c_iq, s_maxcnt, h_z, h_M);
   if (allow_big_lds()) return VH_E_LIMIT;
   const int g = mlp_grid((slots + 16 * MLP_NB - 1) / (16 * MLP_NB));
   prefilter_score_kernel<<<g, 64 * MLP_WAVES, BL_LDS, (hipStream_t)stream>>>((const unsigned char*)blob, F, slots,

@@ -37,7 +37,7 @@ _SIGS = {
     "vh_k2": ([PP, PIX, I32, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_candidate_shift": ([I32, P, P, I32, P, P, P, P, P, P, P], I32),
     "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
-    "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_mlp_pack": ([P, P, P, P, P], I32),
     "vh_mlp_forward": ([P, I64, P, P, P, P, P], I32),
     "vh_mlp_workspace_bytes": ([I64], SZ),

@@ -59,7 +59,7 @@ class SearchParams:
     def bin_shift(self) -> int:
         """f32 bit-pattern bin: key >> shift; a bin spans 2^(shift-23) relative (shift 7 = 15.3 ppm)."""
         s = int(math.floor(23 + math.log2(self.bin_ppm * 1e-6)))
-        return max(4, min(16, s))
+        return max(4, min(15, s))
 
     def maxfz_of(self, z):
         mfc = self.max_fragment_charge

@@ -54,11 +54,15 @@ class Workload:
     cfg: int = 0
     index_cfg: int = 0
     extra: dict = field(default_factory=dict)
+    geometry: dict = field(default_factory=dict)   # index / K1 geometry tuned for this scale (chunk_size, ...)
 
     def search_params(self) -> SearchParams:
-        return SearchParams(precursor_ppm=(-10.0, 10.0), fragment_ppm=(-0.02, 0.02), fragment_unit="da",
-                            max_fragment_charge=self.max_fragment_charge, topk=self.topk,
-                            precursor_charge=(min(self.charges), max(self.charges)))
+        p = SearchParams(precursor_ppm=(-10.0, 10.0), fragment_ppm=(-0.02, 0.02), fragment_unit="da",
+                         max_fragment_charge=self.max_fragment_charge, topk=self.topk,
+                         precursor_charge=(min(self.charges), max(self.charges)))
+        for k, v in self.geometry.items():
+            setattr(p, k, v)
+        return p
 
     def index_params(self) -> dict:
         return {"enzyme": {"cleave_at": "KR", "restrict": "P", "c_terminal": True, "missed_cleavages": self.missed,
@@ -78,7 +82,7 @@ WORKLOADS = {
     4: Workload("cfg4-phospho-prefilter", 20_000, 1, {"S": PHOSPHO, "T": PHOSPHO, "Y": PHOSPHO, "M": OX}, 5,
                 {"S": 3, "T": 3, "Y": 3, "M": 2}, 200_000, 2, charges=(2, 3, 4, 5, 6),
                 charge_p=(0.4, 0.3, 0.15, 0.1, 0.05), n_runs=16, peak_range=(20, 500), calib_ppm=5.0, topk=200,
-                cfg=4, index_cfg=4),
+                cfg=4, index_cfg=4, geometry={"chunk_size": 1024, "bin_ppm": 320.0, "tile": 1024}),
 }
 
 

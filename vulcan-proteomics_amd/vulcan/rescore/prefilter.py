@@ -34,8 +34,8 @@ from ..gpu.host import SearchParams
 
 N_PARAMS = 68353
 BLOB_BYTES = 281728
-FEATURES = ("k1_count", "count_over_best", "log2_matched_intensity", "precursor_ppm", "length", "charge",
-            "isotope_error", "n_var")
+FEATURES = ("k1_count", "count_over_best", "b_ion_count", "log2_b_intensity", "log2_y_intensity", "precursor_ppm",
+            "length", "charge")
 _OFF = {"W1": (0, (256, 8)), "b1": (2048, (256,)), "W2": (2304, (256, 256)), "b2": (67840, (256,)),
         "w3": (68096, (256,)), "b3": (68352, (1,))}
 
@@ -184,7 +184,8 @@ class FeatureLists:
         e = lambda m, dt: torch.empty(m, dtype=dt, device=dev)  # noqa: E731
         self.n_cand = e(n, torch.int32)
         self.row, self.hyp, self.cnt = e(n * topk, torch.int32), e(n * topk, torch.int32), e(n * topk, torch.int32)
-        self.iq = e(n * topk, torch.int16)
+        self.nb = e(n * topk, torch.int16)
+        self.iq = e(n * topk, torch.int32)
         self.maxcnt = e(n, torch.int32)
         self.score = e(n * topk, torch.float32)
 
@@ -194,23 +195,24 @@ class FeatureLists:
                                        K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(hy["hyp_off"]),
                                        K.ptr(hy["maxfz"]), K.ptr(hy["mode"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
                                        self.stride, K.ptr(self.n_cand), K.ptr(self.row), K.ptr(self.hyp),
-                                       K.ptr(self.cnt), K.ptr(self.iq), K.ptr(self.maxcnt), K.ptr(ds.scored),
+                                       K.ptr(self.cnt), K.ptr(self.nb), K.ptr(self.iq), K.ptr(self.maxcnt),
+                                       K.ptr(ds.scored),
                                        K.ptr(ds.matched), K.ptr(ds.status), K.stream()), "k1_features")
 
     def features(self, ds: DeviceSearch, n: int) -> torch.Tensor:
         out = torch.empty((max(n, 1) * self.stride, 8), dtype=torch.float32, device=ds.gi.dev)
         hy = ds.hyp
         K.check(K.lib().vh_prefilter_features(C.byref(ds.gi.view), n, self.stride, K.ptr(self.n_cand),
-                                              K.ptr(self.row), K.ptr(self.hyp), K.ptr(self.cnt), K.ptr(self.iq),
-                                              K.ptr(self.maxcnt), K.ptr(hy["z"]), K.ptr(hy["iso"]), K.ptr(hy["M"]),
+                                              K.ptr(self.row), K.ptr(self.hyp), K.ptr(self.cnt), K.ptr(self.nb),
+                                              K.ptr(self.iq), K.ptr(self.maxcnt), K.ptr(hy["z"]), K.ptr(hy["M"]),
                                               K.ptr(out), K.stream()), "prefilter_features")
         return out[: n * self.stride]
 
     def score_with(self, ds: DeviceSearch, blob: torch.Tensor, n: int):
         hy = ds.hyp
         K.check(K.lib().vh_prefilter_score(C.byref(ds.gi.view), K.ptr(blob), n, self.stride, K.ptr(self.n_cand),
-                                           K.ptr(self.row), K.ptr(self.hyp), K.ptr(self.cnt), K.ptr(self.iq),
-                                           K.ptr(self.maxcnt), K.ptr(hy["z"]), K.ptr(hy["iso"]), K.ptr(hy["M"]),
+                                           K.ptr(self.row), K.ptr(self.hyp), K.ptr(self.cnt), K.ptr(self.nb),
+                                           K.ptr(self.iq), K.ptr(self.maxcnt), K.ptr(hy["z"]), K.ptr(hy["M"]),
                                            K.ptr(self.score), K.stream()), "prefilter_score")
 
 
