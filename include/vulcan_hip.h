@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 8
+#define VH_ABI_VERSION 9
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -152,6 +152,14 @@ int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz,
 size_t vh_spectrum_order_workspace_bytes(int32_t n_spec);
 int vh_spectrum_order(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo, int32_t* order, void* ws,
                       size_t ws_bytes, void* stream);
+/* Work-ordered variant for uneven batches: spectra by (band = first window row >> band_log2 ascending, estimated
+ * K1 work descending, spectrum), with work = n_peaks * sum_h max(maxfz_h - 1, 1) + (window rows) / 4 saturated
+ * at 2^20 - 1. band_log2 >= 28 gives one band (pure longest-first order, the classic list-scheduling rule for a
+ * persistent launch); smaller values keep precursor-mass locality. n_spec < 2^24. Same workspace as
+ * vh_spectrum_order. No reference counterpart. */
+int vh_spectrum_order_work(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo, const int64_t* h_hi,
+                           const int8_t* h_maxfz, const int64_t* pk_off, int32_t band_log2, int32_t* order, void* ws,
+                           size_t ws_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------------- K1 */
 /* Candidate generation, append form (run_k1, engine.py:250-281; k1_kernel, kernels.cu:112-190): for every
@@ -171,12 +179,15 @@ int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_
  * the top-K (count desc, row desc, hypothesis asc) of its appended entries, their partner rows (count -1),
  * deduplicated by row (lowest hypothesis, highest count), written in row order at s*cand_stride.
  * n_cand[s] = number of candidates (<= cand_stride). status[0] is set nonzero if a spectrum exceeded a
- * limit (its list is then empty). */
+ * limit (its list is then empty). queue (NULL = grid launch, one wave per position of `order`): a device u32
+ * that makes the launch persistent -- as many waves as stay resident loop over `order` through this atomic
+ * counter (reset by the call itself on the stream), which keeps waves busy when spectra differ widely in work
+ * (uneven batches). Results do not depend on the mode. */
 int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
                const int64_t* pk_off, const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz,
-               const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp,
-               int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
-               void* stream);
+               const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi, int32_t cand_stride,
+               int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, uint32_t* hyp_scored,
+               unsigned long long* hyp_matched, int32_t* status, uint32_t* queue, void* stream);
 
 /* Index top-K with per-candidate features for the learned prefilter (no reference counterpart; the top-K set and
  * order are those of vh_k1_topk without closure, engine.py:284-312 at pair_closure off). Per spectrum s: the
@@ -193,7 +204,7 @@ int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, con
                    const int8_t* h_maxfz, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
                    int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt,
                    int16_t* c_nb, uint32_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored,
-                   unsigned long long* hyp_matched, int32_t* status, void* stream);
+                   unsigned long long* hyp_matched, int32_t* status, uint32_t* queue, void* stream);
 
 /* Brute-force counts for given (hypothesis, row) pairs without the index (k1_brute_kernel,
  * kernels.cu:195-232): the definition K1 must equal. */
@@ -211,12 +222,12 @@ int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32
  * c_shift (NULL = closed search) = observed - candidate mass: a candidate with a nonzero shift scores shifted +
  * unshifted ions and the delta is placed on the residue that maximises the hyperscore (vk_k2_shift,
  * kernels.cu:291-363, 461-472; peptides up to 63 residues). One wave64 per group; `order` (NULL =
- * 0..n_groups-1) is the processing order of the groups. */
+ * 0..n_groups-1) is the processing order of the groups; queue as in vh_k1_topk (persistent launch). */
 int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* order, const int32_t* grp_spec,
           const int64_t* grp_start, const int32_t* grp_count, int32_t stride, const int32_t* c_row,
           const int32_t* c_hyp, const int8_t* h_maxfz, const float* c_shift, const int64_t* pk_off,
           const float* pk_mass, const float* pk_int, double* o_hs, float* o_sb, float* o_sy, float* o_ppm,
-          int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, void* stream);
+          int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, uint32_t* queue, void* stream);
 
 /* Per-candidate precursor delta for vh_k2's shifted scoring: c_shift[c] = h_M[h] - mass32[row] when h is an
  * offset or open hypothesis, 0 for a closed one (search_modes, vulcan/ptm/search.py:187-189). Groups as in

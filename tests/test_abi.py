@@ -50,9 +50,9 @@ def test_argument_errors_are_reported_without_a_gpu():
     vp = make_params(SearchParams(), {}, {})
     ix = VhIndexView()
     assert L.vh_k1_topk(C.byref(vp), C.byref(ix), 1, None, None, None, None, None, None, None, None, 2, None, None,
-                        None, None, None, None, None, None) == -1
+                        None, None, None, None, None, None, None) == -1
     assert L.vh_k2(C.byref(vp), C.byref(ix), 1, None, None, None, None, 0, None, None, None, None, None, None, None,
-                   None, None, None, None, None, None, None, None, None) == -1
+                   None, None, None, None, None, None, None, None, None, None) == -1
     assert L.vh_spectrum_order(3, None, None, None, None, 0, None) == -1
     assert L.vh_mlp_pack(None, None, None, None, None) == -1
     assert L.vh_mlp_forward(None, 10, None, None, None, None, None) == -1
@@ -64,7 +64,7 @@ def test_argument_errors_are_reported_without_a_gpu():
     ixp = VhIndexView()
     ixp.partner = 8
     assert L.vh_prefilter_select(C.byref(ixp), 1, 200, *[None] * 5, 16, 1, 31, *[None] * 4, None) == -2
-    assert L.vh_k1_features(C.byref(vp), C.byref(ix), 1, *[None] * 9, 10, *[None] * 10, None) == -1
+    assert L.vh_k1_features(C.byref(vp), C.byref(ix), 1, *[None] * 9, 10, *[None] * 11, None) == -1
     vp.max_peaks = 4096
     assert L.vh_k0(C.byref(vp), 1, None, None, C.c_void_p(8), C.c_void_p(8), None, None, C.c_void_p(8),
                    C.c_void_p(8), 10, None, 0, None) == -4

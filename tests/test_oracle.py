@@ -448,3 +448,24 @@ def test_open_search_oracle_finds_shifted_spectra():
     assert len(got) > 20 and np.mean(got) > 0.5
     hit = [rows[sel[s]] == truth[s] for s in shifted if sel[s] >= 0]
     assert np.mean(hit) > 0.5
+
+
+def test_spectrum_classes_bins():
+    """Class = charge bin (2, 3, 4, 5+) x kept-peak bin (< 64, 64-149, >= 150), kept = peaks inside
+    [fmin, fmax] capped at max_peaks, as K0 keeps them."""
+    import numpy as np
+    from vulcan.gpu.host import SearchParams, Spectra
+    from vulcan.gpu.schedule import spectrum_classes
+    p = SearchParams(max_peaks=150)
+    counts = [10, 63, 64, 149, 150, 400]
+    charges = [2, 3, 4, 5, 6, 0]
+    mz, off = [], [0]
+    for c in counts:
+        mz += [500.0] * c + [100.0, 2500.0, float("nan")]     # three peaks K0 drops
+        off.append(len(mz))
+    n = len(counts)
+    sp = Spectra("t", np.arange(n, dtype=np.uint32), np.arange(n, dtype=np.uint32), np.zeros(n),
+                 np.full(n, 500.0), np.array(charges, np.int8), np.array(mz, np.float32),
+                 np.ones(len(mz), np.float32), np.array(off, np.int64))
+    cls = spectrum_classes(sp, p)
+    assert cls.tolist() == [0 * 3 + 0, 1 * 3 + 0, 2 * 3 + 1, 3 * 3 + 1, 3 * 3 + 2, 0 * 3 + 2]

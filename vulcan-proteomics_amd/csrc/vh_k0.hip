@@ -289,10 +289,33 @@ __global__ __launch_bounds__(256) void order_keys_kernel(int n_spec, const int64
   keys[s] = (lo << 32) | (uint64_t)(uint32_t)s;
 }
 
-__global__ __launch_bounds__(256) void order_out_kernel(int n_spec, const uint64_t* __restrict__ keys,
+// work-ordered key: mass band of the first window, then estimated K1 work descending, then the spectrum
+__global__ __launch_bounds__(256) void order_work_keys_kernel(int n_spec, const int64_t* __restrict__ hyp_off,
+                                                              const int64_t* __restrict__ h_lo,
+                                                              const int64_t* __restrict__ h_hi,
+                                                              const int8_t* __restrict__ h_maxfz,
+                                                              const int64_t* __restrict__ pk_off, int band_log2,
+                                                              uint64_t* keys) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_spec) return;
+  const int64_t h0 = hyp_off[s], h1 = hyp_off[s + 1];
+  const int64_t np = pk_off[s + 1] - pk_off[s];
+  int64_t zsum = 0, rows = 0;
+  for (int64_t h = h0; h < h1; ++h) {
+    zsum += h_maxfz[h] > 1 ? h_maxfz[h] - 1 : 1;
+    rows += h_hi[h] - h_lo[h];
+  }
+  int64_t est = np * zsum + rows / 4;
+  est = est < 0xFFFFF ? est : 0xFFFFF;
+  const uint64_t band = h1 > h0 ? ((uint64_t)h_lo[h0] >> band_log2) : 0xFFFFFULL;
+  keys[s] = ((band < 0xFFFFFULL ? band : 0xFFFFFULL) << 44) | ((uint64_t)(0xFFFFF - est) << 24) | (uint64_t)s;
+}
+
+// the spectrum sits in the low `bits` bits of a sorted key (32 for the mass order, 24 for the work order)
+__global__ __launch_bounds__(256) void order_out_kernel(int n_spec, const uint64_t* __restrict__ keys, int bits,
                                                         int32_t* order) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_spec) order[i] = (int32_t)(uint32_t)keys[i];
+  if (i < n_spec) order[i] = (int32_t)(keys[i] & ((1ULL << bits) - 1ULL));
 }
 
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -319,7 +342,30 @@ int vh_spectrum_order(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_l
   uint64_t* res = a;
   int rc = vh_radix_sort_u64(a, b, n_spec, 32, 60, (char*)ws + 2 * kb, ws_bytes - 2 * kb, st, &res);
   if (rc) return rc;
-  order_out_kernel<<<g, 256, 0, st>>>(n_spec, res, order);
+  order_out_kernel<<<g, 256, 0, st>>>(n_spec, res, 32, order);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_spectrum_order_work(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo, const int64_t* h_hi,
+                           const int8_t* h_maxfz, const int64_t* pk_off, int32_t band_log2, int32_t* order, void* ws,
+                           size_t ws_bytes, void* stream) {
+  if (n_spec < 0 || band_log2 < 0 || band_log2 > 40) return VH_E_ARG;
+  if (n_spec && (!hyp_off || !h_lo || !h_hi || !h_maxfz || !pk_off || !order || !ws)) return VH_E_ARG;
+  if (n_spec == 0) return 0;
+  if (n_spec >= (1 << 24)) return VH_E_LIMIT;
+  if (ws_bytes < vh_spectrum_order_workspace_bytes(n_spec)) return VH_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  size_t kb = a256((size_t)n_spec * 8);
+  uint64_t* a = (uint64_t*)ws;
+  uint64_t* b = (uint64_t*)((char*)ws + kb);
+  unsigned g = (unsigned)((n_spec + 255) / 256);
+  order_work_keys_kernel<<<g, 256, 0, st>>>(n_spec, hyp_off, h_lo, h_hi, h_maxfz, pk_off, band_log2, a);
+  VH_CHECK_LAUNCH();
+  uint64_t* res = a;
+  int rc = vh_radix_sort_u64(a, b, n_spec, 24, 64, (char*)ws + 2 * kb, ws_bytes - 2 * kb, st, &res);
+  if (rc) return rc;
+  order_out_kernel<<<g, 256, 0, st>>>(n_spec, res, 24, order);
   VH_CHECK_LAUNCH();
   return 0;
 }

@@ -134,6 +134,7 @@ struct K1Out {
   unsigned long long* hyp_matched;
   int32_t* status;
   int wave_bytes, hist_bytes;
+  uint32_t* queue;   // persistent mode work counter (NULL: grid mode)
   // feature mode
   const float* pk_int;
   int16_t* c_nb;
@@ -141,21 +142,14 @@ struct K1Out {
   int32_t* s_maxcnt;
 };
 
+// one spectrum s, processed by the calling wave in its LDS area wb
 template <int MODE>
-__global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, const int32_t* __restrict__ order,
-                                                 const int64_t* __restrict__ pk_off,
-                                                 const float* __restrict__ pk_mass,
-                                                 const int64_t* __restrict__ hyp_off,
-                                                 const int8_t* __restrict__ h_maxfz,
-                                                 const uint8_t* __restrict__ h_mode,
-                                                 const int64_t* __restrict__ h_lo,
-                                                 const int64_t* __restrict__ h_hi, K1Out O) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int w = threadIdx.x >> 6, lane = lane_id();
-  const int wv = xcd_band_position(P.xcd_group) * (blockDim.x >> 6) + w;
-  if (wv >= n_spec) return;   // a whole wave leaves; no block-level barrier is used below
-  const int s = order ? order[wv] : wv;
-  unsigned char* wb = smem + (size_t)w * O.wave_bytes;
+__device__ void k1_spectrum(const DevP& P, const DevIdx& X, int s, unsigned char* wb,
+                            const int64_t* __restrict__ pk_off, const float* __restrict__ pk_mass,
+                            const int64_t* __restrict__ hyp_off, const int8_t* __restrict__ h_maxfz,
+                            const uint8_t* __restrict__ h_mode, const int64_t* __restrict__ h_lo,
+                            const int64_t* __restrict__ h_hi, const K1Out& O) {
+  const int lane = lane_id();
   uint32_t* hist = (uint32_t*)wb;
   unsigned long long* hist64 = (unsigned long long*)wb;   // feature mode
   Seg* segs = (Seg*)(wb + O.hist_bytes);
@@ -507,6 +501,36 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   if (lane == 0) O.n_cand[s] = nout;
 }
 
+// Grid mode: wave wv takes order[wv] (XCD-banded block order). Persistent mode (O.queue set): every wave takes the
+// next position of `order` from an atomic counter until the list is exhausted, so a wave that finishes a short
+// spectrum immediately starts another instead of idling until its block's slowest wave is done.
+template <int MODE>
+__global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, const int32_t* __restrict__ order,
+                                                 const int64_t* __restrict__ pk_off,
+                                                 const float* __restrict__ pk_mass,
+                                                 const int64_t* __restrict__ hyp_off,
+                                                 const int8_t* __restrict__ h_maxfz,
+                                                 const uint8_t* __restrict__ h_mode,
+                                                 const int64_t* __restrict__ h_lo,
+                                                 const int64_t* __restrict__ h_hi, K1Out O) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  unsigned char* wb = smem + (size_t)w * O.wave_bytes;
+  int wv = O.queue ? 0 : xcd_band_position(P.xcd_group) * (blockDim.x >> 6) + w;
+  for (;;) {
+    if (O.queue) {
+      if (lane == 0) wv = (int)atomicAdd(O.queue, 1u);
+      wv = __shfl(wv, 0);
+    }
+    // a whole wave leaves (no block-level barrier is used); in persistent mode every wave reaches this exit once
+    // the counter passes n_spec
+    if (wv >= n_spec) return;
+    k1_spectrum<MODE>(P, X, order ? order[wv] : wv, wb, pk_off, pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
+    if (!O.queue) return;
+    wave_sync();
+  }
+}
+
 // ------------------------------------------------------------------------------------------------ brute force
 // k1_brute_kernel (kernels.cu:195-232): one thread per (hypothesis, row) pair, fragments generated on the fly,
 // every peak and charge compared with plain f32 bounds.
@@ -551,6 +575,20 @@ __global__ __launch_bounds__(256) void k1_brute_kernel(DevP P, DevIdx X, int64_t
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ULL; }
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0u; }
+
+// grid of a K1 launch: grid mode = one wave per spectrum (XCD-banded); persistent mode = as many blocks as stay
+// resident on the device at once (occupancy query), each wave looping over the work counter
+static unsigned k1_launch_grid(const void* kern, int waves, size_t lds, int32_t n_spec, const VhParams* p,
+                               bool persistent) {
+  const unsigned blocks = (unsigned)((n_spec + waves - 1) / waves);
+  if (!persistent) return xcd_band_grid(blocks, p->xcd_group);
+  int dev = 0, cus = 256, per = 1;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * waves, lds) != hipSuccess || per < 1) per = 1;
+  const unsigned g = (unsigned)cus * (unsigned)per;
+  return g < blocks ? g : blocks;
+}
 
 // ------------------------------------------------------------------------------------------------ launchers
 static int pow2_at_least(int x) {
@@ -615,8 +653,8 @@ int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_
 int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
                const int64_t* pk_off, const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz,
                const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi, int32_t cand_stride,
-               int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status,
-               void* stream) {
+               int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, uint32_t* hyp_scored,
+               unsigned long long* hyp_matched, int32_t* status, uint32_t* queue, void* stream) {
   int rc = k1_check(p, ix);
   if (rc) return rc;
   if (n_spec < 0 || !n_cand || !c_row || !c_hyp || !c_cnt || !hyp_scored || !hyp_matched || !status)
@@ -634,13 +672,15 @@ int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const i
   O.hyp_scored = hyp_scored;
   O.hyp_matched = hyp_matched;
   O.status = status;
+  O.queue = queue;
   int waves;
   k1_geometry(p, 1, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
-  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
-  k1_kernel<1><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
-                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode,
-                                                                      h_lo, h_hi, O);
+  const size_t lds = (size_t)O.wave_bytes * waves;
+  unsigned g = k1_launch_grid((const void*)k1_kernel<1>, waves, lds, n_spec, p, queue != nullptr);
+  if (queue) zero_u32_kernel<<<1, 1, 0, st>>>(queue);
+  k1_kernel<1><<<g, 64 * waves, lds, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off, pk_mass,
+                                           hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
   VH_CHECK_LAUNCH();
   return 0;
 }
@@ -650,7 +690,7 @@ int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, con
                    const int8_t* h_maxfz, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
                    int32_t cand_stride, int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt,
                    int16_t* c_nb, uint32_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored,
-                   unsigned long long* hyp_matched, int32_t* status, void* stream) {
+                   unsigned long long* hyp_matched, int32_t* status, uint32_t* queue, void* stream) {
   int rc = k1_check(p, ix);
   if (rc) return rc;
   if (n_spec < 0 || !n_cand || !c_row || !c_hyp || !c_cnt || !c_nb || !c_iq || !s_maxcnt || !hyp_scored ||
@@ -673,13 +713,15 @@ int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, con
   O.c_nb = c_nb;
   O.c_iq = c_iq;
   O.s_maxcnt = s_maxcnt;
+  O.queue = queue;
   int waves;
   k1_geometry(p, 2, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
-  unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
-  k1_kernel<2><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
-                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode,
-                                                                      h_lo, h_hi, O);
+  const size_t lds = (size_t)O.wave_bytes * waves;
+  unsigned g = k1_launch_grid((const void*)k1_kernel<2>, waves, lds, n_spec, p, queue != nullptr);
+  if (queue) zero_u32_kernel<<<1, 1, 0, st>>>(queue);
+  k1_kernel<2><<<g, 64 * waves, lds, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off, pk_mass,
+                                           hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
   VH_CHECK_LAUNCH();
   return 0;
 }

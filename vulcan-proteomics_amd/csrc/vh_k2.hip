@@ -44,6 +44,7 @@ struct K2Buf {
   double* hs;
   float *sb, *sy, *ppm;
   int16_t *mb, *my, *lb, *ly;
+  uint32_t* queue;   // persistent mode work counter (NULL: grid mode)
 };
 
 // Mass-sorted peaks of one spectrum in LDS with the coarse lookup table.
@@ -116,8 +117,14 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
   for (int i = threadIdx.x; i < K2_LNF; i += blockDim.x) lnf[i] = lnfact_dev(i);
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = lane_id();
-  const int wv = xcd_band_position(P.xcd_group) * K2_WAVES + w;
-  if (wv >= n_groups) return;
+  // grid mode: one group per wave (XCD-banded); persistent mode: groups from the atomic counter until exhausted
+  int wv = O.queue ? 0 : xcd_band_position(P.xcd_group) * K2_WAVES + w;
+  for (;;) {
+  if (O.queue) {
+    if (lane == 0) wv = (int)atomicAdd(O.queue, 1u);
+    wv = __shfl(wv, 0);
+  }
+  if (wv >= n_groups) return;   // every wave of a persistent grid reaches this once the counter passes n_groups
   const int g = order ? order[wv] : wv;
   const int s = grp_spec ? grp_spec[g] : g;
   const int64_t c0 = grp_start ? grp_start[g] : (int64_t)g * stride;
@@ -239,6 +246,9 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
     O.lb[c] = (int16_t)rb.longest;
     O.ly[c] = (int16_t)ry.longest;
   }
+  if (!O.queue) return;
+  wave_sync();   // the next group reloads this wave's LDS peaks
+  }
 }
 
 // ------------------------------------------------------------------------------------------------ shifts
@@ -320,19 +330,31 @@ __global__ __launch_bounds__(256) void rank_kernel(DevP P, int n_groups, const i
   }
 }
 
+__global__ void k2_zero_u32_kernel(uint32_t* p) { *p = 0u; }
+
 extern "C" {
 
 int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* order, const int32_t* grp_spec,
           const int64_t* grp_start, const int32_t* grp_count, int32_t stride, const int32_t* c_row,
           const int32_t* c_hyp, const int8_t* h_maxfz, const float* c_shift, const int64_t* pk_off,
           const float* pk_mass, const float* pk_int, double* o_hs, float* o_sb, float* o_sy, float* o_ppm,
-          int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, void* stream) {
+          int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, uint32_t* queue, void* stream) {
   if (!p || !ix || n_groups < 0 || !grp_count) return VH_E_ARG;
   if (!grp_start && stride <= 0) return VH_E_ARG;
   if (n_groups == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  K2Buf O = {o_hs, o_sb, o_sy, o_ppm, o_mb, o_my, o_lb, o_ly};
-  unsigned g = xcd_band_grid((n_groups + K2_WAVES - 1) / K2_WAVES, p->xcd_group);
+  K2Buf O = {o_hs, o_sb, o_sy, o_ppm, o_mb, o_my, o_lb, o_ly, queue};
+  const unsigned blocks = (unsigned)((n_groups + K2_WAVES - 1) / K2_WAVES);
+  unsigned g = xcd_band_grid(blocks, p->xcd_group);
+  if (queue) {
+    int dev = 0, cus = 256, per = 1;
+    const void* kern = c_shift ? (const void*)k2_kernel<true> : (const void*)k2_kernel<false>;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * K2_WAVES, 0) != hipSuccess || per < 1) per = 1;
+    g = (unsigned)cus * (unsigned)per;
+    g = g < blocks ? g : blocks;
+    k2_zero_u32_kernel<<<1, 1, 0, st>>>(queue);
+  }
   if (c_shift)
     k2_kernel<true><<<g, 64 * K2_WAVES, 0, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_groups, order, grp_spec,
                                                  grp_start, grp_count, stride, c_row, c_hyp, h_maxfz, c_shift,

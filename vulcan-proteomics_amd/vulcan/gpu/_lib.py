@@ -31,13 +31,14 @@ _SIGS = {
     "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_spectrum_order_workspace_bytes": ([I32], SZ),
     "vh_spectrum_order": ([I32, P, P, P, P, SZ, P], I32),
+    "vh_spectrum_order_work": ([I32, P, P, P, P, P, I32, P, P, SZ, P], I32),
     "vh_k1": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I64, P, P, P, P, P, P, P], I32),
-    "vh_k1_topk": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P], I32),
+    "vh_k1_topk": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P], I32),
     "vh_k1_brute": ([PP, PIX, I64, P, P, P, P, P, P, P, P], I32),
-    "vh_k2": ([PP, PIX, I32, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_k2": ([PP, PIX, I32, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_candidate_shift": ([I32, P, P, I32, P, P, P, P, P, P, P], I32),
     "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
-    "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_mlp_pack": ([P, P, P, P, P], I32),
     "vh_mlp_forward": ([P, I64, P, P, P, P, P], I32),
     "vh_mlp_workspace_bytes": ([I64], SZ),

@@ -312,14 +312,14 @@ def _k2_out(n, dev):
 
 
 def _k2_call(gi, vp, n_groups, grp_spec, grp_start, grp_count, stride, row, hyp, maxfz, peaks, out, order=None,
-             shift=None):
+             shift=None, queue=None):
     L = K.lib()
     K.check(L.vh_k2(C.byref(vp), C.byref(gi.view), n_groups, K.ptr(order), K.ptr(grp_spec), K.ptr(grp_start),
                     K.ptr(grp_count),
                     stride, K.ptr(row), K.ptr(hyp), K.ptr(maxfz), K.ptr(shift), K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
                     K.ptr(peaks["int"]), K.ptr(out["hyperscore"]), K.ptr(out["summed_b"]), K.ptr(out["summed_y"]),
                     K.ptr(out["ppm"]), K.ptr(out["matched_b"]), K.ptr(out["matched_y"]), K.ptr(out["longest_b"]),
-                    K.ptr(out["longest_y"]), K.stream()), "k2")
+                    K.ptr(out["longest_y"]), K.ptr(queue), K.stream()), "k2")
 
 
 def run_k2(gi: GpuIndex, h, r, d_spec, d_maxfz, peaks, p: SearchParams, shift: torch.Tensor | None = None):
@@ -459,8 +459,15 @@ class DeviceSearch:
     returns the per-spectrum rank-1 result; no host synchronisation, no allocation, hipGraph-capturable."""
 
     def __init__(self, gi: GpuIndex, p: SearchParams, n_spec: int, report: int = 1, offsets=(), open_da=None,
-                 k1_threshold_open: int = 1):
+                 k1_threshold_open: int = 1, persistent: bool = False, order_mode: str = "mass",
+                 band_log2: int = 28):
         self.gi, self.p, self.n_spec, self.report = gi, p, n_spec, report
+        # persistent: K1 and K2 run as resident waves pulling spectra from an atomic work counter.
+        # order_mode: "mass" (first window row, vh_spectrum_order), "work" (mass band of 2^band_log2 rows, then
+        # estimated work descending, vh_spectrum_order_work) or "fixed" (self.order is set by the caller)
+        if order_mode not in ("mass", "work", "fixed"):
+            raise ValueError(order_mode)
+        self.order_mode, self.band_log2 = order_mode, band_log2
         self.vp = gi.params_for(p)
         self.modes = bool(len(offsets) or open_da is not None)
         if self.modes:
@@ -488,6 +495,7 @@ class DeviceSearch:
         self.c_row, self.c_hyp, self.c_cnt = e(cap_c, torch.int32), e(cap_c, torch.int32), e(cap_c, torch.int32)
         self.scored, self.matched = e(cap_h, torch.int32), e(cap_h, torch.int64)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.queue = torch.zeros(1, dtype=torch.int32, device=dev) if persistent else None
         self.k2 = _k2_out(cap_c, dev)
         self.sel = e(n * report, torch.int64)
         self.nxt = e(n * report, torch.float64)
@@ -521,8 +529,13 @@ class DeviceSearch:
                                 K.ptr(hy["ws"]), hy["ws"].numel(), st), "hypotheses")
         mark("hyp")
         mark("order")
-        K.check(L.vh_spectrum_order(n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(self.order),
-                                    K.ptr(self.order_ws), self.order_ws.numel(), st), "spectrum_order")
+        if self.order_mode == "mass":
+            K.check(L.vh_spectrum_order(n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(self.order),
+                                        K.ptr(self.order_ws), self.order_ws.numel(), st), "spectrum_order")
+        elif self.order_mode == "work":
+            K.check(L.vh_spectrum_order_work(n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
+                                             K.ptr(hy["maxfz"]), K.ptr(pk["off"]), self.band_log2, K.ptr(self.order),
+                                             K.ptr(self.order_ws), self.order_ws.numel(), st), "spectrum_order_work")
         mark("order")
 
     def candidate_stage(self, n: int, mark):
@@ -535,7 +548,7 @@ class DeviceSearch:
                              K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["mode"]),
                              K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand), K.ptr(self.c_row),
                              K.ptr(self.c_hyp), K.ptr(self.c_cnt), K.ptr(self.scored), K.ptr(self.matched),
-                             K.ptr(self.status), K.stream()), "k1_topk")
+                             K.ptr(self.status), K.ptr(self.queue), K.stream()), "k1_topk")
         mark("k1_topk")
 
     def back(self, n: int, mark):
@@ -550,7 +563,7 @@ class DeviceSearch:
                                          K.ptr(self.c_hyp), K.ptr(hy["mode"]), K.ptr(hy["M"]), K.ptr(gi.mass32),
                                          K.ptr(self.c_shift), st), "candidate_shift")
         _k2_call(gi, vp, n, None, None, self.n_cand, self.stride, self.c_row, self.c_hyp, hy["maxfz"], pk, self.k2,
-                 order=self.order, shift=self.c_shift)
+                 order=self.order, shift=self.c_shift, queue=self.queue)
         mark("k2")
         mark("rank")
         K.check(L.vh_rank(C.byref(vp), n, None, K.ptr(self.n_cand), self.stride, K.ptr(self.c_row),

@@ -197,7 +197,8 @@ class FeatureLists:
                                        self.stride, K.ptr(self.n_cand), K.ptr(self.row), K.ptr(self.hyp),
                                        K.ptr(self.cnt), K.ptr(self.nb), K.ptr(self.iq), K.ptr(self.maxcnt),
                                        K.ptr(ds.scored),
-                                       K.ptr(ds.matched), K.ptr(ds.status), K.stream()), "k1_features")
+                                       K.ptr(ds.matched), K.ptr(ds.status), K.ptr(ds.queue), K.stream()),
+                "k1_features")
 
     def features(self, ds: DeviceSearch, n: int) -> torch.Tensor:
         out = torch.empty((max(n, 1) * self.stride, 8), dtype=torch.float32, device=ds.gi.dev)
@@ -220,12 +221,12 @@ class LearnedSearch(DeviceSearch):
     """DeviceSearch whose candidate stage is index top-K -> MLP -> top `keep` + closure (module docstring)."""
 
     def __init__(self, gi: GpuIndex, p: SearchParams, n_spec: int, model: PrefilterModel,
-                 pp: PrefilterParams | None = None, report: int = 1):
+                 pp: PrefilterParams | None = None, report: int = 1, persistent: bool = False):
         self.pp = pp or PrefilterParams()
         q = SearchParams(**p.to_dict())
         q.topk = self.pp.keep
         q.pair_closure = self.pp.pair_closure
-        super().__init__(gi, q, n_spec, report=report)
+        super().__init__(gi, q, n_spec, report=report, persistent=persistent)
         self.vpf = _feature_params(self.vp, self.pp.index_topk)
         self.lists = FeatureLists(n_spec, self.pp.index_topk, gi.dev)
         self.model = model
