@@ -359,7 +359,7 @@ def main():
                 "xcd_group": p.xcd_group}
     achieved = ab[dom] / (stage_ms[dom] * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{dom}_cfg{a.cfg}_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_{w.name}.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         traffic = tj.get("hbm_bytes_per_launch")

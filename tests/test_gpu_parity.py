@@ -148,6 +148,51 @@ def test_k0_edge_cases_parity(env0):
     assert np.array_equal(pk["valid"], o["valid"].astype(bool))
 
 
+def test_k0_sorted_fast_path_parity(env0):
+    """m/z-sorted spectra (the K0 fast path: bitwise top-K search + ballot compaction) with more in-range peaks
+    than max_peaks, many tied intensities (NaN included), exact-threshold ties, and unsorted twins of the same
+    spectra (the general path), against the oracle."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu._abi import make_params
+    from vulcan.gpu.host import SearchParams, Spectra
+    rng = np.random.default_rng(12)
+    peaks = []
+    for n in (0, 1, 20, 149, 150, 151, 200, 300, 500, 512):
+        for variant in range(4):
+            mz = np.sort(rng.uniform(140, 2010, n)).astype(np.float32)
+            mz = np.unique(mz) if variant < 3 else mz          # strictly ascending for the fast path
+            k = len(mz)
+            if variant == 0:
+                it = rng.lognormal(0, 1, k).astype(np.float32)
+            elif variant == 1:
+                it = rng.integers(1, 6, k).astype(np.float32)   # heavy ties at the selection threshold
+            elif variant == 2:
+                it = rng.integers(1, 4, k).astype(np.float32)
+                if k > 3:
+                    it[rng.choice(k, 3, replace=False)] = np.nan
+            else:
+                it = rng.integers(1, 4, k).astype(np.float32)
+                mz = mz[rng.permutation(k)]                     # unsorted: the general path
+            peaks.append((mz, it))
+    off = np.zeros(len(peaks) + 1, np.int64)
+    np.cumsum([len(m) for m, _ in peaks], out=off[1:])
+    n_sp = len(peaks)
+    sp = Spectra("f", np.arange(n_sp, dtype=np.uint32), np.arange(n_sp, dtype=np.uint32), np.zeros(n_sp),
+                 np.full(n_sp, 600.0), np.full(n_sp, 2, np.int8), np.concatenate([m for m, _ in peaks]),
+                 np.concatenate([i for _, i in peaks]), off)
+    for max_peaks in (150, 64):
+        p = SearchParams(max_peaks=max_peaks, min_peaks=10)
+        vp = make_params(p, STATIC, {})
+        o = _O().k0(vp, sp)
+        pk = E.k0_preprocess(sp, p, env0.gi.dev, E.Stats(), None)
+        m = int(o["off"][-1])
+        assert np.array_equal(_np(pk["off"]), o["off"])
+        assert np.array_equal(_np(pk["mass"])[:m], o["mass"])
+        assert np.array_equal(_np(pk["int"])[:m], o["int"], equal_nan=True)
+        assert np.array_equal(pk["valid"], o["valid"].astype(bool))
+        assert np.allclose(_np(pk["tic"])[~np.isnan(o["tic"])], o["tic"][~np.isnan(o["tic"])], rtol=1e-6)
+
+
 def test_hypotheses_parity(env1):
     from vulcan.gpu import engine as E
     st = E.Stats()

@@ -9,7 +9,9 @@
 // first) and -0 onto +0, and the mass key is the order-preserving u32 image of the float. A peak's rank = number
 // of peaks before it; candidates are broadcast with v_readlane (no LDS traffic in the O(n^2) passes). The kept
 // peaks (rank < max_peaks) then get their mass-ordered slot as #(kept with (mass bits, rank) smaller). Spectra
-// with more than 512 in-range peaks rank straight from global memory.
+// with more than 512 in-range peaks rank straight from global memory. A spectrum whose in-range masses are
+// strictly ascending (m/z-sorted input) takes a fast path: a 64-step bitwise search for the max_peaks-th key and
+// a ballot compaction in input order give the same kept set and slots without the O(n^2) passes.
 // Hypotheses restate make_hypotheses (vulcan/gpu/host.py:64-91) with the same f32 operation order.
 #include "vh_common.h"
 
@@ -98,6 +100,61 @@ __global__ __launch_bounds__(256) void k0_emit_kernel(DevP P, int n_spec, const 
       mb[c] = ord_key(m);
       ik[c] = int_key(fi[c]);
       rk[c] = 0;
+    }
+    // Fast path: masses strictly ascending in input order (the usual m/z-sorted spectrum), so the mass order
+    // of the kept peaks is their input order. The kept set {rank < K} is found by a bitwise search for the
+    // K-th largest selection key T: kept = key > T, plus the first K - #(key > T) peaks with key == T.
+    bool unsorted = false;
+    for (int d = lane; d + 1 < n_in; d += 64) unsorted |= !(lm[d] < lm[d + 1]);
+    if (!ballot(unsorted)) {
+      bool kept[K0_REG];
+      if (K == n_in) {
+#pragma unroll
+        for (int c = 0; c < K0_REG; ++c) kept[c] = c * 64 + lane < n_in;
+      } else {
+        uint64_t T = 0;
+        for (int bit = 63; bit >= 0; --bit) {
+          const uint64_t cand = T | (1ULL << bit);
+          int cnt = 0;
+#pragma unroll
+          for (int c = 0; c < K0_REG; ++c)
+            if (c < nch)
+              cnt += __popcll(ballot(c * 64 + lane < n_in && (((uint64_t)ik[c] << 32) | mb[c]) >= cand));
+          if (cnt >= K) T = cand;
+        }
+        int gt = 0;
+#pragma unroll
+        for (int c = 0; c < K0_REG; ++c)
+          if (c < nch) gt += __popcll(ballot(c * 64 + lane < n_in && (((uint64_t)ik[c] << 32) | mb[c]) > T));
+        const int need = K - gt;
+        int eq_before = 0;
+#pragma unroll
+        for (int c = 0; c < K0_REG; ++c) {
+          const uint64_t key = ((uint64_t)ik[c] << 32) | mb[c];
+          const bool v = c < nch && c * 64 + lane < n_in;
+          const uint64_t eb = ballot(v && key == T);
+          kept[c] = v && (key > T || (key == T && eq_before + __popcll(eb & lanemask_lt()) < need));
+          eq_before += __popcll(eb);
+        }
+      }
+      int slot = 0;
+#pragma unroll
+      for (int c = 0; c < K0_REG; ++c) {
+        const uint64_t kb = ballot(c < nch && kept[c]);
+        if (c < nch && kept[c]) {
+          const int ps = slot + __popcll(kb & lanemask_lt());
+          pk_mass[o + ps] = ord_inv(mb[c]);
+          pk_int[o + ps] = fi[c];
+          t += (double)fi[c];
+        }
+        slot += __popcll(kb);
+      }
+      t = wave_sum(t);
+      if (lane == 0) {
+        tic[s] = (float)t;
+        valid[s] = K >= P.min_peaks;
+      }
+      return;
     }
     // rank = #(j before i): key_j > key_i, or equal keys and j < i
 #pragma unroll

@@ -504,7 +504,7 @@ __device__ void k1_spectrum(const DevP& P, const DevIdx& X, int s, unsigned char
 // Grid mode: wave wv takes order[wv] (XCD-banded block order). Persistent mode (O.queue set): every wave takes the
 // next position of `order` from an atomic counter until the list is exhausted, so a wave that finishes a short
 // spectrum immediately starts another instead of idling until its block's slowest wave is done.
-template <int MODE>
+template <int MODE, bool PERSIST>
 __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, const int32_t* __restrict__ order,
                                                  const int64_t* __restrict__ pk_off,
                                                  const float* __restrict__ pk_mass,
@@ -516,17 +516,18 @@ __global__ __launch_bounds__(256) void k1_kernel(DevP P, DevIdx X, int n_spec, c
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = threadIdx.x >> 6, lane = lane_id();
   unsigned char* wb = smem + (size_t)w * O.wave_bytes;
-  int wv = O.queue ? 0 : xcd_band_position(P.xcd_group) * (blockDim.x >> 6) + w;
-  for (;;) {
-    if (O.queue) {
-      if (lane == 0) wv = (int)atomicAdd(O.queue, 1u);
-      wv = __shfl(wv, 0);
-    }
-    // a whole wave leaves (no block-level barrier is used); in persistent mode every wave reaches this exit once
-    // the counter passes n_spec
-    if (wv >= n_spec) return;
+  if (!PERSIST) {
+    const int wv = xcd_band_position(P.xcd_group) * (blockDim.x >> 6) + w;
+    if (wv >= n_spec) return;   // a whole wave leaves; no block-level barrier is used
     k1_spectrum<MODE>(P, X, order ? order[wv] : wv, wb, pk_off, pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
-    if (!O.queue) return;
+    return;
+  }
+  for (;;) {
+    int wv = 0;
+    if (lane == 0) wv = (int)atomicAdd(O.queue, 1u);
+    wv = __shfl(wv, 0);
+    if (wv >= n_spec) return;   // every wave reaches this exit once the counter passes n_spec
+    k1_spectrum<MODE>(P, X, order ? order[wv] : wv, wb, pk_off, pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
     wave_sync();
   }
 }
@@ -643,9 +644,8 @@ int vh_k1(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_
   k1_geometry(p, 0, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
   unsigned g = xcd_band_grid((n_spec + waves - 1) / waves, p->xcd_group);
-  k1_kernel<0><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec,
-                                                                      order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode,
-                                                                      h_lo, h_hi, O);
+  k1_kernel<0, false><<<g, 64 * waves, (size_t)O.wave_bytes * waves, st>>>(
+      vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
   VH_CHECK_LAUNCH();
   return 0;
 }
@@ -677,10 +677,16 @@ int vh_k1_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const i
   k1_geometry(p, 1, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
   const size_t lds = (size_t)O.wave_bytes * waves;
-  unsigned g = k1_launch_grid((const void*)k1_kernel<1>, waves, lds, n_spec, p, queue != nullptr);
-  if (queue) zero_u32_kernel<<<1, 1, 0, st>>>(queue);
-  k1_kernel<1><<<g, 64 * waves, lds, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off, pk_mass,
-                                           hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
+  if (queue) {
+    unsigned g = k1_launch_grid((const void*)k1_kernel<1, true>, waves, lds, n_spec, p, true);
+    zero_u32_kernel<<<1, 1, 0, st>>>(queue);
+    k1_kernel<1, true><<<g, 64 * waves, lds, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off,
+                                                   pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
+  } else {
+    unsigned g = k1_launch_grid((const void*)k1_kernel<1, false>, waves, lds, n_spec, p, false);
+    k1_kernel<1, false><<<g, 64 * waves, lds, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off,
+                                                    pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
+  }
   VH_CHECK_LAUNCH();
   return 0;
 }
@@ -718,10 +724,16 @@ int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, con
   k1_geometry(p, 2, O, waves);
   if (O.wave_bytes > 65536) return VH_E_LIMIT;
   const size_t lds = (size_t)O.wave_bytes * waves;
-  unsigned g = k1_launch_grid((const void*)k1_kernel<2>, waves, lds, n_spec, p, queue != nullptr);
-  if (queue) zero_u32_kernel<<<1, 1, 0, st>>>(queue);
-  k1_kernel<2><<<g, 64 * waves, lds, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off, pk_mass,
-                                           hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
+  if (queue) {
+    unsigned g = k1_launch_grid((const void*)k1_kernel<2, true>, waves, lds, n_spec, p, true);
+    zero_u32_kernel<<<1, 1, 0, st>>>(queue);
+    k1_kernel<2, true><<<g, 64 * waves, lds, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off,
+                                                   pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
+  } else {
+    unsigned g = k1_launch_grid((const void*)k1_kernel<2, false>, waves, lds, n_spec, p, false);
+    k1_kernel<2, false><<<g, 64 * waves, lds, st>>>(vh_make_devp(p), vh_make_devidx(ix), n_spec, order, pk_off,
+                                                    pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, O);
+  }
   VH_CHECK_LAUNCH();
   return 0;
 }

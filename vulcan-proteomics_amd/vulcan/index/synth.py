@@ -72,14 +72,14 @@ class Workload:
 
 
 WORKLOADS = {
-    0: Workload("cfg0-plain-cpu", 2000, 0, {}, 0, None, 10_000, 1, cfg=0, index_cfg=0),
-    1: Workload("cfg1-proteome-plain", 20_000, 1, {"M": OX}, 2, None, 100_000, 2, cfg=1, index_cfg=1),
-    2: Workload("cfg2-uneven-batched", 20_000, 1, {"M": OX}, 2, None, 500_000, 2, charges=(2, 3, 4, 5, 6),
+    0: Workload("plain-2k-proteins-10k-spectra", 2000, 0, {}, 0, None, 10_000, 1, cfg=0, index_cfg=0),
+    1: Workload("proteome-plain-20k-proteins-100k-spectra", 20_000, 1, {"M": OX}, 2, None, 100_000, 2, cfg=1, index_cfg=1),
+    2: Workload("uneven-16-runs-500k-spectra", 20_000, 1, {"M": OX}, 2, None, 500_000, 2, charges=(2, 3, 4, 5, 6),
                 charge_p=(0.4, 0.3, 0.15, 0.1, 0.05), n_runs=16, peak_range=(20, 500), calib_ppm=5.0, cfg=2,
                 index_cfg=1),
-    3: Workload("cfg3-open", 20_000, 1, {"M": OX}, 2, None, 50_000, 2, shift_frac=0.4, topk=100,
+    3: Workload("open-500da-50k-spectra", 20_000, 1, {"M": OX}, 2, None, 50_000, 2, shift_frac=0.4, topk=100,
                 open_da=(-500.0, 500.0), cfg=3, index_cfg=1),
-    4: Workload("cfg4-phospho-prefilter", 20_000, 1, {"S": PHOSPHO, "T": PHOSPHO, "Y": PHOSPHO, "M": OX}, 5,
+    4: Workload("phospho-prefilter-20k-proteins-200k-spectra", 20_000, 1, {"S": PHOSPHO, "T": PHOSPHO, "Y": PHOSPHO, "M": OX}, 5,
                 {"S": 3, "T": 3, "Y": 3, "M": 2}, 200_000, 2, charges=(2, 3, 4, 5, 6),
                 charge_p=(0.4, 0.3, 0.15, 0.1, 0.05), n_runs=16, peak_range=(20, 500), calib_ppm=5.0, topk=200,
                 cfg=4, index_cfg=4, geometry={"chunk_size": 1024, "bin_ppm": 320.0, "tile": 1024}),
