@@ -71,7 +71,7 @@ def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
     from vulcan.rescore import prefilter as PF
     pp = PF.PrefilterParams(index_topk=p.topk, keep=a.keep, steps=a.train_steps)
     t0 = time.perf_counter()
-    sp_tr, _, _ = synth.spectra(l1, lab, a.cfg, n=a.n_train, seed=900 + a.cfg)
+    sp_tr, _, _ = synth.spectra(l1, lab, a.workload, n=a.n_train, seed=900 + a.workload)
     raw_tr = E.upload_spectra(sp_tr, gi.dev)
     x, y, _, _ = PF.training_data(gi, p, raw_tr, pp)
     torch.cuda.synchronize()
@@ -231,7 +231,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cfg", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=1,
+                    help="index into vulcan.index.synth.WORKLOADS (1 = the proteome-scale plain search)")
     ap.add_argument("--n-spectra", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -267,12 +268,12 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes_each_way": n * 4, "launches": 5}))
         return
-    w = synth.WORKLOADS[a.cfg]
+    w = synth.WORKLOADS[a.workload]
     t0 = time.perf_counter()
-    l1, lab = synth.build_index(a.cfg)
+    l1, lab = synth.build_index(a.workload)
     log(f"table: {len(l1)} peptidoforms ({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
-    sp, truth, _ = synth.spectra(l1, lab, a.cfg, n=a.n_spectra)
+    sp, truth, _ = synth.spectra(l1, lab, a.workload, n=a.n_spectra)
     n = len(sp)
     log(f"spectra: {n} ({time.perf_counter() - t0:.1f} s)")
     p = w.search_params()
@@ -288,7 +289,7 @@ def main():
     gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
     log(f"index: {gi.n_postings} postings, {gi.n_buckets} buckets x {gi.n_bins} bins, build {gi.build_s:.3f} s")
     raw = E.upload_spectra(sp, dev)
-    if a.cfg == 2:
+    if w.leg == "uneven":
         res, sched = uneven_leg(a, w, l1, gi, p, sp, raw, n)
         ab = algorithmic_bytes(sched, gi, n, l1)
         ach = ab["k1_topk"] / (res["k1_ms_sum"] * 1e-3) / 1e9
@@ -311,7 +312,7 @@ def main():
             out["rank1_equal_cpu_sample"] = bool(np.array_equal(sched.rank1_rows(n)[:ncpu], cres["row"][:ncpu]))
         print(json.dumps(out), flush=True)
         return
-    if a.prefilter or a.cfg == 4:
+    if a.prefilter or w.leg == "prefilter":
         res, ls = prefilter_leg(a, w, l1, lab, gi, p, raw, n)
         L = res["learned"]
         st = L["stage_ms"]

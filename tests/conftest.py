@@ -18,8 +18,8 @@ STATIC = {"C": 57.021464}
 
 
 @pytest.fixture(scope="session")
-def cfg0():
-    """cfg0-shaped table (2,000 proteins) and 2,000 of its spectra."""
+def small_plain():
+    """small_plain-shaped table (2,000 proteins) and 2,000 of its spectra."""
     from vulcan.index import synth
     l1, lab = synth.build_index(0)
     sp, truth, _ = synth.spectra(l1, lab, 0, n=2000)
@@ -29,8 +29,8 @@ def cfg0():
 
 
 @pytest.fixture(scope="session")
-def cfg1_small():
-    """cfg1 semantics (1 missed cleavage, Met-ox <= 2, 2+ fragments for z >= 3) on 3,000 proteins."""
+def proteome_small():
+    """workload-1 semantics (1 missed cleavage, Met-ox <= 2, 2+ fragments for z >= 3) on 3,000 proteins."""
     from vulcan.index import synth
     l1, lab = synth.build_index(1, n_proteins=3000)
     sp, truth, _ = synth.spectra(l1, lab, 1, n=3000)

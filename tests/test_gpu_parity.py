@@ -38,15 +38,15 @@ class Env:
 
 
 @pytest.fixture(scope="module")
-def env0(cfg0):
-    print("[gpu] building cfg0 env", flush=True)
-    return Env(cfg0)
+def env0(small_plain):
+    print("[gpu] building small_plain env", flush=True)
+    return Env(small_plain)
 
 
 @pytest.fixture(scope="module")
-def env1(cfg1_small):
-    print("[gpu] building cfg1-small env", flush=True)
-    return Env(cfg1_small)
+def env1(proteome_small):
+    print("[gpu] building proteome-small env", flush=True)
+    return Env(proteome_small)
 
 
 def _np(t):
@@ -388,7 +388,7 @@ def test_spectrum_order_sorted_permutation(env1):
 # ------------------------------------------------------------------------------------------------ search modes
 @pytest.fixture(scope="module")
 def env_modes():
-    """300-protein cfg1-semantics table, spectra with 40% precursor-only shifts (the open-search workload's
+    """300-protein workload-1-semantics table, spectra with 40% precursor-only shifts (the open-search workload's
     generator), open window (-150, 500) Da and two mass offsets."""
     from vulcan.gpu import engine as E
     from vulcan.gpu._abi import make_params, max_hyp_per_spectrum, set_modes

@@ -67,15 +67,15 @@ def _take(sp, n):
 
 
 @pytest.fixture(scope="module")
-def fenv0(cfg0):
-    print("[gpu] prefilter env cfg0", flush=True)
-    return FEnv(cfg0, topk=200)
+def fenv0(small_plain):
+    print("[gpu] prefilter env small_plain", flush=True)
+    return FEnv(small_plain, topk=200)
 
 
 @pytest.fixture(scope="module")
-def fenv1(cfg1_small):
-    print("[gpu] prefilter env cfg1-small", flush=True)
-    return FEnv(cfg1_small, topk=64)
+def fenv1(proteome_small):
+    print("[gpu] prefilter env proteome-small", flush=True)
+    return FEnv(proteome_small, topk=64)
 
 
 @pytest.mark.parametrize("which", ["fenv0", "fenv1"])
@@ -263,11 +263,11 @@ def test_select_equals_oracle(fenv0):
 
 # ------------------------------------------------------------------------------------------------ end to end
 @pytest.fixture(scope="module")
-def trained(cfg1_small):
-    """Train on the first 2,000 spectra of the cfg1-small set, evaluate on the other 1,000."""
+def trained(proteome_small):
+    """Train on the first 2,000 spectra of the proteome-small set, evaluate on the other 1,000."""
     from vulcan.gpu import engine as E
     from vulcan.rescore import prefilter as PF
-    d = cfg1_small
+    d = proteome_small
     p = type(d["p"])(**d["p"].to_dict())
     gi = E.GpuIndex(d["l1"], p, d["static"], d["var"])
     sp = d["sp"]

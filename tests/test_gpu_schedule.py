@@ -18,11 +18,11 @@ def _np(t):
 
 
 @pytest.fixture(scope="module")
-def uneven(cfg1_small):
-    """cfg2-shaped spectra (16 runs, 20-500 peaks, charges 2-6, calibration offsets) on the cfg1-small table."""
+def uneven(proteome_small):
+    """uneven-workload-shaped spectra (16 runs, 20-500 peaks, charges 2-6, calibration offsets) on the proteome-small table."""
     from vulcan.gpu import engine as E
     from vulcan.index import synth
-    d = cfg1_small
+    d = proteome_small
     sp, truth, _ = synth.spectra(d["l1"], d["labels"], 2, n=4000)
     p = synth.WORKLOADS[2].search_params()
     gi = E.GpuIndex(d["l1"], p, STATIC, d["var"])

@@ -236,11 +236,11 @@ def test_k1_known_answer_count():
     assert got[_row_of(l1, "PEPTIDEK")] == want
 
 
-def test_k1_fast_equals_brute_force(cfg1_small):
+def test_k1_fast_equals_brute_force(proteome_small):
     """The reference's own K1 invariant (test_k1_k2_small.py:49-65): counts of every window pair agree."""
     from vulcan.gpu._abi import max_hyp_per_spectrum
     O = _O()
-    d = cfg1_small
+    d = proteome_small
     vp = _vp(d["p"], d["static"], d["var"])
     pk = O.k0(vp, d["sp"])
     hyp = O.hypotheses(vp, d["sp"], pk["valid"], d["l1"].mass32, max_hyp_per_spectrum(d["p"]))
@@ -257,10 +257,10 @@ def test_k1_fast_equals_brute_force(cfg1_small):
 
 
 # ------------------------------------------------------------------------------------------------ top-K
-def test_topk_matches_python_brute_force_and_closure_is_symmetric(cfg0):
+def test_topk_matches_python_brute_force_and_closure_is_symmetric(small_plain):
     from vulcan.gpu._abi import max_hyp_per_spectrum
     O = _O()
-    d = cfg0
+    d = small_plain
     p = d["p"]
     vp = _vp(p)
     vp.topk = 3
@@ -313,9 +313,9 @@ def test_rank_matches_python_sort():
 
 
 # ------------------------------------------------------------------------------------------------ end to end
-def test_search_recovers_truth_and_matches_components(cfg0):
+def test_search_recovers_truth_and_matches_components(small_plain):
     O = _O()
-    d = cfg0
+    d = small_plain
     vp = _vp(d["p"])
     hix = O.HostIndex(d["l1"])
     s = O.search(vp, hix, d["sp"])
@@ -344,8 +344,8 @@ def test_label_blindness_role_swap_and_permutation():
         assert np.array_equal(ra[k], rb[k])
 
 
-def test_builder_invariants(cfg0):
-    l1 = cfg0["l1"]
+def test_builder_invariants(small_plain):
+    l1 = small_plain["l1"]
     assert np.all(np.diff(l1.mass32) >= 0)
     same = np.diff(l1.mass32) == 0
     assert np.all(np.diff(l1.pf_id.astype(np.float64))[same] > 0) or not same.any() or \
@@ -359,13 +359,13 @@ def test_builder_invariants(cfg0):
 
 
 # ------------------------------------------------------------------------------------------------ search modes
-def test_mode_hypotheses_match_host_restatement(cfg1_small):
+def test_mode_hypotheses_match_host_restatement(proteome_small):
     """Closed, mass-offset and open hypotheses (ptm/search.py:48-77): the C++ oracle and the host numpy
     restatement agree field by field, and every spectrum's hypotheses keep the order closed, offsets, open."""
     from vulcan.gpu._abi import make_params, max_hyp_per_spectrum, set_modes
     from vulcan.ptm.search import ModeParams, make_mode_hypotheses
     O = _O()
-    d = cfg1_small
+    d = proteome_small
     p = d["p"]
     mp = ModeParams(offsets=(15.994915, 42.010565), open_da=(-150.0, 500.0))
     vp = set_modes(make_params(p, d["static"], d["var"]), mp.offsets, mp.open_da)

@@ -117,19 +117,19 @@ def test_iq_quant_maps_the_largest_peak_into_16_bits():
 
 # ------------------------------------------------------------------------------------------------ K1 features
 @pytest.fixture(scope="module")
-def feat_env(cfg0):
+def feat_env(small_plain):
     from vulcan.gpu._abi import make_params, max_hyp_per_spectrum
     O = _O()
-    sp = cfg0["sp"]
-    p = type(cfg0["p"])(**cfg0["p"].to_dict())
+    sp = small_plain["sp"]
+    p = type(small_plain["p"])(**small_plain["p"].to_dict())
     p.topk = 40
     p.pair_closure = False
     vp = make_params(p, STATIC, {})
-    hix = O.HostIndex(cfg0["l1"])
+    hix = O.HostIndex(small_plain["l1"])
     pk = O.k0(vp, sp)
-    hyp = O.hypotheses(vp, sp, pk["valid"], cfg0["l1"].mass32, max_hyp_per_spectrum(p))
+    hyp = O.hypotheses(vp, sp, pk["valid"], small_plain["l1"].mass32, max_hyp_per_spectrum(p))
     k1 = O.k1_counts(vp, hix, pk, hyp)
-    return {"vp": vp, "hix": hix, "pk": pk, "hyp": hyp, "k1": k1, "p": p, "l1": cfg0["l1"]}
+    return {"vp": vp, "hix": hix, "pk": pk, "hyp": hyp, "k1": k1, "p": p, "l1": small_plain["l1"]}
 
 
 def test_topk_feat_lists_equal_the_plain_topk_without_closure(feat_env):

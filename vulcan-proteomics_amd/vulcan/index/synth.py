@@ -1,18 +1,19 @@
 """Seeded synthetic proteomes, L1 tables and MS2 spectra for the benchmark workloads.
 
-Workloads (numpy PCG64 `default_rng`; proteins seed 100+cfg, spectra 200+cfg):
-  cfg0  2,000 proteins, 0 missed cleavages, length 7-30, fixed C+57.021464, reversed decoys (~1e5 rows);
-        10,000 spectra: 70% from a random target peptidoform (charge 2/3/4 at 60/30/10%), 40% of b/y 1+ ions
-        kept, +-10 ppm fragment jitter, lognormal intensities, 100 uniform noise peaks in 150-2000, top-150
-        peaks kept, precursor jitter U(-5,5) ppm; 30% noise spectra with a random precursor.
+Workloads (WORKLOADS[i]; numpy PCG64 `default_rng`; proteins seed 100+i, spectra 200+i):
+  0 plain-2k-proteins-10k-spectra: 2,000 proteins, 0 missed cleavages, length 7-30, fixed C+57.021464, reversed
+        decoys (~1e5 rows); 10,000 spectra: 70% from a random target peptidoform (charge 2/3/4 at 60/30/10%),
+        40% of b/y 1+ ions kept, +-10 ppm fragment jitter, lognormal intensities, 100 uniform noise peaks in
+        150-2000, top-150 peaks kept, precursor jitter U(-5,5) ppm; 30% noise spectra with a random precursor.
         Search: precursor +-10 ppm, fragment +-0.02 Th (Da units), top-k 50.
-  cfg1  20,000 proteins, 1 missed cleavage, variable Met-ox <= 2 (~3M rows); 100,000 spectra as cfg0 plus
-        b/y 2+ ions for precursor charge >= 3 (max_fragment_charge 2).
-  cfg2  cfg1 table; 500,000 spectra from 16 runs, peak counts U(20,500), charges 2-6, per-run calibration
-        offset U(-5,5) ppm.
-  cfg3  cfg1 table; 50,000 spectra as cfg1, 40% with a precursor-only mass shift U(-150,+400) Da; open window
-        +-500 Da, top-k 100.
-  cfg4  20,000 proteins, variable STY phospho <= 3 and Met-ox <= 2; 200,000 spectra as cfg2.
+  1 proteome-plain-20k-proteins-100k-spectra: 20,000 proteins, 1 missed cleavage, variable Met-ox <= 2 (~3M
+        rows); 100,000 spectra as workload 0 plus b/y 2+ ions for precursor charge >= 3 (max_fragment_charge 2).
+  2 uneven-16-runs-500k-spectra: the table of workload 1; 500,000 spectra from 16 runs, peak counts U(20,500),
+        charges 2-6, per-run calibration offset U(-5,5) ppm.
+  3 open-500da-50k-spectra: the table of workload 1; 50,000 spectra as workload 1, 40% with a precursor-only
+        mass shift U(-150,+400) Da; open window +-500 Da, top-k 100.
+  4 phospho-prefilter-20k-proteins-200k-spectra: 20,000 proteins, variable STY phospho <= 3 and Met-ox <= 2
+        (~36M rows); 200,000 spectra as workload 2; index top-200.
 """
 from __future__ import annotations
 
@@ -45,16 +46,17 @@ class Workload:
     charges: tuple = (2, 3, 4)
     charge_p: tuple = (0.6, 0.3, 0.1)
     n_runs: int = 1
-    peak_range: tuple | None = None        # cfg2/4: total peaks per spectrum U(lo, hi)
+    peak_range: tuple | None = None        # workloads 2/4: total peaks per spectrum U(lo, hi)
     calib_ppm: float = 0.0
     shift_frac: float = 0.0
     shift_range: tuple = (-150.0, 400.0)
     topk: int = 50
     open_da: tuple | None = None
-    cfg: int = 0
-    index_cfg: int = 0
+    key: int = 0
+    table_of: int = 0
     extra: dict = field(default_factory=dict)
     geometry: dict = field(default_factory=dict)   # index / K1 geometry tuned for this scale (chunk_size, ...)
+    leg: str = "plain"                             # benchmark leg: plain | uneven | open | prefilter
 
     def search_params(self) -> SearchParams:
         p = SearchParams(precursor_ppm=(-10.0, 10.0), fragment_ppm=(-0.02, 0.02), fragment_unit="da",
@@ -72,17 +74,18 @@ class Workload:
 
 
 WORKLOADS = {
-    0: Workload("plain-2k-proteins-10k-spectra", 2000, 0, {}, 0, None, 10_000, 1, cfg=0, index_cfg=0),
-    1: Workload("proteome-plain-20k-proteins-100k-spectra", 20_000, 1, {"M": OX}, 2, None, 100_000, 2, cfg=1, index_cfg=1),
+    0: Workload("plain-2k-proteins-10k-spectra", 2000, 0, {}, 0, None, 10_000, 1, key=0, table_of=0),
+    1: Workload("proteome-plain-20k-proteins-100k-spectra", 20_000, 1, {"M": OX}, 2, None, 100_000, 2, key=1, table_of=1),
     2: Workload("uneven-16-runs-500k-spectra", 20_000, 1, {"M": OX}, 2, None, 500_000, 2, charges=(2, 3, 4, 5, 6),
-                charge_p=(0.4, 0.3, 0.15, 0.1, 0.05), n_runs=16, peak_range=(20, 500), calib_ppm=5.0, cfg=2,
-                index_cfg=1),
+                charge_p=(0.4, 0.3, 0.15, 0.1, 0.05), n_runs=16, peak_range=(20, 500), calib_ppm=5.0, key=2,
+                table_of=1, leg="uneven"),
     3: Workload("open-500da-50k-spectra", 20_000, 1, {"M": OX}, 2, None, 50_000, 2, shift_frac=0.4, topk=100,
-                open_da=(-500.0, 500.0), cfg=3, index_cfg=1),
+                open_da=(-500.0, 500.0), key=3, table_of=1, leg="open"),
     4: Workload("phospho-prefilter-20k-proteins-200k-spectra", 20_000, 1, {"S": PHOSPHO, "T": PHOSPHO, "Y": PHOSPHO, "M": OX}, 5,
                 {"S": 3, "T": 3, "Y": 3, "M": 2}, 200_000, 2, charges=(2, 3, 4, 5, 6),
                 charge_p=(0.4, 0.3, 0.15, 0.1, 0.05), n_runs=16, peak_range=(20, 500), calib_ppm=5.0, topk=200,
-                cfg=4, index_cfg=4, geometry={"chunk_size": 1024, "bin_ppm": 320.0, "tile": 1024}),
+                key=4, table_of=4, geometry={"chunk_size": 1024, "bin_ppm": 320.0, "tile": 1024},
+                leg="prefilter"),
 }
 
 
@@ -98,10 +101,10 @@ def proteome(n: int, seed: int) -> list[str]:
     return [raw[off[i]:off[i + 1]] for i in range(n)]
 
 
-def build_index(cfg: int, n_proteins: int | None = None, **kw):
-    """L1 table of workload `cfg` (n_proteins overrides the size for small tests)."""
-    w = WORKLOADS[WORKLOADS[cfg].index_cfg]
-    prots = proteome(n_proteins or w.n_proteins, 100 + w.cfg)
+def build_index(workload: int, n_proteins: int | None = None, **kw):
+    """L1 table of workload `workload` (n_proteins overrides the size for small tests)."""
+    w = WORKLOADS[WORKLOADS[workload].table_of]
+    prots = proteome(n_proteins or w.n_proteins, 100 + w.key)
     return B.build_tables(prots, w.index_params(), **kw)
 
 
@@ -134,13 +137,13 @@ def _ladders(l1, rows: np.ndarray, static: dict, var: dict):
     return b, y, valid, M
 
 
-def spectra(l1, labels: dict, cfg: int, n: int | None = None, seed: int | None = None,
+def spectra(l1, labels: dict, workload: int, n: int | None = None, seed: int | None = None,
             index_params: dict | None = None):
-    """Synthetic spectra of workload `cfg` against table `l1`. Returns (Spectra, truth_row, meta)."""
-    w = WORKLOADS[cfg]
+    """Synthetic spectra of workload `workload` against table `l1`. Returns (Spectra, truth_row, meta)."""
+    w = WORKLOADS[workload]
     n = n or w.n_spectra
-    rng = np.random.default_rng(200 + cfg if seed is None else seed)
-    ip = index_params or WORKLOADS[w.index_cfg].index_params()
+    rng = np.random.default_rng(200 + workload if seed is None else seed)
+    ip = index_params or WORKLOADS[w.table_of].index_params()
     z = rng.choice(np.array(w.charges), size=n, p=np.array(w.charge_p))
     sig = rng.random(n) >= 0.3
     targets = np.nonzero(~labels["is_decoy"])[0]
