@@ -189,6 +189,37 @@ def algorithmic_bytes(ds, gi, n, l1):
     return {"k1_topk": k1, "k2": k2, "n_hyp": nh, "n_rescored": int(len(rows)), "matched": int(matched.sum())}
 
 
+def algorithmic_bytes_join(ds, gi, n):
+    """Per-launch algorithmic HBM bytes of the bucket-join K1 stages on this batch.
+    k1j_count: 6 B per posting of every block with at least one pair (key + row, read once) + 8 B block bounds,
+        per pair its hypothesis (id 4, spectrum 4, window 16, maxfz 1, slot 8) and the spectrum's peak masses
+        (+ intensities in feature mode), and 4 B (16 B feature mode) per window-row slot written;
+    k1j_select: the slots read back, 12 B per candidate written, per hypothesis window/slot/mode in (25 B) and
+        scored/matched out (12 B)."""
+    import numpy as np
+    j = ds.join
+    b = gi.block_arrays()
+    bp = j["blk_pairs"].cpu().numpy().view(np.uint32).astype(np.int64)
+    off = b["off"].cpu().numpy().view(np.uint32).astype(np.int64)
+    per_blk = np.diff(bp)
+    touched = per_blk > 0
+    n_pairs = int(bp[-1])
+    hyp_off = ds.hyp["hyp_off"][: n + 1].cpu().numpy()
+    nh = int(hyp_off[-1])
+    pairs = j["pairs"][:n_pairs].cpu().numpy().view(np.uint32).astype(np.int64)
+    spec = ds.hyp["spec"][:nh].cpu().numpy()
+    npk = np.diff(ds.peaks["off"][: n + 1].cpu().numpy())
+    slots = int(j["h_slot"][nh].item())
+    sw = 16 if j["feat"] else 4
+    pk_b = 8 if j["feat"] else 4
+    count = int(6 * np.diff(off)[touched].sum() + 8 * touched.sum() + 33 * n_pairs
+                + pk_b * npk[spec[pairs]].sum() + sw * slots)
+    nc = ds.n_cand[:n].cpu().numpy().astype(np.int64)
+    select = int(sw * slots + 12 * nc.sum() + 37 * nh)
+    return {"k1j_count": count, "k1j_select": select, "n_pairs": n_pairs, "n_slots": slots,
+            "blocks_touched": int(touched.sum())}
+
+
 def cpu_baseline(vp, l1, sp, target_s: float):
     """The oracle's whole-search (OpenMP over spectra) on a bounded prefix of the same spectra."""
     import numpy as np
@@ -240,6 +271,10 @@ def main():
     ap.add_argument("--chunk", type=int, default=None, help="override rows per index bucket")
     ap.add_argument("--bin-ppm", type=float, default=None, help="override directory bin width")
     ap.add_argument("--tile", type=int, default=None, help="override K1 LDS tile")
+    ap.add_argument("--k1", choices=("auto", "tile", "join"), default=None,
+                    help="K1 implementation (default: the workload's search parameters, auto)")
+    ap.add_argument("--block-q", type=int, default=None, help="join K1: log2 postings per index block")
+    ap.add_argument("--block-rows", type=int, default=None, help="join K1: log2 max rows per index block")
     ap.add_argument("--prefilter", action="store_true",
                     help="plain vs learned prefilter comparison (default for the phospho-prefilter workload)")
     ap.add_argument("--keep", type=int, default=16, help="prefilter: candidates kept before closure")
@@ -283,7 +318,8 @@ def main():
         mp = ModeParams(open_da=w.open_da)
         p = mode_search_params(p, mp)
         modes = {"open_da": mp.open_da, "k1_threshold_open": mp.k1_threshold_open}
-    for k, v in (("chunk_size", a.chunk), ("bin_ppm", a.bin_ppm), ("tile", a.tile)):
+    for k, v in (("chunk_size", a.chunk), ("bin_ppm", a.bin_ppm), ("tile", a.tile), ("k1_impl", a.k1),
+                 ("block_postings_log2", a.block_q), ("block_rows_log2", a.block_rows)):
         if v is not None:
             setattr(p, k, v)
     gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
@@ -355,9 +391,18 @@ def main():
     sig = truth >= 0
     recovery = float(np.mean(best[sig] == truth[sig]))
     ab = algorithmic_bytes(ds, gi, n, l1)
-    dom = max(("k1_topk", "k2"), key=lambda k: stage_ms[k])
+    kernels = ("k1_topk", "k2")
+    if ds.k1_impl == "join":
+        ab.update(algorithmic_bytes_join(ds, gi, n))
+        stage_ms["k1"] = sum(stage_ms[k] for k in ("k1j_pairs", "k1j_count", "k1j_select"))
+        kernels = ("k1j_count", "k1j_select", "k2")
+    dom = max(kernels, key=lambda k: stage_ms[k])
     geometry = {"chunk_size": 1 << gi.bucket_log2, "bin_shift": gi.bin_shift, "n_bins": gi.n_bins, "tile": p.tile,
-                "xcd_group": p.xcd_group}
+                "xcd_group": p.xcd_group, "k1_impl": ds.k1_impl}
+    if ds.k1_impl == "join":
+        bv = gi.blocks()
+        geometry.update(block_postings_log2=p.block_postings_log2, block_rows_log2=p.block_rows_log2,
+                        n_blocks=bv.n_blocks, max_block_rows=bv.max_rows, max_block_postings=bv.max_postings)
     achieved = ab[dom] / (stage_ms[dom] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_{w.name}.json")

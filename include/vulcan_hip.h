@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 9
+#define VH_ABI_VERSION 10
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -100,9 +100,23 @@ typedef struct VhIndexView {
   int64_t n_postings;
 } VhIndexView;
 
+/* Block layout of the same fragment index (bucket-join K1): rows are cut into blocks of consecutive rows, a row
+ * starting a new block when floor(post_off[row] / 2^q_log2) or floor(row / 2^row_log2) changes, so a block holds
+ * fewer than 2^q_log2 + 63 postings and at most 2^row_log2 rows. Inside a block the postings are sorted by key
+ * (f32 bit pattern of the fragment mass). key[i] = that bit pattern; row[i] = ion << 15 | (row - row0[block]). */
+typedef struct VhBlockView {
+  int64_t n_blocks;
+  int32_t max_rows;        /* rows of the largest block */
+  int32_t max_postings;    /* postings of the largest block */
+  const int32_t* row0;     /* n_blocks + 1: first row of each block; row0[n_blocks] = n_rows */
+  const uint32_t* off;     /* n_blocks + 1: first posting of each block; off[n_blocks] = n_postings */
+  const uint32_t* key;
+  const uint16_t* row;
+} VhBlockView;
+
 /* ---------------------------------------------------------------------------------------------- misc */
 int vh_version(void);
-/* sizeof(VhParams) (which = 0) or sizeof(VhIndexView) (which = 1), to check a binding's struct layout. */
+/* sizeof(VhParams) (which = 0), VhIndexView (1) or VhBlockView (2), to check a binding's struct layout. */
 size_t vh_abi_sizeof(int which);
 /* Scratch bytes of the prefix scans used inside vh_k0 / vh_hypotheses / vh_frag_count for n items. */
 size_t vh_scan_workspace_bytes(int64_t n);
@@ -123,6 +137,17 @@ size_t vh_index_workspace_bytes(int64_t n_postings, int64_t n_buckets, int32_t n
  * cumsum CSR assembly of engine.py:129-148. Needs bucket_log2 <= 16, bin_shift <= 15 and the key to fit 64 bits. */
 int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, uint32_t* offsets,
                    uint32_t* postings, void* ws, size_t ws_bytes, void* stream);
+/* Block layout (VhBlockView) of the index, built in two calls: vh_index_block_count writes stats[0] = n_blocks,
+ * stats[1] = rows of the largest block, stats[2] = postings of the largest block (device u64[3]); the caller
+ * allocates row0 / off (n_blocks + 1) and key / row (n_postings) and calls vh_index_build_blocks with the same
+ * q_log2 (6..15) and row_log2 (4..15). Same fragments as vh_index_build (vk_frag_emit, kernels.cu:73-101; CSR
+ * assembly of engine.py:129-148), a second ordering of them: by (block, key). Workspace: one query for both. */
+size_t vh_index_blocks_workspace_bytes(int64_t n_rows, int64_t n_postings);
+int vh_index_block_count(const VhIndexView* ix, const int64_t* post_off, int32_t q_log2, int32_t row_log2,
+                         unsigned long long* stats, void* ws, size_t ws_bytes, void* stream);
+int vh_index_build_blocks(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, int32_t q_log2,
+                          int32_t row_log2, int64_t n_blocks, int32_t* row0, uint32_t* off, uint32_t* key,
+                          uint16_t* row, void* ws, size_t ws_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------------- K0 */
 /* Peak view (engine.py:171-210): keep fmin <= m/z <= fmax (not NaN), mass = m/z - PROTON (f32), top
@@ -211,6 +236,39 @@ int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, con
 int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32_t* c_hyp, const int32_t* c_row,
                 const int32_t* h_spec, const int8_t* h_maxfz, const int64_t* pk_off, const float* pk_mass,
                 int32_t* out_cnt, void* stream);
+
+/* Bucket-join K1: the candidate lists of vh_k1_topk / vh_k1_features (same definition, bit-identical output)
+ * computed block-major instead of spectrum-major, for closed and mass-offset searches (precursor windows of
+ * a few hundred rows). Three calls on the block layout (VhBlockView):
+ *  1. vh_k1j_pairs: for every hypothesis h < hyp_off[n_spec]: its slots (one per window row; h_slot, max_hyp + 1
+ *     entries, h_slot[h] = first slot of h, h_slot[n_hyp] = total) and one (block, h) pair per block its window
+ *     overlaps, grouped by block: pairs[blk_pairs[b] .. blk_pairs[b + 1]) are the hypotheses of block b
+ *     (blk_pairs: n_blocks + 1). status |= 8 when pair_cap or slot_cap is too small (results then incomplete).
+ *  2. vh_k1j_count: per block, the block's postings are staged once in LDS and every pair's window rows in the
+ *     block get their shared-peak count (the K1 count of vh_k1) at counts[h_slot[h] + row - lo]: u32 (mode 1)
+ *     or, mode 2, two u64 per slot (b then y: count | quantised intensity sum << 24, as vh_k1_features).
+ *  3. vh_k1j_topk (mode 1 counts) / vh_k1j_features (mode 2 counts): one wave per spectrum reads its slots,
+ *     writes scored / matched per hypothesis and the top-K lists exactly as vh_k1_topk / vh_k1_features.
+ * The reference computes the same lists with k1_kernel + select_topk (kernels.cu:112-190, engine.py:284-312). */
+size_t vh_k1j_workspace_bytes(int64_t n_blocks, int64_t max_hyp);
+int vh_k1j_pairs(const VhBlockView* bv, int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo,
+                 const int64_t* h_hi, int64_t max_hyp, int64_t* h_slot, int64_t slot_cap, uint32_t* blk_pairs,
+                 uint32_t* pairs, int64_t pair_cap, int32_t* status, void* ws, size_t ws_bytes, void* stream);
+int vh_k1j_count(const VhParams* p, const VhBlockView* bv, int32_t mode, const uint32_t* blk_pairs,
+                 const uint32_t* pairs, int64_t pair_cap, const int32_t* h_spec, const int64_t* h_lo,
+                 const int64_t* h_hi, const int8_t* h_maxfz, const int64_t* h_slot, int64_t slot_cap,
+                 const int64_t* pk_off, const float* pk_mass, const float* pk_int, void* counts, void* stream);
+int vh_k1j_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
+                const int64_t* hyp_off, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
+                const int64_t* h_slot, int64_t slot_cap, const uint32_t* counts, int32_t cand_stride,
+                int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, uint32_t* hyp_scored,
+                unsigned long long* hyp_matched, int32_t* status, uint32_t* queue, void* stream);
+int vh_k1j_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
+                    const int64_t* hyp_off, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
+                    const int64_t* h_slot, int64_t slot_cap, const unsigned long long* counts, int32_t cand_stride,
+                    int32_t* n_cand, int32_t* c_row, int32_t* c_hyp, int32_t* c_cnt, int16_t* c_nb,
+                    uint32_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored, unsigned long long* hyp_matched,
+                    int32_t* status, uint32_t* queue, void* stream);
 
 /* ---------------------------------------------------------------------------------------------- K2 */
 /* Exact b/y rescoring (k2_kernel, kernels.cu:263-376): all b_k / y_k (k = 0..L-2) at fragment charges

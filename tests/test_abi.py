@@ -19,7 +19,8 @@ def test_header_declares_the_hot_path():
     for need in ("vh_frag_count", "vh_index_build", "vh_k0", "vh_hypotheses", "vh_k1", "vh_k1_topk", "vh_k1_brute",
                  "vh_k2", "vh_rank", "vh_version", "vh_abi_sizeof", "vh_k1_features", "vh_mlp_pack",
                  "vh_mlp_forward", "vh_mlp_backward", "vh_mlp_adam", "vh_prefilter_features", "vh_prefilter_score",
-                 "vh_prefilter_select"):
+                 "vh_prefilter_select", "vh_index_block_count", "vh_index_build_blocks", "vh_k1j_pairs",
+                 "vh_k1j_count", "vh_k1j_topk", "vh_k1j_features"):
         assert need in names
 
 
@@ -33,10 +34,11 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_layout_and_version():
     from vulcan.gpu import _lib
-    from vulcan.gpu._abi import VhIndexView, VhParams
+    from vulcan.gpu._abi import VhBlockView, VhIndexView, VhParams
     L = _lib.load()
     assert L.vh_abi_sizeof(0) == C.sizeof(VhParams)
     assert L.vh_abi_sizeof(1) == C.sizeof(VhIndexView)
+    assert L.vh_abi_sizeof(2) == C.sizeof(VhBlockView)
     m = re.search(r"#define VH_ABI_VERSION (\d+)", open(HEADER).read())
     assert L.vh_version() == int(m.group(1))
 
@@ -44,7 +46,7 @@ def test_abi_layout_and_version():
 def test_argument_errors_are_reported_without_a_gpu():
     """Null / inconsistent arguments are rejected on the host before any launch (negative VH_E_* codes)."""
     from vulcan.gpu import _lib
-    from vulcan.gpu._abi import VhIndexView, make_params
+    from vulcan.gpu._abi import VhBlockView, VhIndexView, make_params
     from vulcan.gpu.host import SearchParams
     L = _lib.load()
     vp = make_params(SearchParams(), {}, {})
@@ -65,6 +67,21 @@ def test_argument_errors_are_reported_without_a_gpu():
     ixp.partner = 8
     assert L.vh_prefilter_select(C.byref(ixp), 1, 200, *[None] * 5, 16, 1, 31, *[None] * 4, None) == -2
     assert L.vh_k1_features(C.byref(vp), C.byref(ix), 1, *[None] * 9, 10, *[None] * 11, None) == -1
+    # block layout / join K1
+    ix.n_rows = 100
+    assert L.vh_index_block_count(C.byref(ix), C.c_void_p(8), 5, 10, C.c_void_p(8), None, 0, None) == -1   # q < 6
+    assert L.vh_index_block_count(C.byref(ix), C.c_void_p(8), 12, 10, C.c_void_p(8), None, 0, None) == -3
+    assert L.vh_index_blocks_workspace_bytes(1000, 5000) > 2 * 5000 * 8
+    bv = VhBlockView()
+    assert L.vh_k1j_pairs(None, 1, None, None, None, 10, None, 0, None, None, 0, None, None, 0, None) == -1
+    assert L.vh_k1j_pairs(C.byref(bv), 1, None, None, None, 10, None, 0, None, None, 0, None, None, 0, None) == -1
+    bv.n_blocks, bv.max_rows, bv.max_postings, bv.row0, bv.off = 4, 64, 70000, 8, 8
+    assert L.vh_k1j_count(C.byref(vp), C.byref(bv), 1, *[None] * 2, 0, *[None] * 5, 0, *[None] * 4, None) == -4
+    bv.max_postings = 4096
+    assert L.vh_k1j_count(C.byref(vp), C.byref(bv), 3, *[None] * 2, 0, *[None] * 5, 0, *[None] * 4, None) == -1
+    assert L.vh_k1j_workspace_bytes(1000, 5000) >= 4000
+    assert L.vh_k1j_topk(C.byref(vp), C.byref(ix), 1, *[None] * 6, 0, None, 2, *[None] * 8, None) == -1
+    assert L.vh_k1j_features(C.byref(vp), C.byref(ix), 1, *[None] * 6, 0, None, 2, *[None] * 11, None) == -1
     vp.max_peaks = 4096
     assert L.vh_k0(C.byref(vp), 1, None, None, C.c_void_p(8), C.c_void_p(8), None, None, C.c_void_p(8),
                    C.c_void_p(8), 10, None, 0, None) == -4

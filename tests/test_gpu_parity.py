@@ -102,6 +102,44 @@ def test_index_build_matches_numpy(env0):
     assert np.array_equal(_np(gi.offsets).view(np.uint32).astype(np.int64), want)
 
 
+@pytest.mark.parametrize("q_rb", [(12, 10), (7, 4)])
+def test_block_layout_matches_numpy(env0, q_rb):
+    """vh_index_block_count + vh_index_build_blocks == numpy: block cuts where floor(post_off / 2^q) or
+    floor(row / 2^rb) changes, postings of a block stably sorted by f32 bits (emission order inside equal keys),
+    row field ion << 15 | row - block start, per-block posting offsets and the size statistics."""
+    from vulcan.gpu import engine as E
+    from vulcan.index.build import L1Index
+    l1 = env0.l1
+    n = 3000
+    sub = L1Index(arrays={"pf_id": l1.pf_id[:n], "mass32": l1.mass32[:n], "seq_off": l1.seq_off[:n],
+                          "length": l1.length[:n], "mask": l1.mask[:n], "n_var": l1.n_var[:n],
+                          "missed": l1.missed[:n], "partner": np.arange(n), "residues": l1.residues})
+    q, rb = q_rb
+    p2 = type(env0.p)(**{**env0.p.to_dict(), "chunk_size": 512, "block_postings_log2": q, "block_rows_log2": rb})
+    gi = E.GpuIndex(sub, p2, STATIC, {})
+    bv = gi.blocks()
+    b = gi.block_arrays()
+    post = _postings_numpy(sub, gi.vp, gi)
+    rows = (post[:, 0] << gi.bucket_log2) + post[:, 2]
+    po = _np(gi.post_off)
+    r = np.arange(n)
+    flag = np.ones(n, bool)
+    flag[1:] = ((po[1:n] >> q) != (po[:n - 1] >> q)) | ((r[1:] >> rb) != (r[:-1] >> rb))
+    row0 = np.concatenate([np.flatnonzero(flag), [n]])
+    nb = len(row0) - 1
+    assert bv.n_blocks == nb
+    assert np.array_equal(_np(b["row0"]), row0)
+    assert np.array_equal(_np(b["off"]).view(np.uint32).astype(np.int64), po[row0])
+    blk = np.searchsorted(row0, rows, side="right") - 1
+    o = np.lexsort((post[:, 1], blk))                       # stable
+    assert np.array_equal(_np(b["key"]).view(np.uint32)[:len(o)].astype(np.int64), post[o, 1])
+    want_row = (post[o, 3] << 15) | (rows[o] - row0[blk[o]])
+    assert np.array_equal(_np(b["row"]).view(np.uint16)[:len(o)].astype(np.int64), want_row)
+    sizes = np.diff(po[row0])
+    assert bv.max_rows == np.diff(row0).max() and bv.max_postings == sizes.max()
+    assert np.diff(row0).max() <= 2 ** rb and sizes.max() < 2 ** q + 64
+
+
 # ------------------------------------------------------------------------------------------------ K0 / hyps
 def test_k0_parity(env0):
     from vulcan.gpu import engine as E
@@ -238,7 +276,10 @@ def test_k1_brute_device_equals_oracle(env1):
 def _topk_device(env, **over):
     from vulcan.gpu import engine as E
     p = type(env.p)(**{**env.p.to_dict(), **over})
-    ds = E.DeviceSearch(env.gi, p, len(env.d["sp"]))
+    gi = env.gi
+    if (p.block_postings_log2, p.block_rows_log2) != (env.p.block_postings_log2, env.p.block_rows_log2):
+        gi = E.GpuIndex(env.l1, p, env.d["static"], env.d["var"])     # another block layout
+    ds = E.DeviceSearch(gi, p, len(env.d["sp"]))
     raw = E.upload_spectra(env.d["sp"], env.gi.dev)
     ds.run(raw)
     torch.cuda.synchronize()
@@ -246,7 +287,10 @@ def _topk_device(env, **over):
     return ds, p
 
 
-@pytest.mark.parametrize("over", [{}, {"topk": 3}, {"topk": 3, "tile": 64}, {"topk": 200, "tile": 256}])
+@pytest.mark.parametrize("over", [{"k1_impl": "tile"}, {"k1_impl": "tile", "topk": 3},
+                                  {"k1_impl": "tile", "topk": 3, "tile": 64}, {"k1_impl": "tile", "topk": 200, "tile": 256},
+                                  {"k1_impl": "join"}, {"k1_impl": "join", "topk": 3}, {"k1_impl": "join", "topk": 200},
+                                  {"k1_impl": "join", "block_postings_log2": 8, "block_rows_log2": 5}])
 @pytest.mark.parametrize("which", ["env0", "env1"])
 def test_k1_topk_fused_equals_oracle(which, over, request):
     from vulcan.gpu._abi import make_params
@@ -263,6 +307,64 @@ def test_k1_topk_fused_equals_oracle(which, over, request):
     nh = len(env.hyp["spec"])
     assert np.array_equal(_np(ds.scored)[:nh].astype(np.uint32), env.k1["scored"])
     assert np.array_equal(_np(ds.matched)[:nh].astype(np.uint64), env.k1["matched"])
+
+
+@pytest.mark.parametrize("which", ["env0", "env1"])
+def test_join_pairs_and_counts_equal_oracle(which, request):
+    """vh_k1j_pairs: slots = window rows in hypothesis order (the oracle's window layout), every block's pair
+    list = the hypotheses whose window overlaps it; vh_k1j_count: every slot == the oracle's K1 count."""
+    from vulcan.gpu import engine as E
+    env = request.getfixturevalue(which)
+    ds, p = _topk_device(env, k1_impl="join")
+    j = ds.join
+    nh = len(env.hyp["spec"])
+    hs = _np(j["h_slot"])
+    assert np.array_equal(hs[:nh + 1], env.k1["win_off"])
+    assert np.all(hs[nh:] == env.k1["win_off"][-1])
+    b = env.gi.block_arrays()
+    row0 = _np(b["row0"]).astype(np.int64)
+    nb = len(row0) - 1
+    bp = _np(j["blk_pairs"]).view(np.uint32).astype(np.int64)
+    pairs = _np(j["pairs"]).view(np.uint32).astype(np.int64)
+    lo, hi = env.hyp["lo"], env.hyp["hi"]
+    want = [[] for _ in range(nb)]
+    for h in range(nh):
+        if hi[h] > lo[h]:
+            b0 = np.searchsorted(row0, lo[h], side="right") - 1
+            b1 = np.searchsorted(row0, hi[h] - 1, side="right") - 1
+            for bb in range(b0, b1 + 1):
+                want[bb].append(h)
+    assert bp[nb] == sum(len(w) for w in want)
+    for bb in range(nb):
+        assert sorted(pairs[bp[bb]:bp[bb + 1]].tolist()) == want[bb]
+    tot = int(env.k1["win_off"][-1])
+    assert tot > 1000
+    assert np.array_equal(_np(j["counts"])[:tot], env.k1["counts"])
+
+
+def test_join_capacity_fit_and_overflow_flag(env1):
+    """Pair / slot buffers far too small: the first run grows them to the batch (status clean, lists equal to
+    the tile K1); a later batch that needs more than the fitted capacity raises status bit 8."""
+    from vulcan.gpu import engine as E
+    n = len(env1.d["sp"])
+    raw = E.upload_spectra(env1.d["sp"], env1.gi.dev)
+    ref = E.DeviceSearch(env1.gi, env1.p, n, k1_impl="tile")
+    ref.run(raw)
+    ds = E.DeviceSearch(env1.gi, env1.p, n, k1_impl="join")
+    ds._join_alloc(ds.join, 16, 16)
+    ds.run(raw)
+    torch.cuda.synchronize()
+    assert int(ds.status.item()) == 0
+    assert ds.join["slot_cap"] >= int(env1.k1["win_off"][-1])
+    nc = _np(ds.n_cand)[:n]
+    assert np.array_equal(nc, _np(ref.n_cand)[:n])
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    for a, c in ((ds.c_row, ref.c_row), (ds.c_hyp, ref.c_hyp), (ds.c_cnt, ref.c_cnt)):
+        assert np.array_equal(_np(a)[:n * ds.stride][m], _np(c)[:n * ds.stride][m])
+    ds._join_alloc(ds.join, 16, 16)                    # fitted already: no growth, overflow is flagged
+    ds.run(raw)
+    torch.cuda.synchronize()
+    assert int(ds.status.item()) & 8
 
 
 def test_select_topk_torch_equals_oracle(env1):
@@ -473,3 +575,25 @@ def test_search_modes_recovers_shifted_spectra_and_is_label_blind(env_modes):
     ho = lambda t: t.set_column(t.column_names.index("hyp_offset"), "hyp_offset",  # noqa: E731
                                 pa.array(t.column("hyp_offset").to_numpy().view(np.uint32)))
     assert ho(tbl).equals(ho(tb))
+
+
+def test_modes_join_equals_oracle(env_modes):
+    """The join K1 forced on the mode search (mass offsets + a 650 Da open window: each open hypothesis spans
+    many blocks) gives the oracle's candidate lists."""
+    from vulcan.gpu import engine as E
+    e = env_modes
+    ds = E.DeviceSearch(e["gi"], e["p"], len(e["sp"]), offsets=e["mp"].offsets, open_da=e["mp"].open_da,
+                        k1_impl="join")
+    ds.run(E.upload_spectra(e["sp"], e["gi"].dev))
+    torch.cuda.synchronize()
+    assert int(ds.status.item()) == 0
+    tk = _O().topk(e["vp"], e["hix"], e["hyp"], e["k1"], ds.stride)
+    n = len(e["sp"])
+    nc = _np(ds.n_cand)[:n]
+    assert np.array_equal(nc, tk["n_cand"])
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    for dv, k in ((ds.c_row, "row"), (ds.c_hyp, "hyp"), (ds.c_cnt, "cnt")):
+        assert np.array_equal(_np(dv)[:n * ds.stride][m], tk[k][m]), k
+    nh = len(e["hyp"]["spec"])
+    assert np.array_equal(_np(ds.scored)[:nh].astype(np.uint32), e["k1"]["scored"])
+    assert np.array_equal(_np(ds.matched)[:nh].astype(np.uint64), e["k1"]["matched"])

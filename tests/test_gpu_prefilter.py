@@ -30,7 +30,7 @@ def _P():
 class FEnv:
     """Device front end (K0, hypotheses, order) + K1 feature lists on a table/spectra pair, and the oracle twin."""
 
-    def __init__(self, d, topk=200, n=None):
+    def __init__(self, d, topk=200, n=None, impl="tile"):
         from vulcan.gpu import engine as E
         from vulcan.gpu._abi import make_params, max_hyp_per_spectrum
         from vulcan.rescore import prefilter as PF
@@ -42,7 +42,7 @@ class FEnv:
         self.gi = E.GpuIndex(self.l1, self.p, d["static"], d["var"])
         self.n = len(sp)
         self.raw = E.upload_spectra(sp, self.gi.dev)
-        self.ds = E.DeviceSearch(self.gi, self.p, self.n)
+        self.ds = E.DeviceSearch(self.gi, self.p, self.n, k1_impl=impl, join_features=True)
         self.ds.front(self.raw, self.n, E._marker(None))
         self.vpf = PF._feature_params(self.ds.vp, topk)
         self.fl = PF.FeatureLists(self.n, topk, self.gi.dev)
@@ -78,7 +78,19 @@ def fenv1(proteome_small):
     return FEnv(proteome_small, topk=64)
 
 
-@pytest.mark.parametrize("which", ["fenv0", "fenv1"])
+@pytest.fixture(scope="module")
+def fenv0j(small_plain):
+    print("[gpu] prefilter env small_plain, join K1", flush=True)
+    return FEnv(small_plain, topk=200, impl="join")
+
+
+@pytest.fixture(scope="module")
+def fenv1j(proteome_small):
+    print("[gpu] prefilter env proteome-small, join K1", flush=True)
+    return FEnv(proteome_small, topk=64, impl="join")
+
+
+@pytest.mark.parametrize("which", ["fenv0", "fenv1", "fenv0j", "fenv1j"])
 def test_k1_features_equal_oracle(which, request):
     e = request.getfixturevalue(which)
     O = _O()

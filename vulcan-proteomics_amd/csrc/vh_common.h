@@ -51,6 +51,7 @@ DevIdx vh_make_devidx(const VhIndexView* ix);
 int vh_scan_inclusive_i64(int64_t* a, int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
 int vh_scan_exclusive_u32(uint32_t* a, int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
 size_t vh_scan_ws_u32(int64_t n);
+size_t vh_scan_ws_i64(int64_t n);
 size_t vh_radix_ws_bytes(int64_t n);
 int vh_radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bit_lo, int bit_hi, void* ws, size_t ws_bytes,
                       hipStream_t st, uint64_t** result);

@@ -40,11 +40,21 @@ struct KeyGeom {
   int S;      // bin_shift
   int nbk;    // bucket bits
   uint32_t bin_base;
+  // block layout (vh_index_build_blocks): sort key = block << (16 + nd) | (key - fbase) << 16 | ion << 15 | row
+  // in block, with the block of each row in bid and the first row of each block in brow0
+  int block_mode;
+  uint32_t fbase;     // f32 bits of fragment_min_mz
+  int nd;             // bits of bits(fmax) - bits(fmin)
+  const int64_t* bid;
+  const int32_t* brow0;
+  int64_t n_blocks;
 };
 
 __device__ __forceinline__ uint64_t posting_key(float f, uint64_t bucket, uint32_t local, uint32_t ion,
                                                 const KeyGeom& G) {
   const uint32_t k = fbits(f);
+  if (G.block_mode)
+    return (bucket << (16 + G.nd)) | ((uint64_t)(k - G.fbase) << 16) | ((uint64_t)ion << 15) | (uint64_t)local;
   const uint64_t bin = (uint64_t)((k >> G.S) - G.bin_base);
   return (bin << (17 + G.S + G.nbk)) | (bucket << (17 + G.S)) | ((uint64_t)(k & ((1u << G.S) - 1u)) << 17) |
          ((uint64_t)ion << 16) | (uint64_t)local;
@@ -64,6 +74,11 @@ __global__ __launch_bounds__(256) void frag_emit_kernel(DevP P, DevIdx X, KeyGeo
   float b = nt, y = X.mass32[i] - nt;
   uint64_t bucket = (uint64_t)i >> X.bucket_log2;
   uint32_t local = (uint32_t)(i & ((1LL << X.bucket_log2) - 1));
+  if (G.block_mode) {   // (block ids beyond the caller's count are clamped: no out-of-bounds access)
+    const int64_t bk = G.bid[i] - 1;
+    bucket = (uint64_t)(bk < G.n_blocks ? bk : G.n_blocks - 1);
+    local = (uint32_t)(i - G.brow0[bucket]) & 0x7FFFu;
+  }
   int64_t o = post_off[i];
   for (int k = 0; k < L - 1; ++k) {
     float r = rm[((mk >> k) & 1ULL) * 26 + (s[k] - 'A')];
@@ -183,6 +198,64 @@ __global__ __launch_bounds__(256) void directory_kernel(const uint64_t* __restri
     for (int64_t b = d + 1; b <= n_dir; ++b) offsets[b] = (uint32_t)n;
 }
 
+// ------------------------------------------------------------------------------------------------ blocks
+// Block boundaries: row r starts a block when floor(post_off[r] / 2^q) or floor(r / 2^rb) differs from row r-1's,
+// so a block holds fewer than 2^q + (postings of one row) postings and at most 2^rb rows.
+__global__ __launch_bounds__(256) void block_flag_kernel(const int64_t* __restrict__ post_off, int64_t n, int q,
+                                                         int rb, int64_t* flag) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  flag[r] = r == 0 || (post_off[r] >> q) != (post_off[r - 1] >> q) || (r >> rb) != ((r - 1) >> rb) ? 1 : 0;
+}
+
+// bid = inclusive scan of the flags (block of row r = bid[r] - 1): first row of each block, the row count as the
+// sentinel, and (stats != NULL) the block count, largest row count and largest posting count
+// (row0 has cap + 1 entries: writes beyond are dropped, so a wrong block count cannot write out of bounds)
+__global__ __launch_bounds__(256) void block_rows_kernel(const int64_t* __restrict__ post_off,
+                                                         const int64_t* __restrict__ bid, int64_t n, int64_t cap,
+                                                         int32_t* row0) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  if ((r == 0 || bid[r] != bid[r - 1]) && bid[r] - 1 <= cap) row0[bid[r] - 1] = (int32_t)r;
+  if (r == n - 1 && bid[r] <= cap) row0[bid[r]] = (int32_t)n;
+}
+
+__global__ __launch_bounds__(256) void block_stats_kernel(const int64_t* __restrict__ post_off,
+                                                          const int32_t* __restrict__ row0,
+                                                          const int64_t* __restrict__ bid, int64_t n,
+                                                          unsigned long long* stats) {
+  const int64_t nb = bid[n - 1];
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) stats[0] = (unsigned long long)nb;
+  if (b >= nb) return;
+  atomicMax(&stats[1], (unsigned long long)(row0[b + 1] - row0[b]));
+  atomicMax(&stats[2], (unsigned long long)(post_off[row0[b + 1]] - post_off[row0[b]]));
+}
+
+__global__ void block_stats_zero_kernel(unsigned long long* stats) {
+  if (threadIdx.x < 3) stats[threadIdx.x] = 0ULL;
+}
+
+// sorted block-layout keys -> first posting of each block, key (f32 bits) and ion << 15 | row in block
+__global__ __launch_bounds__(256) void block_split_kernel(const uint64_t* __restrict__ keys, int64_t n, KeyGeom G,
+                                                          int64_t n_blocks, uint32_t* off, uint32_t* key,
+                                                          uint16_t* row) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n == 0) {
+    for (int64_t b = i; b <= n_blocks; b += (int64_t)gridDim.x * blockDim.x) off[b] = 0;
+    return;
+  }
+  if (i >= n) return;
+  const uint64_t kk = keys[i];
+  const int64_t d = (int64_t)(kk >> (16 + G.nd));
+  const int64_t prev = i ? (int64_t)(keys[i - 1] >> (16 + G.nd)) : -1;
+  for (int64_t b = prev + 1; b <= d && b <= n_blocks; ++b) off[b] = (uint32_t)i;
+  key[i] = (uint32_t)((kk >> 16) & ((1ULL << G.nd) - 1ULL)) + G.fbase;
+  row[i] = (uint16_t)(kk & 0xFFFFULL);
+  if (i == n - 1)
+    for (int64_t b = d + 1; b <= n_blocks; ++b) off[b] = (uint32_t)n;
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int64_t n_tiles_of(int64_t n) { return (n + RS_TILE - 1) / RS_TILE; }
@@ -258,6 +331,12 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
   KeyGeom G;
   G.S = ix->bin_shift;
   G.bin_base = ix->bin_base;
+  G.block_mode = 0;
+  G.fbase = fb0;
+  G.nd = 1;
+  G.bid = nullptr;
+  G.brow0 = nullptr;
+  G.n_blocks = 0;
   G.nbk = 0;
   while (G.nbk < 32 && ((uint64_t)(ix->n_buckets - 1) >> G.nbk) != 0) ++G.nbk;
   int nbn = 1;
@@ -281,6 +360,103 @@ int vh_index_build(const VhParams* p, const VhIndexView* ix, const int64_t* post
   if (rc) return rc;
   unsigned g2 = (unsigned)((n > 0 ? n : 1024) + 255) / 256;
   directory_kernel<<<g2, 256, 0, st>>>(src, n, X, G, offsets, postings, n_dir);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+size_t vh_index_blocks_workspace_bytes(int64_t n_rows, int64_t n_postings) {
+  const size_t n = (size_t)(n_rows > 0 ? n_rows : 1);
+  return 2 * align256((size_t)(n_postings > 0 ? n_postings : 1) * 8) + vh_radix_ws_bytes(n_postings) +
+         align256(n * 8) + align256((n + 1) * 4) + align256(vh_scan_ws_i64(n_rows)) + 256;
+}
+
+// flags + scan: bid (n_rows i64) and row0 (n_blocks + 1, at most n_rows + 1 entries)
+static int block_layout(const int64_t* post_off, int64_t n, int q, int rb, int64_t* bid, int64_t cap, int32_t* row0,
+                        void* sws, size_t sws_bytes, hipStream_t st) {
+  const unsigned g = (unsigned)((n + 255) / 256);
+  block_flag_kernel<<<g, 256, 0, st>>>(post_off, n, q, rb, bid);
+  VH_CHECK_LAUNCH();
+  int rc = vh_scan_inclusive_i64(bid, n, sws, sws_bytes, st);
+  if (rc) return rc;
+  block_rows_kernel<<<g, 256, 0, st>>>(post_off, bid, n, cap, row0);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+static int block_args(const VhIndexView* ix, const int64_t* post_off, int32_t q_log2, int32_t row_log2) {
+  if (!ix || !post_off || ix->n_rows < 1) return VH_E_ARG;
+  if (q_log2 < 6 || q_log2 > 15 || row_log2 < 4 || row_log2 > 15) return VH_E_ARG;
+  if (ix->n_rows >= (1LL << 31) || ix->n_postings < 0 || ix->n_postings >= (1LL << 32)) return VH_E_LIMIT;
+  return 0;
+}
+
+int vh_index_block_count(const VhIndexView* ix, const int64_t* post_off, int32_t q_log2, int32_t row_log2,
+                         unsigned long long* stats, void* ws, size_t ws_bytes, void* stream) {
+  int rc = block_args(ix, post_off, q_log2, row_log2);
+  if (rc) return rc;
+  if (!stats) return VH_E_ARG;
+  if (ws_bytes < vh_index_blocks_workspace_bytes(ix->n_rows, 0)) return VH_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = ix->n_rows;
+  char* w = (char*)ws;
+  int64_t* bid = (int64_t*)w;
+  int32_t* row0 = (int32_t*)(w + align256((size_t)n * 8));
+  void* sws = w + align256((size_t)n * 8) + align256((size_t)(n + 1) * 4);
+  rc = block_layout(post_off, n, q_log2, row_log2, bid, n, row0, sws, vh_scan_ws_i64(n), st);
+  if (rc) return rc;
+  block_stats_zero_kernel<<<1, 64, 0, st>>>(stats);
+  block_stats_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(post_off, row0, bid, n, stats);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_index_build_blocks(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, int32_t q_log2,
+                          int32_t row_log2, int64_t n_blocks, int32_t* row0, uint32_t* off, uint32_t* key,
+                          uint16_t* row, void* ws, size_t ws_bytes, void* stream) {
+  int rc = block_args(ix, post_off, q_log2, row_log2);
+  if (rc) return rc;
+  if (!p || !row0 || !off || (ix->n_postings && (!key || !row)) || n_blocks < 1 || n_blocks > ix->n_rows)
+    return VH_E_ARG;
+  if (!(p->fmin > 0.f) || !(p->fmax >= p->fmin)) return VH_E_ARG;
+  if (ws_bytes < vh_index_blocks_workspace_bytes(ix->n_rows, ix->n_postings)) return VH_E_WORKSPACE;
+  uint32_t fb0, fb1;
+  memcpy(&fb0, &p->fmin, 4);
+  memcpy(&fb1, &p->fmax, 4);
+  KeyGeom G;
+  G.S = 0;
+  G.bin_base = 0;
+  G.block_mode = 1;
+  G.fbase = fb0;
+  G.nd = 1;
+  while (G.nd < 32 && ((uint64_t)(fb1 - fb0) >> G.nd) != 0) ++G.nd;
+  G.nbk = 1;
+  while (G.nbk < 40 && ((uint64_t)n_blocks >> G.nbk) != 0) ++G.nbk;
+  if (16 + G.nd + G.nbk > 64) return VH_E_LIMIT;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nr = ix->n_rows;
+  const int64_t n = ix->n_postings;
+  char* w = (char*)ws;
+  const size_t kb = align256((size_t)(n > 0 ? n : 1) * 8);
+  uint64_t* ka = (uint64_t*)w;
+  uint64_t* kb2 = (uint64_t*)(w + kb);
+  char* w2 = w + 2 * kb + vh_radix_ws_bytes(n);
+  int64_t* bid = (int64_t*)w2;
+  void* sws = w2 + align256((size_t)nr * 8) + align256((size_t)(nr + 1) * 4);
+  // the caller's row0 has n_blocks + 1 entries: n_blocks must be the count vh_index_block_count reported
+  rc = block_layout(post_off, nr, q_log2, row_log2, bid, n_blocks, row0, sws, vh_scan_ws_i64(nr), st);
+  if (rc) return rc;
+  G.bid = bid;
+  G.brow0 = row0;
+  G.n_blocks = n_blocks;
+  DevP P = vh_make_devp(p);
+  DevIdx X = vh_make_devidx(ix);
+  frag_emit_kernel<<<(unsigned)((nr + 255) / 256), 256, 0, st>>>(P, X, G, post_off, ka);
+  VH_CHECK_LAUNCH();
+  uint64_t* src = ka;
+  rc = vh_radix_sort_u64(ka, kb2, n, 16, 16 + G.nd + G.nbk, w + 2 * kb, vh_radix_ws_bytes(n), st, &src);
+  if (rc) return rc;
+  const unsigned g2 = (unsigned)((n > 0 ? n : 1024) + 255) / 256;
+  block_split_kernel<<<g2, 256, 0, st>>>(src, n, G, n_blocks, off, key, row);
   VH_CHECK_LAUNCH();
   return 0;
 }

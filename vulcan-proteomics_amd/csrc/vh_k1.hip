@@ -22,7 +22,7 @@
 // the high 40 bits (one u64 atomic per match). A winner's per-ion counts and intensity pseudo-logs travel in a
 // payload array beside the top-K key buffer (moved with the keys by the bitonic compaction); the keys themselves
 // are those of MODE 1, so the top-K set and order are exactly MODE 1's; no closure.
-#include "vh_common.h"
+#include "vh_topk.h"
 
 #define K1_MAXSEG 32
 #define K1_MAXBK 32
@@ -36,110 +36,6 @@ struct BkEnt {   // a bucket touched by a tile: its segments (bit mask), their r
   int bk, zmax, rlo, rhi;
   uint32_t mask;
   int pad0, pad1, pad2;
-};
-
-__device__ void bitonic_sort_u64(uint64_t* a, int n, bool descending) {
-  const int lane = lane_id();
-  for (int k = 2; k <= n; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < n; i += 64) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          uint64_t x = a[i], y = a[ixj];
-          bool up = ((i & k) == 0) != descending;   // ascending run in this block
-          if (up ? (x > y) : (x < y)) {
-            a[i] = y;
-            a[ixj] = x;
-          }
-        }
-      }
-      wave_sync();
-    }
-  }
-}
-
-// bitonic sort of keys a (descending) carrying the payloads b along
-__device__ void bitonic_sort_u64_pair(uint64_t* a, uint64_t* b, int n) {
-  const int lane = lane_id();
-  for (int k = 2; k <= n; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < n; i += 64) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          uint64_t x = a[i], y = a[ixj];
-          bool up = (i & k) != 0;
-          if (up ? (x > y) : (x < y)) {
-            a[i] = y;
-            a[ixj] = x;
-            uint64_t t = b[i];
-            b[i] = b[ixj];
-            b[ixj] = t;
-          }
-        }
-      }
-      wave_sync();
-    }
-  }
-}
-
-// sort the buffer descending, keep the best K; returns the new fill and updates the pruning threshold.
-// pay (feature mode): payload array moved with the keys
-__device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int capb, int nbuf, int K,
-                                            uint64_t& thresh) {
-  for (int i = nbuf + lane_id(); i < capb; i += 64) buf[i] = 0ULL;
-  wave_sync();
-  if (pay)
-    bitonic_sort_u64_pair(buf, pay, capb);
-  else
-    bitonic_sort_u64(buf, capb, true);
-  int n = nbuf < K ? nbuf : K;
-  thresh = n == K ? buf[K - 1] : 0ULL;
-  wave_sync();
-  return n;
-}
-
-// Intensity quantisation of the feature mode: each peak's intensity scaled by the power of two that maps the
-// spectrum's largest finite intensity into [2^15, 2^16), truncated to an integer (non-finite, negative, denormal
-// and NaN intensities count 0). vh_k1_features documents the same rule.
-__device__ __forceinline__ int iq_shift(float maxI) {
-  if (!(maxI >= 1.17549435e-38f) || maxI > 3.40282347e38f) return -1000;
-  int x;
-  frexpf(maxI, &x);
-  return 16 - x;
-}
-
-__device__ __forceinline__ uint32_t iq_quant(float I, int e) {
-  if (e == -1000 || !(I >= 1.17549435e-38f)) return 0u;
-  const float v = ldexpf(I, e);
-  return v >= 1.f ? (uint32_t)v : 0u;
-}
-
-// 14-bit pseudo-log2 of 1 + sum: exponent << 8 | the 8 bits below the leading one
-__device__ __forceinline__ uint32_t iq_log(uint64_t sum) {
-  const uint64_t x = sum + 1ULL;
-  const int e = 63 - __builtin_clzll(x);
-  const uint32_t m = e >= 8 ? (uint32_t)((x >> (e - 8)) & 0xFFULL) : (uint32_t)((x << (8 - e)) & 0xFFULL);
-  return ((uint32_t)e << 8) | m;
-}
-
-struct K1Out {
-  // append mode
-  int64_t cap;
-  unsigned long long* n_out;
-  int32_t *out_hyp, *out_row, *out_cnt;
-  // top-K mode
-  int cand_stride, capb, clos_n;
-  int32_t *n_cand, *c_row, *c_hyp, *c_cnt;
-  uint32_t* hyp_scored;
-  unsigned long long* hyp_matched;
-  int32_t* status;
-  int wave_bytes, hist_bytes;
-  uint32_t* queue;   // persistent mode work counter (NULL: grid mode)
-  // feature mode
-  const float* pk_int;
-  int16_t* c_nb;
-  uint32_t* c_iq;
-  int32_t* s_maxcnt;
 };
 
 // one spectrum s, processed by the calling wave in its LDS area wb
@@ -362,10 +258,7 @@ __device__ void k1_spectrum(const DevP& P, const DevIdx& X, int s, unsigned char
         if (MODE == 2) {
           const unsigned long long cb = act ? hist64[2 * (g.base + j)] : 0ULL;
           const unsigned long long cy = act ? hist64[2 * (g.base + j) + 1] : 0ULL;
-          const uint32_t nb = (uint32_t)(cb & 0xFFFFFFULL);
-          c = nb + (uint32_t)(cy & 0xFFFFFFULL);
-          pl = ((uint64_t)(nb < 32767u ? nb : 32767u) << 28) | ((uint64_t)iq_log(cb >> 24) << 14) |
-               (uint64_t)iq_log(cy >> 24);
+          pl = feat_payload(cb, cy, c);
         } else {
           c = act ? hist[g.base + j] : 0u;
         }
@@ -388,30 +281,7 @@ __device__ void k1_spectrum(const DevP& P, const DevIdx& X, int s, unsigned char
             }
           }
         } else {
-          uint64_t key = 0;
-          if (flag) {
-            uint32_t cc = c < 32767u ? c : 32767u;
-            clamped |= c > 32767u;
-            key = ((uint64_t)cc << 48) | ((uint64_t)(uint32_t)(g.r0 + j) << 20) | ((uint64_t)(63 - g_hl) << 14);
-          }
-          bool ins = flag && key > thresh;
-          uint64_t bal = ballot(ins);
-          int cnt = __popcll(bal);
-          if (cnt) {
-            if (nbuf + cnt > O.capb) {
-              nbuf = topk_compact(buf, pay, O.capb, nbuf, K, thresh);
-              ins = flag && key > thresh;
-              bal = ballot(ins);
-              cnt = __popcll(bal);
-            }
-            if (ins) {
-              const int at = nbuf + __popcll(bal & lanemask_lt());
-              buf[at] = key;
-              if (MODE == 2) pay[at] = pl;
-            }
-            nbuf += cnt;
-            wave_sync();
-          }
+          topk_offer<MODE>(flag, c, g.r0 + j, g_hl, pl, buf, pay, O.capb, K, nbuf, thresh, clamped);
         }
       }
       sc = wave_sum(sc);
@@ -429,76 +299,8 @@ __device__ void k1_spectrum(const DevP& P, const DevIdx& X, int s, unsigned char
   }
   if (MODE == 0) return;
 
-  // ---- final top-K, pair closure, dedupe by row
-  if (ballot(clamped)) {
-    if (lane == 0) atomicOr(O.status, 2);
-  }
-  const int ntop = topk_compact(buf, pay, O.capb, nbuf, K, thresh);
-  if (MODE == 2 && lane == 0) O.s_maxcnt[s] = ntop ? (int32_t)(buf[0] >> 48) : 0;
-  uint64_t* ck = (uint64_t*)wb;                      // reuses the histogram area
-  int32_t* cv = (int32_t*)(ck + O.clos_n);
-  const int ncl = (MODE == 1 && P.pair_closure) ? 2 * ntop : ntop;
-  for (int i = lane; i < O.clos_n; i += 64) {
-    uint64_t v = ~0ULL;
-    if (i < ntop) {
-      uint64_t key = buf[i];
-      uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
-      uint32_t hl = 63u - (uint32_t)((key >> 14) & 63ULL);
-      cv[i] = (int32_t)(key >> 48);
-      v = ((uint64_t)row << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
-    } else if (i < ncl) {
-      uint64_t key = buf[i - ntop];
-      uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
-      uint32_t hl = 63u - (uint32_t)((key >> 14) & 63ULL);
-      uint32_t pr = (uint32_t)X.partner[row];
-      cv[i] = -1;
-      v = ((uint64_t)pr << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
-    }
-    ck[i] = v;
-  }
-  wave_sync();
-  bitonic_sort_u64(ck, O.clos_n, false);
-  int nout = 0;
-  int64_t ob = (int64_t)s * O.cand_stride;
-  for (int i0 = 0; i0 < ncl; i0 += 64) {
-    const int i = i0 + lane;
-    bool first = false;
-    uint32_t row = 0, hl = 0;
-    int32_t cc = -1;
-    uint32_t best = 0;
-    if (i < ncl) {
-      uint64_t v = ck[i];
-      row = (uint32_t)(v >> 32);
-      hl = (uint32_t)((v >> 24) & 0xFFu);
-      first = i == 0 || (uint32_t)(ck[i - 1] >> 32) != row;
-      if (first) {
-        best = (uint32_t)(v & 0xFFFFFFu);
-        cc = cv[best];
-        for (int j = i + 1; j < ncl && (uint32_t)(ck[j] >> 32) == row; ++j) {
-          const uint32_t q = (uint32_t)(ck[j] & 0xFFFFFFu);
-          // the duplicate with the larger count (ties: lower hypothesis) gives the feature payload
-          if (cv[q] > cc) {
-            cc = cv[q];
-            best = q;
-          }
-        }
-      }
-    }
-    uint64_t bal = ballot(first);
-    if (first) {
-      int pos = nout + __popcll(bal & lanemask_lt());
-      O.c_row[ob + pos] = (int32_t)row;
-      O.c_hyp[ob + pos] = (int32_t)(h0 + hl);
-      O.c_cnt[ob + pos] = cc;
-      if (MODE == 2) {
-        const uint64_t pv = pay[best];
-        O.c_nb[ob + pos] = (int16_t)((pv >> 28) & 0x7FFFULL);
-        O.c_iq[ob + pos] = (uint32_t)(((pv >> 14) & 0x3FFFULL) << 16) | (uint32_t)(pv & 0x3FFFULL);
-      }
-    }
-    nout += __popcll(bal);
-  }
-  if (lane == 0) O.n_cand[s] = nout;
+  // ---- final top-K, pair closure, dedupe by row (the closure keys reuse the histogram area)
+  k1_finish<MODE>(P, X, s, h0, buf, pay, nbuf, thresh, clamped, (uint64_t*)wb, O);
 }
 
 // Grid mode: wave wv takes order[wv] (XCD-banded block order). Persistent mode (O.queue set): every wave takes the

@@ -153,6 +153,7 @@ static size_t scan_ws(int64_t n, size_t elem) {
 }
 
 size_t vh_scan_ws_u32(int64_t n) { return scan_ws(n, sizeof(uint32_t)); }
+size_t vh_scan_ws_i64(int64_t n) { return scan_ws(n, sizeof(int64_t)); }
 
 template <typename T, bool INCLUSIVE>
 static int scan_any(T* a, int64_t n, void* ws, size_t ws_bytes, hipStream_t st) {
@@ -192,7 +193,9 @@ extern "C" {
 
 int vh_version(void) { return VH_ABI_VERSION; }
 
-size_t vh_abi_sizeof(int which) { return which == 0 ? sizeof(VhParams) : sizeof(VhIndexView); }
+size_t vh_abi_sizeof(int which) {
+  return which == 0 ? sizeof(VhParams) : which == 1 ? sizeof(VhIndexView) : which == 2 ? sizeof(VhBlockView) : 0;
+}
 
 size_t vh_scan_workspace_bytes(int64_t n) { return scan_ws(n, sizeof(int64_t)); }
 

@@ -29,6 +29,11 @@ class VhIndexView(C.Structure):
                 ("n_buckets", I64), ("offsets", P), ("postings", P), ("n_postings", I64)]
 
 
+class VhBlockView(C.Structure):
+    _fields_ = [("n_blocks", I64), ("max_rows", I32), ("max_postings", I32), ("row0", P), ("off", P), ("key", P),
+                ("row", P)]
+
+
 def mod_arrays(static_mods: dict, var_mods: dict):
     """f32 static/variable tables and the N-term delta as the reference sets them (engine.py:71-82)."""
     stat = np.zeros(26, np.float32)

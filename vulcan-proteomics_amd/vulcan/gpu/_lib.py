@@ -12,12 +12,13 @@ import ctypes as C
 import torch
 
 from .. import _build
-from ._abi import VhIndexView, VhParams
+from ._abi import VhBlockView, VhIndexView, VhParams
 
 P = C.c_void_p
 I32, I64, SZ, F32 = C.c_int32, C.c_int64, C.c_size_t, C.c_float
 PP = C.POINTER(VhParams)
 PIX = C.POINTER(VhIndexView)
+PBV = C.POINTER(VhBlockView)
 
 _SIGS = {
     "vh_version": ([], I32),
@@ -27,6 +28,9 @@ _SIGS = {
     "vh_frag_count": ([PP, PIX, P, P, SZ, P], I32),
     "vh_index_workspace_bytes": ([I64, I64, I32], SZ),
     "vh_index_build": ([PP, PIX, P, P, P, P, SZ, P], I32),
+    "vh_index_blocks_workspace_bytes": ([I64, I64], SZ),
+    "vh_index_block_count": ([PIX, P, I32, I32, P, P, SZ, P], I32),
+    "vh_index_build_blocks": ([PP, PIX, P, I32, I32, I64, P, P, P, P, P, SZ, P], I32),
     "vh_k0": ([PP, I32, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_spectrum_order_workspace_bytes": ([I32], SZ),
@@ -39,6 +43,11 @@ _SIGS = {
     "vh_candidate_shift": ([I32, P, P, I32, P, P, P, P, P, P, P], I32),
     "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
     "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_k1j_workspace_bytes": ([I64, I64], SZ),
+    "vh_k1j_pairs": ([PBV, I32, P, P, P, I64, P, I64, P, P, I64, P, P, SZ, P], I32),
+    "vh_k1j_count": ([PP, PBV, I32, P, P, I64, P, P, P, P, P, I64, P, P, P, P, P], I32),
+    "vh_k1j_topk": ([PP, PIX, I32, P, P, P, P, P, P, I64, P, I32, P, P, P, P, P, P, P, P, P], I32),
+    "vh_k1j_features": ([PP, PIX, I32, P, P, P, P, P, P, I64, P, I32, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_mlp_pack": ([P, P, P, P, P], I32),
     "vh_mlp_forward": ([P, I64, P, P, P, P, P], I32),
     "vh_mlp_workspace_bytes": ([I64], SZ),

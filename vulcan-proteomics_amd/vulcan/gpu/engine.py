@@ -19,7 +19,7 @@ import pyarrow as pa
 import torch
 
 from . import _lib as K
-from ._abi import VhIndexView, index_geometry, make_params, max_hyp_per_spectrum, set_modes
+from ._abi import VhBlockView, VhIndexView, index_geometry, make_params, max_hyp_per_spectrum, set_modes
 from .host import NEUTRON, PROTON, SAGE_MONO, SearchParams, Spectra, make_hypotheses, pre_hypotheses  # noqa: F401
 
 ROW_BITS = 28   # peptidoform rows < 2^28 (engine.py:35 of the reference)
@@ -105,11 +105,50 @@ class GpuIndex:
         torch.cuda.synchronize()
         del ws
         self.slices = [Slice(0, n_buckets, self.offsets, self.postings)]
+        self._blocks = None
         self.inv_logw = None
         self.build_s = time.perf_counter() - t0
         self.device_bytes = sum(t.numel() * t.element_size() for t in
                                 [self.seq_off, self.length, self.mask, self.mass32, self.residues, self.partner,
                                  self.offsets, self.postings])
+
+    def blocks(self) -> VhBlockView:
+        """Block layout of the index for the bucket-join K1 (vh_index_build_blocks), built on first use:
+        consecutive rows cut into blocks of ~2^block_postings_log2 postings (at most 2^block_rows_log2 rows),
+        postings key-sorted inside a block."""
+        if self._blocks is not None:
+            return self._blocks["view"]
+        t0 = time.perf_counter()
+        L = K.lib()
+        d, st = self.dev, K.stream()
+        q, rb = int(self.p.block_postings_log2), int(self.p.block_rows_log2)
+        wsb = int(L.vh_index_blocks_workspace_bytes(self.n, self.n_postings))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=d)
+        stats = torch.zeros(3, dtype=torch.int64, device=d)
+        K.check(L.vh_index_block_count(C.byref(self.view), K.ptr(self.post_off), q, rb, K.ptr(stats), K.ptr(ws), wsb,
+                                       st), "index_block_count")
+        nb, max_rows, max_post = (int(x) for x in stats.tolist())
+        b = {"row0": torch.empty(nb + 1, dtype=torch.int32, device=d),
+             "off": torch.empty(nb + 1, dtype=torch.int32, device=d),
+             "key": torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d),
+             "row": torch.empty(max(self.n_postings, 1), dtype=torch.int16, device=d)}
+        K.check(L.vh_index_build_blocks(C.byref(self.vp), C.byref(self.view), K.ptr(self.post_off), q, rb, nb,
+                                        K.ptr(b["row0"]), K.ptr(b["off"]), K.ptr(b["key"]), K.ptr(b["row"]),
+                                        K.ptr(ws), wsb, st), "index_build_blocks")
+        torch.cuda.synchronize()
+        del ws
+        v = VhBlockView()
+        v.n_blocks, v.max_rows, v.max_postings = nb, max_rows, max_post
+        v.row0, v.off, v.key, v.row = (K.ptr(b[k]) for k in ("row0", "off", "key", "row"))
+        b["view"] = v
+        b["build_s"] = time.perf_counter() - t0
+        self._blocks = b
+        self.device_bytes += sum(b[k].numel() * b[k].element_size() for k in ("row0", "off", "key", "row"))
+        return v
+
+    def block_arrays(self) -> dict:
+        self.blocks()
+        return self._blocks
 
     def params_for(self, p: SearchParams):
         """VhParams for a SearchParams that shares this index's residue tables (topk / tile / tolerance may
@@ -460,8 +499,16 @@ class DeviceSearch:
 
     def __init__(self, gi: GpuIndex, p: SearchParams, n_spec: int, report: int = 1, offsets=(), open_da=None,
                  k1_threshold_open: int = 1, persistent: bool = False, order_mode: str = "mass",
-                 band_log2: int = 28):
+                 band_log2: int = 28, k1_impl: str | None = None, join_features: bool = False):
         self.gi, self.p, self.n_spec, self.report = gi, p, n_spec, report
+        # k1_impl: "tile" (vh_k1_topk, spectrum-major) or "join" (vh_k1j_*, block-major); "auto" picks join
+        # unless an open-search window is on (its windows span 10^5 rows: per-row slots would not pay)
+        impl = k1_impl or p.k1_impl
+        if impl == "auto":
+            impl = "tile" if open_da is not None else "join"
+        if impl not in ("tile", "join"):
+            raise ValueError(impl)
+        self.k1_impl = impl
         # persistent: K1 and K2 run as resident waves pulling spectra from an atomic work counter.
         # order_mode: "mass" (first window row, vh_spectrum_order), "work" (mass band of 2^band_log2 rows, then
         # estimated work descending, vh_spectrum_order_work) or "fixed" (self.order is set by the caller)
@@ -499,6 +546,75 @@ class DeviceSearch:
         self.k2 = _k2_out(cap_c, dev)
         self.sel = e(n * report, torch.int64)
         self.nxt = e(n * report, torch.float64)
+        # join_features: the join K1 keeps per-ion counts and intensity sums per slot (vh_k1j_features)
+        self.join = self._join_buffers(cap_h, maxh, join_features) if impl == "join" else None
+
+    # ---------------------------------------------------------------------------------------- bucket-join K1
+    def _join_buffers(self, cap_h: int, maxh: int, feat: bool = False) -> dict:
+        """Pair / slot buffers of the join K1. Capacities start from the index's mean precursor window (rows
+        within the precursor tolerance of each row's mass, averaged over the table) and are fitted to the batch
+        on the first run (one host read of the totals)."""
+        gi, dev = self.gi, self.gi.dev
+        bv = gi.blocks()
+        L = K.lib()
+        m = gi.mass32
+        p = self.p
+        lo = m * (1.0 + p.precursor_ppm[0] * 1e-6)
+        hi = m * (1.0 + p.precursor_ppm[1] * 1e-6)
+        rows = (torch.searchsorted(m, hi, right=True) - torch.searchsorted(m, lo)).float()
+        mean_rows = float(rows.mean().item()) if len(m) else 1.0
+        rows_per_block = max(gi.n / max(bv.n_blocks, 1), 1.0)
+        n_h = max(self.n_spec, 1) * min(maxh, 4)       # typical hypotheses per spectrum; fitted on the first run
+        slot_cap = int(n_h * (1.5 * mean_rows + 8)) + 4096
+        pair_cap = int(n_h * (1.5 * mean_rows / rows_per_block + 2)) + 4096
+        j = {"feat": feat, "cap_h": cap_h,
+             "h_slot": torch.empty(cap_h + 1, dtype=torch.int64, device=dev),
+             "blk_pairs": torch.empty(bv.n_blocks + 1, dtype=torch.int32, device=dev),
+             "ws": torch.empty(int(L.vh_k1j_workspace_bytes(bv.n_blocks, cap_h)), dtype=torch.uint8, device=dev),
+             "fitted": False}
+        self._join_alloc(j, slot_cap, pair_cap)
+        return j
+
+    def _join_alloc(self, j: dict, slot_cap: int, pair_cap: int):
+        dev = self.gi.dev
+        j["slot_cap"], j["pair_cap"] = slot_cap, pair_cap
+        j["pairs"] = torch.empty(max(pair_cap, 1), dtype=torch.int32, device=dev)
+        if j["feat"]:
+            j["counts"] = torch.empty(2 * max(slot_cap, 1), dtype=torch.int64, device=dev)
+        else:
+            j["counts"] = torch.empty(max(slot_cap, 1), dtype=torch.int32, device=dev)
+
+    def _join_pairs(self, n: int):
+        L = K.lib()
+        j, hy = self.join, self.hyp
+        bv = self.gi.blocks()
+        K.check(L.vh_k1j_pairs(C.byref(bv), n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]), j["cap_h"],
+                               K.ptr(j["h_slot"]), j["slot_cap"], K.ptr(j["blk_pairs"]), K.ptr(j["pairs"]),
+                               j["pair_cap"], K.ptr(self.status), K.ptr(j["ws"]), j["ws"].numel(), K.stream()),
+                "k1j_pairs")
+
+    def join_front(self, n: int, mark):
+        """Pairs + per-block counts of the join K1 (fits the pair / slot capacities on the first call)."""
+        L = K.lib()
+        j, hy, pk = self.join, self.hyp, self.peaks
+        bv = self.gi.blocks()
+        mark("k1j_pairs")
+        self._join_pairs(n)
+        if not j["fitted"]:
+            tot_slots = int(j["h_slot"][j["cap_h"]].item())
+            tot_pairs = int(j["blk_pairs"][bv.n_blocks].item())
+            if tot_slots > j["slot_cap"] or tot_pairs > j["pair_cap"]:
+                self._join_alloc(j, max(tot_slots * 5 // 4, j["slot_cap"]), max(tot_pairs * 5 // 4, j["pair_cap"]))
+                self.status.zero_()
+                self._join_pairs(n)
+            j["fitted"] = True
+        mark("k1j_pairs")
+        mark("k1j_count")
+        K.check(L.vh_k1j_count(C.byref(self.vp), C.byref(bv), 2 if j["feat"] else 1, K.ptr(j["blk_pairs"]),
+                               K.ptr(j["pairs"]), j["pair_cap"], K.ptr(hy["spec"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
+                               K.ptr(hy["maxfz"]), K.ptr(j["h_slot"]), j["slot_cap"], K.ptr(pk["off"]),
+                               K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(j["counts"]), K.stream()), "k1j_count")
+        mark("k1j_count")
 
     def run(self, raw: dict, ev: dict | None = None):
         """ev (optional): stage name -> list, receives a (start, end) pair of torch.cuda.Event per stage,
@@ -543,6 +659,18 @@ class DeviceSearch:
         L = K.lib()
         gi, vp = self.gi, self.vp
         pk, hy = self.peaks, self.hyp
+        if self.k1_impl == "join":
+            self.join_front(n, mark)
+            j = self.join
+            mark("k1j_select")
+            K.check(L.vh_k1j_topk(C.byref(vp), C.byref(gi.view), n, K.ptr(self.order), K.ptr(hy["hyp_off"]),
+                                  K.ptr(hy["mode"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]), K.ptr(j["h_slot"]),
+                                  j["slot_cap"], K.ptr(j["counts"]), self.stride, K.ptr(self.n_cand),
+                                  K.ptr(self.c_row), K.ptr(self.c_hyp), K.ptr(self.c_cnt), K.ptr(self.scored),
+                                  K.ptr(self.matched), K.ptr(self.status), K.ptr(self.queue), K.stream()),
+                    "k1j_topk")
+            mark("k1j_select")
+            return
         mark("k1_topk")
         K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.view), n, K.ptr(self.order), K.ptr(pk["off"]),
                              K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["mode"]),
@@ -597,8 +725,11 @@ def device_table(ds: DeviceSearch, sp: Spectra, gi: GpuIndex, l1, p: SearchParam
     k2["spec"] = torch.repeat_interleave(torch.arange(n, device=dev), ds.n_cand[:n].long())
     hyp = hyp_to_host(ds.hyp)
     tbl, hh = psm_table(sp, gi, l1, hyp, h, r, c, k2, ds.scored, ds.matched, ds.peaks, p, report, return_hyp=True)
-    if int(ds.status.item()) & 1:
+    stt = int(ds.status.item())
+    if stt & 1:
         raise RuntimeError("k1_topk: a spectrum has more hypotheses than one wave handles")
+    if stt & 8:
+        raise RuntimeError("k1j: pair / slot capacity exceeded")
     return tbl, hh, {"h": h, "r": r, "k1c": c, "k2": k2, "hyp": hyp}
 
 

@@ -63,7 +63,7 @@ class ClassSchedule(DeviceSearch):
     parameters (`table`: class -> {"tile", "xcd_group", "persistent"}; missing classes use the defaults)."""
 
     def __init__(self, gi: GpuIndex, p: SearchParams, sp: Spectra, table: dict | None = None, report: int = 1):
-        super().__init__(gi, p, len(sp), report=report, persistent=True, order_mode="fixed")
+        super().__init__(gi, p, len(sp), report=report, persistent=True, order_mode="fixed", k1_impl="tile")
         cls = spectrum_classes(sp, p)
         o = np.lexsort((np.asarray(sp.precursor_mz), cls))
         self.order.copy_(torch.from_numpy(o.astype(np.int32)).to(gi.dev))
