@@ -192,8 +192,8 @@ def algorithmic_bytes(ds, gi, n, l1):
 def algorithmic_bytes_join(ds, gi, n):
     """Per-launch algorithmic HBM bytes of the bucket-join K1 stages on this batch.
     k1j_count: 6 B per posting of every block with at least one pair (key + row, read once) + 8 B block bounds,
-        per pair its hypothesis (id 4, spectrum 4, window 16, maxfz 1, slot 8) and the spectrum's peak masses
-        (+ intensities in feature mode), and 4 B (16 B feature mode) per window-row slot written;
+        per pair its 32-byte record (peaks, slot, window, peak count, maxfz) and the spectrum's peak masses
+        (+ intensities in feature mode), and 4 B (8 B feature mode) per window-row slot written;
     k1j_select: the slots read back, 12 B per candidate written, per hypothesis window/slot/mode in (25 B) and
         scored/matched out (12 B)."""
     import numpy as np
@@ -206,14 +206,13 @@ def algorithmic_bytes_join(ds, gi, n):
     n_pairs = int(bp[-1])
     hyp_off = ds.hyp["hyp_off"][: n + 1].cpu().numpy()
     nh = int(hyp_off[-1])
-    pairs = j["pairs"][:n_pairs].cpu().numpy().view(np.uint32).astype(np.int64)
-    spec = ds.hyp["spec"][:nh].cpu().numpy()
-    npk = np.diff(ds.peaks["off"][: n + 1].cpu().numpy())
+    rec = j["pairs"][:n_pairs].cpu().numpy()
+    np_pair = rec[:, 3] & 0xFFFFFFFF                                              # peak count field
     slots = int(j["h_slot"][nh].item())
-    sw = 16 if j["feat"] else 4
+    sw = 8 if j["feat"] else 4
     pk_b = 8 if j["feat"] else 4
-    count = int(6 * np.diff(off)[touched].sum() + 8 * touched.sum() + 33 * n_pairs
-                + pk_b * npk[spec[pairs]].sum() + sw * slots)
+    count = int(6 * np.diff(off)[touched].sum() + 8 * touched.sum() + 32 * n_pairs + pk_b * np_pair.sum()
+                + sw * slots)
     nc = ds.n_cand[:n].cpu().numpy().astype(np.int64)
     select = int(sw * slots + 12 * nc.sum() + 37 * nh)
     return {"k1j_count": count, "k1j_select": select, "n_pairs": n_pairs, "n_slots": slots,

@@ -231,6 +231,19 @@ int vh_k1_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, con
                    int16_t* c_nb, uint32_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored,
                    unsigned long long* hyp_matched, int32_t* status, uint32_t* queue, void* stream);
 
+/* The per-candidate features of vh_k1_features (c_nb, c_iq, s_maxcnt; same definitions) computed after the
+ * selection for the listed candidates only: for each candidate row the b / y matches of its indexed fragments with
+ * the spectrum's kept peaks (tol_bound(mass * z, lo) <= f <= tol_bound(mass * z, hi), z < maxfz), counted and
+ * intensity-summed (quantised as in vh_k1_features) against the mass-sorted peaks; among the spectrum's hypotheses
+ * whose window holds the row, the largest count (ties: lower hypothesis) gives the features. The lists are those of
+ * vh_k1_topk / vh_k1j_topk without pair closure (cand_stride slots per spectrum); status |= 32 if a recomputed
+ * count differs from c_cnt (clamped at 32767). */
+int vh_k1_candidate_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
+                             const int64_t* pk_off, const float* pk_mass, const float* pk_int,
+                             const int64_t* hyp_off, const int8_t* h_maxfz, const int64_t* h_lo, const int64_t* h_hi,
+                             int32_t cand_stride, const int32_t* n_cand, const int32_t* c_row, const int32_t* c_cnt,
+                             int16_t* c_nb, uint32_t* c_iq, int32_t* s_maxcnt, int32_t* status, void* stream);
+
 /* Brute-force counts for given (hypothesis, row) pairs without the index (k1_brute_kernel,
  * kernels.cu:195-232): the definition K1 must equal. */
 int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32_t* c_hyp, const int32_t* c_row,
@@ -242,22 +255,30 @@ int vh_k1_brute(const VhParams* p, const VhIndexView* ix, int64_t n, const int32
  * a few hundred rows). Three calls on the block layout (VhBlockView):
  *  1. vh_k1j_pairs: for every hypothesis h < hyp_off[n_spec]: its slots (one per window row; h_slot, max_hyp + 1
  *     entries, h_slot[h] = first slot of h, h_slot[n_hyp] = total) and one (block, h) pair per block its window
- *     overlaps, grouped by block: pairs[blk_pairs[b] .. blk_pairs[b + 1]) are the hypotheses of block b
- *     (blk_pairs: n_blocks + 1). status |= 8 when pair_cap or slot_cap is too small (results then incomplete).
+ *     overlaps, grouped by block: pairs[blk_pairs[b] .. blk_pairs[b + 1]) are the pairs of block b (blk_pairs:
+ *     n_blocks + 1), each a 32-byte record {i64 first peak of the spectrum, i64 h_slot[h], i32 lo, i32 hi,
+ *     i32 peak count, i32 maxfz | (s_qe + 2048) << 16}. status |= 8 when pair_cap or slot_cap is too small
+ *     (results then incomplete; slots past slot_cap are never written).
  *  2. vh_k1j_count: per block, the block's postings are staged once in LDS and every pair's window rows in the
  *     block get their shared-peak count (the K1 count of vh_k1) at counts[h_slot[h] + row - lo]: u32 (mode 1)
- *     or, mode 2, two u64 per slot (b then y: count | quantised intensity sum << 24, as vh_k1_features).
+ *     or, mode 2, one u64 per slot: count << 43 | min(b-ion count, 32767) << 28 | iq_log(b intensity sum) << 14 |
+ *     iq_log(y intensity sum) (the per-ion terms of vh_k1_features); a count >= 2^20 is clamped and sets
+ *     status bit 16.
  *  3. vh_k1j_topk (mode 1 counts) / vh_k1j_features (mode 2 counts): one wave per spectrum reads its slots,
  *     writes scored / matched per hypothesis and the top-K lists exactly as vh_k1_topk / vh_k1_features.
  * The reference computes the same lists with k1_kernel + select_topk (kernels.cu:112-190, engine.py:284-312). */
 size_t vh_k1j_workspace_bytes(int64_t n_blocks, int64_t max_hyp);
-int vh_k1j_pairs(const VhBlockView* bv, int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo,
-                 const int64_t* h_hi, int64_t max_hyp, int64_t* h_slot, int64_t slot_cap, uint32_t* blk_pairs,
-                 uint32_t* pairs, int64_t pair_cap, int32_t* status, void* ws, size_t ws_bytes, void* stream);
+int vh_k1j_pairs(const VhBlockView* bv, int32_t n_spec, const int64_t* hyp_off, const int32_t* h_spec,
+                 const int64_t* h_lo, const int64_t* h_hi, const int8_t* h_maxfz, const int64_t* pk_off,
+                 const int32_t* s_qe, int64_t max_hyp, int64_t* h_slot, int64_t slot_cap, uint32_t* blk_pairs,
+                 void* pairs, int64_t pair_cap, int32_t* status, void* ws, size_t ws_bytes, void* stream);
 int vh_k1j_count(const VhParams* p, const VhBlockView* bv, int32_t mode, const uint32_t* blk_pairs,
-                 const uint32_t* pairs, int64_t pair_cap, const int32_t* h_spec, const int64_t* h_lo,
-                 const int64_t* h_hi, const int8_t* h_maxfz, const int64_t* h_slot, int64_t slot_cap,
-                 const int64_t* pk_off, const float* pk_mass, const float* pk_int, void* counts, void* stream);
+                 const void* pairs, int64_t pair_cap, const float* pk_mass, const float* pk_int, void* counts,
+                 int32_t* status, void* stream);
+/* Feature mode helper, called before vh_k1j_pairs: s_qe[s] = the intensity scale shift of spectrum s's kept
+ * peaks (the power of two that brings its largest finite intensity into [2^15, 2^16), -1000 if none), the
+ * quantisation of vh_k1_features; the pair records carry it (s_qe may be NULL for mode 1). */
+int vh_k1j_iq_shift(int32_t n_spec, const int64_t* pk_off, const float* pk_int, int32_t* s_qe, void* stream);
 int vh_k1j_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order,
                 const int64_t* hyp_off, const uint8_t* h_mode, const int64_t* h_lo, const int64_t* h_hi,
                 const int64_t* h_slot, int64_t slot_cap, const uint32_t* counts, int32_t cand_stride,

@@ -73,12 +73,13 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert L.vh_index_block_count(C.byref(ix), C.c_void_p(8), 12, 10, C.c_void_p(8), None, 0, None) == -3
     assert L.vh_index_blocks_workspace_bytes(1000, 5000) > 2 * 5000 * 8
     bv = VhBlockView()
-    assert L.vh_k1j_pairs(None, 1, None, None, None, 10, None, 0, None, None, 0, None, None, 0, None) == -1
-    assert L.vh_k1j_pairs(C.byref(bv), 1, None, None, None, 10, None, 0, None, None, 0, None, None, 0, None) == -1
+    assert L.vh_k1j_pairs(None, 1, *[None] * 7, 10, None, 0, None, None, 0, None, None, 0, None) == -1
+    assert L.vh_k1j_pairs(C.byref(bv), 1, *[None] * 7, 10, None, 0, None, None, 0, None, None, 0, None) == -1
     bv.n_blocks, bv.max_rows, bv.max_postings, bv.row0, bv.off = 4, 64, 70000, 8, 8
-    assert L.vh_k1j_count(C.byref(vp), C.byref(bv), 1, *[None] * 2, 0, *[None] * 5, 0, *[None] * 4, None) == -4
+    assert L.vh_k1j_count(C.byref(vp), C.byref(bv), 1, None, None, 0, *[None] * 4, None) == -4
     bv.max_postings = 4096
-    assert L.vh_k1j_count(C.byref(vp), C.byref(bv), 3, *[None] * 2, 0, *[None] * 5, 0, *[None] * 4, None) == -1
+    assert L.vh_k1j_count(C.byref(vp), C.byref(bv), 3, None, None, 0, *[None] * 4, None) == -1
+    assert L.vh_k1j_iq_shift(2, None, None, None, None) == -1
     assert L.vh_k1j_workspace_bytes(1000, 5000) >= 4000
     assert L.vh_k1j_topk(C.byref(vp), C.byref(ix), 1, *[None] * 6, 0, None, 2, *[None] * 8, None) == -1
     assert L.vh_k1j_features(C.byref(vp), C.byref(ix), 1, *[None] * 6, 0, None, 2, *[None] * 11, None) == -1

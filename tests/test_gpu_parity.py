@@ -325,8 +325,17 @@ def test_join_pairs_and_counts_equal_oracle(which, request):
     row0 = _np(b["row0"]).astype(np.int64)
     nb = len(row0) - 1
     bp = _np(j["blk_pairs"]).view(np.uint32).astype(np.int64)
-    pairs = _np(j["pairs"]).view(np.uint32).astype(np.int64)
+    rec = _np(j["pairs"])
     lo, hi = env.hyp["lo"], env.hyp["hi"]
+    # records: (first peak, slot, lo | hi << 32, peak count | maxfz << 32); the slot names the hypothesis
+    pairs = np.searchsorted(hs[:nh + 1], rec[:, 1], side="right") - 1
+    w32 = rec[:, 2:].copy().view(np.int32)
+    po = _np(ds.peaks["off"])
+    spec = env.hyp["spec"][pairs[:bp[nb]]]
+    assert np.array_equal(rec[:bp[nb], 0], po[spec])
+    assert np.array_equal(w32[:bp[nb], 0], lo[pairs[:bp[nb]]]) and np.array_equal(w32[:bp[nb], 1], hi[pairs[:bp[nb]]])
+    assert np.array_equal(w32[:bp[nb], 2], po[spec + 1] - po[spec])
+    assert np.array_equal(w32[:bp[nb], 3] & 0xFF, env.hyp["maxfz"][pairs[:bp[nb]]])
     want = [[] for _ in range(nb)]
     for h in range(nh):
         if hi[h] > lo[h]:

@@ -42,7 +42,7 @@ class FEnv:
         self.gi = E.GpuIndex(self.l1, self.p, d["static"], d["var"])
         self.n = len(sp)
         self.raw = E.upload_spectra(sp, self.gi.dev)
-        self.ds = E.DeviceSearch(self.gi, self.p, self.n, k1_impl=impl, join_features=True)
+        self.ds = E.DeviceSearch(self.gi, self.p, self.n, k1_impl=impl)
         self.ds.front(self.raw, self.n, E._marker(None))
         self.vpf = PF._feature_params(self.ds.vp, topk)
         self.fl = PF.FeatureLists(self.n, topk, self.gi.dev)

@@ -410,10 +410,10 @@ static int k1_check(const VhParams* p, const VhIndexView* ix) {
 
 static void k1_geometry(const VhParams* p, int mode, K1Out& O, int& waves) {
   int K = p->topk > 0 ? p->topk : 1;
-  O.capb = mode ? pow2_at_least(K + 64) : 0;
+  O.capb = mode ? pow2_at_least(K + 64 > 2 * K ? K + 64 : 2 * K) : 0;
   O.clos_n = mode ? pow2_at_least((2 * K) > 64 ? 2 * K : 64) : 0;
   int hb = p->tile * (mode == 2 ? 16 : 4);
-  int cb = O.clos_n * 12;
+  int cb = O.clos_n * 8;
   O.hist_bytes = ((hb > cb ? hb : cb) + 15) & ~15;
   O.wave_bytes = O.hist_bytes + K1_MAXSEG * (int)sizeof(Seg) + K1_MAXBK * (int)sizeof(BkEnt) + 64 * 4 + 64 * 8 +
                  O.capb * (mode == 2 ? 16 : 8);

@@ -50,19 +50,51 @@ __device__ void bitonic_sort_u64_pair(uint64_t* a, uint64_t* b, int n) {
   }
 }
 
-// sort the buffer descending, keep the best K; returns the new fill and updates the pruning threshold.
-// pay (feature mode): payload array moved with the keys
+// Keep the best K keys of the buffer (keys are unique: (row, local hypothesis) occurs once per spectrum), return
+// the new fill and set the pruning threshold (the K-th largest key once K are kept, else 0); pay (feature mode):
+// payloads moved with the keys. The kept keys stay in buffer order (unsorted): the K-th largest key T is found by
+// a bitwise search over bits 62..14 (key bits 13..0 and 63 are zero; each step counts the keys >= the candidate
+// with one LDS pass over the buffer), then keys >= T are ballot-compacted in place (every read of a chunk happens
+// before its writes, and writes never pass the read position).
 __device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int capb, int nbuf, int K,
                                             uint64_t& thresh) {
-  for (int i = nbuf + lane_id(); i < capb; i += 64) buf[i] = 0ULL;
-  wave_sync();
-  if (pay)
-    bitonic_sort_u64_pair(buf, pay, capb);
-  else
-    bitonic_sort_u64(buf, capb, true);
-  int n = nbuf < K ? nbuf : K;
-  thresh = n == K ? buf[K - 1] : 0ULL;
-  wave_sync();
+  (void)capb;
+  const int lane = lane_id();
+  if (nbuf <= K) {
+    uint64_t mn = ~0ULL;
+    for (int i = lane; i < nbuf; i += 64) mn = buf[i] < mn ? buf[i] : mn;
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t v = __shfl_xor(mn, o);
+      mn = v < mn ? v : mn;
+    }
+    thresh = nbuf == K ? mn : 0ULL;
+    return nbuf;
+  }
+  uint64_t T = 0;
+  for (int bit = 62; bit >= 14; --bit) {
+    const uint64_t c = T | (1ULL << bit);
+    int cnt = 0;
+    for (int i = lane; i < nbuf; i += 64) cnt += buf[i] >= c ? 1 : 0;
+    cnt = wave_sum(cnt);
+    if (cnt >= K) T = c;
+  }
+  int n = 0;
+  for (int i0 = 0; i0 < nbuf; i0 += 64) {
+    const int i = i0 + lane;
+    const uint64_t k = i < nbuf ? buf[i] : 0ULL;
+    const uint64_t pv = (pay && i < nbuf) ? pay[i] : 0ULL;
+    const bool keep = k >= T;   // T > 0, so lanes past the fill never keep
+    const uint64_t bal = ballot(keep);
+    wave_sync();
+    if (keep) {
+      const int at = n + __popcll(bal & lanemask_lt());
+      buf[at] = k;
+      if (pay) pay[at] = pv;
+    }
+    n += __popcll(bal);
+    wave_sync();
+  }
+  thresh = T;
   return n;
 }
 
@@ -154,7 +186,7 @@ __device__ __forceinline__ uint64_t feat_payload(unsigned long long cb, unsigned
 
 // Final stage of one spectrum: top-K of the buffer, pair closure (MODE 1 with pair_closure: every winner's
 // partner row with count -1), dedupe by row (lowest hypothesis, highest count), candidates in row order at
-// s * cand_stride. ck: clos_n u64 followed by clos_n i32 of wave-private LDS.
+// s * cand_stride. ck: clos_n u64 of wave-private LDS; the sorted entries go back into buf (capb >= 2K).
 template <int MODE>
 __device__ __forceinline__ void k1_finish(const DevP& P, const DevIdx& X, int s, int64_t h0, uint64_t* buf,
                                           uint64_t* pay, int nbuf, uint64_t thresh, bool clamped, uint64_t* ck,
@@ -165,58 +197,79 @@ __device__ __forceinline__ void k1_finish(const DevP& P, const DevIdx& X, int s,
     if (lane == 0) atomicOr(O.status, 2);
   }
   const int ntop = topk_compact(buf, pay, O.capb, nbuf, K, thresh);
-  if (MODE == 2 && lane == 0) O.s_maxcnt[s] = ntop ? (int32_t)(buf[0] >> 48) : 0;
-  int32_t* cv = (int32_t*)(ck + O.clos_n);
-  const int ncl = (MODE == 1 && P.pair_closure) ? 2 * ntop : ntop;
-  for (int i = lane; i < O.clos_n; i += 64) {
-    uint64_t v = ~0ULL;
-    if (i < ntop) {
-      uint64_t key = buf[i];
-      uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
-      uint32_t hl = 63u - (uint32_t)((key >> 14) & 63ULL);
-      cv[i] = (int32_t)(key >> 48);
-      v = ((uint64_t)row << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
-    } else if (i < ncl) {
-      uint64_t key = buf[i - ntop];
-      uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
-      uint32_t hl = 63u - (uint32_t)((key >> 14) & 63ULL);
-      uint32_t pr = (uint32_t)X.partner[row];
-      cv[i] = -1;
-      v = ((uint64_t)pr << 32) | ((uint64_t)hl << 24) | (uint64_t)i;
+  if (MODE == 2) {
+    uint64_t mx = 0;
+    for (int i = lane; i < ntop; i += 64) mx = buf[i] > mx ? buf[i] : mx;
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t v = __shfl_xor(mx, o);
+      mx = v > mx ? v : mx;
     }
-    ck[i] = v;
+    if (lane == 0) O.s_maxcnt[s] = (int32_t)(mx >> 48);
+  }
+  // closure entries: row << 36 | hypothesis << 30 | (count + 1) << 11 | entry (count -1 = partner row)
+  const int ncl = (MODE == 1 && P.pair_closure) ? 2 * ntop : ntop;
+  for (int i = lane; i < ncl; i += 64) {
+    const uint64_t key = buf[i < ntop ? i : i - ntop];
+    uint32_t row = (uint32_t)((key >> 20) & 0xFFFFFFFULL);
+    const uint32_t hl = 63u - (uint32_t)((key >> 14) & 63ULL);
+    uint32_t c1 = (uint32_t)(key >> 48) + 1u;
+    if (i >= ntop) {
+      row = (uint32_t)X.partner[row];
+      c1 = 0u;
+    }
+    ck[i] = ((uint64_t)row << 36) | ((uint64_t)hl << 30) | ((uint64_t)c1 << 11) | (uint64_t)i;
   }
   wave_sync();
-  bitonic_sort_u64(ck, O.clos_n, false);
+  // rank sort into the (no longer needed) top-K buffer: entries are unique, rank = #smaller entries
+  uint64_t* cs = buf;
+  for (int i0 = 0; i0 < ncl; i0 += 256) {
+    uint64_t mine[4];
+    int rk[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      mine[u] = i0 + u * 64 + lane < ncl ? ck[i0 + u * 64 + lane] : ~0ULL;
+      rk[u] = 0;
+    }
+    for (int j = 0; j < ncl; ++j) {
+      const uint64_t v = ck[j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) rk[u] += v < mine[u] ? 1 : 0;
+    }
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u * 64 + lane < ncl) cs[rk[u]] = mine[u];
+  }
+  wave_sync();
+  // dedupe by row: the first entry (lowest hypothesis) names the hypothesis, the largest count wins (ties: the
+  // earlier entry, which also gives the feature payload)
   int nout = 0;
-  int64_t ob = (int64_t)s * O.cand_stride;
+  const int64_t ob = (int64_t)s * O.cand_stride;
   for (int i0 = 0; i0 < ncl; i0 += 64) {
     const int i = i0 + lane;
     bool first = false;
-    uint32_t row = 0, hl = 0;
+    uint32_t row = 0, hl = 0, best = 0;
     int32_t cc = -1;
-    uint32_t best = 0;
     if (i < ncl) {
-      uint64_t v = ck[i];
-      row = (uint32_t)(v >> 32);
-      hl = (uint32_t)((v >> 24) & 0xFFu);
-      first = i == 0 || (uint32_t)(ck[i - 1] >> 32) != row;
+      const uint64_t v = cs[i];
+      row = (uint32_t)(v >> 36);
+      hl = (uint32_t)((v >> 30) & 63ULL);
+      first = i == 0 || (uint32_t)(cs[i - 1] >> 36) != row;
       if (first) {
-        best = (uint32_t)(v & 0xFFFFFFu);
-        cc = cv[best];
-        for (int j = i + 1; j < ncl && (uint32_t)(ck[j] >> 32) == row; ++j) {
-          const uint32_t q = (uint32_t)(ck[j] & 0xFFFFFFu);
-          // the duplicate with the larger count (ties: lower hypothesis) gives the feature payload
-          if (cv[q] > cc) {
-            cc = cv[q];
-            best = q;
+        best = (uint32_t)(v & 0x7FFULL);
+        cc = (int32_t)((v >> 11) & 0x7FFFFULL) - 1;
+        for (int j = i + 1; j < ncl && (uint32_t)(cs[j] >> 36) == row; ++j) {
+          const int32_t cj = (int32_t)((cs[j] >> 11) & 0x7FFFFULL) - 1;
+          if (cj > cc) {
+            cc = cj;
+            best = (uint32_t)(cs[j] & 0x7FFULL);
           }
         }
       }
     }
-    uint64_t bal = ballot(first);
+    const uint64_t bal = ballot(first);
     if (first) {
-      int pos = nout + __popcll(bal & lanemask_lt());
+      const int pos = nout + __popcll(bal & lanemask_lt());
       O.c_row[ob + pos] = (int32_t)row;
       O.c_hyp[ob + pos] = (int32_t)(h0 + hl);
       O.c_cnt[ob + pos] = cc;

@@ -44,8 +44,10 @@ _SIGS = {
     "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
     "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_k1j_workspace_bytes": ([I64, I64], SZ),
-    "vh_k1j_pairs": ([PBV, I32, P, P, P, I64, P, I64, P, P, I64, P, P, SZ, P], I32),
-    "vh_k1j_count": ([PP, PBV, I32, P, P, I64, P, P, P, P, P, I64, P, P, P, P, P], I32),
+    "vh_k1j_pairs": ([PBV, I32, P, P, P, P, P, P, P, I64, P, I64, P, P, I64, P, P, SZ, P], I32),
+    "vh_k1j_count": ([PP, PBV, I32, P, P, I64, P, P, P, P, P], I32),
+    "vh_k1j_iq_shift": ([I32, P, P, P, P], I32),
+    "vh_k1_candidate_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P], I32),
     "vh_k1j_topk": ([PP, PIX, I32, P, P, P, P, P, P, I64, P, I32, P, P, P, P, P, P, P, P, P], I32),
     "vh_k1j_features": ([PP, PIX, I32, P, P, P, P, P, P, I64, P, I32, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_mlp_pack": ([P, P, P, P, P], I32),

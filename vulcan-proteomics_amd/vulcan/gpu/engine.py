@@ -568,6 +568,7 @@ class DeviceSearch:
         slot_cap = int(n_h * (1.5 * mean_rows + 8)) + 4096
         pair_cap = int(n_h * (1.5 * mean_rows / rows_per_block + 2)) + 4096
         j = {"feat": feat, "cap_h": cap_h,
+             "s_qe": torch.empty(max(self.n_spec, 1), dtype=torch.int32, device=dev) if feat else None,
              "h_slot": torch.empty(cap_h + 1, dtype=torch.int64, device=dev),
              "blk_pairs": torch.empty(bv.n_blocks + 1, dtype=torch.int32, device=dev),
              "ws": torch.empty(int(L.vh_k1j_workspace_bytes(bv.n_blocks, cap_h)), dtype=torch.uint8, device=dev),
@@ -578,17 +579,18 @@ class DeviceSearch:
     def _join_alloc(self, j: dict, slot_cap: int, pair_cap: int):
         dev = self.gi.dev
         j["slot_cap"], j["pair_cap"] = slot_cap, pair_cap
-        j["pairs"] = torch.empty(max(pair_cap, 1), dtype=torch.int32, device=dev)
+        j["pairs"] = torch.empty((max(pair_cap, 1), 4), dtype=torch.int64, device=dev)    # 32-byte records
         if j["feat"]:
-            j["counts"] = torch.empty(2 * max(slot_cap, 1), dtype=torch.int64, device=dev)
+            j["counts"] = torch.empty(max(slot_cap, 1), dtype=torch.int64, device=dev)
         else:
             j["counts"] = torch.empty(max(slot_cap, 1), dtype=torch.int32, device=dev)
 
     def _join_pairs(self, n: int):
         L = K.lib()
-        j, hy = self.join, self.hyp
+        j, hy, pk = self.join, self.hyp, self.peaks
         bv = self.gi.blocks()
-        K.check(L.vh_k1j_pairs(C.byref(bv), n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]), j["cap_h"],
+        K.check(L.vh_k1j_pairs(C.byref(bv), n, K.ptr(hy["hyp_off"]), K.ptr(hy["spec"]), K.ptr(hy["lo"]),
+                               K.ptr(hy["hi"]), K.ptr(hy["maxfz"]), K.ptr(pk["off"]), K.ptr(j["s_qe"]), j["cap_h"],
                                K.ptr(j["h_slot"]), j["slot_cap"], K.ptr(j["blk_pairs"]), K.ptr(j["pairs"]),
                                j["pair_cap"], K.ptr(self.status), K.ptr(j["ws"]), j["ws"].numel(), K.stream()),
                 "k1j_pairs")
@@ -599,6 +601,9 @@ class DeviceSearch:
         j, hy, pk = self.join, self.hyp, self.peaks
         bv = self.gi.blocks()
         mark("k1j_pairs")
+        if j["feat"]:
+            K.check(L.vh_k1j_iq_shift(n, K.ptr(pk["off"]), K.ptr(pk["int"]), K.ptr(j["s_qe"]), K.stream()),
+                    "k1j_iq_shift")
         self._join_pairs(n)
         if not j["fitted"]:
             tot_slots = int(j["h_slot"][j["cap_h"]].item())
@@ -611,9 +616,8 @@ class DeviceSearch:
         mark("k1j_pairs")
         mark("k1j_count")
         K.check(L.vh_k1j_count(C.byref(self.vp), C.byref(bv), 2 if j["feat"] else 1, K.ptr(j["blk_pairs"]),
-                               K.ptr(j["pairs"]), j["pair_cap"], K.ptr(hy["spec"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
-                               K.ptr(hy["maxfz"]), K.ptr(j["h_slot"]), j["slot_cap"], K.ptr(pk["off"]),
-                               K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(j["counts"]), K.stream()), "k1j_count")
+                               K.ptr(j["pairs"]), j["pair_cap"], K.ptr(pk["mass"]), K.ptr(pk["int"]),
+                               K.ptr(j["counts"]), K.ptr(self.status), K.stream()), "k1j_count")
         mark("k1j_count")
 
     def run(self, raw: dict, ev: dict | None = None):

@@ -192,17 +192,21 @@ class FeatureLists:
     def k1(self, ds: DeviceSearch, vpf: VhParams, n: int):
         pk, hy = ds.peaks, ds.hyp
         if ds.k1_impl == "join":
-            if not ds.join["feat"]:
-                raise ValueError("the DeviceSearch was built without join_features")
+            # block-major counts, index top-K without closure, then the features of the selected candidates
             ds.join_front(n, _marker(None))
             j = ds.join
-            K.check(K.lib().vh_k1j_features(C.byref(vpf), C.byref(ds.gi.view), n, K.ptr(ds.order),
-                                            K.ptr(hy["hyp_off"]), K.ptr(hy["mode"]), K.ptr(hy["lo"]),
-                                            K.ptr(hy["hi"]), K.ptr(j["h_slot"]), j["slot_cap"], K.ptr(j["counts"]),
-                                            self.stride, K.ptr(self.n_cand), K.ptr(self.row), K.ptr(self.hyp),
-                                            K.ptr(self.cnt), K.ptr(self.nb), K.ptr(self.iq), K.ptr(self.maxcnt),
-                                            K.ptr(ds.scored), K.ptr(ds.matched), K.ptr(ds.status), K.ptr(ds.queue),
-                                            K.stream()), "k1j_features")
+            L = K.lib()
+            K.check(L.vh_k1j_topk(C.byref(vpf), C.byref(ds.gi.view), n, K.ptr(ds.order), K.ptr(hy["hyp_off"]),
+                                  K.ptr(hy["mode"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]), K.ptr(j["h_slot"]),
+                                  j["slot_cap"], K.ptr(j["counts"]), self.stride, K.ptr(self.n_cand),
+                                  K.ptr(self.row), K.ptr(self.hyp), K.ptr(self.cnt), K.ptr(ds.scored),
+                                  K.ptr(ds.matched), K.ptr(ds.status), K.ptr(ds.queue), K.stream()), "k1j_topk")
+            K.check(L.vh_k1_candidate_features(C.byref(vpf), C.byref(ds.gi.view), n, K.ptr(ds.order),
+                                               K.ptr(pk["off"]), K.ptr(pk["mass"]), K.ptr(pk["int"]),
+                                               K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["lo"]),
+                                               K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand), K.ptr(self.row),
+                                               K.ptr(self.cnt), K.ptr(self.nb), K.ptr(self.iq), K.ptr(self.maxcnt),
+                                               K.ptr(ds.status), K.stream()), "k1_candidate_features")
             return
         K.check(K.lib().vh_k1_features(C.byref(vpf), C.byref(ds.gi.view), n, K.ptr(ds.order), K.ptr(pk["off"]),
                                        K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(hy["hyp_off"]),
@@ -239,7 +243,7 @@ class LearnedSearch(DeviceSearch):
         q = SearchParams(**p.to_dict())
         q.topk = self.pp.keep
         q.pair_closure = self.pp.pair_closure
-        super().__init__(gi, q, n_spec, report=report, persistent=persistent, join_features=True)
+        super().__init__(gi, q, n_spec, report=report, persistent=persistent)
         self.vpf = _feature_params(self.vp, self.pp.index_topk)
         self.lists = FeatureLists(n_spec, self.pp.index_topk, gi.dev)
         self.model = model
@@ -266,7 +270,7 @@ def training_data(gi: GpuIndex, p: SearchParams, raw: dict, pp: PrefilterParams)
     """Features and rescorer labels of every index top-K candidate of the spectra in `raw` (device tensors):
     returns (x [m, 8] f32, y [m] f32, spectrum [m] i64, hyperscore [m] f64) over the non-empty slots."""
     n = raw["n"]
-    ds = DeviceSearch(gi, p, n, join_features=True)
+    ds = DeviceSearch(gi, p, n)
     mark = _marker(None)
     ds.front(raw, n, mark)
     vpf = _feature_params(ds.vp, pp.index_topk)
