@@ -60,16 +60,18 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
-    """Plain (index top-K + closure -> K2) vs learned prefilter (index top-K -> MLP -> top keep + closure -> K2)
-    vs the K1-count rule at the same K2 budget (index top keep + closure), on the same spectra; the network is
-    trained first on separately seeded spectra of the same workload (labels from the plain K2)."""
+    """Plain (index top-K + closure -> K2) vs learned prefilter (index top-K -> MLP -> best keep_min, then every
+    candidate within margin of the best score up to keep, + closure -> K2) vs the K1-count rule (index top-16 +
+    closure), on the same spectra; the network is trained first on separately seeded spectra of the same workload
+    (labels from the plain K2)."""
     import numpy as np
     import torch
 
     from vulcan.gpu import engine as E
     from vulcan.index import synth
     from vulcan.rescore import prefilter as PF
-    pp = PF.PrefilterParams(index_topk=p.topk, keep=a.keep, steps=a.train_steps)
+    pp = PF.PrefilterParams(index_topk=p.topk, keep=a.keep, keep_min=a.keep_min, margin=a.margin,
+                            steps=a.train_steps)
     t0 = time.perf_counter()
     sp_tr, _, _ = synth.spectra(l1, lab, a.workload, n=a.n_train, seed=900 + a.workload)
     raw_tr = E.upload_spectra(sp_tr, gi.dev)
@@ -83,7 +85,7 @@ def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
     log(f"trained {pp.steps} steps in {train_s:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
     del raw_tr, x, y
     q_plain = type(p)(**{**p.to_dict(), "topk": pp.index_topk, "pair_closure": True})
-    q_rule = type(p)(**{**p.to_dict(), "topk": pp.keep, "pair_closure": True})
+    q_rule = type(p)(**{**p.to_dict(), "topk": 16, "pair_closure": True})
     out = {}
     rows = {}
     for name, mk in (("plain", lambda: E.DeviceSearch(gi, q_plain, n)),
@@ -111,7 +113,8 @@ def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
         log(f"{name}: {n / dt:,.0f} spectra/s ({dt * 1e3:.2f} ms/step)")
     ag = {k: float(np.mean(rows[k][rows["plain"] >= 0] == rows["plain"][rows["plain"] >= 0]))
           for k in ("learned", "count_rule")}
-    res = {"index_topk": pp.index_topk, "keep": pp.keep, "pair_closure": True,
+    res = {"index_topk": pp.index_topk, "keep_max": pp.keep, "keep_min": pp.keep_min, "margin": pp.margin,
+           "pair_closure": True,
            "top1_agreement": ag["learned"], "count_rule_top1_agreement": ag["count_rule"],
            "speedup_vs_plain": out["learned"]["spectra_per_s"] / out["plain"]["spectra_per_s"],
            "train": {"spectra": a.n_train, "steps": pp.steps, "batch": pp.batch, "train_s": train_s,
@@ -276,7 +279,10 @@ def main():
     ap.add_argument("--block-rows", type=int, default=None, help="join K1: log2 max rows per index block")
     ap.add_argument("--prefilter", action="store_true",
                     help="plain vs learned prefilter comparison (default for the phospho-prefilter workload)")
-    ap.add_argument("--keep", type=int, default=16, help="prefilter: candidates kept before closure")
+    ap.add_argument("--keep", type=int, default=64, help="prefilter: at most this many candidates kept before closure")
+    ap.add_argument("--keep-min", type=int, default=8, help="prefilter: always keep the best this many by score")
+    ap.add_argument("--margin", type=float, default=3.0,
+                    help="prefilter: also keep candidates scoring within this margin of the best")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--n-train", type=int, default=20000, help="prefilter: training spectra (own seed)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -356,7 +362,8 @@ def main():
                "warmup": a.warmup, "ms_per_step": L["ms_per_step"], "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
                "config": {"workload": w.name, "n_spectra": n, "n_peptidoforms": len(l1),
-                          "n_postings": gi.n_postings, "index_topk": res["index_topk"], "keep": res["keep"],
+                          "n_postings": gi.n_postings, "index_topk": res["index_topk"], "keep_min": res["keep_min"],
+                          "keep_max": res["keep_max"], "keep_margin": res["margin"],
                           "pair_closure": True, "precursor_ppm": list(p.precursor_ppm),
                           "fragment_da": list(p.fragment_ppm), "max_fragment_charge": p.max_fragment_charge,
                           "parallelism": "single-gpu"},

@@ -367,13 +367,14 @@ int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, 
                        const int16_t* c_nb, const uint32_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z,
                        const float* h_M, float* score, void* stream);
 
-/* Per spectrum: the `keep` best slots by (score desc, NaN last; ties: earlier slot), then the reference's pair
- * closure and dedupe (partner rows with count -1; by row: lowest hypothesis, highest count; engine.py:284-312),
- * written in row order at s*out_stride. stride_in <= 1024, keep <= 512. */
+/* Per spectrum: the kept slots by (score desc, NaN last; ties: earlier slot) -- the best keep_min, then every
+ * slot whose score is >= best score - margin (f32), at most keep_max (keep_min = keep_max: a fixed top-keep) --
+ * then the reference's pair closure and dedupe (partner rows with count -1; by row: lowest hypothesis, highest
+ * count; engine.py:284-312), written in row order at s*out_stride. stride_in <= 1024, keep_max <= 512. */
 int vh_prefilter_select(const VhIndexView* ix, int32_t n_spec, int32_t stride_in, const int32_t* n_cand,
                         const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const float* score,
-                        int32_t keep, int32_t pair_closure, int32_t out_stride, int32_t* out_n, int32_t* out_row,
-                        int32_t* out_hyp, int32_t* out_cnt, void* stream);
+                        int32_t keep_min, int32_t keep_max, float margin, int32_t pair_closure, int32_t out_stride,
+                        int32_t* out_n, int32_t* out_row, int32_t* out_hyp, int32_t* out_cnt, void* stream);
 
 #ifdef __cplusplus
 }

@@ -10,8 +10,9 @@ Only tests/ (and __graft_entry__.smoke()) import this module, as the checker of 
             nearest even, f32 bias adds and ReLU), sums in float64: the GPU must agree within f32 accumulation
             error, which a wrong layout, index or mask would exceed by orders of magnitude;
   adam      f32 restatement of vh_mlp_adam (bit-exact target);
-  select    vh_prefilter_select: top `keep` by (score desc, NaN last, earlier slot first), then the reference's
-            pair closure / dedupe (vulcan/gpu/engine.py:284-312) (bit-exact target).
+  select    vh_prefilter_select: the best keep_min by (score desc, NaN last, earlier slot first), then every slot
+            within margin of the best score (at most keep), then the reference's pair closure / dedupe
+            (vulcan/gpu/engine.py:284-312) (bit-exact target).
 """
 from __future__ import annotations
 
@@ -143,7 +144,11 @@ def score_ord(x: np.ndarray) -> np.ndarray:
     return np.where(np.isnan(x), 0, o).astype(np.uint64)
 
 
-def select(n_cand, row, hyp, cnt, score, stride_in: int, keep: int, closure: bool, partner, out_stride: int):
+def select(n_cand, row, hyp, cnt, score, stride_in: int, keep: int, closure: bool, partner, out_stride: int,
+           keep_min: int | None = None, margin: float = 0.0):
+    """keep_min (default keep): the best keep_min by score, then every slot scoring >= best - margin (f32), at most
+    keep."""
+    keep_min = keep if keep_min is None else keep_min
     n_spec = len(n_cand)
     on = np.zeros(n_spec, np.int32)
     orow = np.zeros(n_spec * out_stride, np.int32)
@@ -154,7 +159,14 @@ def select(n_cand, row, hyp, cnt, score, stride_in: int, keep: int, closure: boo
         b = s * stride_in
         sc = np.asarray(score[b:b + n], np.float32)
         key = score_ord(sc)
-        idx = np.lexsort((np.arange(n), -key.astype(np.float64)))[:min(keep, n)]
+        order = np.lexsort((np.arange(n), -key.astype(np.float64)))
+        lim = min(keep, n)
+        if n:
+            thr = np.float32(sc[order[0]] - np.float32(margin))
+            c = int(np.sum(sc[order[:lim]] >= thr))
+        else:
+            c = 0
+        idx = order[:max(c, min(keep_min, n))]
         ent = [(int(row[b + i]), int(hyp[b + i]), int(cnt[b + i])) for i in idx]
         if closure:
             ent += [(int(partner[r]), h, -1) for r, h, _ in ent[:len(idx)]]

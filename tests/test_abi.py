@@ -61,11 +61,13 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert L.vh_mlp_forward(C.c_void_p(8), 10, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8), None, None) == -1
     assert L.vh_mlp_backward(C.c_void_p(8), 100, *[C.c_void_p(8)] * 7, 16, None) == -3
     assert L.vh_mlp_workspace_bytes(-1) == 0 and L.vh_mlp_workspace_bytes(1000) > 1000 * 256 * 6
-    assert L.vh_prefilter_select(C.byref(ix), 1, 2000, *[None] * 5, 16, 1, 32, *[None] * 4, None) == -1
-    assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 0, 1, 32, *[None] * 4, None) == -1
+    assert L.vh_prefilter_select(C.byref(ix), 1, 2000, *[None] * 5, 16, 16, 0.0, 1, 32, *[None] * 4, None) == -1
+    assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 0, 16, 0.0, 1, 32, *[None] * 4, None) == -1
+    assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 16, 8, 0.0, 1, 32, *[None] * 4, None) == -1
+    assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 8, 16, -1.0, 1, 32, *[None] * 4, None) == -1
     ixp = VhIndexView()
     ixp.partner = 8
-    assert L.vh_prefilter_select(C.byref(ixp), 1, 200, *[None] * 5, 16, 1, 31, *[None] * 4, None) == -2
+    assert L.vh_prefilter_select(C.byref(ixp), 1, 200, *[None] * 5, 16, 16, 0.0, 1, 31, *[None] * 4, None) == -2
     assert L.vh_k1_features(C.byref(vp), C.byref(ix), 1, *[None] * 9, 10, *[None] * 11, None) == -1
     # block layout / join K1
     ix.n_rows = 100

@@ -257,16 +257,19 @@ def test_select_equals_oracle(fenv0):
     sc[::97] = np.nan
     sc[5::13] = sc[4::13][: len(sc[5::13])]          # ties
     fl.score.copy_(torch.from_numpy(sc))
-    for keep, clos in ((16, True), (1, True), (16, False), (300, True)):
+    for kmin, keep, margin, clos in ((16, 16, 0.0, True), (1, 1, 0.0, True), (16, 16, 0.0, False),
+                                     (300, 300, 0.0, True), (4, 40, 0.5, True), (8, 64, 1.5, False),
+                                     (1, 200, 0.0, True)):
         ostr = 2 * keep if clos else keep
         dev = e.gi.dev
         on = torch.empty(e.n, dtype=torch.int32, device=dev)
         orow, ohyp, ocnt = (torch.empty(e.n * ostr, dtype=torch.int32, device=dev) for _ in range(3))
         K.check(K.lib().vh_prefilter_select(C.byref(e.gi.view), e.n, e.topk, K.ptr(fl.n_cand), K.ptr(fl.row),
-                                            K.ptr(fl.hyp), K.ptr(fl.cnt), K.ptr(fl.score), keep, int(clos), ostr,
+                                            K.ptr(fl.hyp), K.ptr(fl.cnt), K.ptr(fl.score), kmin, keep, margin,
+                                            int(clos), ostr,
                                             K.ptr(on), K.ptr(orow), K.ptr(ohyp), K.ptr(ocnt), K.stream()), "select")
         o = P.select(_np(fl.n_cand)[:e.n], _np(fl.row), _np(fl.hyp), _np(fl.cnt), sc, e.topk, keep, clos,
-                     e.l1.partner, ostr)
+                     e.l1.partner, ostr, keep_min=kmin, margin=margin)
         assert np.array_equal(_np(on), o["n"]), keep
         mm = (np.arange(ostr)[None, :] < o["n"][:, None]).ravel()
         for dv, ov in ((orow, o["row"]), (ohyp, o["hyp"]), (ocnt, o["cnt"])):

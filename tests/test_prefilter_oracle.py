@@ -233,6 +233,34 @@ def test_select_oracle_invariants():
         assert len(got) == len(set(top) | set(partner[top]))
 
 
+def test_select_oracle_adaptive_keep():
+    """keep_min / margin: per spectrum exactly max(min(keep_min, n), #{score >= best - margin} among the first keep)
+    slots are kept, and they are the best-scoring ones (no closure)."""
+    P = _P()
+    rng = np.random.default_rng(8)
+    n_spec, stride = 40, 60
+    n_cand = rng.integers(0, stride + 1, n_spec).astype(np.int32)
+    row = np.zeros(n_spec * stride, np.int32)
+    for s in range(n_spec):
+        row[s * stride:(s + 1) * stride] = np.sort(rng.choice(5000, stride, replace=False))
+    hyp = np.zeros(n_spec * stride, np.int32)
+    cnt = np.ones(n_spec * stride, np.int32)
+    score = (rng.standard_normal(n_spec * stride) * 2).astype(np.float32)
+    score[::23] = np.nan
+    for kmin, keep, margin in ((3, 30, 1.0), (1, 5, 10.0), (10, 10, 0.0), (2, 60, 0.0)):
+        out = P.select(n_cand, row, hyp, cnt, score, stride, keep, False, None, keep, keep_min=kmin, margin=margin)
+        for s in range(n_spec):
+            n = int(n_cand[s])
+            sc = score[s * stride:s * stride + n].astype(np.float64)
+            scn = np.where(np.isnan(sc), -np.inf, sc)
+            o = np.argsort(-scn, kind="stable")
+            within = int(np.sum(scn[o[:min(keep, n)]] >= (np.float32(scn[o[0]] - np.float32(margin)) if n else 0)))
+            want_n = max(min(kmin, n), within)
+            assert out["n"][s] == want_n, (kmin, keep, margin, s)
+            got = set(out["row"][s * keep:s * keep + want_n].tolist())
+            assert got == set(row[s * stride:s * stride + n][o[:want_n]].tolist())
+
+
 def test_features_restatement_known_values():
     P = _P()
     mass32 = np.array([1000.0, 2000.0], np.float32)

@@ -493,7 +493,8 @@ __device__ void bitonic_u64(uint64_t* a, int n, bool descending) {
 }
 
 struct SelArgs {
-  int32_t n_spec, stride_in, keep, closure, out_stride, sort_n, clos_n, wave_bytes;
+  int32_t n_spec, stride_in, keep_min, keep_max, closure, out_stride, sort_n, clos_n, wave_bytes;
+  float margin;
   const int32_t *n_cand, *c_row, *c_hyp, *c_cnt;
   const float* score;
   const int64_t* partner;
@@ -517,7 +518,21 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs A) {
     keys[i] = i < n ? ((uint64_t)score_ord(A.score[base + i]) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)i) : 0ULL;
   wave_sync();
   bitonic_u64(keys, A.sort_n, true);
-  const int nt = n < A.keep ? n : A.keep;
+  // kept: the best keep_min, then every candidate scoring within margin of the best, at most keep_max
+  int nt;
+  {
+    const int lim = n < A.keep_max ? n : A.keep_max;
+    const float top = n > 0 ? A.score[base + (0xFFFFFFFFu - (uint32_t)keys[0])] : 0.f;
+    const float thr = top - A.margin;
+    int cnt = 0;
+    for (int t0 = 0; t0 < lim; t0 += 64) {
+      const int t = t0 + lane;
+      const bool ok = t < lim && A.score[base + (0xFFFFFFFFu - (uint32_t)keys[t])] >= thr;
+      cnt += __popcll(ballot(ok));
+    }
+    const int lo = n < A.keep_min ? n : A.keep_min;
+    nt = cnt > lo ? cnt : lo;
+  }
   const int ncl = A.closure ? 2 * nt : nt;
   for (int i = lane; i < A.clos_n; i += 64) {
     uint64_t v = ~0ULL;
@@ -743,16 +758,20 @@ int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, 
 
 int vh_prefilter_select(const VhIndexView* ix, int32_t n_spec, int32_t stride_in, const int32_t* n_cand,
                         const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const float* score,
-                        int32_t keep, int32_t pair_closure, int32_t out_stride, int32_t* out_n, int32_t* out_row,
-                        int32_t* out_hyp, int32_t* out_cnt, void* stream) {
-  if (!ix || n_spec < 0 || stride_in < 1 || stride_in > 1024 || keep < 1 || keep > 512) return VH_E_ARG;
+                        int32_t keep_min, int32_t keep_max, float margin, int32_t pair_closure, int32_t out_stride,
+                        int32_t* out_n, int32_t* out_row, int32_t* out_hyp, int32_t* out_cnt, void* stream) {
+  if (!ix || n_spec < 0 || stride_in < 1 || stride_in > 1024 || keep_min < 1 || keep_max < keep_min ||
+      keep_max > 512 || !(margin >= 0.f))
+    return VH_E_ARG;
+  const int keep = keep_max;
   if (pair_closure && !ix->partner) return VH_E_ARG;
   if (out_stride < (pair_closure ? 2 * keep : keep)) return VH_E_CAPACITY;
   if (n_spec == 0) return 0;
   if (!n_cand || !c_row || !c_hyp || !c_cnt || !score || !out_n || !out_row || !out_hyp || !out_cnt)
     return VH_E_ARG;
   SelArgs A;
-  A.n_spec = n_spec; A.stride_in = stride_in; A.keep = keep; A.closure = pair_closure ? 1 : 0;
+  A.n_spec = n_spec; A.stride_in = stride_in; A.keep_min = keep_min; A.keep_max = keep_max; A.margin = margin;
+  A.closure = pair_closure ? 1 : 0;
   A.out_stride = out_stride;
   A.sort_n = pow2_ge(stride_in < 64 ? 64 : stride_in);
   const int k = keep < stride_in ? keep : stride_in;

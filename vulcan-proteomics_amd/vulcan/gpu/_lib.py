@@ -57,7 +57,7 @@ _SIGS = {
     "vh_mlp_adam": ([P, P, P, P, I64, F32, F32, F32, F32, F32, F32, P], I32),
     "vh_prefilter_features": ([PIX, I32, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_prefilter_score": ([PIX, P, I32, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
-    "vh_prefilter_select": ([PIX, I32, I32, P, P, P, P, P, I32, I32, I32, P, P, P, P, P], I32),
+    "vh_prefilter_select": ([PIX, I32, I32, P, P, P, P, P, I32, I32, F32, I32, I32, P, P, P, P, P], I32),
 }
 
 ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "workspace too small", -4: "size limit exceeded"}

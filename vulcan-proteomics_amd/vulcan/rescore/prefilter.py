@@ -43,7 +43,9 @@ _OFF = {"W1": (0, (256, 8)), "b1": (2048, (256,)), "W2": (2304, (256, 256)), "b2
 @dataclass
 class PrefilterParams:
     index_topk: int = 200      # K1 shortlist scored by the network
-    keep: int = 16             # candidates kept per spectrum before pair closure
+    keep: int = 64             # at most this many candidates kept per spectrum before pair closure
+    keep_min: int = 8          # always the best keep_min by score ...
+    margin: float = 3.0        # ... then every candidate scoring within margin of the best (up to keep)
     pair_closure: bool = True
     steps: int = 3000
     batch: int = 8192
@@ -260,7 +262,8 @@ class LearnedSearch(DeviceSearch):
         mark("mlp")
         mark("select")
         K.check(L.vh_prefilter_select(C.byref(self.gi.view), n, fl.stride, K.ptr(fl.n_cand), K.ptr(fl.row),
-                                      K.ptr(fl.hyp), K.ptr(fl.cnt), K.ptr(fl.score), self.pp.keep,
+                                      K.ptr(fl.hyp), K.ptr(fl.cnt), K.ptr(fl.score),
+                                      min(self.pp.keep_min, self.pp.keep), self.pp.keep, float(self.pp.margin),
                                       int(self.pp.pair_closure), self.stride, K.ptr(self.n_cand), K.ptr(self.c_row),
                                       K.ptr(self.c_hyp), K.ptr(self.c_cnt), K.stream()), "prefilter_select")
         mark("select")
