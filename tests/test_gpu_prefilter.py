@@ -45,7 +45,7 @@ class FEnv:
         self.ds = E.DeviceSearch(self.gi, self.p, self.n, k1_impl=impl)
         self.ds.front(self.raw, self.n, E._marker(None))
         self.vpf = PF._feature_params(self.ds.vp, topk)
-        self.fl = PF.FeatureLists(self.n, topk, self.gi.dev)
+        self.fl = PF.FeatureLists(self.n, topk, self.gi.dev, source="full")
         self.fl.k1(self.ds, self.vpf, self.n)
         torch.cuda.synchronize()
         self.topk = topk
@@ -107,6 +107,22 @@ def test_k1_features_equal_oracle(which, request):
     assert np.array_equal(iq, tf["iq"][:e.n * e.topk][m])
     assert np.array_equal(_np(fl.nb)[:e.n * e.topk][m], tf["nb"][:e.n * e.topk][m])
     assert (iq >> 16).max() > 0 and (iq & 0xFFFF).max() > 0
+
+
+def test_count_only_feature_source(fenv1j):
+    """source "counts": the same lists as "full", b-ion count and intensity fields zero, best count per spectrum."""
+    from vulcan.rescore import prefilter as PF
+    e = fenv1j
+    fl = PF.FeatureLists(e.n, e.topk, e.gi.dev, source="counts")
+    fl.k1(e.ds, e.vpf, e.n)
+    torch.cuda.synchronize()
+    nc = _np(fl.n_cand)[:e.n]
+    assert np.array_equal(nc, _np(e.fl.n_cand)[:e.n])
+    m = (np.arange(e.topk)[None, :] < nc[:, None]).ravel()
+    for a, b in ((fl.row, e.fl.row), (fl.hyp, e.fl.hyp), (fl.cnt, e.fl.cnt)):
+        assert np.array_equal(_np(a)[:e.n * e.topk][m], _np(b)[:e.n * e.topk][m])
+    assert not _np(fl.nb)[:e.n * e.topk][m].any() and not _np(fl.iq)[:e.n * e.topk][m].any()
+    assert np.array_equal(_np(fl.maxcnt)[:e.n], _np(e.fl.maxcnt)[:e.n])
 
 
 def test_k1_features_lists_equal_fused_topk_without_closure(fenv0):

@@ -46,6 +46,8 @@ class PrefilterParams:
     keep: int = 64             # at most this many candidates kept per spectrum before pair closure
     keep_min: int = 8          # always the best keep_min by score ...
     margin: float = 3.0        # ... then every candidate scoring within margin of the best (up to keep)
+    features: str = "counts"   # "counts": count features only (b-ion count and intensity fields zero);
+                               # "full": + per-candidate b-ion count and per-ion intensity pseudo-logs
     pair_closure: bool = True
     steps: int = 3000
     batch: int = 8192
@@ -178,9 +180,14 @@ def _feature_params(vp: VhParams, topk: int) -> VhParams:
 
 
 class FeatureLists:
-    """Device buffers of vh_k1_features' per-spectrum lists (stride = index top-K)."""
+    """Device buffers of vh_k1_features' per-spectrum lists (stride = index top-K). source "full": b-ion count and
+    per-ion intensity pseudo-logs per candidate; "counts": those fields are zero (the network sees count, count /
+    best count, precursor ppm, length and charge)."""
 
-    def __init__(self, n_spec: int, topk: int, dev):
+    def __init__(self, n_spec: int, topk: int, dev, source: str = "counts"):
+        if source not in ("counts", "full"):
+            raise ValueError(source)
+        self.source = source
         n = max(n_spec, 1)
         self.stride = topk
         e = lambda m, dt: torch.empty(m, dtype=dt, device=dev)  # noqa: E731
@@ -203,6 +210,9 @@ class FeatureLists:
                                   j["slot_cap"], K.ptr(j["counts"]), self.stride, K.ptr(self.n_cand),
                                   K.ptr(self.row), K.ptr(self.hyp), K.ptr(self.cnt), K.ptr(ds.scored),
                                   K.ptr(ds.matched), K.ptr(ds.status), K.ptr(ds.queue), K.stream()), "k1j_topk")
+            if self.source == "counts":
+                self._count_only(n)
+                return
             K.check(L.vh_k1_candidate_features(C.byref(vpf), C.byref(ds.gi.view), n, K.ptr(ds.order),
                                                K.ptr(pk["off"]), K.ptr(pk["mass"]), K.ptr(pk["int"]),
                                                K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["lo"]),
@@ -218,6 +228,17 @@ class FeatureLists:
                                        K.ptr(ds.scored),
                                        K.ptr(ds.matched), K.ptr(ds.status), K.ptr(ds.queue), K.stream()),
                 "k1_features")
+        if self.source == "counts":
+            self._count_only(n)
+
+    def _count_only(self, n: int):
+        """Count-only features: zero b-ion counts / intensity fields, best count per spectrum (device ops only)."""
+        self.nb.zero_()
+        self.iq.zero_()
+        S = self.stride
+        cnt = self.cnt[:n * S].view(n, S)
+        valid = torch.arange(S, device=cnt.device)[None, :] < self.n_cand[:n, None]
+        self.maxcnt[:n] = torch.where(valid, cnt, torch.zeros_like(cnt)).amax(1)
 
     def features(self, ds: DeviceSearch, n: int) -> torch.Tensor:
         out = torch.empty((max(n, 1) * self.stride, 8), dtype=torch.float32, device=ds.gi.dev)
@@ -247,7 +268,7 @@ class LearnedSearch(DeviceSearch):
         q.pair_closure = self.pp.pair_closure
         super().__init__(gi, q, n_spec, report=report, persistent=persistent)
         self.vpf = _feature_params(self.vp, self.pp.index_topk)
-        self.lists = FeatureLists(n_spec, self.pp.index_topk, gi.dev)
+        self.lists = FeatureLists(n_spec, self.pp.index_topk, gi.dev, self.pp.features)
         self.model = model
         self.blob = model.blob(gi.dev)
 
@@ -277,7 +298,7 @@ def training_data(gi: GpuIndex, p: SearchParams, raw: dict, pp: PrefilterParams)
     mark = _marker(None)
     ds.front(raw, n, mark)
     vpf = _feature_params(ds.vp, pp.index_topk)
-    fl = FeatureLists(n, pp.index_topk, gi.dev)
+    fl = FeatureLists(n, pp.index_topk, gi.dev, pp.features)
     fl.k1(ds, vpf, n)
     x = fl.features(ds, n)
     out = _k2_out(n * fl.stride, gi.dev)
