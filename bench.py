@@ -124,8 +124,9 @@ def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
 
 
 def uneven_leg(a, w, l1, gi, p, sp, raw, n):
-    """Packed multi-run batch: default eager step (grid launches, mass order) vs the autotuned per-class
-    schedule (persistent / grid per class, tile, XCD grouping) replayed as one hipGraph."""
+    """Packed multi-run batch: default eager step (grid launches, mass order) vs the tuned step (the fastest of
+    grid / persistent launches, mass / work order, XCD grouping and the per-class schedule, timed on this batch)
+    replayed as one hipGraph."""
     import numpy as np
     import torch
 
@@ -136,19 +137,25 @@ def uneven_leg(a, w, l1, gi, p, sp, raw, n):
     rows_base = base.rank1_rows(n)
     log(f"eager default: {n / dt_base:,.0f} spectra/s ({dt_base * 1e3:.2f} ms)")
     del base
-    sched = S.ClassSchedule(gi, p, sp)
-    tune = S.autotune(sched, raw)
-    log(f"autotuned {len(sched.classes)} classes over {tune['tried']} configurations in {tune['seconds']:.1f} s")
+    t0 = time.perf_counter()
+    sched, tried = S.tune_step(gi, p, sp, raw)
+    tune_s = time.perf_counter() - t0
+    cfg = min(tried, key=lambda t: t[1])[0]
+    log(f"tuned the step over {len(tried)} configurations in {tune_s:.1f} s: {cfg}")
     ev: dict = {}
     sched.run(raw)
     for _ in range(3):
         sched.run(raw, ev)
     torch.cuda.synchronize()
-    cms = sched.class_ms(ev)
-    k1_ms = sum(float(np.mean([x.elapsed_time(y) for x, y in zip(ev[f"k1_c{c}"][0::2], ev[f"k1_c{c}"][1::2])]))
-                for c, _, _ in sched.classes)
-    k2_ms = sum(float(np.mean([x.elapsed_time(y) for x, y in zip(ev[f"k2_c{c}"][0::2], ev[f"k2_c{c}"][1::2])]))
-                for c, _, _ in sched.classes)
+    stage = _stages(ev)
+    classes = []
+    if isinstance(sched, S.ClassSchedule):
+        cms = sched.class_ms(ev)
+        classes = [{"class": c, "charge_bin": S.Z_BINS[c // (len(S.P_BINS) + 1)], "peak_bin": c % (len(S.P_BINS) + 1),
+                    "spectra": cnt, "config": sched.table[c], "k1_k2_ms": cms[c]} for c, _, cnt in sched.classes]
+    k1_ms = sum(v for k, v in stage.items() if k.startswith("k1"))
+    k2_ms = sum(v for k, v in stage.items() if k.startswith("k2"))
+    join_ms = {k: stage[k] for k in ("k1j_pairs", "k1j_count") if k in stage}
     g = S.GraphStep(sched, raw)
     for _ in range(a.warmup):
         g.replay()
@@ -161,12 +168,11 @@ def uneven_leg(a, w, l1, gi, p, sp, raw, n):
     rows = sched.rank1_rows(n)
     same = bool(np.array_equal(rows, rows_base))
     log(f"graph schedule: {n / dt:,.0f} spectra/s ({dt * 1e3:.2f} ms), rank-1 equal to eager default: {same}")
-    classes = [{"class": c, "charge_bin": S.Z_BINS[c // (len(S.P_BINS) + 1)], "peak_bin": c % (len(S.P_BINS) + 1),
-                "spectra": cnt, "config": sched.table[c], "k1_k2_ms": cms[c]} for c, _, cnt in sched.classes]
     return {"graph_ms_per_step": dt * 1e3, "graph_spectra_per_s": n / dt, "eager_default_ms": dt_base * 1e3,
             "eager_default_spectra_per_s": n / dt_base, "eager_default_stage_ms": st_base,
-            "rank1_equal_default": same, "autotune_s": tune["seconds"], "configs_tried": tune["tried"],
-            "k1_ms_sum": k1_ms, "k2_ms_sum": k2_ms, "classes": classes}, sched
+            "rank1_equal_default": same, "tune_s": tune_s, "configs": [{"config": c, "eager_ms": t} for c, t in tried],
+            "chosen": cfg, "stage_ms": stage, "k1_ms_sum": k1_ms, "k2_ms_sum": k2_ms, "k1_impl": sched.k1_impl,
+            "join_stage_ms": join_ms, "classes": classes}, sched
 
 
 def algorithmic_bytes(ds, gi, n, l1):
@@ -333,7 +339,12 @@ def main():
     if w.leg == "uneven":
         res, sched = uneven_leg(a, w, l1, gi, p, sp, raw, n)
         ab = algorithmic_bytes(sched, gi, n, l1)
-        ach = ab["k1_topk"] / (res["k1_ms_sum"] * 1e-3) / 1e9
+        if sched.k1_impl == "join":
+            ab.update(algorithmic_bytes_join(sched, gi, n))
+            rk, rb, rms = "k1j_count", ab["k1j_count"], res["join_stage_ms"]["k1j_count"]
+        else:
+            rk, rb, rms = "k1_topk (sum over classes)", ab["k1_topk"], res["k1_ms_sum"]
+        ach = rb / (rms * 1e-3) / 1e9
         out = {"metric": METRIC, "value": res["graph_spectra_per_s"], "unit": "spectra/s", "n_gpus": 1,
                "steps": a.steps, "warmup": a.warmup, "ms_per_step": res["graph_ms_per_step"],
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -342,9 +353,9 @@ def main():
                           "precursor_ppm": list(p.precursor_ppm), "fragment_da": list(p.fragment_ppm),
                           "max_fragment_charge": p.max_fragment_charge, "precursor_charge": list(p.precursor_charge),
                           "parallelism": "single-gpu", "launch": "one hipGraph per step, per-class autotuned"},
-               "roofline": {"bound": "hbm", "kernel": "k1_topk (sum over classes)", "achieved": ach,
+               "roofline": {"bound": "hbm", "kernel": rk, "achieved": ach,
                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                            "algorithmic_bytes_per_launch": ab["k1_topk"]},
+                            "algorithmic_bytes_per_launch": rb},
                "uneven": res, "counts": ab, "cpu_baseline": None}
         if not a.no_cpu:
             log("cpu baseline ...")

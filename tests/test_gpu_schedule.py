@@ -94,13 +94,14 @@ def test_persistent_and_work_order_equal_default(uneven, mode):
         assert np.array_equal(np.sort(o), np.arange(u["n"]))
 
 
-def test_class_schedule_graph_replays_equal_eager(uneven):
+@pytest.mark.parametrize("k1_impl", ["join", "tile"])
+def test_class_schedule_graph_replays_equal_eager(uneven, k1_impl):
     from vulcan.gpu import engine as E
     from vulcan.gpu import schedule as S
     u = uneven
     rng = np.random.default_rng(3)
     cands = S.DEFAULT_CANDIDATES
-    sched = S.ClassSchedule(u["gi"], u["p"], u["sp"])
+    sched = S.ClassSchedule(u["gi"], u["p"], u["sp"], k1_impl=k1_impl)
     assert len(sched.classes) >= 6
     sched.set_table({c: cands[int(rng.integers(len(cands)))] for c, _, _ in sched.classes})
     sched.run(u["raw"])
@@ -132,3 +133,15 @@ def test_autotune_picks_a_configuration_for_every_class(uneven):
     sched.run(u["raw"])
     torch.cuda.synchronize()
     _same(sched, u["base"], u["n"])
+
+
+def test_tune_step_picks_a_configuration_with_identical_results(uneven):
+    from vulcan.gpu import schedule as S
+    u = uneven
+    cands = [{"kind": "step", "persistent": True, "order_mode": "work", "xcd_group": 0}, {"kind": "classes"}]
+    obj, tried = S.tune_step(u["gi"], u["p"], u["sp"], u["raw"], candidates=cands, reps=1)
+    assert [c for c, _ in tried] == cands and all(t > 0 for _, t in tried)
+    obj.run(u["raw"])
+    torch.cuda.synchronize()
+    assert int(obj.status.item()) == 0
+    _same(obj, u["base"], u["n"])

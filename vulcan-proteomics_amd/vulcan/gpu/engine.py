@@ -95,22 +95,52 @@ class GpuIndex:
         if self.n_postings >= 2 ** 32 or self.n_postings > params.max_slice_entries:
             raise ValueError("more than 2^32 - 1 postings")
         v.n_postings = self.n_postings
-        self.offsets = torch.empty(n_buckets * n_bins + 1, dtype=torch.int32, device=d)
-        self.postings = torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d)
-        wsb = int(L.vh_index_workspace_bytes(self.n_postings, n_buckets, n_bins))
-        ws = torch.empty(wsb, dtype=torch.uint8, device=d)
-        K.check(L.vh_index_build(C.byref(self.vp), C.byref(v), K.ptr(self.post_off), K.ptr(self.offsets),
-                                 K.ptr(self.postings), K.ptr(ws), wsb, st), "index_build")
-        v.offsets, v.postings = K.ptr(self.offsets), K.ptr(self.postings)
         torch.cuda.synchronize()
-        del ws
-        self.slices = [Slice(0, n_buckets, self.offsets, self.postings)]
+        self._bin = None
         self._blocks = None
         self.inv_logw = None
         self.build_s = time.perf_counter() - t0
         self.device_bytes = sum(t.numel() * t.element_size() for t in
                                 [self.seq_off, self.length, self.mask, self.mass32, self.residues, self.partner,
-                                 self.offsets, self.postings])
+                                 self.post_off])
+        if params.k1_impl == "tile":
+            self.tile_view()
+
+    def tile_view(self) -> VhIndexView:
+        """The index view with the bin-major postings and directory the spectrum-major (tile) K1 reads
+        (vh_index_build), built on first use: the bucket-join K1 reads the block layout instead (blocks())."""
+        if self._bin is None:
+            t0 = time.perf_counter()
+            L = K.lib()
+            v, d, st = self.view, self.dev, K.stream()
+            nb, nn = self.n_buckets, self.n_bins
+            off = torch.empty(nb * nn + 1, dtype=torch.int32, device=d)
+            post = torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d)
+            wsb = int(L.vh_index_workspace_bytes(self.n_postings, nb, nn))
+            ws = torch.empty(wsb, dtype=torch.uint8, device=d)
+            K.check(L.vh_index_build(C.byref(self.vp), C.byref(v), K.ptr(self.post_off), K.ptr(off), K.ptr(post),
+                                     K.ptr(ws), wsb, st), "index_build")
+            v.offsets, v.postings = K.ptr(off), K.ptr(post)
+            torch.cuda.synchronize()
+            del ws
+            self._bin = {"offsets": off, "postings": post, "build_s": time.perf_counter() - t0}
+            self.device_bytes += off.numel() * 4 + post.numel() * 4
+        return self.view
+
+    @property
+    def offsets(self) -> torch.Tensor:
+        self.tile_view()
+        return self._bin["offsets"]
+
+    @property
+    def postings(self) -> torch.Tensor:
+        self.tile_view()
+        return self._bin["postings"]
+
+    @property
+    def slices(self):
+        self.tile_view()
+        return [Slice(0, self.n_buckets, self._bin["offsets"], self._bin["postings"])]
 
     def blocks(self) -> VhBlockView:
         """Block layout of the index for the bucket-join K1 (vh_index_build_blocks), built on first use:
@@ -301,7 +331,7 @@ def run_k1(gi: GpuIndex, hyp, peaks, p: SearchParams, stats: Stats, k1_threshold
         raise ValueError("hypotheses reference spectra beyond the peak view")
     if "mode" in hd:
         vp.k1_threshold_open = int(p.k1_threshold if k1_threshold_open is None else k1_threshold_open)
-    K.check(L.vh_k1(C.byref(vp), C.byref(gi.view), n_spec, None, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
+    K.check(L.vh_k1(C.byref(vp), C.byref(gi.tile_view()), n_spec, None, K.ptr(peaks["off"]), K.ptr(peaks["mass"]),
                     K.ptr(hd["hyp_off"]), K.ptr(hd["maxfz"]), K.ptr(hd.get("mode")), K.ptr(hd["lo"]),
                     K.ptr(hd["hi"]), cap, K.ptr(n_out),
                     K.ptr(o_h), K.ptr(o_r), K.ptr(o_c), K.ptr(scored), K.ptr(matched), K.stream()), "k1")
@@ -509,6 +539,8 @@ class DeviceSearch:
         if impl not in ("tile", "join"):
             raise ValueError(impl)
         self.k1_impl = impl
+        if impl == "tile":
+            gi.tile_view()     # the bin-major layout is built here, never inside a (captured) step
         # persistent: K1 and K2 run as resident waves pulling spectra from an atomic work counter.
         # order_mode: "mass" (first window row, vh_spectrum_order), "work" (mass band of 2^band_log2 rows, then
         # estimated work descending, vh_spectrum_order_work) or "fixed" (self.order is set by the caller)
@@ -676,7 +708,7 @@ class DeviceSearch:
             mark("k1j_select")
             return
         mark("k1_topk")
-        K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.view), n, K.ptr(self.order), K.ptr(pk["off"]),
+        K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.tile_view()), n, K.ptr(self.order), K.ptr(pk["off"]),
                              K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["mode"]),
                              K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand), K.ptr(self.c_row),
                              K.ptr(self.c_hyp), K.ptr(self.c_cnt), K.ptr(self.scored), K.ptr(self.matched),

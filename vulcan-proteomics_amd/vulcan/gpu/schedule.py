@@ -11,12 +11,16 @@ overlaps runs on several streams. Here:
 * packing: spectra are ordered by (class, precursor mass) on the host; each class is one contiguous range of
   the order array, so K1 and K2 of a class are launched over `order[start:start+count]` with that class's
   parameters, and precursor-mass locality is kept inside a class;
-* schedule: per class a K1 LDS tile (rows per pass), the XCD grouping of the block order, and grid vs
-  persistent launch (resident waves pulling spectra from an atomic counter, vh_k1_topk / vh_k2 `queue`);
+* schedule: per class a K1 LDS tile (rows per pass; tile K1 only), the XCD grouping of the block order, and grid
+  vs persistent launch (resident waves pulling spectra from an atomic counter, vh_k1_topk / vh_k1j_topk / vh_k2
+  `queue`); with the bucket-join K1 the block-major counting runs once for the whole batch and the per-class
+  launches are its per-spectrum select and K2;
 * autotune: every candidate configuration is applied to all classes at once, the step runs, and each class
   keeps the configuration under which its own K1 + K2 launches were fastest (HIP events per launch);
 * graph: `GraphStep` captures K0 .. rank of the packed batch into one hipGraph (no allocation, no host
-  synchronisation and no memset node inside: every counter is cleared by a kernel), replayed per step.
+  synchronisation and no memset node inside: every counter is cleared by a kernel), replayed per step;
+* tune_step: the whole-step configuration (grid / persistent launches, processing order, XCD grouping, or the
+  per-class schedule) is chosen by timing each candidate on the batch; the winner is the one captured.
 
 Results never depend on the schedule: candidate lists and scores are per spectrum.
 """
@@ -42,6 +46,9 @@ DEFAULT_CANDIDATES = [
     {"tile": t, "xcd_group": g, "persistent": q}
     for t, g, q in itertools.product((256, 512, 1024, 2048), (0, 32), (False, True))
 ]
+# with the bucket-join K1 the counts are block-major for the whole batch; per class only the select and K2
+# launches are scheduled (the LDS tile does not apply)
+JOIN_CANDIDATES = [{"tile": 512, "xcd_group": g, "persistent": q} for g, q in itertools.product((0, 32), (False, True))]
 
 
 def spectrum_classes(sp: Spectra, p: SearchParams) -> np.ndarray:
@@ -62,8 +69,9 @@ class ClassSchedule(DeviceSearch):
     """DeviceSearch over a packed multi-run batch whose K1 / K2 are launched per spectrum class with per-class
     parameters (`table`: class -> {"tile", "xcd_group", "persistent"}; missing classes use the defaults)."""
 
-    def __init__(self, gi: GpuIndex, p: SearchParams, sp: Spectra, table: dict | None = None, report: int = 1):
-        super().__init__(gi, p, len(sp), report=report, persistent=True, order_mode="fixed", k1_impl="tile")
+    def __init__(self, gi: GpuIndex, p: SearchParams, sp: Spectra, table: dict | None = None, report: int = 1,
+                 k1_impl: str | None = None):
+        super().__init__(gi, p, len(sp), report=report, persistent=True, order_mode="fixed", k1_impl=k1_impl)
         cls = spectrum_classes(sp, p)
         o = np.lexsort((np.asarray(sp.precursor_mz), cls))
         self.order.copy_(torch.from_numpy(o.astype(np.int32)).to(gi.dev))
@@ -84,11 +92,26 @@ class ClassSchedule(DeviceSearch):
     def candidate_stage(self, n: int, mark):
         L = K.lib()
         gi, pk, hy = self.gi, self.peaks, self.hyp
+        if self.k1_impl == "join":
+            self.join_front(n, mark)
+            j = self.join
+            for c, s0, cnt in self.classes:
+                vp = self.vps[c]
+                q = self.queue if self.table[c]["persistent"] else None
+                mark(f"k1_c{c}")
+                K.check(L.vh_k1j_topk(C.byref(vp), C.byref(gi.view), cnt, K.ptr(self.order[s0:s0 + cnt]),
+                                      K.ptr(hy["hyp_off"]), K.ptr(hy["mode"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
+                                      K.ptr(j["h_slot"]), j["slot_cap"], K.ptr(j["counts"]), self.stride,
+                                      K.ptr(self.n_cand), K.ptr(self.c_row), K.ptr(self.c_hyp), K.ptr(self.c_cnt),
+                                      K.ptr(self.scored), K.ptr(self.matched), K.ptr(self.status), K.ptr(q),
+                                      K.stream()), "k1j_topk")
+                mark(f"k1_c{c}")
+            return
         for c, s0, cnt in self.classes:
             vp = self.vps[c]
             q = self.queue if self.table[c]["persistent"] else None
             mark(f"k1_c{c}")
-            K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.view), cnt, K.ptr(self.order[s0:s0 + cnt]),
+            K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.tile_view()), cnt, K.ptr(self.order[s0:s0 + cnt]),
                                  K.ptr(pk["off"]), K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]),
                                  K.ptr(hy["mode"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride,
                                  K.ptr(self.n_cand), K.ptr(self.c_row), K.ptr(self.c_hyp), K.ptr(self.c_cnt),
@@ -128,7 +151,7 @@ class ClassSchedule(DeviceSearch):
 def autotune(sched: ClassSchedule, raw: dict, candidates=None, reps: int = 2) -> dict:
     """Per class, the candidate configuration with the fastest K1 + K2 launches. Returns {"table": {class:
     config}, "class_ms": {class: ms}, "tried": n, "seconds": s}; the schedule is left set to the table."""
-    cands = candidates or DEFAULT_CANDIDATES
+    cands = candidates or (JOIN_CANDIDATES if sched.k1_impl == "join" else DEFAULT_CANDIDATES)
     t0 = time.perf_counter()
     best = {c: (float("inf"), None) for c, _, _ in sched.classes}
     for cfg in cands:
@@ -145,6 +168,47 @@ def autotune(sched: ClassSchedule, raw: dict, candidates=None, reps: int = 2) ->
     sched.set_table(table)
     return {"table": table, "class_ms": {c: t for c, (t, _) in best.items()}, "tried": len(cands),
             "seconds": time.perf_counter() - t0}
+
+
+# whole-step configurations tried by tune_step: launch form (grid / persistent), processing order (precursor mass /
+# mass band then estimated work) and XCD grouping of the block order; "classes" = the per-class schedule
+STEP_CANDIDATES = [
+    {"kind": "step", "persistent": False, "order_mode": "mass", "xcd_group": 32},
+    {"kind": "step", "persistent": False, "order_mode": "mass", "xcd_group": 0},
+    {"kind": "step", "persistent": True, "order_mode": "mass", "xcd_group": 32},
+    {"kind": "step", "persistent": True, "order_mode": "work", "xcd_group": 32},
+    {"kind": "classes"},
+]
+
+
+def tune_step(gi: GpuIndex, p: SearchParams, sp: Spectra, raw: dict, candidates=None, reps: int = 3):
+    """Time every candidate configuration of the whole search step on this batch (eager, after a warm-up run)
+    and return (the fastest step object, [(config, ms)]); a "classes" candidate is a ClassSchedule autotuned per
+    class first. Results never depend on the configuration."""
+    n = len(sp)
+    out = []
+    best = (float("inf"), None)
+    for cfg in candidates or STEP_CANDIDATES:
+        if cfg["kind"] == "classes":
+            obj = ClassSchedule(gi, p, sp)
+            autotune(obj, raw, reps=1)
+        else:
+            q = SearchParams(**p.to_dict())
+            q.xcd_group = int(cfg["xcd_group"])
+            obj = DeviceSearch(gi, q, n, persistent=bool(cfg["persistent"]), order_mode=cfg["order_mode"])
+        obj.run(raw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            obj.run(raw)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        out.append((dict(cfg), ms))
+        if ms < best[0]:
+            best = (ms, obj)
+        else:
+            del obj
+    return best[1], out
 
 
 def save_table(path: str, table: dict):

@@ -220,7 +220,7 @@ class FeatureLists:
                                                K.ptr(self.cnt), K.ptr(self.nb), K.ptr(self.iq), K.ptr(self.maxcnt),
                                                K.ptr(ds.status), K.stream()), "k1_candidate_features")
             return
-        K.check(K.lib().vh_k1_features(C.byref(vpf), C.byref(ds.gi.view), n, K.ptr(ds.order), K.ptr(pk["off"]),
+        K.check(K.lib().vh_k1_features(C.byref(vpf), C.byref(ds.gi.tile_view()), n, K.ptr(ds.order), K.ptr(pk["off"]),
                                        K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(hy["hyp_off"]),
                                        K.ptr(hy["maxfz"]), K.ptr(hy["mode"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
                                        self.stride, K.ptr(self.n_cand), K.ptr(self.row), K.ptr(self.hyp),
