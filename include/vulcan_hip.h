@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 10
+#define VH_ABI_VERSION 11
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -361,11 +361,13 @@ int vh_prefilter_features(const VhIndexView* ix, int32_t n_spec, int32_t stride,
                           float* out, void* stream);
 
 /* MLP score of every non-empty candidate slot (features as vh_prefilter_features, computed in registers):
- * score[slot]; empty slots are not written. */
+ * score[slot]; empty slots are not written. Only the filled slots are evaluated (their compact order is built
+ * in the workspace: vh_prefilter_score_workspace_bytes). n_spec * stride < 2^32 - 64. */
+size_t vh_prefilter_score_workspace_bytes(int32_t n_spec, int32_t stride);
 int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, int32_t stride,
                        const int32_t* n_cand, const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt,
                        const int16_t* c_nb, const uint32_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z,
-                       const float* h_M, float* score, void* stream);
+                       const float* h_M, float* score, void* ws, size_t ws_bytes, void* stream);
 
 /* Per spectrum: the kept slots by (score desc, NaN last; ties: earlier slot) -- the best keep_min, then every
  * slot whose score is >= best score - margin (f32), at most keep_max (keep_min = keep_max: a fixed top-keep) --

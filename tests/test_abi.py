@@ -19,6 +19,7 @@ def test_header_declares_the_hot_path():
     for need in ("vh_frag_count", "vh_index_build", "vh_k0", "vh_hypotheses", "vh_k1", "vh_k1_topk", "vh_k1_brute",
                  "vh_k2", "vh_rank", "vh_version", "vh_abi_sizeof", "vh_k1_features", "vh_mlp_pack",
                  "vh_mlp_forward", "vh_mlp_backward", "vh_mlp_adam", "vh_prefilter_features", "vh_prefilter_score",
+                 "vh_prefilter_score_workspace_bytes",
                  "vh_prefilter_select", "vh_index_block_count", "vh_index_build_blocks", "vh_k1j_pairs",
                  "vh_k1j_count", "vh_k1j_topk", "vh_k1j_features"):
         assert need in names
@@ -61,6 +62,12 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert L.vh_mlp_forward(C.c_void_p(8), 10, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8), None, None) == -1
     assert L.vh_mlp_backward(C.c_void_p(8), 100, *[C.c_void_p(8)] * 7, 16, None) == -3
     assert L.vh_mlp_workspace_bytes(-1) == 0 and L.vh_mlp_workspace_bytes(1000) > 1000 * 256 * 6
+    assert L.vh_prefilter_score_workspace_bytes(-1, 8) == 0 and L.vh_prefilter_score_workspace_bytes(10, 0) == 0
+    assert L.vh_prefilter_score_workspace_bytes(1000, 200) >= 1001 * 4 + (1000 * 200 // 64) * 4
+    ws_small = L.vh_prefilter_score_workspace_bytes(1000, 200) - 1
+    # pointers that are never dereferenced: both calls return before any launch
+    assert L.vh_prefilter_score(C.byref(ix), 1, 1000, 200, *[1] * 10, 1, ws_small, None) == -3   # workspace short
+    assert L.vh_prefilter_score(C.byref(ix), 1, 1000, 200, *[1] * 10, None, 10 ** 9, None) == -1  # no workspace
     assert L.vh_prefilter_select(C.byref(ix), 1, 2000, *[None] * 5, 16, 16, 0.0, 1, 32, *[None] * 4, None) == -1
     assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 0, 16, 0.0, 1, 32, *[None] * 4, None) == -1
     assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 16, 8, 0.0, 1, 32, *[None] * 4, None) == -1

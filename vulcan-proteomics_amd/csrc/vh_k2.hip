@@ -47,21 +47,40 @@ struct K2Buf {
   uint32_t* queue;   // persistent mode work counter (NULL: grid mode)
 };
 
-// Mass-sorted peaks of one spectrum in LDS with the coarse lookup table.
+// Lookup cell of mass x: clamp((x - base) * inv_w, 0, K2_CELLS - 1) truncated (= floor on the clamped range;
+// NaN goes to cell 0). Monotone in x, so the table built with it never passes the first peak >= x.
+__device__ __forceinline__ int k2_cell(float x, float base, float inv_w) {
+  return (int)fminf(fmaxf((x - base) * inv_w, 0.f), (float)(K2_CELLS - 1));
+}
+
+// f / z for a fragment charge z: exact for z = 1, 2 (a power-of-two scale rounds like the division)
+__device__ __forceinline__ float div_charge(float f, int z) {
+  if (z == 1) return f;
+  if (z == 2) return f * 0.5f;
+  return f / (float)z;
+}
+
+// Mass-sorted peaks of one spectrum in LDS as (mass, intensity) pairs, pk[np] = (NaN, 0) ends every scan, with
+// the coarse lookup table (tb[c] = first peak whose cell is >= c).
 struct PeakView {
-  const float* sm;
-  const float* si;
+  const float2* pk;
   const uint16_t* tb;
   int np;
   float base, inv_w;
 
-  __device__ __forceinline__ int first_at_least(float a) const {
-    float fc = floorf((a - base) * inv_w);
-    int lo = np ? (int)tb[fc < 0.f ? 0 : (fc > (float)(K2_CELLS - 1) ? K2_CELLS - 1 : (int)fc)] : 0;
-    while (lo < np && sm[lo] < a) ++lo;
-    return lo;
-  }
+  __device__ __forceinline__ int cell_start(float a) const { return tb[k2_cell(a, base, inv_w)]; }
 };
+
+// The most intense peak with a <= mass <= bd (ties -> later peak; intensities compared with >= against the
+// running maximum, kernels.cu:312-329). Starting at the cell of a, peaks below a are passed over and the scan
+// stops at the first peak above bd (or the NaN sentinel).
+__device__ __forceinline__ void best_in_window(const PeakView& V, float a, float bd, float& mx, int& best) {
+  for (int j = V.cell_start(a);; ++j) {
+    const float2 q = V.pk[j];
+    if (!(q.x <= bd)) break;
+    if (q.x >= a && q.y >= mx) { mx = q.y; best = j; }
+  }
+}
 
 // One tolerance window at m/z `mz`: the most intense peak (ties -> later peak) updates the running (mx, best)
 // shared across the unshifted/shifted windows and the window's own (mx_a, best_a) (kernels.cu:312-329).
@@ -69,10 +88,12 @@ __device__ __forceinline__ void window_scan(const PeakView& V, const DevP& P, fl
                                             float& best_mz, float& mx_a, int& best_a) {
   const float a = tol_bound(mz, P.tol_lo, P.tol_da);
   const float bd = tol_bound(mz, P.tol_hi, P.tol_da);
-  for (int j = V.first_at_least(a); j < V.np && V.sm[j] <= bd; ++j) {
-    const float v = V.si[j];
-    if (v >= mx) { mx = v; best = j; best_mz = mz; }
-    if (v >= mx_a) { mx_a = v; best_a = j; }
+  for (int j = V.cell_start(a);; ++j) {
+    const float2 q = V.pk[j];
+    if (!(q.x <= bd)) break;
+    if (q.x < a) continue;
+    if (q.y >= mx) { mx = q.y; best = j; best_mz = mz; }
+    if (q.y >= mx_a) { mx_a = q.y; best_a = j; }
   }
 }
 
@@ -88,10 +109,10 @@ __device__ __forceinline__ void ion_alt_sums(const PeakView& V, const DevP& P, f
       float mx_a = 0.f;
       int best_a = -1;
       const float base = alt == 0 ? cum : cum + shift;
-      window_scan(V, P, base / (float)z, mx, best, bmz, mx_a, best_a);
+      window_scan(V, P, div_charge(base, z), mx, best, bmz, mx_a, best_a);
       if (best_a >= 0) {
-        if (alt == 0) { iu = iu + V.si[best_a]; nu += 1; }
-        else { is = is + V.si[best_a]; ns += 1; }
+        if (alt == 0) { iu = iu + V.pk[best_a].y; nu += 1; }
+        else { is = is + V.pk[best_a].y; ns += 1; }
       }
     }
   }
@@ -110,8 +131,7 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
                                                  const float* __restrict__ pk_int, K2Buf O) {
   __shared__ float rm[52];
   __shared__ double lnf[K2_LNF];
-  __shared__ float s_m[K2_WAVES][VH_K2_MAX_PEAKS];
-  __shared__ float s_i[K2_WAVES][VH_K2_MAX_PEAKS];
+  __shared__ float2 s_p[K2_WAVES][VH_K2_MAX_PEAKS + 1];
   __shared__ uint16_t s_t[K2_WAVES][K2_CELLS];
   load_rm(rm, P);
   for (int i = threadIdx.x; i < K2_LNF; i += blockDim.x) lnf[i] = lnfact_dev(i);
@@ -132,33 +152,22 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
   const int64_t q0 = pk_off[s];
   int np = (int)(pk_off[s + 1] - q0);
   np = np < VH_K2_MAX_PEAKS ? np : VH_K2_MAX_PEAKS;
-  float* sm = s_m[w];
-  float* si = s_i[w];
+  float2* pk = s_p[w];
   uint16_t* tb = s_t[w];
-  for (int j = lane; j < np; j += 64) {
-    sm[j] = pk_mass[q0 + j];
-    si[j] = pk_int[q0 + j];
-  }
+  for (int j = lane; j <= np; j += 64)
+    pk[j] = j < np ? make_float2(pk_mass[q0 + j], pk_int[q0 + j]) : make_float2(__builtin_nanf(""), 0.f);
   wave_sync();
-  // coarse table: cell(x) = floor((x - base) * inv_w), clamped to [0, K2_CELLS - 1]
-  const float base = np ? sm[0] : 0.f;
-  const float span = np ? sm[np - 1] - base : 0.f;
+  // coarse table over [first peak, last peak]
+  const float base = np ? pk[0].x : 0.f;
+  const float span = np ? pk[np - 1].x - base : 0.f;
   const float inv_w = span > 0.f ? (float)(K2_CELLS - 2) / span : 1.f;
   for (int j = lane; j <= np; j += 64) {
-    int cj = K2_CELLS;   // sentinel past the last cell for j == np
-    if (j < np) {
-      float f = floorf((sm[j] - base) * inv_w);
-      cj = f < 0.f ? 0 : (f > (float)(K2_CELLS - 1) ? K2_CELLS - 1 : (int)f);
-    }
-    int cp = -1;
-    if (j > 0) {
-      float f = floorf((sm[j - 1] - base) * inv_w);
-      cp = f < 0.f ? 0 : (f > (float)(K2_CELLS - 1) ? K2_CELLS - 1 : (int)f);
-    }
+    const int cj = j < np ? k2_cell(pk[j].x, base, inv_w) : K2_CELLS;   // j == np: past the last cell
+    const int cp = j > 0 ? k2_cell(pk[j - 1].x, base, inv_w) : -1;
     for (int c = cp + 1; c <= cj && c < K2_CELLS; ++c) tb[c] = (uint16_t)j;
   }
   wave_sync();
-  const PeakView V = {sm, si, tb, np, base, inv_w};
+  const PeakView V = {pk, tb, np, base, inv_w};
   for (int64_t c = c0 + lane; c < c0 + nc; c += 64) {
     const int r = c_row[c];
     const int maxfz = h_maxfz[c_hyp[c]];
@@ -173,11 +182,30 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
     Run rb = {0, 0, 0, 0}, ry = {0, 0, 0, 0};
     float bs_init = 0.f, yu_init = 0.f;   // sum_k is[0][k] and sum_k iu[1][k] (k ascending)
     int nbs_init = 0, nyu_init = 0;
+    // one fragment window at m/z mz: the best peak is matched (ppm, count, intensity, ladder run)
+    auto match = [&](float mz, int kind, int k) {
+      float mx = 0.f;
+      int best = -1;
+      best_in_window(V, tol_bound(mz, P.tol_lo, P.tol_da), tol_bound(mz, P.tol_hi, P.tol_da), mx, best);
+      if (best >= 0) {
+        const float2 q = pk[best];
+        const float num = (q.y * fabsf(mz - q.x)) * 2.0e6f;
+        ppm = ppm + num / (mz + q.x);
+        if (kind == 0) { mb += 1; sb = sb + q.y; run_matched(rb, k); }
+        else { my += 1; sy = sy + q.y; run_matched(ry, k); }
+      }
+    };
     for (int kind = 0; kind < 2; ++kind) {
       float cum = kind == 0 ? nt : X.mass32[r] - nt;
       for (int k = 0; k < L - 1; ++k) {
         const float rmv = rm[((mk >> k) & 1ULL) * 26 + (sq[k] - 'A')];
         cum = kind == 0 ? cum + rmv : cum - rmv;
+        if (!SHIFT || n_alt == 1) {   // charges ascending: z = 1, 2 without a division
+          if (maxfz > 1) match(cum, kind, k);
+          if (maxfz > 2) match(cum * 0.5f, kind, k);
+          for (int z = 3; z < maxfz; ++z) match(cum / (float)z, kind, k);
+          continue;
+        }
         float iu = 0.f, is = 0.f;
         int nu = 0, ns = 0;
         for (int z = 1; z < maxfz; ++z) {
@@ -187,14 +215,14 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
             float mx_a = 0.f;
             int best_a = -1;
             const float b = alt == 0 ? cum : cum + shift;
-            window_scan(V, P, b / (float)z, mx, best, mz, mx_a, best_a);
-            if (SHIFT && n_alt == 2 && best_a >= 0) {
-              if (alt == 0) { iu = iu + si[best_a]; nu += 1; }
-              else { is = is + si[best_a]; ns += 1; }
+            window_scan(V, P, div_charge(b, z), mx, best, mz, mx_a, best_a);
+            if (best_a >= 0) {
+              if (alt == 0) { iu = iu + pk[best_a].y; nu += 1; }
+              else { is = is + pk[best_a].y; ns += 1; }
             }
           }
           if (best >= 0) {
-            const float pmass = sm[best], pint = si[best];
+            const float pmass = pk[best].x, pint = pk[best].y;
             const float num = (pint * fabsf(mz - pmass)) * 2.0e6f;
             ppm = ppm + num / (mz + pmass);
             if (kind == 0) { mb += 1; sb = sb + pint; run_matched(rb, k); }

@@ -24,6 +24,7 @@ typedef __attribute__((ext_vector_type(8))) unsigned short u16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2_t;
+typedef __attribute__((ext_vector_type(2))) short i16x2;
 
 // parameter vector (f32 master weights): W1 [256][8] (out, in), b1, W2 [256][256] (out, in), b2, w3, b3
 #define TH_W1 0
@@ -59,8 +60,24 @@ __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float
 
 // two f32 -> packed bf16 (lo = a), round to nearest even: one v_cvt_pk_bf16_f32 on gfx950 (same bits as f2bf)
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
-  const bf16x2_t v = {(__bf16)a, (__bf16)b};
-  return __builtin_bit_cast(uint32_t, v);
+  typedef __attribute__((ext_vector_type(2))) float f32x2;
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+
+// max(x, 0.f) on the bit pattern: a float with the sign bit set is a negative int32 (one v_max_i32, no NaN
+// canonicalisation; a positive NaN passes through)
+__device__ __forceinline__ float relu_f32(float x) {
+  const int b = __float_as_int(x);
+  return __int_as_float(b > 0 ? b : 0);
+}
+
+// ReLU of two packed bf16: a bf16 with the sign bit set is a negative int16, so max(x, 0) as int16 pairs
+// (v_pk_max_i16) zeroes it and keeps the rest. Rounding f32 -> bf16 keeps the sign, so this equals rounding
+// max(x, 0.f).
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t w) {
+  const i16x2 z = {0, 0};
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, w), z));
 }
 
 __device__ __forceinline__ f32x4 mfma_bf16(u16x8 a, u16x8 b, f32x4 c) {
@@ -80,11 +97,9 @@ struct FeatIn {
   const uint64_t* mask;
 };
 
-// the 8 raw features of candidate slot c (false for an empty slot); vh_prefilter_features documents them
-__device__ __forceinline__ bool cand_features(const FeatIn& F, int64_t c, float* f) {
-  const int64_t s = c / F.stride;
-  const int i = (int)(c - s * F.stride);
-  if (i >= F.n_cand[s]) return false;
+// the 8 raw features of slot c = s * stride + i of spectrum s (a filled slot); vh_prefilter_features documents
+// them
+__device__ __forceinline__ void slot_features(const FeatIn& F, int64_t s, int64_t c, float* f) {
   const int row = F.c_row[c], h = F.c_hyp[c], cnt = F.c_cnt[c], mx = F.s_maxcnt[s];
   const float m32 = F.mass32[row], M = F.h_M[h];
   const uint32_t iq = F.c_iq[c];
@@ -96,6 +111,14 @@ __device__ __forceinline__ bool cand_features(const FeatIn& F, int64_t c, float*
   f[5] = ((M - m32) / m32) * 1000000.0f;
   f[6] = (float)F.length[row];
   f[7] = (float)F.h_z[h];
+}
+
+// the 8 raw features of candidate slot c (false for an empty slot)
+__device__ __forceinline__ bool cand_features(const FeatIn& F, int64_t c, float* f) {
+  const int64_t s = c / F.stride;
+  const int i = (int)(c - s * F.stride);
+  if (i >= F.n_cand[s]) return false;
+  slot_features(F, s, c, f);
   return true;
 }
 
@@ -134,20 +157,17 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* lds, const u16x8 (
   const float* b2 = (const float*)(lds + BL_B2);
   const float* w3 = (const float*)(lds + BL_W3);
   const float b3 = *(const float*)(lds + BL_B3);
-  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   // h1[nb][c] = bf16 pairs of K slice c: words 0-1 from neuron block 2c, words 2-3 from block 2c+1
   uint32_t h1[NB][8][4];
 #pragma unroll
   for (int mb = 0; mb < 16; ++mb) {
     const u16x8 a = w1p[mb * 64 + lane];
+    const f32x4 bb = *(const f32x4*)(b1 + 16 * mb + 4 * g);   // the bias enters as the accumulator's start
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      const f32x4 acc = mfma_bf16(a, xb[nb], zero);
-      const f32x4 bb = *(const f32x4*)(b1 + 16 * mb + 4 * g);
-      float v[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) v[r] = fmaxf(acc[r] + bb[r], 0.f);
-      const uint32_t w0 = pack_bf16x2(v[0], v[1]), w1 = pack_bf16x2(v[2], v[3]);
+      const f32x4 acc = mfma_bf16(a, xb[nb], bb);
+      const uint32_t w0 = relu_bf16x2(pack_bf16x2(acc[0], acc[1]));
+      const uint32_t w1 = relu_bf16x2(pack_bf16x2(acc[2], acc[3]));
       h1[nb][mb >> 1][2 * (mb & 1)] = w0;
       h1[nb][mb >> 1][2 * (mb & 1) + 1] = w1;
       if (STORE) {
@@ -166,26 +186,24 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* lds, const u16x8 (
   for (int nb = 0; nb < NB; ++nb) part[nb] = 0.f;
 #pragma unroll 1
   for (int mb2 = 0; mb2 < 16; ++mb2) {
+    const f32x4 bb = *(const f32x4*)(b2 + 16 * mb2 + 4 * g);   // the bias enters as the accumulator's start
+    const f32x4 ww = *(const f32x4*)(w3 + 16 * mb2 + 4 * g);
     f32x4 acc[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[nb] = zero;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const u16x8 a = w2p[(mb2 * 8 + c) * 64 + lane];
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
         const u32x4 hw = {h1[nb][c][0], h1[nb][c][1], h1[nb][c][2], h1[nb][c][3]};
-        acc[nb] = mfma_bf16(a, __builtin_bit_cast(u16x8, hw), acc[nb]);
+        acc[nb] = mfma_bf16(a, __builtin_bit_cast(u16x8, hw), c == 0 ? bb : acc[nb]);
       }
     }
-    const f32x4 bb = *(const f32x4*)(b2 + 16 * mb2 + 4 * g);
-    const f32x4 ww = *(const f32x4*)(w3 + 16 * mb2 + 4 * g);
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
       float v4[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float v = fmaxf(acc[nb][r] + bb[r], 0.f);
+        const float v = relu_f32(acc[nb][r]);
         v4[r] = v;
         part[nb] = fmaf(ww[r], v, part[nb]);
       }
@@ -249,30 +267,63 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void mlp_fwd_kernel(const unsigned 
   }
 }
 
+// Scoring runs over the filled slots only: off = exclusive prefix of the per-spectrum counts (off[n_spec] = the
+// total), tile_spec[t] = the spectrum holding candidate 64 t of that compact order.
+__global__ __launch_bounds__(256) void score_count_kernel(const int32_t* __restrict__ n_cand, int n_spec,
+                                                          int stride, uint32_t* off) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > n_spec) return;
+  const int v = s < n_spec ? n_cand[s] : 0;
+  off[s] = (uint32_t)(v < 0 ? 0 : (v > stride ? stride : v));
+}
+
+__global__ __launch_bounds__(256) void score_tiles_kernel(const uint32_t* __restrict__ off, int n_spec,
+                                                          uint32_t* tile_spec) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_spec) return;
+  const uint32_t a = off[s], b = off[s + 1];
+  for (uint32_t t = (a + 63u) >> 6; (t << 6) < b; ++t) tile_spec[t] = (uint32_t)s;
+}
+
+// Persistent: waves take 64-candidate tiles of the compact order; lane l computes the features of candidate
+// 64 t + l, and the four 16-candidate B operands of layer 1 are gathered from those lanes.
 __global__ __launch_bounds__(64 * MLP_WAVES) void prefilter_score_kernel(const unsigned char* __restrict__ blob,
-                                                                         FeatIn F, int64_t n_slots, float* score) {
+                                                                         FeatIn F,
+                                                                         const uint32_t* __restrict__ off,
+                                                                         const uint32_t* __restrict__ tile_spec,
+                                                                         int n_spec, float* score) {
   extern __shared__ __align__(16) unsigned char lds[];
   stage_blob(lds, blob);
   const int w = threadIdx.x >> 6, lane = lane_id(), g = lane >> 4, cl = lane & 15;
-  const int64_t tile = 16 * MLP_NB, n_tiles = (n_slots + tile - 1) / tile;
+  const uint32_t total = off[n_spec];
+  const int64_t n_tiles = ((int64_t)total + 63) >> 6;
   for (int64_t t = (int64_t)blockIdx.x * MLP_WAVES + w; t < n_tiles; t += (int64_t)gridDim.x * MLP_WAVES) {
-    const int64_t c0 = t * tile;
+    const uint32_t q = (uint32_t)(t << 6) + (uint32_t)lane;
+    const bool ok = q < total;
+    int64_t c = 0;
+    float f[8];
+    if (ok) {
+      int s = (int)tile_spec[t];
+      while (off[s + 1] <= q) ++s;   // spectra without candidates are passed over; off[n_spec] > q ends it
+      c = (int64_t)s * F.stride + (q - off[s]);
+      slot_features(F, s, c, f);
+    }
+    const u32x4 x = __builtin_bit_cast(u32x4, feature_operand(lds, f, ok));
     u16x8 xb[MLP_NB];
-    bool ok[MLP_NB];
 #pragma unroll
     for (int nb = 0; nb < MLP_NB; ++nb) {
-      const int64_t c = c0 + 16 * nb + cl;
-      float f[8];
-      ok[nb] = g == 0 && c < n_slots && cand_features(F, c, f);
-      xb[nb] = feature_operand(lds, f, ok[nb]);
+      u32x4 y;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t v = (uint32_t)__shfl((int)x[j], 16 * nb + cl);
+        y[j] = g == 0 ? v : 0u;   // lanes 16..63: the zero K padding of layer 1
+      }
+      xb[nb] = __builtin_bit_cast(u16x8, y);
     }
     float sc[MLP_NB];
-    mlp_tile<MLP_NB, false>(lds, xb, sc, c0, n_slots, nullptr, nullptr);
-    if (g == 0) {
-#pragma unroll
-      for (int nb = 0; nb < MLP_NB; ++nb)
-        if (ok[nb]) score[c0 + 16 * nb + cl] = sc[nb];
-    }
+    mlp_tile<MLP_NB, false>(lds, xb, sc, 0, 0, nullptr, nullptr);
+    const float mine = g == 0 ? sc[0] : g == 1 ? sc[1] : g == 2 ? sc[2] : sc[3];   // candidate 16 g + cl = lane
+    if (ok) score[c] = mine;
   }
 }
 
@@ -739,19 +790,40 @@ int vh_prefilter_features(const VhIndexView* ix, int32_t n_spec, int32_t stride,
   return 0;
 }
 
+static size_t score_a256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t vh_prefilter_score_workspace_bytes(int32_t n_spec, int32_t stride) {
+  if (n_spec < 0 || stride < 1) return 0;
+  const int64_t tiles = ((int64_t)n_spec * stride + 63) / 64 + 1;
+  return score_a256((size_t)(n_spec + 1) * 4) + score_a256((size_t)tiles * 4) + vh_scan_ws_u32(n_spec + 1) + 256;
+}
+
 int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, int32_t stride,
                        const int32_t* n_cand, const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt,
                        const int16_t* c_nb, const uint32_t* c_iq, const int32_t* s_maxcnt, const int8_t* h_z,
-                       const float* h_M, float* score, void* stream) {
+                       const float* h_M, float* score, void* ws, size_t ws_bytes, void* stream) {
   if (!ix || !blob || n_spec < 0 || stride < 1 || !score) return VH_E_ARG;
   if (n_spec == 0) return 0;
-  if (!n_cand || !c_row || !c_hyp || !c_cnt || !c_nb || !c_iq || !s_maxcnt || !h_z || !h_M) return VH_E_ARG;
-  const int64_t slots = (int64_t)n_spec * stride;
+  if (!n_cand || !c_row || !c_hyp || !c_cnt || !c_nb || !c_iq || !s_maxcnt || !h_z || !h_M || !ws) return VH_E_ARG;
+  if ((int64_t)n_spec * stride >= ((int64_t)1 << 32) - 64) return VH_E_LIMIT;
+  if (ws_bytes < vh_prefilter_score_workspace_bytes(n_spec, stride)) return VH_E_WORKSPACE;
   FeatIn F = make_feat(ix, stride, n_cand, c_row, c_hyp, c_cnt, c_nb, c_iq, s_maxcnt, h_z, h_M);
   if (allow_big_lds()) return VH_E_LIMIT;
-  const int g = mlp_grid((slots + 16 * MLP_NB - 1) / (16 * MLP_NB));
-  prefilter_score_kernel<<<g, 64 * MLP_WAVES, BL_LDS, (hipStream_t)stream>>>((const unsigned char*)blob, F, slots,
-                                                                             score);
+  hipStream_t st = (hipStream_t)stream;
+  unsigned char* p = (unsigned char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  uint32_t* off = (uint32_t*)p;
+  uint32_t* tile_spec = (uint32_t*)(p + score_a256((size_t)(n_spec + 1) * 4));
+  const int64_t tiles = ((int64_t)n_spec * stride + 63) / 64 + 1;
+  void* sws = p + score_a256((size_t)(n_spec + 1) * 4) + score_a256((size_t)tiles * 4);
+  score_count_kernel<<<(unsigned)((n_spec + 256) / 256), 256, 0, st>>>(n_cand, n_spec, stride, off);
+  VH_CHECK_LAUNCH();
+  int rc = vh_scan_exclusive_u32(off, (int64_t)n_spec + 1, sws, vh_scan_ws_u32(n_spec + 1), st);
+  if (rc) return rc;
+  score_tiles_kernel<<<(unsigned)((n_spec + 255) / 256), 256, 0, st>>>(off, n_spec, tile_spec);
+  VH_CHECK_LAUNCH();
+  const int g = mlp_grid(tiles);
+  prefilter_score_kernel<<<g, 64 * MLP_WAVES, BL_LDS, st>>>((const unsigned char*)blob, F, off, tile_spec, n_spec,
+                                                            score);
   VH_CHECK_LAUNCH();
   return 0;
 }

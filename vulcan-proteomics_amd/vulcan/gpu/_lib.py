@@ -56,7 +56,8 @@ _SIGS = {
     "vh_mlp_backward": ([P, I64, P, P, P, P, P, P, P, SZ, P], I32),
     "vh_mlp_adam": ([P, P, P, P, I64, F32, F32, F32, F32, F32, F32, P], I32),
     "vh_prefilter_features": ([PIX, I32, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
-    "vh_prefilter_score": ([PIX, P, I32, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_prefilter_score_workspace_bytes": ([I32, I32], SZ),
+    "vh_prefilter_score": ([PIX, P, I32, I32, P, P, P, P, P, P, P, P, P, P, P, SZ, P], I32),
     "vh_prefilter_select": ([PIX, I32, I32, P, P, P, P, P, I32, I32, F32, I32, I32, P, P, P, P, P], I32),
 }
 

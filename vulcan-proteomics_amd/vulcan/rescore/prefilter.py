@@ -197,6 +197,7 @@ class FeatureLists:
         self.iq = e(n * topk, torch.int32)
         self.maxcnt = e(n, torch.int32)
         self.score = e(n * topk, torch.float32)
+        self.score_ws = e(int(K.lib().vh_prefilter_score_workspace_bytes(n, topk)), torch.uint8)
 
     def k1(self, ds: DeviceSearch, vpf: VhParams, n: int):
         pk, hy = ds.peaks, ds.hyp
@@ -254,7 +255,8 @@ class FeatureLists:
         K.check(K.lib().vh_prefilter_score(C.byref(ds.gi.view), K.ptr(blob), n, self.stride, K.ptr(self.n_cand),
                                            K.ptr(self.row), K.ptr(self.hyp), K.ptr(self.cnt), K.ptr(self.nb),
                                            K.ptr(self.iq), K.ptr(self.maxcnt), K.ptr(hy["z"]), K.ptr(hy["M"]),
-                                           K.ptr(self.score), K.stream()), "prefilter_score")
+                                           K.ptr(self.score), K.ptr(self.score_ws), self.score_ws.numel(),
+                                           K.stream()), "prefilter_score")
 
 
 class LearnedSearch(DeviceSearch):
