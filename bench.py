@@ -42,17 +42,22 @@ def _stages(ev):
 
 
 def time_steps(obj, raw, steps, warmup):
-    """Warm up, then time `steps` runs (wall clock between synchronisations) with per-stage HIP events."""
+    """Warm up, time `steps` runs (wall clock between synchronisations, no instrumentation in the timed loop),
+    then run `steps` more with per-stage HIP events for the stage breakdown."""
     import torch
     for _ in range(warmup):
         obj.run(raw)
     torch.cuda.synchronize()
-    ev: dict = {}
     t0 = time.perf_counter()
+    for _ in range(steps):
+        obj.run(raw)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ev: dict = {}
     for _ in range(steps):
         obj.run(raw, ev)
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps, _stages(ev)
+    return dt, _stages(ev)
 
 
 MLP_FLOP_PER_CANDIDATE = 2 * (8 * 256 + 256 * 256 + 256)   # 135,680
@@ -384,22 +389,9 @@ def main():
         print(json.dumps(out), flush=True)
         return
     ds = E.DeviceSearch(gi, p, n, **modes)
-    for _ in range(a.warmup):
-        ds.run(raw)
-    torch.cuda.synchronize()
-    log("warmup done")
-    ev: dict = {}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        ds.run(raw, ev)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    dt, stage_ms = time_steps(ds, raw, a.steps, a.warmup)   # timed loop without events; stages from a second pass
+    wall = dt * a.steps
     log(f"timed {a.steps} steps: {wall:.3f} s")
-    stage_ms = {}
-    for k, lst in ev.items():
-        pairs = list(zip(lst[0::2], lst[1::2]))
-        stage_ms[k] = float(np.mean([s.elapsed_time(e) for s, e in pairs]))
     if int(ds.status.item()) != 0:
         raise RuntimeError(f"k1_topk status {int(ds.status.item())}")
     sel = ds.sel[:n].cpu().numpy()

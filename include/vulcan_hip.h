@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 11
+#define VH_ABI_VERSION 12
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -170,13 +170,15 @@ int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz,
                   int8_t* h_maxfz, uint8_t* h_mode, float* h_off, int64_t hyp_capacity, void* ws, size_t ws_bytes,
                   void* stream);
 
-/* Order of the spectra for the K1/K2 launches: ascending first-hypothesis window row (i.e. precursor mass;
- * spectra without hypotheses last), by a device radix sort. Waves then visit the index in mass order and the
- * kernels map contiguous ranges of this order onto each XCD, so an XCD's L2 serves a narrow band of buckets.
- * No reference counterpart (the reference launches work items in hypothesis order, engine.py:213-229). */
+/* Order of the spectra for the K1/K2 launches: by the band of the first hypothesis' window start row (65,536
+ * equal bands of the n_rows index rows, i.e. of precursor mass; spectra without hypotheses last; spectrum order
+ * inside a band), by two device radix passes. n_rows < 2^47. Waves then visit the index in mass order and the kernels map
+ * contiguous ranges of this order onto each XCD, so an XCD's L2 serves a narrow band of rows. Results do not
+ * depend on the order. No reference counterpart (the reference launches work items in hypothesis order,
+ * engine.py:213-229). */
 size_t vh_spectrum_order_workspace_bytes(int32_t n_spec);
-int vh_spectrum_order(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo, int32_t* order, void* ws,
-                      size_t ws_bytes, void* stream);
+int vh_spectrum_order(int32_t n_spec, int64_t n_rows, const int64_t* hyp_off, const int64_t* h_lo, int32_t* order,
+                      void* ws, size_t ws_bytes, void* stream);
 /* Work-ordered variant for uneven batches: spectra by (band = first window row >> band_log2 ascending, estimated
  * K1 work descending, spectrum), with work = n_peaks * sum_h max(maxfz_h - 1, 1) + (window rows) / 4 saturated
  * at 2^20 - 1. band_log2 >= 28 gives one band (pure longest-first order, the classic list-scheduling rule for a

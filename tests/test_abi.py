@@ -56,7 +56,7 @@ def test_argument_errors_are_reported_without_a_gpu():
                         None, None, None, None, None, None, None) == -1
     assert L.vh_k2(C.byref(vp), C.byref(ix), 1, None, None, None, None, 0, None, None, None, None, None, None, None,
                    None, None, None, None, None, None, None, None, None, None) == -1
-    assert L.vh_spectrum_order(3, None, None, None, None, 0, None) == -1
+    assert L.vh_spectrum_order(3, 10, None, None, None, None, 0, None) == -1
     assert L.vh_mlp_pack(None, None, None, None, None) == -1
     assert L.vh_mlp_forward(None, 10, None, None, None, None, None) == -1
     assert L.vh_mlp_forward(C.c_void_p(8), 10, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8), None, None) == -1

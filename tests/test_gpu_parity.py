@@ -481,15 +481,17 @@ def test_label_blindness_gpu():
 
 
 def test_spectrum_order_sorted_permutation(env1):
-    """vh_spectrum_order: a permutation of the spectra by first-hypothesis window row (spectra without
-    hypotheses last)."""
+    """vh_spectrum_order: a permutation of the spectra by the band of the first-hypothesis window row (65,536
+    equal bands of the index rows; spectra without hypotheses last), stable."""
     ds, _ = _topk_device(env1)
     n = len(env1.d["sp"])
     order = _np(ds.order)[:n]
     assert np.array_equal(np.sort(order), np.arange(n))
     off = env1.hyp["hyp_off"]
-    key = np.where(off[1:] > off[:-1], env1.hyp["lo"][np.minimum(off[:-1], max(len(env1.hyp["lo"]) - 1, 0))],
-                   2 ** 28 - 1)
+    n_rows = int(ds.gi.n)
+    lo = env1.hyp["lo"][np.minimum(off[:-1], max(len(env1.hyp["lo"]) - 1, 0))].astype(np.int64)
+    key = np.where(off[1:] > off[:-1], (np.clip(lo, 0, n_rows - 1) << 16) // n_rows, 65535)
+    assert len(np.unique(key)) > 8                   # the bands spread the spectra
     k = key[order]
     assert np.all(np.diff(k) >= 0)
     same = np.diff(k) == 0

@@ -91,6 +91,26 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wave-wide bitonic sort of a[0, n) in LDS (n a power of two, wave-private): every stage visits its n / 2
+// compare pairs (i with bit j clear, i | j), so no lane idles on the upper element of a pair.
+__device__ __forceinline__ void wave_bitonic_u64(uint64_t* a, int n, bool descending) {
+  const int lane = lane_id();
+  const int half = n >> 1;
+  for (int k = 2; k <= n; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int p = lane; p < half; p += 64) {
+        const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+        const uint64_t x = a[i], y = a[i | j];
+        const bool up = ((i & k) == 0) != descending;
+        if (up ? (x > y) : (x < y)) {
+          a[i] = y;
+          a[i | j] = x;
+        }
+      }
+      wave_sync();
+    }
+}
+
 __device__ __forceinline__ uint64_t ballot(bool b) { return __ballot(b); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ULL << lane_id()) - 1ULL; }
 

@@ -253,22 +253,28 @@ __global__ __launch_bounds__(256) void hyp_count_kernel(DevP P, int n_spec, cons
   hyp_off[s + 1] = n_hyp_of(P, pmz[s], charge[s], valid[s]);
 }
 
-__device__ __forceinline__ int64_t lower_bound_f(const float* a, int64_t n, float x) {
-  int64_t L = 0, R = n;
-  while (L < R) {
-    int64_t M = (L + R) >> 1;
-    if (a[M] < x) L = M + 1; else R = M;
+// Window bounds of every hypothesis, thread per hypothesis: hyp_emit_kernel leaves the f32 mass bounds in h_lo /
+// h_hi (bit patterns); they become lower_bound(lo) / upper_bound(hi) over mass32, the two searches advancing in
+// lockstep so their loads are in flight together.
+__global__ __launch_bounds__(256) void hyp_bounds_kernel(int n_spec, const int64_t* __restrict__ hyp_off,
+                                                         const float* __restrict__ mass32, int64_t n_rows,
+                                                         int64_t cap, int64_t* h_lo, int64_t* h_hi) {
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= cap || h >= hyp_off[n_spec]) return;
+  const float lo = __int_as_float((int)h_lo[h]), hi = __int_as_float((int)h_hi[h]);
+  int64_t L0 = 0, R0 = n_rows, L1 = 0, R1 = n_rows;
+  while (L0 < R0 || L1 < R1) {
+    if (L0 < R0) {
+      const int64_t M = (L0 + R0) >> 1;
+      if (mass32[M] < lo) L0 = M + 1; else R0 = M;
+    }
+    if (L1 < R1) {
+      const int64_t M = (L1 + R1) >> 1;
+      if (!(hi < mass32[M])) L1 = M + 1; else R1 = M;
+    }
   }
-  return L;
-}
-
-__device__ __forceinline__ int64_t upper_bound_f(const float* a, int64_t n, float x) {
-  int64_t L = 0, R = n;
-  while (L < R) {
-    int64_t M = (L + R) >> 1;
-    if (!(x < a[M])) L = M + 1; else R = M;
-  }
-  return L;
+  h_lo[h] = L0;
+  h_hi[h] = L1;
 }
 
 __global__ __launch_bounds__(256) void hyp_emit_kernel(DevP P, int n_spec, const double* __restrict__ pmz,
@@ -298,8 +304,8 @@ __global__ __launch_bounds__(256) void hyp_emit_kernel(DevP P, int n_spec, const
       h_iso[o] = (int8_t)iso;
       h_pm[o] = pm;
       h_M[o] = M;
-      h_lo[o] = lower_bound_f(mass32, n_rows, lo);
-      h_hi[o] = upper_bound_f(mass32, n_rows, hi);
+      h_lo[o] = (int64_t)__float_as_int(lo);   // mass bounds; rows by hyp_bounds_kernel
+      h_hi[o] = (int64_t)__float_as_int(hi);
       h_maxfz[o] = (int8_t)(mf > 2 ? mf : 2);
       if (h_mode) { h_mode[o] = VH_MODE_CLOSED; h_off[o] = 0.f; }
     }
@@ -327,8 +333,8 @@ __global__ __launch_bounds__(256) void hyp_emit_kernel(DevP P, int n_spec, const
       h_iso[o] = 0;
       h_pm[o] = pm;
       h_M[o] = M0;
-      h_lo[o] = lower_bound_f(mass32, n_rows, lo);
-      h_hi[o] = upper_bound_f(mass32, n_rows, hi);
+      h_lo[o] = (int64_t)__float_as_int(lo);
+      h_hi[o] = (int64_t)__float_as_int(hi);
       h_maxfz[o] = (int8_t)(mf > 2 ? mf : 2);
       h_mode[o] = open ? VH_MODE_OPEN : VH_MODE_OFFSET;
       h_off[o] = open ? __int_as_float(0x7fc00000) : P.offsets[q];
@@ -337,13 +343,22 @@ __global__ __launch_bounds__(256) void hyp_emit_kernel(DevP P, int n_spec, const
 }
 
 // ------------------------------------------------------------------------------------------------ spectrum order
-__global__ __launch_bounds__(256) void order_keys_kernel(int n_spec, const int64_t* __restrict__ hyp_off,
+// Mass order for locality only (results do not depend on it): spectra by their first window row in
+// 2^ORDER_BITS equal row bands (two radix passes), spectra without hypotheses last; the sort is stable, so a band
+// keeps spectrum order.
+#define ORDER_BITS 16
+__global__ __launch_bounds__(256) void order_keys_kernel(int n_spec, int64_t n_rows, const int64_t* __restrict__ hyp_off,
                                                          const int64_t* __restrict__ h_lo, uint64_t* keys) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_spec) return;
   int64_t h0 = hyp_off[s];
-  uint64_t lo = hyp_off[s + 1] > h0 ? (uint64_t)h_lo[h0] : 0x0FFFFFFFULL;
-  keys[s] = (lo << 32) | (uint64_t)(uint32_t)s;
+  const uint64_t top = (1ULL << ORDER_BITS) - 1ULL;
+  uint64_t band = top;
+  if (hyp_off[s + 1] > h0 && n_rows > 0) {
+    const int64_t lo = h_lo[h0] < 0 ? 0 : (h_lo[h0] >= n_rows ? n_rows - 1 : h_lo[h0]);
+    band = ((uint64_t)lo << ORDER_BITS) / (uint64_t)n_rows;   // lo < n_rows < 2^47
+  }
+  keys[s] = (band << 32) | (uint64_t)(uint32_t)s;
 }
 
 // work-ordered key: mass band of the first window, then estimated K1 work descending, then the spectrum
@@ -384,9 +399,10 @@ size_t vh_spectrum_order_workspace_bytes(int32_t n_spec) {
   return 2 * a256(n * 8) + vh_radix_ws_bytes((int64_t)n);
 }
 
-int vh_spectrum_order(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo, int32_t* order, void* ws,
-                      size_t ws_bytes, void* stream) {
-  if (n_spec < 0 || (n_spec && (!hyp_off || !h_lo || !order || !ws))) return VH_E_ARG;
+int vh_spectrum_order(int32_t n_spec, int64_t n_rows, const int64_t* hyp_off, const int64_t* h_lo, int32_t* order,
+                      void* ws, size_t ws_bytes, void* stream) {
+  if (n_spec < 0 || n_rows < 0 || (n_spec && (!hyp_off || !h_lo || !order || !ws))) return VH_E_ARG;
+  if (n_rows >= ((int64_t)1 << 47)) return VH_E_LIMIT;
   if (n_spec == 0) return 0;
   if (ws_bytes < vh_spectrum_order_workspace_bytes(n_spec)) return VH_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
@@ -394,10 +410,10 @@ int vh_spectrum_order(int32_t n_spec, const int64_t* hyp_off, const int64_t* h_l
   uint64_t* a = (uint64_t*)ws;
   uint64_t* b = (uint64_t*)((char*)ws + kb);
   unsigned g = (unsigned)((n_spec + 255) / 256);
-  order_keys_kernel<<<g, 256, 0, st>>>(n_spec, hyp_off, h_lo, a);
+  order_keys_kernel<<<g, 256, 0, st>>>(n_spec, n_rows, hyp_off, h_lo, a);
   VH_CHECK_LAUNCH();
   uint64_t* res = a;
-  int rc = vh_radix_sort_u64(a, b, n_spec, 32, 60, (char*)ws + 2 * kb, ws_bytes - 2 * kb, st, &res);
+  int rc = vh_radix_sort_u64(a, b, n_spec, 32, 32 + ORDER_BITS, (char*)ws + 2 * kb, ws_bytes - 2 * kb, st, &res);
   if (rc) return rc;
   order_out_kernel<<<g, 256, 0, st>>>(n_spec, res, 32, order);
   VH_CHECK_LAUNCH();
@@ -471,6 +487,9 @@ int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz,
   if (rc) return rc;
   hyp_emit_kernel<<<g, 256, 0, st>>>(P, n_spec, precursor_mz, charge, hyp_off, mass32, n_rows, h_spec, h_z, h_iso,
                                      h_pm, h_M, h_lo, h_hi, h_maxfz, h_mode, h_off);
+  VH_CHECK_LAUNCH();
+  const int64_t cap = (int64_t)n_spec * maxh;   // <= hyp_capacity
+  hyp_bounds_kernel<<<(unsigned)((cap + 255) / 256), 256, 0, st>>>(n_spec, hyp_off, mass32, n_rows, cap, h_lo, h_hi);
   VH_CHECK_LAUNCH();
   return 0;
 }

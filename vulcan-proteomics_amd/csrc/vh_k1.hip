@@ -14,13 +14,13 @@
 // entries of consecutive buckets side by side (one line for a group of four), each bucket's postings in that bin
 // are key-sorted and scanned until the key passes the high bound, and postings whose row falls in a segment of
 // that bucket increment their slot with an LDS atomic. The scan of the histogram feeds a
-// wave-private top-K buffer (ballot-compacted inserts, pruned by the current K-th key, bitonic compaction when
-// full). Closure and dedupe are one more bitonic sort. Nothing leaves the wave but the final candidate list.
+// wave-private top-K buffer (ballot-compacted inserts, pruned by the current K-th key, compacted by a bitwise
+// K-th-key search when full). Closure and dedupe follow one bitonic sort. Nothing leaves the wave but the final candidate list.
 //
 // Feature mode (MODE 2, the learned prefilter's input): each slot has two u64 LDS counters, one per ion type
 // (the posting's ion bit), count in the low 24 bits and the sum of the matching peaks' quantised intensities in
 // the high 40 bits (one u64 atomic per match). A winner's per-ion counts and intensity pseudo-logs travel in a
-// payload array beside the top-K key buffer (moved with the keys by the bitonic compaction); the keys themselves
+// payload array beside the top-K key buffer (moved with the keys by the compaction); the keys themselves
 // are those of MODE 1, so the top-K set and order are exactly MODE 1's; no closure.
 #include "vh_topk.h"
 

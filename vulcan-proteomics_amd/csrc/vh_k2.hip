@@ -71,17 +71,6 @@ struct PeakView {
   __device__ __forceinline__ int cell_start(float a) const { return tb[k2_cell(a, base, inv_w)]; }
 };
 
-// The most intense peak with a <= mass <= bd (ties -> later peak; intensities compared with >= against the
-// running maximum, kernels.cu:312-329). Starting at the cell of a, peaks below a are passed over and the scan
-// stops at the first peak above bd (or the NaN sentinel).
-__device__ __forceinline__ void best_in_window(const PeakView& V, float a, float bd, float& mx, int& best) {
-  for (int j = V.cell_start(a);; ++j) {
-    const float2 q = V.pk[j];
-    if (!(q.x <= bd)) break;
-    if (q.x >= a && q.y >= mx) { mx = q.y; best = j; }
-  }
-}
-
 // One tolerance window at m/z `mz`: the most intense peak (ties -> later peak) updates the running (mx, best)
 // shared across the unshifted/shifted windows and the window's own (mx_a, best_a) (kernels.cu:312-329).
 __device__ __forceinline__ void window_scan(const PeakView& V, const DevP& P, float mz, float& mx, int& best,
@@ -182,27 +171,45 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
     Run rb = {0, 0, 0, 0}, ry = {0, 0, 0, 0};
     float bs_init = 0.f, yu_init = 0.f;   // sum_k is[0][k] and sum_k iu[1][k] (k ascending)
     int nbs_init = 0, nyu_init = 0;
-    // one fragment window at m/z mz: the best peak is matched (ppm, count, intensity, ladder run)
-    auto match = [&](float mz, int kind, int k) {
+    // one fragment window [a, bd] at m/z mz, its scan starting at peak j (= q): the best peak is matched (ppm,
+    // count, intensity, ladder run)
+    auto window = [&](float mz, float a, float bd, int j, float2 q, int kind, int k) {
       float mx = 0.f;
       int best = -1;
-      best_in_window(V, tol_bound(mz, P.tol_lo, P.tol_da), tol_bound(mz, P.tol_hi, P.tol_da), mx, best);
-      if (best >= 0) {
-        const float2 q = pk[best];
-        const float num = (q.y * fabsf(mz - q.x)) * 2.0e6f;
-        ppm = ppm + num / (mz + q.x);
-        if (kind == 0) { mb += 1; sb = sb + q.y; run_matched(rb, k); }
-        else { my += 1; sy = sy + q.y; run_matched(ry, k); }
+      while (q.x <= bd) {   // the NaN sentinel ends every scan
+        if (q.x >= a && q.y >= mx) { mx = q.y; best = j; }
+        q = pk[++j];
       }
+      if (best >= 0) {
+        const float2 b = pk[best];
+        const float num = (b.y * fabsf(mz - b.x)) * 2.0e6f;
+        ppm = ppm + num / (mz + b.x);
+        if (kind == 0) { mb += 1; sb = sb + b.y; run_matched(rb, k); }
+        else { my += 1; sy = sy + b.y; run_matched(ry, k); }
+      }
+    };
+    auto match = [&](float mz, int kind, int k) {
+      const float a = tol_bound(mz, P.tol_lo, P.tol_da), bd = tol_bound(mz, P.tol_hi, P.tol_da);
+      const int j = V.cell_start(a);
+      window(mz, a, bd, j, pk[j], kind, k);
     };
     for (int kind = 0; kind < 2; ++kind) {
       float cum = kind == 0 ? nt : X.mass32[r] - nt;
+      int nres = L > 1 ? sq[0] : 'A';   // residue k + 1 is loaded while fragment k is matched
       for (int k = 0; k < L - 1; ++k) {
-        const float rmv = rm[((mk >> k) & 1ULL) * 26 + (sq[k] - 'A')];
+        const int res = nres;
+        if (k + 2 < L) nres = sq[k + 1];
+        const float rmv = rm[((mk >> k) & 1ULL) * 26 + (res - 'A')];
         cum = kind == 0 ? cum + rmv : cum - rmv;
         if (!SHIFT || n_alt == 1) {   // charges ascending: z = 1, 2 without a division
-          if (maxfz > 1) match(cum, kind, k);
-          if (maxfz > 2) match(cum * 0.5f, kind, k);
+          const float m2 = cum * 0.5f;
+          const float a1 = tol_bound(cum, P.tol_lo, P.tol_da), b1 = tol_bound(cum, P.tol_hi, P.tol_da);
+          const float a2 = tol_bound(m2, P.tol_lo, P.tol_da), b2 = tol_bound(m2, P.tol_hi, P.tol_da);
+          // both charges' lookups (cell start, first peak) are issued before either window is scanned
+          const int j1 = V.cell_start(a1), j2 = V.cell_start(a2);
+          const float2 q1 = pk[j1], q2 = pk[j2];
+          if (maxfz > 1) window(cum, a1, b1, j1, q1, kind, k);
+          if (maxfz > 2) window(m2, a2, b2, j2, q2, kind, k);
           for (int z = 3; z < maxfz; ++z) match(cum / (float)z, kind, k);
           continue;
         }

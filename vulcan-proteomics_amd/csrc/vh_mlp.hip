@@ -159,13 +159,22 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* lds, const u16x8 (
   const float b3 = *(const float*)(lds + BL_B3);
   // h1[nb][c] = bf16 pairs of K slice c: words 0-1 from neuron block 2c, words 2-3 from block 2c+1
   uint32_t h1[NB][8][4];
-#pragma unroll
-  for (int mb = 0; mb < 16; ++mb) {
+  // layer 1 in neuron blocks of 16: the MFMAs of block mb + 1 are issued before the epilogue (ReLU, bf16
+  // packing) of block mb; the scheduling fences keep two blocks of accumulators live at most
+  f32x4 la[2][NB];
+  auto l1_mfma = [&](int mb, f32x4(&acc)[NB]) {
     const u16x8 a = w1p[mb * 64 + lane];
     const f32x4 bb = *(const f32x4*)(b1 + 16 * mb + 4 * g);   // the bias enters as the accumulator's start
 #pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = mfma_bf16(a, xb[nb], bb);
+  };
+  l1_mfma(0, la[0]);
+#pragma unroll
+  for (int mb = 0; mb < 16; ++mb) {
+    if (mb + 1 < 16) l1_mfma(mb + 1, la[(mb + 1) & 1]);
+#pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      const f32x4 acc = mfma_bf16(a, xb[nb], bb);
+      const f32x4 acc = la[mb & 1][nb];
       const uint32_t w0 = relu_bf16x2(pack_bf16x2(acc[0], acc[1]));
       const uint32_t w1 = relu_bf16x2(pack_bf16x2(acc[2], acc[3]));
       h1[nb][mb >> 1][2 * (mb & 1)] = w0;
@@ -179,16 +188,15 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* lds, const u16x8 (
         }
       }
     }
-    __builtin_amdgcn_sched_barrier(0);   // layer 1 block by block: bounded live accumulators
+    __builtin_amdgcn_sched_barrier(0);
   }
   float part[NB];
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) part[nb] = 0.f;
-#pragma unroll 1
-  for (int mb2 = 0; mb2 < 16; ++mb2) {
+  // layer 2 in neuron blocks of 16, software-pipelined: the MFMAs of block mb2 + 1 are issued before the
+  // epilogue (ReLU, layer-3 dot product) of block mb2, so the VALU work runs while the matrix core is busy
+  auto l2_mfma = [&](int mb2, f32x4(&acc)[NB]) {
     const f32x4 bb = *(const f32x4*)(b2 + 16 * mb2 + 4 * g);   // the bias enters as the accumulator's start
-    const f32x4 ww = *(const f32x4*)(w3 + 16 * mb2 + 4 * g);
-    f32x4 acc[NB];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const u16x8 a = w2p[(mb2 * 8 + c) * 64 + lane];
@@ -198,6 +206,9 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* lds, const u16x8 (
         acc[nb] = mfma_bf16(a, __builtin_bit_cast(u16x8, hw), c == 0 ? bb : acc[nb]);
       }
     }
+  };
+  auto l2_epi = [&](int mb2, const f32x4(&acc)[NB]) {
+    const f32x4 ww = *(const f32x4*)(w3 + 16 * mb2 + 4 * g);
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
       float v4[4];
@@ -215,7 +226,19 @@ __device__ __forceinline__ void mlp_tile(const unsigned char* lds, const u16x8 (
         }
       }
     }
+  };
+  f32x4 acc0[NB], acc1[NB];
+  l2_mfma(0, acc0);
+#pragma unroll 1
+  for (int mb2 = 0; mb2 < 14; mb2 += 2) {
+    l2_mfma(mb2 + 1, acc1);
+    l2_epi(mb2, acc0);
+    l2_mfma(mb2 + 2, acc0);
+    l2_epi(mb2 + 1, acc1);
   }
+  l2_mfma(15, acc1);
+  l2_epi(14, acc0);
+  l2_epi(15, acc1);
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
     float p = part[nb];
@@ -285,41 +308,60 @@ __global__ __launch_bounds__(256) void score_tiles_kernel(const uint32_t* __rest
   for (uint32_t t = (a + 63u) >> 6; (t << 6) < b; ++t) tile_spec[t] = (uint32_t)s;
 }
 
-// Persistent: waves take 64-candidate tiles of the compact order; lane l computes the features of candidate
-// 64 t + l, and the four 16-candidate B operands of layer 1 are gathered from those lanes.
+// Thread per filled slot in the compact order q: its slot index (cidx[q] = s * stride + i) and its normalised
+// bf16 layer-1 operand (feat[q], 16 B: the 8 features as vh_prefilter_features, (f - mu) * isd).
+__global__ __launch_bounds__(256) void score_feat_kernel(const unsigned char* __restrict__ blob, FeatIn F,
+                                                         const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ tile_spec, int n_spec,
+                                                         u32x4* feat, uint32_t* cidx) {
+  const uint32_t total = off[n_spec];
+  const int64_t q64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q64 >= total) return;
+  const uint32_t q = (uint32_t)q64;
+  int s = (int)tile_spec[q >> 6];
+  while (off[s + 1] <= q) ++s;   // spectra without candidates are passed over; off[n_spec] > q ends it
+  const int64_t c = (int64_t)s * F.stride + (q - off[s]);
+  float f[8];
+  slot_features(F, s, c, f);
+  feat[q] = __builtin_bit_cast(u32x4, feature_operand(blob, f, true));
+  cidx[q] = (uint32_t)c;
+}
+
+// Persistent: waves take 64-candidate tiles of the compact order; lanes 0-15 load the four 16-candidate B
+// operands of layer 1 (lanes 16-63 carry its zero K padding), and lane l writes the score of candidate 64 t + l.
+// The next tile's operands are loaded before the current tile's MFMA work.
 __global__ __launch_bounds__(64 * MLP_WAVES) void prefilter_score_kernel(const unsigned char* __restrict__ blob,
-                                                                         FeatIn F,
                                                                          const uint32_t* __restrict__ off,
-                                                                         const uint32_t* __restrict__ tile_spec,
+                                                                         const u32x4* __restrict__ feat,
+                                                                         const uint32_t* __restrict__ cidx,
                                                                          int n_spec, float* score) {
   extern __shared__ __align__(16) unsigned char lds[];
   stage_blob(lds, blob);
   const int w = threadIdx.x >> 6, lane = lane_id(), g = lane >> 4, cl = lane & 15;
   const uint32_t total = off[n_spec];
   const int64_t n_tiles = ((int64_t)total + 63) >> 6;
-  for (int64_t t = (int64_t)blockIdx.x * MLP_WAVES + w; t < n_tiles; t += (int64_t)gridDim.x * MLP_WAVES) {
-    const uint32_t q = (uint32_t)(t << 6) + (uint32_t)lane;
-    const bool ok = q < total;
-    int64_t c = 0;
-    float f[8];
-    if (ok) {
-      int s = (int)tile_spec[t];
-      while (off[s + 1] <= q) ++s;   // spectra without candidates are passed over; off[n_spec] > q ends it
-      c = (int64_t)s * F.stride + (q - off[s]);
-      slot_features(F, s, c, f);
-    }
-    const u32x4 x = __builtin_bit_cast(u32x4, feature_operand(lds, f, ok));
-    u16x8 xb[MLP_NB];
+  const int64_t step = (int64_t)gridDim.x * MLP_WAVES;
+  u32x4 nx[MLP_NB];
+  uint32_t nc = 0;
+  auto load = [&](int64_t t) {
+    const uint32_t q0 = (uint32_t)(t << 6);
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int nb = 0; nb < MLP_NB; ++nb) {
-      u32x4 y;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t v = (uint32_t)__shfl((int)x[j], 16 * nb + cl);
-        y[j] = g == 0 ? v : 0u;   // lanes 16..63: the zero K padding of layer 1
-      }
-      xb[nb] = __builtin_bit_cast(u16x8, y);
+      const uint32_t q = q0 + 16u * nb + (uint32_t)cl;
+      nx[nb] = (g == 0 && q < total) ? feat[q] : zero4;
     }
+    nc = q0 + (uint32_t)lane < total ? cidx[q0 + lane] : 0u;
+  };
+  int64_t t = (int64_t)blockIdx.x * MLP_WAVES + w;
+  if (t < n_tiles) load(t);
+  for (; t < n_tiles; t += step) {
+    u16x8 xb[MLP_NB];
+#pragma unroll
+    for (int nb = 0; nb < MLP_NB; ++nb) xb[nb] = __builtin_bit_cast(u16x8, nx[nb]);
+    const uint32_t c = nc;
+    const bool ok = (uint32_t)(t << 6) + (uint32_t)lane < total;
+    if (t + step < n_tiles) load(t + step);
     float sc[MLP_NB];
     mlp_tile<MLP_NB, false>(lds, xb, sc, 0, 0, nullptr, nullptr);
     const float mine = g == 0 ? sc[0] : g == 1 ? sc[1] : g == 2 ? sc[2] : sc[3];   // candidate 16 g + cl = lane
@@ -524,25 +566,6 @@ __device__ __forceinline__ uint32_t score_ord(float x) {   // order-preserving u
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__device__ void bitonic_u64(uint64_t* a, int n, bool descending) {
-  const int lane = lane_id();
-  for (int k = 2; k <= n; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < n; i += 64) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const uint64_t x = a[i], y = a[ixj];
-          const bool up = ((i & k) == 0) != descending;
-          if (up ? (x > y) : (x < y)) {
-            a[i] = y;
-            a[ixj] = x;
-          }
-        }
-      }
-      wave_sync();
-    }
-}
-
 struct SelArgs {
   int32_t n_spec, stride_in, keep_min, keep_max, closure, out_stride, sort_n, clos_n, wave_bytes;
   float margin;
@@ -568,7 +591,7 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs A) {
   for (int i = lane; i < A.sort_n; i += 64)
     keys[i] = i < n ? ((uint64_t)score_ord(A.score[base + i]) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)i) : 0ULL;
   wave_sync();
-  bitonic_u64(keys, A.sort_n, true);
+  wave_bitonic_u64(keys, A.sort_n, true);
   // kept: the best keep_min, then every candidate scoring within margin of the best, at most keep_max
   int nt;
   {
@@ -599,7 +622,7 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs A) {
     ck[i] = v;
   }
   wave_sync();
-  bitonic_u64(ck, A.clos_n, false);
+  wave_bitonic_u64(ck, A.clos_n, false);
   int nout = 0;
   const int64_t ob = (int64_t)s * A.out_stride;
   for (int i0 = 0; i0 < ncl; i0 += 64) {
@@ -792,10 +815,12 @@ int vh_prefilter_features(const VhIndexView* ix, int32_t n_spec, int32_t stride,
 
 static size_t score_a256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// workspace: off [n_spec + 1], tile_spec [tiles], feat [slots] (16 B), cidx [slots], scan scratch
 size_t vh_prefilter_score_workspace_bytes(int32_t n_spec, int32_t stride) {
   if (n_spec < 0 || stride < 1) return 0;
-  const int64_t tiles = ((int64_t)n_spec * stride + 63) / 64 + 1;
-  return score_a256((size_t)(n_spec + 1) * 4) + score_a256((size_t)tiles * 4) + vh_scan_ws_u32(n_spec + 1) + 256;
+  const int64_t slots = (int64_t)n_spec * stride, tiles = (slots + 63) / 64 + 1;
+  return score_a256((size_t)(n_spec + 1) * 4) + score_a256((size_t)tiles * 4) + score_a256((size_t)slots * 16) +
+         score_a256((size_t)slots * 4) + vh_scan_ws_u32(n_spec + 1) + 256;
 }
 
 int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, int32_t stride,
@@ -811,18 +836,27 @@ int vh_prefilter_score(const VhIndexView* ix, const void* blob, int32_t n_spec, 
   if (allow_big_lds()) return VH_E_LIMIT;
   hipStream_t st = (hipStream_t)stream;
   unsigned char* p = (unsigned char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  const int64_t slots = (int64_t)n_spec * stride, tiles = (slots + 63) / 64 + 1;
   uint32_t* off = (uint32_t*)p;
-  uint32_t* tile_spec = (uint32_t*)(p + score_a256((size_t)(n_spec + 1) * 4));
-  const int64_t tiles = ((int64_t)n_spec * stride + 63) / 64 + 1;
-  void* sws = p + score_a256((size_t)(n_spec + 1) * 4) + score_a256((size_t)tiles * 4);
+  p += score_a256((size_t)(n_spec + 1) * 4);
+  uint32_t* tile_spec = (uint32_t*)p;
+  p += score_a256((size_t)tiles * 4);
+  u32x4* feat = (u32x4*)p;
+  p += score_a256((size_t)slots * 16);
+  uint32_t* cidx = (uint32_t*)p;
+  p += score_a256((size_t)slots * 4);
+  void* sws = p;
   score_count_kernel<<<(unsigned)((n_spec + 256) / 256), 256, 0, st>>>(n_cand, n_spec, stride, off);
   VH_CHECK_LAUNCH();
   int rc = vh_scan_exclusive_u32(off, (int64_t)n_spec + 1, sws, vh_scan_ws_u32(n_spec + 1), st);
   if (rc) return rc;
   score_tiles_kernel<<<(unsigned)((n_spec + 255) / 256), 256, 0, st>>>(off, n_spec, tile_spec);
   VH_CHECK_LAUNCH();
+  score_feat_kernel<<<(unsigned)((slots + 255) / 256), 256, 0, st>>>((const unsigned char*)blob, F, off, tile_spec,
+                                                                     n_spec, feat, cidx);
+  VH_CHECK_LAUNCH();
   const int g = mlp_grid(tiles);
-  prefilter_score_kernel<<<g, 64 * MLP_WAVES, BL_LDS, st>>>((const unsigned char*)blob, F, off, tile_spec, n_spec,
+  prefilter_score_kernel<<<g, 64 * MLP_WAVES, BL_LDS, st>>>((const unsigned char*)blob, off, feat, cidx, n_spec,
                                                             score);
   VH_CHECK_LAUNCH();
   return 0;

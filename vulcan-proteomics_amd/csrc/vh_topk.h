@@ -1,54 +1,10 @@
 // Per-spectrum top-K machinery shared by the K1 kernels (tile K1 and the select stage of the bucket-join K1):
-// wave-wide bitonic sorts, the ballot-compacted top-K buffer, the feature-mode intensity quantisation and the
+// the ballot-compacted top-K buffer, the feature-mode intensity quantisation and the
 // final top-K / pair closure / dedupe stage. Top-K key = count (clamped at 32767) << 48 | row << 20 |
 // (63 - local hypothesis) << 14, so the buffer order is count desc, row desc, lower hypothesis first
 // (select_topk of the reference, vulcan/gpu/engine.py:284-312).
 #pragma once
 #include "vh_common.h"
-
-__device__ void bitonic_sort_u64(uint64_t* a, int n, bool descending) {
-  const int lane = lane_id();
-  for (int k = 2; k <= n; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < n; i += 64) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          uint64_t x = a[i], y = a[ixj];
-          bool up = ((i & k) == 0) != descending;   // ascending run in this block
-          if (up ? (x > y) : (x < y)) {
-            a[i] = y;
-            a[ixj] = x;
-          }
-        }
-      }
-      wave_sync();
-    }
-  }
-}
-
-// bitonic sort of keys a (descending) carrying the payloads b along
-__device__ void bitonic_sort_u64_pair(uint64_t* a, uint64_t* b, int n) {
-  const int lane = lane_id();
-  for (int k = 2; k <= n; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < n; i += 64) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          uint64_t x = a[i], y = a[ixj];
-          bool up = (i & k) != 0;
-          if (up ? (x > y) : (x < y)) {
-            a[i] = y;
-            a[ixj] = x;
-            uint64_t t = b[i];
-            b[i] = b[ixj];
-            b[ixj] = t;
-          }
-        }
-      }
-      wave_sync();
-    }
-  }
-}
 
 // Keep the best K keys of the buffer (keys are unique: (row, local hypothesis) occurs once per spectrum), return
 // the new fill and set the pruning threshold (the K-th largest key once K are kept, else 0); pay (feature mode):
@@ -219,28 +175,13 @@ __device__ __forceinline__ void k1_finish(const DevP& P, const DevIdx& X, int s,
     }
     ck[i] = ((uint64_t)row << 36) | ((uint64_t)hl << 30) | ((uint64_t)c1 << 11) | (uint64_t)i;
   }
+  // ascending sort, padded to a power of two with all-ones entries (they sort last; ck holds clos_n >= that)
+  int np2 = 1;
+  while (np2 < ncl) np2 <<= 1;
+  for (int i = ncl + lane; i < np2; i += 64) ck[i] = ~0ULL;
   wave_sync();
-  // rank sort into the (no longer needed) top-K buffer: entries are unique, rank = #smaller entries
-  uint64_t* cs = buf;
-  for (int i0 = 0; i0 < ncl; i0 += 256) {
-    uint64_t mine[4];
-    int rk[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      mine[u] = i0 + u * 64 + lane < ncl ? ck[i0 + u * 64 + lane] : ~0ULL;
-      rk[u] = 0;
-    }
-    for (int j = 0; j < ncl; ++j) {
-      const uint64_t v = ck[j];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) rk[u] += v < mine[u] ? 1 : 0;
-    }
-    wave_sync();
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (i0 + u * 64 + lane < ncl) cs[rk[u]] = mine[u];
-  }
-  wave_sync();
+  wave_bitonic_u64(ck, np2, false);
+  const uint64_t* cs = ck;
   // dedupe by row: the first entry (lowest hypothesis) names the hypothesis, the largest count wins (ties: the
   // earlier entry, which also gives the feature payload)
   int nout = 0;

@@ -34,7 +34,7 @@ _SIGS = {
     "vh_k0": ([PP, I32, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_spectrum_order_workspace_bytes": ([I32], SZ),
-    "vh_spectrum_order": ([I32, P, P, P, P, SZ, P], I32),
+    "vh_spectrum_order": ([I32, I64, P, P, P, P, SZ, P], I32),
     "vh_spectrum_order_work": ([I32, P, P, P, P, P, I32, P, P, SZ, P], I32),
     "vh_k1": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I64, P, P, P, P, P, P, P], I32),
     "vh_k1_topk": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P], I32),

@@ -682,7 +682,7 @@ class DeviceSearch:
         mark("hyp")
         mark("order")
         if self.order_mode == "mass":
-            K.check(L.vh_spectrum_order(n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(self.order),
+            K.check(L.vh_spectrum_order(n, self.gi.n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(self.order),
                                         K.ptr(self.order_ws), self.order_ws.numel(), st), "spectrum_order")
         elif self.order_mode == "work":
             K.check(L.vh_spectrum_order_work(n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
