@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 12
+#define VH_ABI_VERSION 13
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -116,7 +116,8 @@ typedef struct VhBlockView {
 
 /* ---------------------------------------------------------------------------------------------- misc */
 int vh_version(void);
-/* sizeof(VhParams) (which = 0), VhIndexView (1) or VhBlockView (2), to check a binding's struct layout. */
+/* sizeof(VhParams) (which = 0), VhIndexView (1), VhBlockView (2), VhK2Cols (3) or VhPsmCols (4), to check a
+ * binding's struct layout. */
 size_t vh_abi_sizeof(int which);
 /* Scratch bytes of the prefix scans used inside vh_k0 / vh_hypotheses / vh_frag_count for n items. */
 size_t vh_scan_workspace_bytes(int64_t n);
@@ -324,6 +325,60 @@ int vh_candidate_shift(int32_t n_groups, const int64_t* grp_start, const int32_t
 int vh_rank(const VhParams* p, int32_t n_groups, const int64_t* grp_start, const int32_t* grp_count,
             int32_t stride, const int32_t* c_row, const double* hs, const int16_t* mb, const int16_t* my,
             int32_t report, int64_t* o_sel, double* o_next, void* stream);
+
+/* Per-candidate outputs of vh_k2 (inputs of vh_psm_features). */
+typedef struct VhK2Cols {
+  const double* hs;
+  const float* sb;
+  const float* sy;
+  const float* ppm;
+  const int16_t* mb;
+  const int16_t* my;
+  const int16_t* lb;
+  const int16_t* ly;
+} VhK2Cols;
+
+/* PSM table columns (PSM_SCHEMA.md section 1), one slot per (group, rank), dense [n_groups * report]; a slot
+ * whose vh_rank selection is -1 gets rank 0 and no other column is written. */
+typedef struct VhPsmCols {
+  int64_t* spec_idx;
+  int64_t* pf_row;
+  int32_t* rank;
+  int8_t* charge;
+  int8_t* isotope_error;
+  float* expmass;
+  float* calcmass;
+  float* delta_mass_ppm;
+  double* hyperscore;
+  double* delta_next;
+  double* delta_best;
+  int16_t* matched_b;
+  int16_t* matched_y;
+  int16_t* matched_peaks;
+  float* summed_intensity;
+  float* matched_intensity_pct;
+  float* average_ppm;
+  int16_t* longest_b;
+  int16_t* longest_y;
+  float* longest_y_pct;
+  int16_t* peptide_len;
+  int32_t* k1_count;
+  int32_t* scored_candidates;
+  double* poisson;
+} VhPsmCols;
+
+/* The feature columns of psm_table (vulcan/gpu/engine.py:361-432) for the vh_rank selections sel / next_hs of
+ * n_groups candidate groups (group g = spectrum grp_spec[g], or g when grp_spec is NULL): candidate columns
+ * gathered (c_row, c_hyp, c_cnt, k2), hypothesis columns (h_z, h_iso, h_pm), mass error
+ * |pm - calc - iso*NEUTRON| * 2e6 / (pm - iso*NEUTRON + calc) (f32), delta_next = hs - next, delta_best =
+ * rank-1 hs - hs, matched_intensity_pct = 100 * (sb + sy) / tic (f32), longest_y_pct = longest_y / length,
+ * scored_candidates = sum of h_scored over the spectrum's hypotheses, poisson = log10(lam^k e^-lam / e^lnfact(k))
+ * with lam = sum h_matched / sum h_scored (f64; an infinite probability becomes 0, Sage's 1e-325). */
+int vh_psm_features(const VhIndexView* ix, int32_t n_groups, const int32_t* grp_spec, int32_t report,
+                    const int64_t* sel, const double* next_hs, const int32_t* c_row, const int32_t* c_hyp,
+                    const int32_t* c_cnt, const VhK2Cols* k2, const int64_t* hyp_off, const int8_t* h_z,
+                    const int8_t* h_iso, const float* h_pm, const uint32_t* h_scored,
+                    const unsigned long long* h_matched, const float* tic, const VhPsmCols* out, void* stream);
 
 /* ---------------------------------------------------------------------------------------------- learned prefilter */
 /* No reference counterpart (the reference's learned models, vulcan/rescore/fast.py:198-254 and static.py:25-82,

@@ -19,7 +19,7 @@ def test_header_declares_the_hot_path():
     for need in ("vh_frag_count", "vh_index_build", "vh_k0", "vh_hypotheses", "vh_k1", "vh_k1_topk", "vh_k1_brute",
                  "vh_k2", "vh_rank", "vh_version", "vh_abi_sizeof", "vh_k1_features", "vh_mlp_pack",
                  "vh_mlp_forward", "vh_mlp_backward", "vh_mlp_adam", "vh_prefilter_features", "vh_prefilter_score",
-                 "vh_prefilter_score_workspace_bytes",
+                 "vh_prefilter_score_workspace_bytes", "vh_psm_features",
                  "vh_prefilter_select", "vh_index_block_count", "vh_index_build_blocks", "vh_k1j_pairs",
                  "vh_k1j_count", "vh_k1j_topk", "vh_k1j_features"):
         assert need in names
@@ -35,11 +35,14 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_layout_and_version():
     from vulcan.gpu import _lib
-    from vulcan.gpu._abi import VhBlockView, VhIndexView, VhParams
+    from vulcan.gpu._abi import VhBlockView, VhIndexView, VhK2Cols, VhParams, VhPsmCols
     L = _lib.load()
     assert L.vh_abi_sizeof(0) == C.sizeof(VhParams)
     assert L.vh_abi_sizeof(1) == C.sizeof(VhIndexView)
     assert L.vh_abi_sizeof(2) == C.sizeof(VhBlockView)
+    assert L.vh_abi_sizeof(3) == C.sizeof(VhK2Cols) == 64
+    assert L.vh_abi_sizeof(4) == C.sizeof(VhPsmCols) == 24 * 8
+    assert L.vh_abi_sizeof(5) == 0
     m = re.search(r"#define VH_ABI_VERSION (\d+)", open(HEADER).read())
     assert L.vh_version() == int(m.group(1))
 
@@ -62,6 +65,8 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert L.vh_mlp_forward(C.c_void_p(8), 10, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8), None, None) == -1
     assert L.vh_mlp_backward(C.c_void_p(8), 100, *[C.c_void_p(8)] * 7, 16, None) == -3
     assert L.vh_mlp_workspace_bytes(-1) == 0 and L.vh_mlp_workspace_bytes(1000) > 1000 * 256 * 6
+    assert L.vh_psm_features(C.byref(ix), 1, None, 0, *[None] * 6, *[None] * 7, None, None) == -1  # report 0
+    assert L.vh_psm_features(C.byref(ix), 1, None, 1, *[None] * 6, *[None] * 7, None, None) == -1  # no columns
     assert L.vh_prefilter_score_workspace_bytes(-1, 8) == 0 and L.vh_prefilter_score_workspace_bytes(10, 0) == 0
     assert L.vh_prefilter_score_workspace_bytes(1000, 200) >= 1001 * 4 + (1000 * 200 // 64) * 4
     ws_small = L.vh_prefilter_score_workspace_bytes(1000, 200) - 1

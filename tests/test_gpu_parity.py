@@ -449,6 +449,54 @@ def test_search_report5_matches_oracle_ranking(env1):
     assert got == want
 
 
+def test_psm_features_equal_oracle(env1):
+    """vh_psm_features (the PSM table's feature columns, computed on the device) against the numpy restatement
+    of psm_table's arithmetic on the same K2 outputs and oracle rank selections; the table's column order and
+    types follow the PSM schema."""
+    import pyarrow as pa
+    from oracle import psm as OP
+    from vulcan.gpu import engine as E
+    report = 5
+    tbl, _, dbg = E.search(env1.d["sp"], env1.gi, env1.l1, env1.p, report=report)
+    O = _O()
+    k2 = {k: _np(v) for k, v in dbg["k2"].items()}
+    spec = k2["spec"]
+    us, cnt = np.unique(spec, return_counts=True)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    rows = _np(dbg["r"]).astype(np.int32)
+    sel, nxt = O.rank(env1.vp, start, cnt, rows, k2["hyperscore"], k2["matched_b"], k2["matched_y"], report)
+    hyp = dbg["hyp"]
+    n = len(env1.d["sp"])
+    n_h = len(hyp["spec"])
+    hyp_off = np.concatenate([[0], np.cumsum(np.bincount(np.asarray(hyp["spec"]), minlength=n)[:n])])
+    ds = dbg["device"]
+    ref = OP.features(sel, nxt, us, rows, _np(dbg["h"]), _np(dbg["k1c"]),
+                      {"hs": k2["hyperscore"], "sb": k2["summed_b"], "sy": k2["summed_y"], "ppm": k2["ppm"],
+                       "mb": k2["matched_b"], "my": k2["matched_y"], "lb": k2["longest_b"], "ly": k2["longest_y"]},
+                      hyp_off, hyp["z"], hyp["iso"], hyp["pm"], _np(ds.scored)[:n_h], _np(ds.matched)[:n_h],
+                      _np(dbg["peaks"]["tic"])[:n], env1.l1.mass32, env1.l1.length, report)
+    ok = ref["rank"] > 0
+    assert tbl.num_rows == int(ok.sum()) > 0
+    for k, v in ref.items():
+        got, want = tbl.column(k).to_numpy(), v[ok]
+        if k == "poisson":
+            fin = np.isfinite(want)
+            assert np.array_equal(np.isfinite(got), fin) and np.array_equal(np.isnan(got), np.isnan(want))
+            assert np.allclose(got[fin], want[fin], rtol=HS_RTOL, atol=0), k
+        else:
+            assert got.dtype == want.dtype and np.array_equal(got, want), k
+    names = ["run", "spec_idx", "scan", "rt", "rank", "pf_row", "pf_id", "charge", "isotope_error", "expmass",
+             "calcmass", "delta_mass_ppm", "hyperscore", "delta_next", "delta_best", "matched_b", "matched_y",
+             "matched_peaks", "summed_intensity", "matched_intensity_pct", "average_ppm", "longest_b", "longest_y",
+             "longest_y_pct", "peptide_len", "missed_cleavages", "n_var", "k1_count", "scored_candidates", "poisson"]
+    assert tbl.column_names == names
+    types = {"spec_idx": pa.int64(), "rank": pa.int32(), "pf_row": pa.int64(), "charge": pa.int8(),
+             "isotope_error": pa.int8(), "expmass": pa.float32(), "hyperscore": pa.float64(),
+             "matched_b": pa.int16(), "average_ppm": pa.float32(), "k1_count": pa.int32(), "poisson": pa.float64()}
+    for k, t in types.items():
+        assert tbl.schema.field(k).type == t, k
+
+
 def test_deterministic_rerun(env1):
     a, _ = _topk_device(env1)
     ra = [_np(t).copy() for t in (a.n_cand, a.c_row, a.c_hyp, a.c_cnt, a.k2["hyperscore"], a.sel)]

@@ -367,6 +367,78 @@ __global__ __launch_bounds__(256) void rank_kernel(DevP P, int n_groups, const i
 
 __global__ void k2_zero_u32_kernel(uint32_t* p) { *p = 0u; }
 
+// ------------------------------------------------------------------------------------------------ PSM features
+// Thread per (group, rank) slot: the psm_table columns of the selected candidate (engine.py:361-432), in the
+// f32 / f64 operation order of the reference's numpy expressions.
+__global__ __launch_bounds__(256) void psm_kernel(DevIdx X, int n_groups, const int32_t* __restrict__ grp_spec,
+                                                  int report, const int64_t* __restrict__ sel,
+                                                  const double* __restrict__ next_hs,
+                                                  const int32_t* __restrict__ c_row,
+                                                  const int32_t* __restrict__ c_hyp,
+                                                  const int32_t* __restrict__ c_cnt, VhK2Cols K,
+                                                  const int64_t* __restrict__ hyp_off,
+                                                  const int8_t* __restrict__ h_z, const int8_t* __restrict__ h_iso,
+                                                  const float* __restrict__ h_pm,
+                                                  const uint32_t* __restrict__ h_scored,
+                                                  const unsigned long long* __restrict__ h_matched,
+                                                  const float* __restrict__ tic, VhPsmCols O) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)n_groups * report) return;
+  const int64_t g = q / report;
+  const int i = (int)(q - g * report);
+  const int64_t c = sel[q];
+  if (c < 0) {
+    O.rank[q] = 0;
+    return;
+  }
+  const int s = grp_spec ? grp_spec[g] : (int)g;
+  const int row = c_row[c], h = c_hyp[c];
+  const double hs = K.hs[c];
+  const int mb = K.mb[c], my = K.my[c], ly = K.ly[c];
+  const float sb = K.sb[c], sy = K.sy[c];
+  // hypothesis and mass columns (f32, the reference's order of operations)
+  const int iso = h_iso[h];
+  const float pm = h_pm[h], calc = X.mass32[row];
+  const float ie = (float)iso * VH_NEUTRON;
+  const float dm = (fabsf((pm - calc) - ie) * 2.0e6f) / ((pm - ie) + calc);
+  const int len = X.length[row];
+  // per-spectrum K1 statistics summed over its hypotheses
+  int64_t sc = 0, mt = 0;
+  for (int64_t hh = hyp_off[s]; hh < hyp_off[s + 1]; ++hh) {
+    sc += h_scored[hh];
+    mt += (int64_t)h_matched[hh];
+  }
+  const int kk = mb + my;
+  const double lam = (double)mt / (double)sc;
+  double pois = pow(lam, (double)kk) * exp(-lam) / exp(lnfact_dev(kk));
+  if (isinf(pois)) pois = 0.0;   // Sage's 1e-325, which is 0 in f64
+  const float summed = sb + sy;
+  O.spec_idx[q] = s;
+  O.pf_row[q] = row;
+  O.rank[q] = i + 1;
+  O.charge[q] = h_z[h];
+  O.isotope_error[q] = (int8_t)iso;
+  O.expmass[q] = pm;
+  O.calcmass[q] = calc;
+  O.delta_mass_ppm[q] = dm;
+  O.hyperscore[q] = hs;
+  O.delta_next[q] = hs - next_hs[q];
+  O.delta_best[q] = K.hs[sel[g * report]] - hs;
+  O.matched_b[q] = (int16_t)mb;
+  O.matched_y[q] = (int16_t)my;
+  O.matched_peaks[q] = (int16_t)kk;
+  O.summed_intensity[q] = summed;
+  O.matched_intensity_pct[q] = (100.0f * summed) / tic[s];
+  O.average_ppm[q] = K.ppm[c];
+  O.longest_b[q] = K.lb[c];
+  O.longest_y[q] = (int16_t)ly;
+  O.longest_y_pct[q] = (float)((double)ly / (double)len);
+  O.peptide_len[q] = (int16_t)len;
+  O.k1_count[q] = c_cnt[c];
+  O.scored_candidates[q] = (int32_t)sc;
+  O.poisson[q] = log10(pois);
+}
+
 extern "C" {
 
 int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* order, const int32_t* grp_spec,
@@ -410,6 +482,32 @@ int vh_candidate_shift(int32_t n_groups, const int64_t* grp_start, const int32_t
   if (!c_row || !c_hyp || !h_mode || !h_M || !mass32 || !c_shift) return VH_E_ARG;
   cand_shift_kernel<<<(unsigned)n_groups, 64, 0, (hipStream_t)stream>>>(n_groups, grp_start, grp_count, stride, c_row,
                                                                          c_hyp, h_mode, h_M, mass32, c_shift);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_psm_features(const VhIndexView* ix, int32_t n_groups, const int32_t* grp_spec, int32_t report,
+                    const int64_t* sel, const double* next_hs, const int32_t* c_row, const int32_t* c_hyp,
+                    const int32_t* c_cnt, const VhK2Cols* k2, const int64_t* hyp_off, const int8_t* h_z,
+                    const int8_t* h_iso, const float* h_pm, const uint32_t* h_scored,
+                    const unsigned long long* h_matched, const float* tic, const VhPsmCols* out, void* stream) {
+  if (!ix || n_groups < 0 || report < 1 || !k2 || !out) return VH_E_ARG;
+  if (n_groups == 0) return 0;
+  if (!sel || !next_hs || !c_row || !c_hyp || !c_cnt || !hyp_off || !h_z || !h_iso || !h_pm || !h_scored ||
+      !h_matched || !tic)
+    return VH_E_ARG;
+  const VhK2Cols K = *k2;
+  const VhPsmCols O = *out;
+  if (!K.hs || !K.sb || !K.sy || !K.ppm || !K.mb || !K.my || !K.lb || !K.ly) return VH_E_ARG;
+  if (!O.spec_idx || !O.pf_row || !O.rank || !O.charge || !O.isotope_error || !O.expmass || !O.calcmass ||
+      !O.delta_mass_ppm || !O.hyperscore || !O.delta_next || !O.delta_best || !O.matched_b || !O.matched_y ||
+      !O.matched_peaks || !O.summed_intensity || !O.matched_intensity_pct || !O.average_ppm || !O.longest_b ||
+      !O.longest_y || !O.longest_y_pct || !O.peptide_len || !O.k1_count || !O.scored_candidates || !O.poisson)
+    return VH_E_ARG;
+  const int64_t slots = (int64_t)n_groups * report;
+  psm_kernel<<<(unsigned)((slots + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      vh_make_devidx(ix), n_groups, grp_spec, report, sel, next_hs, c_row, c_hyp, c_cnt, K, hyp_off, h_z, h_iso,
+      h_pm, h_scored, h_matched, tic, O);
   VH_CHECK_LAUNCH();
   return 0;
 }

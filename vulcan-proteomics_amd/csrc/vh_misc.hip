@@ -194,7 +194,14 @@ extern "C" {
 int vh_version(void) { return VH_ABI_VERSION; }
 
 size_t vh_abi_sizeof(int which) {
-  return which == 0 ? sizeof(VhParams) : which == 1 ? sizeof(VhIndexView) : which == 2 ? sizeof(VhBlockView) : 0;
+  switch (which) {
+    case 0: return sizeof(VhParams);
+    case 1: return sizeof(VhIndexView);
+    case 2: return sizeof(VhBlockView);
+    case 3: return sizeof(VhK2Cols);
+    case 4: return sizeof(VhPsmCols);
+    default: return 0;
+  }
 }
 
 size_t vh_scan_workspace_bytes(int64_t n) { return scan_ws(n, sizeof(int64_t)); }

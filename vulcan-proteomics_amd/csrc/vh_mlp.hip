@@ -588,41 +588,82 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs A) {
   const int64_t base = (int64_t)s * A.stride_in;
   int n = A.n_cand[s];
   n = n < A.stride_in ? n : A.stride_in;
-  for (int i = lane; i < A.sort_n; i += 64)
-    keys[i] = i < n ? ((uint64_t)score_ord(A.score[base + i]) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)i) : 0ULL;
+  // keys (unsorted, slot order): score_ord << 32 | ~slot, so key order = score desc (NaN last), earlier slot first
+  uint64_t mx = 0;
+  for (int i = lane; i < n; i += 64) {
+    const uint64_t k = ((uint64_t)score_ord(A.score[base + i]) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)i);
+    keys[i] = k;
+    mx = k > mx ? k : mx;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t v = __shfl_xor(mx, o);
+    mx = v > mx ? v : mx;
+  }
   wave_sync();
-  wave_bitonic_u64(keys, A.sort_n, true);
-  // kept: the best keep_min, then every candidate scoring within margin of the best, at most keep_max
-  int nt;
-  {
-    const int lim = n < A.keep_max ? n : A.keep_max;
-    const float top = n > 0 ? A.score[base + (0xFFFFFFFFu - (uint32_t)keys[0])] : 0.f;
-    const float thr = top - A.margin;
-    int cnt = 0;
-    for (int t0 = 0; t0 < lim; t0 += 64) {
-      const int t = t0 + lane;
-      const bool ok = t < lim && A.score[base + (0xFFFFFFFFu - (uint32_t)keys[t])] >= thr;
-      cnt += __popcll(ballot(ok));
+  // kept: the best keep_min, then every candidate scoring within margin of the best, at most keep_max. Those
+  // scoring >= best - margin are a prefix of the key order, so the kept set is the top nt keys.
+  const float thr = (n > 0 ? A.score[base + (0xFFFFFFFFu - (uint32_t)mx)] : 0.f) - A.margin;
+  int within = 0;
+  for (int i0 = 0; i0 < n; i0 += 64) within += __popcll(ballot(i0 + lane < n && A.score[base + i0 + lane] >= thr));
+  const int lim = n < A.keep_max ? n : A.keep_max;
+  const int cnt = within < lim ? within : lim;
+  const int lo = n < A.keep_min ? n : A.keep_min;
+  const int nt = cnt > lo ? cnt : lo;
+  // the nt-th largest score_ord T (bitwise search), then the lowest slots among the keys scoring exactly T
+  uint32_t T = 0;
+  int above = 0;   // keys with score_ord > T
+  const int mode = nt >= n ? 0 : (nt == within ? 1 : 2);   // 0: all, 1: the prefix {score >= thr}, 2: search
+  if (mode == 2) {
+    {
+      for (int bit = 31; bit >= 0; --bit) {
+        const uint32_t c = T | (1u << bit);
+        int k = 0;
+        for (int i = lane; i < n; i += 64) k += (uint32_t)(keys[i] >> 32) >= c ? 1 : 0;
+        if (wave_sum(k) >= nt) T = c;
+      }
+      int k = 0;
+      for (int i = lane; i < n; i += 64) k += (uint32_t)(keys[i] >> 32) > T ? 1 : 0;
+      above = wave_sum(k);
     }
-    const int lo = n < A.keep_min ? n : A.keep_min;
-    nt = cnt > lo ? cnt : lo;
+  }
+  // compaction of the kept slots (slot order) into closure entries
+  int nk = 0, ties = 0;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + lane;
+    bool keep = false;
+    if (i < n) {
+      if (mode == 0) keep = true;
+      else if (mode == 1) keep = A.score[base + i] >= thr;
+      else {
+        const uint32_t o = (uint32_t)(keys[i] >> 32);
+        keep = o > T;
+        const bool tie = o == T;
+        const uint64_t tb = ballot(tie);
+        if (tie) keep = ties + __popcll(tb & lanemask_lt()) < nt - above;
+        ties += __popcll(tb);
+      }
+    }
+    const uint64_t kb = ballot(keep);
+    if (keep) {
+      const int e = nk + __popcll(kb & lanemask_lt());
+      const int32_t row = A.c_row[base + i];
+      eh[e] = A.c_hyp[base + i];
+      ec[e] = A.c_cnt[base + i];
+      ck[e] = ((uint64_t)(uint32_t)row << 32) | (uint64_t)e;
+      if (A.closure) {
+        eh[nt + e] = A.c_hyp[base + i];
+        ec[nt + e] = -1;
+        ck[nt + e] = ((uint64_t)(uint32_t)A.partner[row] << 32) | (uint64_t)(nt + e);
+      }
+    }
+    nk += __popcll(kb);
   }
   const int ncl = A.closure ? 2 * nt : nt;
-  for (int i = lane; i < A.clos_n; i += 64) {
-    uint64_t v = ~0ULL;
-    if (i < ncl) {
-      const int t = i < nt ? i : i - nt;
-      const uint32_t idx = 0xFFFFFFFFu - (uint32_t)keys[t];
-      const int32_t row = A.c_row[base + idx];
-      const int32_t r = i < nt ? row : (int32_t)A.partner[row];
-      eh[i] = A.c_hyp[base + idx];
-      ec[i] = i < nt ? A.c_cnt[base + idx] : -1;
-      v = ((uint64_t)(uint32_t)r << 32) | (uint64_t)i;
-    }
-    ck[i] = v;
-  }
+  int np2 = 1;
+  while (np2 < ncl) np2 <<= 1;
+  for (int i = ncl + lane; i < np2; i += 64) ck[i] = ~0ULL;
   wave_sync();
-  wave_bitonic_u64(ck, A.clos_n, false);
+  wave_bitonic_u64(ck, np2, false);
   int nout = 0;
   const int64_t ob = (int64_t)s * A.out_stride;
   for (int i0 = 0; i0 < ncl; i0 += 64) {

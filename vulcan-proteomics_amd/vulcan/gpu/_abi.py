@@ -1,4 +1,5 @@
-"""ctypes mirror of include/vulcan_hip.h (VhParams, VhIndexView) and conversion from SearchParams.
+"""ctypes mirror of include/vulcan_hip.h (VhParams, VhIndexView, VhBlockView, VhK2Cols, VhPsmCols) and conversion
+from SearchParams.
 
 Torch-free. The struct layouts are checked against the C side with vh_abi_sizeof (tests).
 """
@@ -32,6 +33,20 @@ class VhIndexView(C.Structure):
 class VhBlockView(C.Structure):
     _fields_ = [("n_blocks", I64), ("max_rows", I32), ("max_postings", I32), ("row0", P), ("off", P), ("key", P),
                 ("row", P)]
+
+
+class VhK2Cols(C.Structure):
+    _fields_ = [(k, P) for k in ("hs", "sb", "sy", "ppm", "mb", "my", "lb", "ly")]
+
+
+PSM_COLS = ("spec_idx", "pf_row", "rank", "charge", "isotope_error", "expmass", "calcmass", "delta_mass_ppm",
+            "hyperscore", "delta_next", "delta_best", "matched_b", "matched_y", "matched_peaks", "summed_intensity",
+            "matched_intensity_pct", "average_ppm", "longest_b", "longest_y", "longest_y_pct", "peptide_len",
+            "k1_count", "scored_candidates", "poisson")
+
+
+class VhPsmCols(C.Structure):
+    _fields_ = [(k, P) for k in PSM_COLS]
 
 
 def mod_arrays(static_mods: dict, var_mods: dict):

@@ -12,13 +12,15 @@ import ctypes as C
 import torch
 
 from .. import _build
-from ._abi import VhBlockView, VhIndexView, VhParams
+from ._abi import VhBlockView, VhIndexView, VhK2Cols, VhParams, VhPsmCols
 
 P = C.c_void_p
 I32, I64, SZ, F32 = C.c_int32, C.c_int64, C.c_size_t, C.c_float
 PP = C.POINTER(VhParams)
 PIX = C.POINTER(VhIndexView)
 PBV = C.POINTER(VhBlockView)
+PK2 = C.POINTER(VhK2Cols)
+PPSM = C.POINTER(VhPsmCols)
 
 _SIGS = {
     "vh_version": ([], I32),
@@ -42,6 +44,7 @@ _SIGS = {
     "vh_k2": ([PP, PIX, I32, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_candidate_shift": ([I32, P, P, I32, P, P, P, P, P, P, P], I32),
     "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
+    "vh_psm_features": ([PIX, I32, P, I32, P, P, P, P, P, PK2, P, P, P, P, P, P, P, PPSM, P], I32),
     "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_k1j_workspace_bytes": ([I64, I64], SZ),
     "vh_k1j_pairs": ([PBV, I32, P, P, P, P, P, P, P, I64, P, I64, P, P, I64, P, P, SZ, P], I32),

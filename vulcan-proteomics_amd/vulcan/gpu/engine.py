@@ -433,14 +433,55 @@ def rank_groups(gi, p: SearchParams, n_groups, grp_start, grp_count, stride, row
     return sel[:n_groups * report], nxt[:n_groups * report]
 
 
+def psm_features(gi: GpuIndex, n_spec: int, grp_spec, sel, nxt, report: int, h, r, k1c, k2, hyp, scored, matched,
+                 peaks) -> dict:
+    """vh_psm_features: the psm_table feature columns (engine.py:361-432) of vh_rank's selections, computed on
+    the device, dense [n_groups * report] (rank 0 = empty slot). hyp: device hypotheses (spec, z, iso, pm)."""
+    from ._abi import PSM_COLS, VhK2Cols, VhPsmCols
+    dev = gi.dev
+    n_groups = len(grp_spec)
+    q = max(n_groups * report, 1)
+    types = {"spec_idx": torch.int64, "pf_row": torch.int64, "rank": torch.int32, "charge": torch.int8,
+             "isotope_error": torch.int8, "hyperscore": torch.float64, "delta_next": torch.float64,
+             "delta_best": torch.float64, "poisson": torch.float64, "matched_b": torch.int16, "matched_y": torch.int16,
+             "matched_peaks": torch.int16, "longest_b": torch.int16, "longest_y": torch.int16,
+             "peptide_len": torch.int16, "k1_count": torch.int32, "scored_candidates": torch.int32}
+    cols = {k: torch.empty(q, dtype=types.get(k, torch.float32), device=dev) for k in PSM_COLS}
+    if n_groups == 0:
+        return {k: v[:0] for k, v in cols.items()}
+    hs_spec = hyp["spec"].long()
+    hyp_off = torch.zeros(n_spec + 1, dtype=torch.int64, device=dev)
+    hyp_off[1:] = torch.cumsum(torch.bincount(hs_spec, minlength=n_spec)[:n_spec], 0)
+    kc = {"hs": k2["hyperscore"], "sb": k2["summed_b"], "sy": k2["summed_y"], "ppm": k2["ppm"],
+          "mb": k2["matched_b"], "my": k2["matched_y"], "lb": k2["longest_b"], "ly": k2["longest_y"]}
+    kc = {k: v.contiguous() for k, v in kc.items()}
+    k2c = VhK2Cols(**{k: K.ptr(v) for k, v in kc.items()})
+    oc = VhPsmCols(**{k: K.ptr(v) for k, v in cols.items()})
+    args = {"grp": grp_spec.to(torch.int32).contiguous(), "row": r.to(torch.int32).contiguous(),
+            "hyp": h.to(torch.int32).contiguous(), "cnt": k1c.to(torch.int32).contiguous(),
+            "z": hyp["z"].to(torch.int8).contiguous(), "iso": hyp["iso"].to(torch.int8).contiguous(),
+            "pm": hyp["pm"].to(torch.float32).contiguous(), "sc": scored.to(torch.int32).contiguous(),
+            "mt": matched.to(torch.int64).contiguous(), "tic": peaks["tic"].to(torch.float32).contiguous()}
+    K.check(K.lib().vh_psm_features(C.byref(gi.view), n_groups, K.ptr(args["grp"]), report, K.ptr(sel), K.ptr(nxt),
+                                    K.ptr(args["row"]), K.ptr(args["hyp"]), K.ptr(args["cnt"]), C.byref(k2c),
+                                    K.ptr(hyp_off), K.ptr(args["z"]), K.ptr(args["iso"]), K.ptr(args["pm"]),
+                                    K.ptr(args["sc"]), K.ptr(args["mt"]), K.ptr(args["tic"]), C.byref(oc),
+                                    K.stream()), "psm_features")
+    return {k: v[:n_groups * report] for k, v in cols.items()}
+
+
 def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched, peaks, p: SearchParams,
               report: int | None = None, return_hyp: bool = False):
-    """Rank candidates per spectrum (vh_rank: hyperscore desc, row asc) and build the PSM table with the
-    reference's columns (engine.py:361-432, PSM_SCHEMA.md section 1)."""
+    """Rank candidates per spectrum (vh_rank: hyperscore desc, row asc), compute the feature columns on the
+    device (vh_psm_features) and build the PSM table with the reference's columns (engine.py:361-432,
+    PSM_SCHEMA.md section 1); the host only gathers the run's scan / rt and the index's pf_id / missed / n_var."""
     report = p.report_psms if report is None else report
     dev = gi.dev
     spec = k2["spec"]
     n = len(spec)
+    n_spec = len(sp)
+    hyp_d = hyp if isinstance(hyp.get("spec"), torch.Tensor) else {
+        k: torch.as_tensor(np.ascontiguousarray(hyp[k]), device=dev) for k in ("spec", "z", "iso", "pm")}
     if n:
         us, cnts = torch.unique_consecutive(spec, return_counts=True)
         start = torch.zeros(len(us), dtype=torch.int64, device=dev)
@@ -449,66 +490,43 @@ def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched
         sel, nxt = rank_groups(gi, p, len(us), start, cnts.to(torch.int32).contiguous(), 0, row32,
                                k2["hyperscore"].contiguous(), k2["matched_b"].contiguous(),
                                k2["matched_y"].contiguous(), report)
-        sel = sel.view(-1, report)
-        nxt = nxt.view(-1, report)
-        ok = sel >= 0
+        cols = psm_features(gi, n_spec, us, sel, nxt, report, h, r, k1c, k2, hyp_d, scored, matched, peaks)
+        ok = cols["rank"] > 0
         idx = sel[ok]
-        rk = (torch.nonzero(ok)[:, 1] + 1).cpu().numpy()
-        best_idx = sel[:, 0].clamp(min=0)
-        best = torch.repeat_interleave(k2["hyperscore"][best_idx], ok.sum(1))
-        dn_next = nxt[ok]
+        dcols = {k: v[ok].cpu().numpy() for k, v in cols.items()}
+        hh = h[idx].cpu().numpy()
     else:
-        idx = torch.zeros(0, dtype=torch.int64, device=dev)
-        rk = np.zeros(0, np.int64)
-        best = torch.zeros(0, dtype=torch.float64, device=dev)
-        dn_next = torch.zeros(0, dtype=torch.float64, device=dev)
-    out = {k: v[idx].cpu().numpy() for k, v in k2.items() if k != "spec"}
-    s = spec[idx].cpu().numpy()
-    hsk = out["hyperscore"]
-    dn = hsk - dn_next.cpu().numpy()
-    db = best.cpu().numpy() - hsk
-    hh = h[idx].cpu().numpy()
-    rows = r[idx].cpu().numpy()
-    k1 = k1c[idx].cpu().numpy()
-    n_spec = len(sp)
-    hyp_h = hyp if not isinstance(hyp.get("spec"), torch.Tensor) else hyp_to_host(hyp)
-    n_h = len(hyp_h["spec"])
-    sc = np.zeros(n_spec, np.int64)
-    mt = np.zeros(n_spec, np.int64)
-    np.add.at(sc, hyp_h["spec"], scored[:n_h].cpu().numpy().astype(np.int64))
-    np.add.at(mt, hyp_h["spec"], matched[:n_h].cpu().numpy().astype(np.int64))
-    tic = peaks["tic"][:n_spec].cpu().numpy()
-    mbk, myk = out["matched_b"].astype(np.int64), out["matched_y"].astype(np.int64)
-    kk = mbk + myk
-    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
-        lam = mt[s] / sc[s]
-        pois = lam ** kk * np.exp(-lam) / np.exp(lnfact(kk))
-        pois = np.where(np.isinf(pois), 1e-325, pois)
-        log_pois = np.log10(pois)
-    z = hyp_h["z"][hh].astype(np.int64)
-    iso = hyp_h["iso"][hh].astype(np.int64)
-    pm = hyp_h["pm"][hh]
-    calc = l1.mass32[rows]
-    ie = (iso.astype(np.float32) * NEUTRON).astype(np.float32)
-    dm = (np.abs(pm - calc - ie) * np.float32(2e6) / (pm - ie + calc)).astype(np.float32)
-    plen = l1.length[rows].astype(np.int64)
-    cols = {
-        "run": pa.array([sp.run] * len(s)),
-        "spec_idx": s.astype(np.int64), "scan": sp.scan[s], "rt": sp.rt[s],
-        "rank": rk.astype(np.int32), "pf_row": rows.astype(np.int64), "pf_id": l1.pf_id[rows],
-        "charge": z.astype(np.int8), "isotope_error": iso.astype(np.int8),
-        "expmass": pm.astype(np.float32), "calcmass": calc.astype(np.float32), "delta_mass_ppm": dm,
-        "hyperscore": hsk, "delta_next": dn, "delta_best": db,
-        "matched_b": mbk.astype(np.int16), "matched_y": myk.astype(np.int16), "matched_peaks": kk.astype(np.int16),
-        "summed_intensity": (out["summed_b"] + out["summed_y"]).astype(np.float32),
-        "matched_intensity_pct": (100.0 * (out["summed_b"] + out["summed_y"]) / tic[s]).astype(np.float32),
-        "average_ppm": out["ppm"], "longest_b": out["longest_b"], "longest_y": out["longest_y"],
-        "longest_y_pct": (out["longest_y"] / plen).astype(np.float32),
-        "peptide_len": plen.astype(np.int16), "missed_cleavages": l1.missed[rows], "n_var": l1.n_var[rows],
-        "k1_count": k1.astype(np.int32), "scored_candidates": sc[s].astype(np.int32), "poisson": log_pois,
-    }
+        empty = torch.zeros(0, dtype=torch.int64, device=dev)
+        cols = psm_features(gi, n_spec, empty, empty, empty, report, h, r, k1c, k2, hyp_d, scored, matched, peaks)
+        dcols = {k: v.cpu().numpy() for k, v in cols.items()}
+        hh = np.zeros(0, np.int64)
+    s = dcols["spec_idx"]
+    rows = dcols["pf_row"]
+    cols = {"run": pa.array([sp.run] * len(s)), "spec_idx": s, "scan": sp.scan[s], "rt": sp.rt[s],
+            "rank": dcols["rank"], "pf_row": rows, "pf_id": l1.pf_id[rows]}
+    for k in ("charge", "isotope_error", "expmass", "calcmass", "delta_mass_ppm", "hyperscore", "delta_next",
+              "delta_best", "matched_b", "matched_y", "matched_peaks", "summed_intensity", "matched_intensity_pct",
+              "average_ppm", "longest_b", "longest_y", "longest_y_pct", "peptide_len"):
+        cols[k] = dcols[k]
+    cols["missed_cleavages"] = l1.missed[rows]
+    cols["n_var"] = l1.n_var[rows]
+    for k in ("k1_count", "scored_candidates", "poisson"):
+        cols[k] = dcols[k]
     tbl = pa.table(cols)
     return (tbl, hh) if return_hyp else tbl
+
+
+def write_psms(tbl: pa.Table, out_dir: str, run: str | None = None) -> str:
+    """Write a PSM table as <out_dir>/<run>.psms.parquet (the reference's file naming, vulcan/gpu/run.py:66)."""
+    import os
+
+    import pyarrow.parquet as pq
+    os.makedirs(out_dir, exist_ok=True)
+    if run is None:
+        run = tbl.column("run")[0].as_py() if tbl.num_rows else "run"
+    path = os.path.join(out_dir, f"{run}.psms.parquet")
+    pq.write_table(tbl, path)
+    return path
 
 
 def _marker(ev: dict | None):
