@@ -51,7 +51,7 @@ struct JArgs {
   const uint32_t* off;
   const uint32_t* key;
   const uint16_t* row;
-  int maxp;        // LDS posting capacity (>= largest block, multiple of 8)
+  int maxp;        // LDS posting capacity (>= largest block + 4 sentinels, multiple of 8)
   int tcells;      // directory cells (power of two)
   int hist_rows;   // histogram slots per wave (>= largest block's rows)
   // pairs and hypotheses
@@ -165,25 +165,24 @@ struct JBlock {
 #define J_PU 8   // peaks per lane held in registers (512 peaks: the K0 limit; more are loaded on demand)
 
 // One (peak, fragment charge) range lookup: the exact f32 key range [bits(lo bound), bits(hi bound)] starts at
-// its low bound's directory cell (cells hold about one posting each); postings are examined four at a time,
-// branch-free, until one passes the high bound, and those in range whose row lies in [ra, rb) add to the
-// histogram.
+// its low bound's directory cell (cells hold about one posting each); postings are examined four at a time
+// (the staged keys end with four all-ones sentinels, so the loads need no bounds test) until one passes the
+// high bound, and those in range whose row lies in [ra, rb) add to the histogram.
 template <int MODE>
 __device__ __forceinline__ void k1j_lookup(const DevP& P, const JBlock& B, float pmu, unsigned long long inc, int z,
                                            int ra, int rb, uint32_t* hist) {
   const float m = pmu * (float)z;
   const float a = tol_bound(m, P.tol_lo, P.tol_da);
   const float bd = tol_bound(m, P.tol_hi, P.tol_da);
-  if (bd < P.fmin || a > P.fmax) return;
+  if (bd < P.fmin || a > P.fmax) return;   // (about half of the charge-2 lookups: above the fragment range)
   const uint32_t ka = a > 0.f ? fbits(a) : 0u;
   const uint32_t kb = fbits(bd);
   if (kb < B.kmin || ka > B.kmax) return;
-  const int n = B.n;
   int e = B.table[ka <= B.kmin ? 0u : (ka - B.kmin) >> B.tsh];   // first posting of ka's cell
   for (;;) {
     uint32_t k4[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) k4[v] = e + v < n ? B.keys[e + v] : 0xFFFFFFFFu;
+    for (int v = 0; v < 4; ++v) k4[v] = B.keys[e + v];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       if (k4[v] >= ka && k4[v] <= kb) {
@@ -197,7 +196,7 @@ __device__ __forceinline__ void k1j_lookup(const DevP& P, const JBlock& B, float
         }
       }
     }
-    if (k4[3] > kb) break;   // (past the end the sentinel stops the scan)
+    if (k4[3] > kb) break;   // (past the end the sentinels stop the scan)
     e += 4;
   }
 }
@@ -322,6 +321,7 @@ __global__ __launch_bounds__(J_THREADS) void k1j_count_kernel(DevP P, JArgs A) {
     }
   }
   if (tid < cnt0) meta[tid] = m0;
+  if (tid < 4) keys[n + tid] = 0xFFFFFFFFu;   // scan sentinels (maxp leaves room for them)
   __syncthreads();
   const uint32_t kmin = n ? keys[0] : 0u, kmax = n ? keys[n - 1] : 0u;
   int tsh = 0;
@@ -553,7 +553,7 @@ static size_t align256j(size_t x) { return (x + 255) & ~(size_t)255; }
 // LDS of a count workgroup: keys (maxp u32) + rows (maxp u16) + directory (tcells + 8 u16) + pair metadata +
 // per-wave histograms of hist_rows slots (u32; mode 2: two u64), at most 512 (mode 2: 256) rows per pass
 static size_t count_lds(const VhBlockView* bv, int mode, int& maxp, int& tcells, int& hist_rows) {
-  maxp = (bv->max_postings + 7) & ~7;
+  maxp = (bv->max_postings + 4 + 7) & ~7;   // + 4 sentinel keys
   tcells = 256;   // about one posting per directory cell, at most 4096 cells
   while (tcells < 4096 && tcells * 2 <= bv->max_postings) tcells *= 2;
   // one pass holds at most 512 rows (its slot words travel in 8 registers per lane); feature mode 256 (LDS)
