@@ -431,8 +431,11 @@ __device__ __forceinline__ void k1j_load_batch(const JSel& J, int64_t slot, int 
   const int lane = lane_id();
 #pragma unroll
   for (int u = 0; u < JU; ++u) {
+    // every lane loads (a clamped, in-window index: no exec-masked load), the value past the window is zeroed
     const int jd = b0 + u * 64 + lane;
-    v[u] = (ok && jd < len) ? c[slot + (len - 1 - jd)] : (SlotT)0;
+    const int jr = len - 1 - jd;
+    const SlotT x = c[slot + (jr > 0 ? jr : 0)];
+    v[u] = (ok && jd < len) ? x : (SlotT)0;
   }
 }
 
@@ -481,6 +484,13 @@ __device__ void k1j_spectrum(const DevP& P, const DevIdx& X, int s, unsigned cha
     unsigned long long mt = 0;
     // rows descending, JU chunks of 64 slots per batch; the next batch's loads are issued before the current
     // batch is processed
+    if (len == 0 || !ok) {   // uniform: nothing to read (an overflowing window is flagged by vh_k1j_pairs)
+      if (lane == 0) {
+        O.hyp_scored[h0 + hc] = 0;
+        O.hyp_matched[h0 + hc] = 0;
+      }
+      continue;
+    }
     SlotT cur[JU], nxt[JU];
     k1j_load_batch(J, slot, len, 0, ok, cur);
     for (int b0 = 0; b0 < len; b0 += 64 * JU) {
