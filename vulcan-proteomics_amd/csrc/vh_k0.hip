@@ -112,15 +112,44 @@ __global__ __launch_bounds__(256) void k0_emit_kernel(DevP P, int n_spec, const 
 #pragma unroll
         for (int c = 0; c < K0_REG; ++c) kept[c] = c * 64 + lane < n_in;
       } else {
-        uint64_t T = 0;
-        for (int bit = 63; bit >= 0; --bit) {
-          const uint64_t cand = T | (1ULL << bit);
+        // the intensity half of T first (32 passes); its low (mass) half is needed only when the peaks whose
+        // intensity key equals that half are more than the places left: otherwise T is the smallest of them
+        uint32_t Ti = 0;
+        for (int bit = 31; bit >= 0; --bit) {
+          const uint32_t cand = Ti | (1u << bit);
           int cnt = 0;
 #pragma unroll
           for (int c = 0; c < K0_REG; ++c)
-            if (c < nch)
-              cnt += __popcll(ballot(c * 64 + lane < n_in && (((uint64_t)ik[c] << 32) | mb[c]) >= cand));
-          if (cnt >= K) T = cand;
+            if (c < nch) cnt += __popcll(ballot(c * 64 + lane < n_in && ik[c] >= cand));
+          if (cnt >= K) Ti = cand;
+        }
+        int gti = 0, eqi = 0;
+        uint32_t mlo = 0xFFFFFFFFu;
+#pragma unroll
+        for (int c = 0; c < K0_REG; ++c) {
+          if (c < nch) {
+            const bool v = c * 64 + lane < n_in;
+            gti += __popcll(ballot(v && ik[c] > Ti));
+            eqi += __popcll(ballot(v && ik[c] == Ti));
+            if (v && ik[c] == Ti && mb[c] < mlo) mlo = mb[c];
+          }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+          const uint32_t x = (uint32_t)__shfl_xor((int)mlo, off);
+          mlo = x < mlo ? x : mlo;
+        }
+        uint64_t T = ((uint64_t)Ti << 32) | mlo;   // every tie at Ti is kept
+        if (gti + eqi > K) {                       // some ties at Ti are not: the mass half by bitwise search
+          T = (uint64_t)Ti << 32;
+          for (int bit = 31; bit >= 0; --bit) {
+            const uint64_t cand = T | (1ULL << bit);
+            int cnt = 0;
+#pragma unroll
+            for (int c = 0; c < K0_REG; ++c)
+              if (c < nch)
+                cnt += __popcll(ballot(c * 64 + lane < n_in && (((uint64_t)ik[c] << 32) | mb[c]) >= cand));
+            if (cnt >= K) T = cand;
+          }
         }
         int gt = 0;
 #pragma unroll
