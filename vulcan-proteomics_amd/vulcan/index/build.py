@@ -10,8 +10,11 @@ The index side is label-free; labels are kept apart (labels.arrow) and only used
 """
 from __future__ import annotations
 
+import argparse
+import hashlib
 import json
 import os
+import sys
 import time
 
 import numpy as np
@@ -21,6 +24,52 @@ from . import _core
 from . import decoys as D
 
 INDEX_VERSION = "vulcan-mi355x-l1-0.1"
+DECOY_TAG = "rev_"   # FASTA entries whose accession contains it are skipped (decoys are made per peptide)
+
+
+def _standard_mask() -> np.ndarray:
+    """Residue letters with a Sage monoisotopic mass (lib.rs:63-65 is_standard): the 20 amino acids plus U, O."""
+    from ..gpu.host import SAGE_MONO
+    m = np.zeros(256, bool)
+    for i in range(26):
+        m[65 + i] = SAGE_MONO[i] > 0
+    return m
+
+
+_STANDARD = _standard_mask()
+
+
+def read_fasta(paths) -> list:
+    """(accession, sequence) of every FASTA entry of `paths` in file order: a '>' line starts an entry (the
+    accession is its first whitespace-separated token), the following lines are its sequence."""
+    out = []
+    for path in ([paths] if isinstance(paths, str) else paths):
+        acc, seq = None, []
+        with open(path) as fh:
+            for line in fh:
+                line = line.strip()
+                if line.startswith(">"):
+                    if acc is not None:
+                        out.append((acc, "".join(seq)))
+                    head = line[1:].split()
+                    acc, seq = (head[0] if head else ""), []
+                elif line and acc is not None:
+                    seq.append(line)
+        if acc is not None:
+            out.append((acc, "".join(seq)))
+    return out
+
+
+def content_hash(fasta_paths, params) -> str:
+    """Hash of the FASTA bytes, the build parameters and the index version (reference build.py:66-75)."""
+    h = hashlib.sha256()
+    for path in fasta_paths:
+        with open(path, "rb") as fh:
+            for b in iter(lambda: fh.read(1 << 22), b""):
+                h.update(b)
+    h.update(json.dumps(params, sort_keys=True).encode())
+    h.update(INDEX_VERSION.encode())
+    return h.hexdigest()[:16]
 
 
 def default_params() -> dict:
@@ -39,7 +88,8 @@ def default_params() -> dict:
 
 
 def digest(proteins, enzyme: dict):
-    """Unique I/L-collapsed tryptic peptides of `proteins` (sorted) and their minimal missed-cleavage count."""
+    """Unique I/L-collapsed tryptic peptides of `proteins` (sorted) and their minimal missed-cleavage count;
+    peptides with a residue outside the Sage mass table (X, B, Z, J, ...) are dropped (lib.rs:117-121)."""
     cut = np.zeros(256, bool)
     for c in enzyme["cleave_at"]:
         cut[ord(c)] = True
@@ -63,6 +113,8 @@ def digest(proteins, enzyme: dict):
             ln = s1 - s0
             ok = np.nonzero((ln >= lo) & (ln <= hi))[0]
             for i in ok:
+                if not _STANDARD[a[s0[i]:s1[i]]].all():
+                    continue
                 k = raw[s0[i]:s1[i]]
                 v = best.get(k)
                 if v is None or m < v:
@@ -204,3 +256,33 @@ def build(proteins, out_dir: str, params: dict | None = None, shuffle_rows_seed:
     l1, labels = build_tables(proteins, params, shuffle_rows_seed, swap_labels)
     l1.save(out_dir, labels)
     return l1.meta
+
+
+def build_fasta(fasta_paths, out_dir: str, params: dict | None = None) -> dict:
+    """The index of FASTA files (reference build.py:78-93): entries whose accession contains DECOY_TAG are
+    skipped, the rest digested and expanded as build(); meta.json records the FASTA paths and content hash."""
+    fasta_paths = [fasta_paths] if isinstance(fasta_paths, str) else list(fasta_paths)
+    p = default_params()
+    p.update(params or {})
+    entries = read_fasta(fasta_paths)
+    proteins = [seq for acc, seq in entries if DECOY_TAG not in acc]
+    l1, labels = build_tables(proteins, p)
+    l1.meta.update(fasta=[os.path.abspath(f) for f in fasta_paths], content_hash=content_hash(fasta_paths, p),
+                   n_proteins=len(proteins), n_skipped_decoy_entries=len(entries) - len(proteins))
+    l1.save(out_dir, labels)
+    return l1.meta
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="Build an L1 peptidoform index from FASTA files.")
+    ap.add_argument("--fasta", nargs="+", required=True)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--params", default="{}", help="JSON overrides of the build parameters")
+    a = ap.parse_args(argv)
+    meta = build_fasta(a.fasta, a.out, json.loads(a.params))
+    print(json.dumps({k: meta[k] for k in ("n_proteins", "n_targets", "n_peptidoforms", "content_hash", "build_s")}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
