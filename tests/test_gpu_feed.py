@@ -1,0 +1,88 @@
+"""Multi-run feed (store read + pinned double-buffered upload overlapped with the previous run's search) vs one
+search per run: the same rank-1 rows and the same PSM tables, runs of different sizes through one preallocated
+DeviceSearch."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("needs a ROCm GPU", allow_module_level=True)
+
+
+@pytest.fixture(scope="module")
+def runs(small_plain, tmp_path_factory):
+    from vulcan.gpu import engine as E
+    from vulcan.index import synth
+    d = small_plain
+    out = tmp_path_factory.mktemp("stores")
+    paths, sps = [], []
+    for i, n in enumerate((700, 1500, 400)):
+        sp, _, _ = synth.spectra(d["l1"], d["labels"], 0, n=n, seed=50 + i)
+        sp.run = f"run{i}"
+        path = str(out / f"run{i}.vspec.arrow")
+        E.write_store(sp, path)
+        paths.append(path)
+        sps.append(E.load_store(path))
+    return d, paths, sps
+
+
+def test_feed_equals_per_run_search(runs, tmp_path):
+    from vulcan.gpu import engine as E
+    from vulcan.gpu.feed import RunFeed
+    d, paths, sps = runs
+    print("[gpu] feed: 3 runs (700 / 1500 / 400 spectra)", flush=True)
+    gi = E.GpuIndex(d["l1"], d["p"], d["static"], d["var"])
+    feed = RunFeed(gi, d["p"], max_spectra=1500, max_peaks=max(int(s.offsets[-1]) for s in sps))
+    # capacities far below the first run's pairs / slots, marked fitted: the kernels bound their writes and
+    # flag the overflow, the feed refits and searches the run again
+    feed.ds._join_alloc(feed.ds.join, 64, 16)
+    feed.ds.join["fitted"] = True
+    res = feed.run(paths, l1=d["l1"], out_dir=str(tmp_path))
+    assert 1 <= feed.refits <= 2          # the 700-spectrum run, perhaps the 1,500 one
+    torch.cuda.synchronize()
+    assert [r["run"] for r in res] == ["run0", "run1", "run2"]
+    for r, sp in zip(res, sps):
+        tbl, _, dbg = E.search(sp, gi, d["l1"], d["p"])
+        rows = dbg["device"].rank1_rows(len(sp))
+        assert r["n_spectra"] == len(sp)
+        assert np.array_equal(r["rows"], rows), r["run"]
+        assert r["table"].num_rows == tbl.num_rows and r["table"].num_rows > 0
+        assert r["table"].schema == tbl.schema
+        for name in tbl.column_names:
+            a, b = r["table"].column(name).to_numpy(), tbl.column(name).to_numpy()
+            if a.dtype.kind == "f":
+                assert np.array_equal(a, b, equal_nan=True), (r["run"], name)
+            else:
+                assert np.array_equal(a, b), (r["run"], name)
+        import pyarrow.parquet as pq
+        assert pq.read_table(r["path"]).num_rows == tbl.num_rows
+
+
+def test_feed_rejects_oversized_run(runs):
+    from vulcan.gpu import engine as E
+    from vulcan.gpu.feed import RunFeed
+    d, paths, sps = runs
+    gi = E.GpuIndex(d["l1"], d["p"], d["static"], d["var"])
+    feed = RunFeed(gi, d["p"], max_spectra=800, max_peaks=10 ** 6)
+    with pytest.raises(ValueError, match="exceeds the feed"):
+        feed.run(paths[:2])
+
+
+def test_search_many_writes_per_run_tables(runs, tmp_path):
+    """batch.search_many (reference batch.py:21-41 result keys) writes the tables one search per run gives."""
+    import pyarrow.parquet as pq
+    from vulcan.gpu import batch
+    from vulcan.gpu import engine as E
+    d, paths, sps = runs
+    print("[gpu] search_many: 3 runs", flush=True)
+    gi = E.GpuIndex(d["l1"], d["p"], d["static"], d["var"])
+    r = batch.search_many(paths, gi, d["l1"], d["p"], str(tmp_path / "out"), io_threads=2)
+    assert r["n_spectra"] == sum(len(s) for s in sps) and [x["run"] for x in r["runs"]] == ["run0", "run1", "run2"]
+    assert all(x["gpu_ms"] > 0 and x["h2d_bytes"] > 0 for x in r["runs"])
+    for sp in sps:
+        tbl, _, _ = E.search(sp, gi, d["l1"], d["p"])
+        got = pq.read_table(str(tmp_path / "out" / f"{sp.run}.psms.parquet"))
+        assert got.num_rows == tbl.num_rows
+        for name in ("spec_idx", "pf_row", "rank", "hyperscore", "matched_peaks"):
+            assert np.array_equal(got.column(name).to_numpy(), tbl.column(name).to_numpy()), (sp.run, name)

@@ -194,6 +194,44 @@ class GpuIndex:
 
 
 # ------------------------------------------------------------------------------------------------ spectra
+def load_store(path: str, out: dict | None = None, threads: int = 4) -> Spectra:
+    """The MS2 spectra of a run's store (`<run>.vspec.arrow`, reference engine.py:157-168): MS1 rows dropped,
+    an unknown charge becomes 0, the run name from the store's metadata. out: optional preallocated arrays
+    {"mz", "int", "off", "pmz", "z"} the peaks and precursors are written into (e.g. pinned host memory);
+    threads: host threads copying the peak record batches."""
+    from ..io import store
+    t = store.read_table(path)
+    keep = t.column("ms_level").to_numpy() == 2
+    idx = None if keep.all() else np.flatnonzero(keep)
+    pk = store.peaks_numpy(t, None if idx is None else keep, out, threads)
+
+    def col(name, dtype, fill=None):
+        a = t.column(name).to_numpy(zero_copy_only=False)
+        if fill is not None:
+            a = np.nan_to_num(np.asarray(a, np.float64), nan=fill)
+        return np.array(a if idx is None else a[idx], dtype)   # writable copies of the per-spectrum columns
+
+    pmz, z = col("precursor_mz", np.float64), col("precursor_charge", np.int8, fill=0)
+    if out is not None:
+        n = len(z)
+        out["pmz"][:n], out["z"][:n] = pmz, z
+        pmz, z = out["pmz"][:n], out["z"][:n]
+    name = store.run_meta(path).get("run_name", path)
+    return Spectra(name, col("scan", np.uint32), col("index", np.uint32), col("rt_min", np.float64), pmz, z,
+                   pk.mz, pk.intensity, pk.offsets)
+
+
+def write_store(sp: Spectra, path: str, run_meta: dict | None = None) -> dict:
+    """Write MS2 spectra as a run store (the inverse of load_store; for synthetic runs and tests)."""
+    from ..io import store
+    w = store.StoreWriter(path, {"run_name": sp.run, **(run_meta or {})})
+    n = len(sp)
+    w.write({"offsets": sp.offsets, "mz": sp.mz, "intensity": sp.intensity, "scan": sp.scan, "rt_min": sp.rt,
+             "precursor_mz": sp.precursor_mz, "precursor_charge": sp.charge, "ms_level": np.full(n, 2, np.uint8),
+             "activation": np.full(n, 2, np.uint8)})
+    return w.close()
+
+
 def upload_spectra(sp: Spectra, dev, stats: Stats | None = None) -> dict:
     """Host run -> device buffers (raw peaks CSR, precursor m/z f64, charge i8)."""
     d = {"mz": torch.from_numpy(np.ascontiguousarray(sp.mz, np.float32)).to(dev),
@@ -262,11 +300,13 @@ def device_hypotheses(gi: GpuIndex, raw: dict, peaks: dict, p: SearchParams, vp=
     return o
 
 
-def hyp_to_host(hd: dict) -> dict:
-    n = int(hd["hyp_off"][-1].item())
+def hyp_to_host(hd: dict, n_spec: int | None = None) -> dict:
+    """Host copy of device hypotheses; n_spec: spectra of the batch when hd's buffers were sized for more."""
+    off = hd["hyp_off"] if n_spec is None else hd["hyp_off"][:n_spec + 1]
+    n = int(off[-1].item())
     out = {k: hd[k][:n].cpu().numpy() for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz", "mode", "off")
            if k in hd}
-    out["hyp_off"] = hd["hyp_off"].cpu().numpy()
+    out["hyp_off"] = off.cpu().numpy()
     return out
 
 
@@ -756,9 +796,9 @@ class DeviceSearch:
 
     def rank1_rows(self, n: int) -> np.ndarray:
         """Host copy of the rank-1 row per spectrum (-1: no PSM)."""
-        sel = self.sel[:n * self.report].view(n, self.report)[:, 0].cpu().numpy()
-        rows = self.c_row[: n * self.stride].cpu().numpy()
-        return np.where(sel >= 0, rows[np.maximum(sel, 0)], -1)
+        sel = self.sel[:n * self.report].view(n, self.report)[:, 0]
+        rows = self.c_row[sel.clamp(min=0)].long()        # gathered on the device: n values cross to the host
+        return torch.where(sel >= 0, rows, -1).cpu().numpy()
 
     def candidates(self, n: int):
         """Flatten the per-spectrum candidate lists into (h, r, c) sorted by (spectrum, row) (device)."""
@@ -777,7 +817,7 @@ def device_table(ds: DeviceSearch, sp: Spectra, gi: GpuIndex, l1, p: SearchParam
     h, r, c, flat = ds.candidates(n)
     k2 = {k: v[flat] for k, v in ds.k2.items()}
     k2["spec"] = torch.repeat_interleave(torch.arange(n, device=dev), ds.n_cand[:n].long())
-    hyp = hyp_to_host(ds.hyp)
+    hyp = hyp_to_host(ds.hyp, n)
     tbl, hh = psm_table(sp, gi, l1, hyp, h, r, c, k2, ds.scored, ds.matched, ds.peaks, p, report, return_hyp=True)
     stt = int(ds.status.item())
     if stt & 1:
