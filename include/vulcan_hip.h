@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 13
+#define VH_ABI_VERSION 14
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -116,8 +116,8 @@ typedef struct VhBlockView {
 
 /* ---------------------------------------------------------------------------------------------- misc */
 int vh_version(void);
-/* sizeof(VhParams) (which = 0), VhIndexView (1), VhBlockView (2), VhK2Cols (3) or VhPsmCols (4), to check a
- * binding's struct layout. */
+/* sizeof(VhParams) (which = 0), VhIndexView (1), VhBlockView (2), VhK2Cols (3), VhPsmCols (4) or VhGbmNode (5),
+ * to check a binding's struct layout. */
 size_t vh_abi_sizeof(int which);
 /* Scratch bytes of the prefix scans used inside vh_k0 / vh_hypotheses / vh_frag_count for n items. */
 size_t vh_scan_workspace_bytes(int64_t n);
@@ -434,6 +434,33 @@ int vh_prefilter_select(const VhIndexView* ix, int32_t n_spec, int32_t stride_in
                         const int32_t* c_row, const int32_t* c_hyp, const int32_t* c_cnt, const float* score,
                         int32_t keep_min, int32_t keep_max, float margin, int32_t pair_closure, int32_t out_stride,
                         int32_t* out_n, int32_t* out_row, int32_t* out_hyp, int32_t* out_cnt, void* stream);
+
+/* ---- gradient-boosted tree scorer (replaces the torch GpuHGB of vulcan/rescore/fast.py:198-254) ---- */
+#define VH_GBM_CHUNK_NODES 1024   /* LDS node chunk: a tree has at most this many nodes */
+#define VH_GBM_MAX_FEATURES 63
+
+/* One tree node, 16 B. Internal: v = numeric threshold, children at left and left + 1 (left: x <= v, right:
+ * x > v; a NaN goes left iff missing_left). Leaf: v = the leaf value. Indexes are into the whole node array. */
+typedef struct {
+  double v;
+  int32_t left;
+  int16_t feature;
+  uint8_t missing_left;
+  uint8_t is_leaf;
+} VhGbmNode;
+
+/* sklearn HistGradientBoosting{Classifier,Regressor} raw prediction, one tree per iteration (binary or
+ * regression; no categorical splits), in f64: out[i] = ((0 + baseline) + v_0(x_i)) + v_1(x_i) + ... in tree
+ * order, v_t the leaf reached from tree_root[t] -- bit-identical to sklearn's decision_function / raw predict.
+ * X: n rows of n_features f64 (row-major). The trees are cut into n_chunks runs of whole trees: chunk c holds
+ * trees [chunk_tree0[c], chunk_tree0[c+1]) whose nodes are [chunk_node0[c], chunk_node0[c+1]), at most
+ * VH_GBM_CHUNK_NODES nodes. max_depth: the deepest leaf's depth (walks stop after max_depth + 1 nodes).
+ * status (zeroed by the caller): bit 1 = a walk left its chunk, met a feature >= n_features or no leaf within
+ * max_depth + 1 nodes (a malformed model; the sample's sum is then wrong). */
+int vh_gbm_decision(const VhGbmNode* nodes, int64_t n_nodes, const int32_t* tree_root, int32_t n_trees,
+                    const int32_t* chunk_node0, const int32_t* chunk_tree0, int32_t n_chunks, int32_t max_depth,
+                    double baseline, const double* X, int64_t n, int32_t n_features, double* out, int32_t* status,
+                    void* stream);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,7 @@ def test_header_declares_the_hot_path():
                  "vh_mlp_forward", "vh_mlp_backward", "vh_mlp_adam", "vh_prefilter_features", "vh_prefilter_score",
                  "vh_prefilter_score_workspace_bytes", "vh_psm_features",
                  "vh_prefilter_select", "vh_index_block_count", "vh_index_build_blocks", "vh_k1j_pairs",
-                 "vh_k1j_count", "vh_k1j_topk", "vh_k1j_features"):
+                 "vh_k1j_count", "vh_k1j_topk", "vh_k1j_features", "vh_gbm_decision"):
         assert need in names
 
 
@@ -42,7 +42,9 @@ def test_abi_layout_and_version():
     assert L.vh_abi_sizeof(2) == C.sizeof(VhBlockView)
     assert L.vh_abi_sizeof(3) == C.sizeof(VhK2Cols) == 64
     assert L.vh_abi_sizeof(4) == C.sizeof(VhPsmCols) == 24 * 8
-    assert L.vh_abi_sizeof(5) == 0
+    from vulcan.gpu._abi import GBM_NODE
+    assert L.vh_abi_sizeof(5) == GBM_NODE.itemsize == 16
+    assert L.vh_abi_sizeof(6) == 0
     m = re.search(r"#define VH_ABI_VERSION (\d+)", open(HEADER).read())
     assert L.vh_version() == int(m.group(1))
 

@@ -200,6 +200,7 @@ size_t vh_abi_sizeof(int which) {
     case 2: return sizeof(VhBlockView);
     case 3: return sizeof(VhK2Cols);
     case 4: return sizeof(VhPsmCols);
+    case 5: return sizeof(VhGbmNode);
     default: return 0;
   }
 }

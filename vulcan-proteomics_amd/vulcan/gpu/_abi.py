@@ -49,6 +49,11 @@ class VhPsmCols(C.Structure):
     _fields_ = [(k, P) for k in PSM_COLS]
 
 
+GBM_NODE = np.dtype([("v", "<f8"), ("left", "<i4"), ("feature", "<i2"), ("missing_left", "u1"), ("is_leaf", "u1")])
+GBM_CHUNK_NODES = 1024    # VH_GBM_CHUNK_NODES
+GBM_MAX_FEATURES = 63     # VH_GBM_MAX_FEATURES
+
+
 def mod_arrays(static_mods: dict, var_mods: dict):
     """f32 static/variable tables and the N-term delta as the reference sets them (engine.py:71-82)."""
     stat = np.zeros(26, np.float32)

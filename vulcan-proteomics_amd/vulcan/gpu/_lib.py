@@ -15,7 +15,7 @@ from .. import _build
 from ._abi import VhBlockView, VhIndexView, VhK2Cols, VhParams, VhPsmCols
 
 P = C.c_void_p
-I32, I64, SZ, F32 = C.c_int32, C.c_int64, C.c_size_t, C.c_float
+I32, I64, SZ, F32, F64 = C.c_int32, C.c_int64, C.c_size_t, C.c_float, C.c_double
 PP = C.POINTER(VhParams)
 PIX = C.POINTER(VhIndexView)
 PBV = C.POINTER(VhBlockView)
@@ -62,6 +62,7 @@ _SIGS = {
     "vh_prefilter_score_workspace_bytes": ([I32, I32], SZ),
     "vh_prefilter_score": ([PIX, P, I32, I32, P, P, P, P, P, P, P, P, P, P, P, SZ, P], I32),
     "vh_prefilter_select": ([PIX, I32, I32, P, P, P, P, P, I32, I32, F32, I32, I32, P, P, P, P, P], I32),
+    "vh_gbm_decision": ([P, I64, P, I32, P, P, I32, I32, F64, P, I64, I32, P, P, P], I32),
 }
 
 ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "workspace too small", -4: "size limit exceeded"}
