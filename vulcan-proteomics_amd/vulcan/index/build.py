@@ -84,6 +84,8 @@ def default_params() -> dict:
         "variable_mod_caps": None,
         "peptide_min_mass": 500.0, "peptide_max_mass": 5000.0,
         "decoy_seed": D.DECOY_SEED, "id_seed": 0,
+        # multi-replicate tables (build_multi_decoy_tables)
+        "decoy_seeds": [D.DECOY_SEED], "entrap_seeds": [],
     }
 
 
@@ -145,7 +147,9 @@ class L1Index:
         self.n_var = np.ascontiguousarray(arrays["n_var"], np.uint8)
         self.missed = np.ascontiguousarray(arrays["missed"], np.uint8)
         self.partner = np.ascontiguousarray(arrays["partner"], np.int64)
-        self.partners = [self.partner]
+        # multi-decoy tables: the pair partner in decoy set j >= 1 as partner_<j> (reference build.py:366-372)
+        self.partners = [self.partner] + [np.ascontiguousarray(arrays[f"partner_{j}"], np.int64)
+                                          for j in range(1, 64) if f"partner_{j}" in arrays]
         self.residues = np.ascontiguousarray(arrays["residues"], np.uint8)
 
     def __len__(self):
@@ -161,6 +165,8 @@ class L1Index:
                         "seq_off": pa.array(self.seq_off, pa.int64()), "length": pa.array(self.length, pa.uint8()),
                         "mask": pa.array(self.mask, pa.uint64()), "n_var": pa.array(self.n_var, pa.uint8()),
                         "missed": pa.array(self.missed, pa.uint8()), "partner": pa.array(self.partner, pa.int64())})
+        for j, pj in enumerate(self.partners[1:], start=1):
+            tbl = tbl.append_column(f"partner_{j}", pa.array(pj, pa.int64()))
         _write(tbl, os.path.join(path, "peptidoforms.arrow"))
         np.save(os.path.join(path, "residues.npy"), self.residues)
         if labels is not None:
@@ -248,6 +254,116 @@ def build_tables(proteins, params: dict | None = None, shuffle_rows_seed: int | 
                          "partner": partner, "residues": residues[gather], "meta": meta})
     labels = {"pf_id": l1.pf_id, "is_decoy": is_decoy}
     return l1, labels
+
+
+FAMILY = {"T": 0, "E": 1, "decoy_T": 2, "decoy_E": 3}
+
+
+def build_multi_decoy_tables(proteins, params: dict | None = None):
+    """Multi-replicate table for family top-2 retention (reference build.py:257-418): rows T ∪ E_s ∪ dec_j(T) ∪
+    dec_j(E_s) for every entrapment seed s of params['entrap_seeds'] and decoy seed j of params['decoy_seeds']
+    (shuffled decoys only). Forward peptides lacking a decoy in some set are dropped. partners[j] pairs each
+    forward row with its set-j decoy (and back); rows of other decoy sets are their own set-j partner.
+    Labels: family (0 T, 1 E, 2 dec(T), 3 dec(E)), entrap_seed, decoy_set (-1 forward), decoy_seed, is_decoy,
+    is_entrapment. Returns (L1Index, labels dict)."""
+    p = default_params()
+    p.update(params or {})
+    dseeds = list(p["decoy_seeds"])
+    J = len(dseeds)
+    if J < 1:
+        raise ValueError("decoy_seeds must name at least one decoy set")
+    t0 = time.perf_counter()
+    e = p["enzyme"]
+    targets, t_missed = digest(proteins, e)
+    t_missed_of = dict(zip(targets, t_missed.tolist()))
+    taken = set(targets)
+    fam_of = {t: (0, 0, t) for t in targets}             # forward peptide -> (family, entrapment seed, target)
+    ent_stats = {}
+    for s_ in p["entrap_seeds"]:
+        m, nr, nd = D.make_entrapment(targets, s_, taken)
+        ent_stats[str(s_)] = {"n": len(m), "redrawn": nr, "dropped": nd}
+        for t, (ent, _) in m.items():
+            fam_of[ent] = (1, s_, t)
+    forward = sorted(fam_of)
+    used: set = set()
+    decs, dec_stats = [], {}
+    for ds in dseeds:
+        dj, nr, nd = D.make_shuffled_decoys(forward, ds, used)
+        decs.append(dj)
+        dec_stats[str(ds)] = {"n": len(dj), "redrawn": nr, "dropped": nd}
+    fw = [f for f in forward if all(f in dj for dj in decs)]
+    res = []
+    for dj in decs:
+        r = _core.expand_pairs(fw, [dj[f][1] for f in fw], p["static_mods"], p["variable_mods"],
+                               p["max_variable_mods"], p["peptide_min_mass"], p["peptide_max_mass"], p["id_seed"],
+                               p.get("variable_mod_caps"))
+        res.append(([dj[f][0] for f in fw], r))
+    r0 = res[0][1]
+    for _, r in res[1:]:
+        if not (np.array_equal(r["seq"], r0["seq"]) and np.array_equal(r["id_f"], r0["id_f"])):
+            raise RuntimeError("the forward side differs between decoy sets")
+    nf = len(r0["seq"])
+    seq_i = r0["seq"].astype(np.int64)
+    lens = np.array([len(x) for x in fw], np.int64)
+    a_off = np.zeros(len(fw) + 1, np.int64)
+    np.cumsum(lens, out=a_off[1:])
+    L = int(a_off[-1])
+    residues = np.concatenate([np.frombuffer("".join(fw).encode(), np.uint8)] +
+                              [np.frombuffer("".join(b).encode(), np.uint8) for b, _ in res])
+    pf_id = np.concatenate([r0["id_f"]] + [r["id_d"] for _, r in res])
+    mass32 = np.concatenate([r0["mass32_f"]] + [r["mass32_d"] for _, r in res])
+    seq_off = np.concatenate([a_off[seq_i] + j * L for j in range(J + 1)])
+    length = np.tile(lens[seq_i], J + 1).astype(np.uint8)
+    mask = np.concatenate([r0["mask_f"]] + [r["mask_d"] for _, r in res])
+    n_var = np.tile(r0["n_var"], J + 1)
+    tgt_of = [fam_of[f][2] for f in fw]
+    missed = np.tile(np.array([t_missed_of[t] for t in tgt_of], np.uint8)[seq_i], J + 1)
+    if len(np.unique(pf_id)) != len(pf_id):
+        raise RuntimeError("peptidoform id collision")
+    N = len(pf_id)
+    srt = np.lexsort((pf_id, mass32))
+    inv = np.empty_like(srt)
+    inv[srt] = np.arange(N)
+    base = np.arange(nf)
+    partners = []
+    for j in range(J):
+        pu = np.arange(N)
+        pu[:nf] = (j + 1) * nf + base
+        pu[(j + 1) * nf:(j + 2) * nf] = base
+        partners.append(inv[pu][srt])
+    lens_s = length[srt].astype(np.int64)
+    new_off = np.zeros(N + 1, np.int64)
+    np.cumsum(lens_s, out=new_off[1:])
+    gather = np.repeat(seq_off[srt] - new_off[:-1], lens_s) + np.arange(new_off[-1])
+    side = np.repeat(np.arange(-1, J), nf).astype(np.int8)
+    fam_fwd = np.array([fam_of[f][0] for f in fw], np.uint8)[seq_i]
+    seed_fwd = np.array([fam_of[f][1] for f in fw], np.int32)[seq_i]
+    fam = np.tile(fam_fwd, J + 1) + np.where(side >= 0, 2, 0).astype(np.uint8)
+    meta = {"index_version": INDEX_VERSION + "+multidecoy", "params": p, "n_targets": len(targets),
+            "n_forward_peptides": len(fw), "n_forward_without_decoy": len(forward) - len(fw),
+            "decoy_sets": dec_stats, "entrapment": ent_stats, "n_decoy_sets": J, "n_peptidoforms": int(N),
+            "n_peptidoforms_by_family": {k: int((fam == v).sum()) for k, v in FAMILY.items()},
+            "build_s": time.perf_counter() - t0}
+    arrays = {"pf_id": pf_id[srt], "mass32": mass32[srt], "seq_off": new_off[:-1], "length": length[srt],
+              "mask": mask[srt], "n_var": n_var[srt], "missed": missed[srt], "partner": partners[0],
+              "residues": residues[gather], "meta": meta}
+    for j in range(1, J):
+        arrays[f"partner_{j}"] = partners[j]
+    l1 = L1Index(arrays=arrays)
+    side_s = side[srt]
+    labels = {"pf_id": l1.pf_id, "family": fam[srt], "entrap_seed": np.tile(seed_fwd, J + 1)[srt],
+              "decoy_set": side_s, "decoy_seed": np.array([0] + dseeds, np.int32)[side_s + 1],
+              "is_decoy": side_s >= 0, "is_entrapment": (fam[srt] % 2) == 1}
+    return l1, labels
+
+
+def build_multi_decoy(proteins, out_dir: str, params: dict | None = None) -> dict:
+    """Build and write a multi-replicate table (same file layout as build(), plus partner_<j> columns and the
+    family / decoy-set label columns)."""
+    l1, labels = build_multi_decoy_tables(proteins, params)
+    l1.path = out_dir
+    l1.save(out_dir, labels)
+    return l1.meta
 
 
 def build(proteins, out_dir: str, params: dict | None = None, shuffle_rows_seed: int | None = None,

@@ -61,3 +61,66 @@ def make_decoys(forward, seed: int = DECOY_SEED, max_tries: int = 50):
             out[pep] = (dec, perm)
             used.add(dec)
     return out, n_shuffled
+
+
+def make_entrapment(targets, seed: int, taken: set, max_tries: int = 200):
+    """Paired shuffled entrapment E_s of every target (reference decoys.py:72-106): the C-term-fixed shuffle
+    seeded by (target, s, attempt), drawn again while it equals the target, a target, a pseudo-reverse of a
+    target, a sequence in `taken` (T and earlier entrapments), a palindrome (its pseudo-reverse is itself) or
+    has a pseudo-reverse that is a target or taken. `taken` gains the new entrapments. Returns
+    ({target: (entrapment, perm)}, n_redrawn, n_dropped)."""
+    tgt = sorted(set(targets))
+    tset = set(tgt)
+    rev_t = {pseudo_reverse(t) for t in tgt}
+
+    def clash(e: str, t: str) -> bool:
+        pe = pseudo_reverse(e)
+        return e == t or e in taken or e in tset or e in rev_t or pe == e or pe in taken or pe in tset
+
+    out, n_redraw, n_drop = {}, 0, 0
+    for t in tgt:
+        k = 0
+        perm = hash_shuffle_perm(t, seed, 0)
+        e = apply_perm(t, perm)
+        while clash(e, t):
+            k += 1
+            if k > max_tries:
+                e = None
+                break
+            perm = hash_shuffle_perm(t, seed, k)
+            e = apply_perm(t, perm)
+        if e is None:
+            n_drop += 1
+            continue
+        n_redraw += k > 0
+        out[t] = (e, perm)
+        taken.add(e)
+    return out, n_redraw, n_drop
+
+
+def make_shuffled_decoys(forward, seed: int, used: set, max_tries: int = 50):
+    """One shuffled decoy set (reference decoys.py:143-168): the C-term-fixed shuffle of every forward peptide
+    (T and E) seeded by (peptide, seed, attempt), drawn again while it equals the peptide, any forward peptide,
+    or a decoy already drawn in this or an earlier set (`used`, updated). Returns ({fwd: (decoy, perm)},
+    n_redrawn, n_dropped)."""
+    fwd = sorted(set(forward))
+    fset = set(fwd)
+    out, n_redraw, n_drop = {}, 0, 0
+    for p in fwd:
+        k = 0
+        perm = hash_shuffle_perm(p, seed, 0)
+        d = apply_perm(p, perm)
+        while d == p or d in fset or d in used:
+            k += 1
+            if k > max_tries:
+                d = None
+                break
+            perm = hash_shuffle_perm(p, seed, k)
+            d = apply_perm(p, perm)
+        if d is None:
+            n_drop += 1
+            continue
+        n_redraw += k > 0
+        out[p] = (d, perm)
+        used.add(d)
+    return out, n_redraw, n_drop
