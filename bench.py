@@ -260,6 +260,115 @@ def cpu_baseline(vp, l1, sp, target_s: float):
             "cpu_model": _cpu_model()}, res, n
 
 
+def gbm_leg(a):
+    """Gradient-boosted tree scorer (vh_gbm_decision via GpuHGB): rows/s on `--gbm-rows` rows of 24 f64 features
+    resident in HBM, a 300-tree binary HistGradientBoosting model (at most 31 leaves per tree) fitted on 50k
+    synthetic rows; sklearn's decision_function on the host cores as the CPU baseline (a bounded sample)."""
+    import numpy as np
+    import torch
+    from sklearn.ensemble import HistGradientBoostingClassifier
+
+    from vulcan.gpu import _lib as K
+    from vulcan.rescore.fast import GpuHGB
+    nf, iters = 24, 300
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(50000, nf))
+    X[rng.random(X.shape) < 0.03] = np.nan
+    xz = np.nan_to_num(X)
+    y = (xz[:, 0] * xz[:, 1] + np.sin(xz[:, 2] * 3) + rng.normal(size=len(X)) * 0.3 > 0).astype(int)
+    m = HistGradientBoostingClassifier(max_iter=iters, max_leaf_nodes=31, early_stopping=False,
+                                       random_state=0).fit(X, y)
+    g = GpuHGB(m)
+    n = a.gbm_rows
+    Xt = rng.normal(size=(n, nf))
+    Xt[rng.random(Xt.shape) < 0.03] = np.nan
+    Xd = torch.from_numpy(Xt).cuda()
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    for _ in range(a.warmup):
+        g.decision_device(Xd, out)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    for _ in range(a.steps):
+        g.decision_device(Xd, out)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / a.steps
+    g.check()
+    chk = min(n, 100_000)
+    equal = bool(np.array_equal(out[:chk].cpu().numpy(), m.decision_function(Xt[:chk])))
+    bytes_per_launch = n * (nf * 8 + 8)                  # feature rows read + one f64 score written
+    ach = bytes_per_launch / (ms * 1e-3) / 1e9
+    res = {"metric": "rows/s (gradient-boosted tree scorer, f64, bit-identical to sklearn)", "value": n / (ms * 1e-3),
+           "unit": "rows/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"gbm-{iters}-trees-{nf}-features-{n}-rows", "n_rows": n, "n_features": nf,
+                      "n_trees": g.T, "n_nodes": g.M, "node_chunks": g.n_chunks, "max_depth": g.depth,
+                      "parallelism": "single-gpu"},
+           "kernel_ms": ms, "tree_walks_per_s": n * g.T / (ms * 1e-3), "equal_sklearn_first_rows": equal,
+           "roofline": {"bound": "hbm", "kernel": "gbm_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": bytes_per_launch,
+                        "note": "walks are bound by dependent LDS reads (node, then feature), not by HBM"},
+           "cpu_baseline": None}
+    if not a.no_cpu:
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ns = min(n, 1_000_000)
+        t0 = time.perf_counter()
+        m.decision_function(Xt[:ns])
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": ns / tc, "unit": "rows/s", "cores": cores, "kind": "reference",
+                               "sample": f"sklearn {__import__('sklearn').__version__} decision_function (the call "
+                                         f"the reference makes on the CPU) on the first {ns} rows"}
+    print(json.dumps(res), flush=True)
+
+
+def family_leg(a):
+    """Family top-2 retention (vulcan.gpu.family.search_family) over a multi-replicate table (T, two entrapment
+    families, two shuffled decoy sets) built from `--family-proteins` synthetic proteins, on `--n-spectra`
+    (default 50,000) synthetic spectra; a step = one whole family search (device K0 -> hypotheses -> K1 ->
+    per-family shortlist + closure -> K2 -> top-2 table on the host)."""
+    import numpy as np
+    import torch
+
+    from vulcan.gpu import engine as E
+    from vulcan.gpu import family as F
+    from vulcan.index import build as B
+    from vulcan.index import synth
+    ip = synth.WORKLOADS[1].index_params()
+    t0 = time.perf_counter()
+    l1, lab = B.build_multi_decoy_tables(synth.proteome(a.family_proteins, 11),
+                                         {"entrap_seeds": [2001, 2002], "decoy_seeds": [1001, 1002], **ip})
+    log(f"multi-replicate table: {len(l1)} rows ({time.perf_counter() - t0:.1f} s)")
+    n = a.n_spectra or 50_000
+    sp, _, _ = synth.spectra(l1, lab, 1, n=n, seed=7, index_params=ip)
+    p = synth.WORKLOADS[1].search_params()
+    gi = E.GpuIndex(l1, p, STATIC, dict(synth.WORKLOADS[1].var_mods))
+    fi = F.FamilyIndex.from_l1(l1, gi, lab)
+    for _ in range(max(a.warmup, 1)):
+        F.search_family(sp, gi, l1, fi, p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stage = {}
+    for _ in range(a.steps):
+        tbl, st = F.search_family(sp, gi, l1, fi, p)
+        for k, v in st.gpu_ms.items():
+            stage[k] = stage.get(k, 0.0) + v / a.steps
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / a.steps
+    res = {"metric": "spectra/s (family top-2 retention search)", "value": n / wall, "unit": "spectra/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"family-top2-{a.family_proteins}-proteins-{n}-spectra", "n_spectra": n,
+                      "n_peptidoforms": len(l1), "decoy_sets": 2, "entrapment_seeds": 2, "topk": p.topk,
+                      "parallelism": "single-gpu"},
+           "stage_ms": stage, "n_table_rows": tbl.num_rows, "counts": st.counts,
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(res), flush=True)
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -298,6 +407,10 @@ def main():
     ap.add_argument("--n-train", type=int, default=20000, help="prefilter: training spectra (own seed)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run the u32 copy kernel on this many MiB (HBM counter calibration)")
+    ap.add_argument("--gbm", action="store_true", help="gradient-boosted tree scorer leg (rows/s) instead")
+    ap.add_argument("--gbm-rows", type=int, default=1_000_000)
+    ap.add_argument("--family", action="store_true", help="family top-2 retention leg instead")
+    ap.add_argument("--family-proteins", type=int, default=2000)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU sharding "
@@ -318,6 +431,12 @@ def main():
             K.check(K.lib().vh_copy_u32(K.ptr(src), K.ptr(dst), n, K.stream()), "copy")
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes_each_way": n * 4, "launches": 5}))
+        return
+    if a.gbm:
+        gbm_leg(a)
+        return
+    if a.family:
+        family_leg(a)
         return
     w = synth.WORKLOADS[a.workload]
     t0 = time.perf_counter()

@@ -17,10 +17,11 @@ exactly as a separate search over that replicate's rows would give it:
     partner of a shortlisted row in set j) with at least min_matched_peaks matches, top-2 per (spectrum, group)
     by (hyperscore desc, row asc).
 
-K0, the hypotheses, the K1 counts and K2 run on the HIP kernels of vulcan.gpu.engine; the selection steps are
-device sorts and scans over the appended K1 list. The family codes come from the label table (the reference's
-sanctioned use of labels for retention, symmetric in the target / decoy roles). Features that depend on the
-whole table (scored_candidates, poisson) cannot be rebuilt per replicate and are not emitted.
+K0, the hypotheses, the K1 counts and K2 run on the HIP kernels of vulcan.gpu.engine (the hypotheses stay on the
+device); the selection steps are device sorts and scans over the appended K1 list. The family codes come from
+the label table (the reference's sanctioned use of labels for retention, symmetric in the target / decoy roles).
+Features that depend on the whole table (scored_candidates, poisson) cannot be rebuilt per replicate and are
+not emitted.
 """
 from __future__ import annotations
 
@@ -79,6 +80,10 @@ class FamilyIndex:
         return [self.group(f, j, d) for f in (0, 1 + s_index) for d in (False, True)]
 
 
+def _dev_i64(a, dev) -> torch.Tensor:
+    return a.long() if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a, np.int64), device=dev)
+
+
 def _lex_order(*keys):
     """Permutation sorting by keys[0] (major) .. keys[-1] (minor), ascending; ties keep their input order."""
     o = torch.argsort(keys[-1], stable=True)
@@ -118,9 +123,7 @@ def select_family(o_h, o_r, o_c, d_spec, hyp, fi: FamilyIndex, p: E.SearchParams
     tk = torch.zeros(n, dtype=torch.int64, device=dev).scatter_reduce_(0, inv, t, reduce="amax")
     us = uk >> ROW_BITS
     ur = uk & ((1 << ROW_BITS) - 1)
-    hspec = torch.as_tensor(np.asarray(hyp["spec"], np.int64), device=dev)
-    hlo = torch.as_tensor(np.asarray(hyp["lo"], np.int64), device=dev)
-    hhi = torch.as_tensor(np.asarray(hyp["hi"], np.int64), device=dev)
+    hspec, hlo, hhi = (_dev_i64(hyp[k], dev) for k in ("spec", "lo", "hi"))
     h0 = torch.searchsorted(hspec, us, side="left")
     h1 = torch.searchsorted(hspec, us, side="right")
     hcan = torch.full((n,), -1, dtype=torch.int64, device=dev)
@@ -141,9 +144,12 @@ def search_family(sp: E.Spectra, gi: E.GpuIndex, l1, fi: FamilyIndex, p: E.Searc
     st = E.Stats()
     dev = gi.dev
     with E.Timer(st, "k0"):
-        peaks = E.k0_preprocess(sp, p, dev, st)
-    with E.Timer(st, "hyp"):
-        hyp = E.make_hypotheses(sp, peaks["valid"], gi.mass32_np, p)
+        peaks = E.k0_preprocess(sp, p, dev, st, gi)
+    with E.Timer(st, "hyp"):              # vh_hypotheses: the host rule (host.py:64-91) on the device
+        hd = E.device_hypotheses(gi, peaks["raw"], peaks, p)
+        n_h = int(hd["hyp_off"][len(sp)].item())
+        hyp = {k: hd[k][:n_h] for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz")}
+        hyp["hyp_off"] = hd["hyp_off"][:len(sp) + 1]
     with E.Timer(st, "k1"):
         o_h, o_r, o_c, scored, matched, d_spec, d_maxfz = E.run_k1(gi, hyp, peaks, p, st)
     with E.Timer(st, "topk"):
@@ -152,8 +158,8 @@ def search_family(sp: E.Spectra, gi: E.GpuIndex, l1, fi: FamilyIndex, p: E.Searc
         k2 = E.run_k2(gi, h, r, d_spec, d_maxfz, peaks, p)
     with E.Timer(st, "features"):
         tbl = family_table(sp, l1, fi, hyp, h, r, intop, spec, ukey, k2, p)
-    st.counts.update(n_spectra=len(sp), n_searched=int(np.asarray(peaks["valid"]).sum()),
-                     n_hypotheses=len(hyp["spec"]), n_rescored=int(len(r)), n_rows=tbl.num_rows)
+    st.counts.update(n_spectra=len(sp), n_searched=int(np.asarray(peaks["valid"]).sum()), n_hypotheses=n_h,
+                     n_rescored=int(len(r)), n_rows=tbl.num_rows)
     return tbl, st
 
 
@@ -180,7 +186,7 @@ def family_table(sp, l1, fi: FamilyIndex, hyp, h, r, intop, spec, ukey, k2, p) -
     rank = _segment_rank(s * 256 + g)
     keep = rank < 2
     idx, g, s, rank = idx[keep], g[keep], s[keep], rank[keep]
-    hh = h[idx].cpu().numpy()
+    hh = h[idx]
     rows = r[idx].cpu().numpy()
     sn = s.cpu().numpy()
     return pa.table({
@@ -189,7 +195,8 @@ def family_table(sp, l1, fi: FamilyIndex, hyp, h, r, intop, spec, ukey, k2, p) -
         "fgroup": g.cpu().numpy().astype(np.int16), "grank": (rank.cpu().numpy() + 1).astype(np.int8),
         "pf_row": rows.astype(np.int64), "pf_id": l1.pf_id[rows],
         "hyperscore": k2["hyperscore"][idx].cpu().numpy(),
-        "charge": np.asarray(hyp["z"])[hh].astype(np.int8), "isotope_error": np.asarray(hyp["iso"])[hh].astype(np.int8),
+        "charge": _dev_i64(hyp["z"], dev)[hh].cpu().numpy().astype(np.int8),
+        "isotope_error": _dev_i64(hyp["iso"], dev)[hh].cpu().numpy().astype(np.int8),
         "matched_b": k2["matched_b"][idx].cpu().numpy(), "matched_y": k2["matched_y"][idx].cpu().numpy(),
     })
 
