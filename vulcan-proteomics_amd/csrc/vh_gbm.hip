@@ -3,9 +3,10 @@
 //
 // A workgroup owns up to GBM_SPW(nf) samples (one per thread). Their feature rows are staged once in LDS
 // (row stride nf | 1 doubles, so lanes reading different rows spread over the banks); the trees are staged in
-// chunks of whole trees (at most VH_GBM_CHUNK_NODES nodes, 16 B each). Every thread walks each tree of the
-// chunk for its sample and adds the leaf value to its f64 sum in tree order -- the summation order of sklearn
-// (0 + baseline, then += tree by tree), so the result is bit-identical. Walks are bounded by max_depth + 1
+// chunks of whole trees (at most VH_GBM_CHUNK_NODES nodes, 16 B each). Every thread walks the trees of the
+// chunk for its sample GBM_WALK at a time (independent chains of LDS reads) and adds the leaf values to its f64
+// sum in tree order -- the summation order of sklearn (0 + baseline, then += tree by tree), so the result is
+// bit-identical. Walks are bounded by max_depth + 1
 // node visits and every node / feature index is checked, so a malformed model sets status bit 1 instead of
 // reading outside the staged arrays or looping.
 #include "vh_common.h"
@@ -13,6 +14,7 @@
 namespace {
 
 constexpr int GBM_X_BYTES = 32768;   // LDS for the staged feature rows
+constexpr int GBM_WALK = 4;          // trees walked at once per thread
 
 __host__ __device__ inline int gbm_nfp(int nf) { return nf | 1; }
 // samples per workgroup: the largest of 256 / 128 / 64 whose rows fit GBM_X_BYTES
@@ -58,25 +60,62 @@ __global__ __launch_bounds__(256) void gbm_kernel(const VhGbmNode* __restrict__ 
     }
     __syncthreads();
     if (tid < ns) {
-      for (int t = t0; t < t1; ++t) {
-        int li = tree_root[t] - c0;
-        double v = 0.0;
-        bool leaf = false;
-        for (int step = 0; step < max_steps; ++step) {
-          if ((unsigned)li >= (unsigned)cn) break;
-          const VhGbmNode nd = sn[li];
-          if (nd.is_leaf) {
-            v = nd.v;
-            leaf = true;
-            break;
+      // GBM_WALK trees at once: independent chains of dependent LDS reads. A walker that has reached its leaf
+      // re-reads it (no branch around the loads); walkers past the chunk's last tree walk that tree and are
+      // not added. Child and feature indexes are checked before use; a walk that leaves the chunk or ends
+      // without a leaf marks the sample bad.
+      for (int t = t0; t < t1; t += GBM_WALK) {
+        int li[GBM_WALK];
+#pragma unroll
+        for (int u = 0; u < GBM_WALK; ++u) li[u] = tree_root[t + u < t1 ? t + u : t1 - 1] - c0;
+        bool at_leaf[GBM_WALK];
+        double v[GBM_WALK];
+#pragma unroll
+        for (int u = 0; u < GBM_WALK; ++u) {
+          if ((unsigned)li[u] >= (unsigned)cn) {
+            li[u] = 0;
+            bad = true;
           }
-          if ((unsigned)nd.feature >= (unsigned)nf) break;
-          const double x = xr[nd.feature];
-          const bool go_left = (x != x) ? (nd.missing_left != 0) : (x <= nd.v);
-          li = nd.left - c0 + (go_left ? 0 : 1);
+          at_leaf[u] = false;
+          v[u] = 0.0;
         }
-        bad |= !leaf;
-        acc += v;
+        for (int step = 0; step < max_steps; ++step) {
+          VhGbmNode nd[GBM_WALK];
+#pragma unroll
+          for (int u = 0; u < GBM_WALK; ++u) nd[u] = sn[li[u]];
+          double x[GBM_WALK];
+#pragma unroll
+          for (int u = 0; u < GBM_WALK; ++u) {
+            const bool fok = (unsigned)nd[u].feature < (unsigned)nf;
+            bad |= !fok && !nd[u].is_leaf;
+            x[u] = xr[fok ? nd[u].feature : 0];
+          }
+          bool all = true;
+#pragma unroll
+          for (int u = 0; u < GBM_WALK; ++u) {
+            if (nd[u].is_leaf) {
+              at_leaf[u] = true;
+              v[u] = nd[u].v;
+            } else {
+              const bool go_left = (x[u] != x[u]) ? (nd[u].missing_left != 0) : (x[u] <= nd[u].v);
+              const int nx = nd[u].left - c0 + (go_left ? 0 : 1);
+              if ((unsigned)nx < (unsigned)cn) {
+                li[u] = nx;
+              } else {
+                bad = true;
+              }
+              all = false;
+            }
+          }
+          if (all) break;
+        }
+#pragma unroll
+        for (int u = 0; u < GBM_WALK; ++u) {
+          if (t + u < t1) {
+            bad |= !at_leaf[u];
+            acc += v[u];   // tree order
+          }
+        }
       }
     }
   }
