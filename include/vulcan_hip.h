@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 14
+#define VH_ABI_VERSION 15
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -439,8 +439,10 @@ int vh_prefilter_select(const VhIndexView* ix, int32_t n_spec, int32_t stride_in
 #define VH_GBM_CHUNK_NODES 1024   /* LDS node chunk: a tree has at most this many nodes */
 #define VH_GBM_MAX_FEATURES 63
 
-/* One tree node, 16 B. Internal: v = numeric threshold, children at left and left + 1 (left: x <= v, right:
- * x > v; a NaN goes left iff missing_left). Leaf: v = the leaf value. Indexes are into the whole node array. */
+/* One tree node, 16 B. Internal: v = numeric threshold, feature = the split feature, children at left and
+ * left + 1 (left: x <= v, right: x > v; a NaN goes left iff missing_left). Leaf: v = the leaf value, left = its
+ * own index, feature = -1 (0xFFFF), is_leaf = 1. left is an index inside the node's chunk (node
+ * chunk_node0[c] + left). */
 typedef struct {
   double v;
   int32_t left;
@@ -455,8 +457,9 @@ typedef struct {
  * X: n rows of n_features f64 (row-major). The trees are cut into n_chunks runs of whole trees: chunk c holds
  * trees [chunk_tree0[c], chunk_tree0[c+1]) whose nodes are [chunk_node0[c], chunk_node0[c+1]), at most
  * VH_GBM_CHUNK_NODES nodes. max_depth: the deepest leaf's depth (walks stop after max_depth + 1 nodes).
- * status (zeroed by the caller): bit 1 = a walk left its chunk, met a feature >= n_features or no leaf within
- * max_depth + 1 nodes (a malformed model; the sample's sum is then wrong). */
+ * status (zeroed by the caller): bit 1 = a walk did not end on a leaf within max_depth + 1 steps (a malformed
+ * model: indexes beyond the chunk / features beyond n_features are clamped, so every read stays in bounds, but
+ * the sample's sum is then wrong). */
 int vh_gbm_decision(const VhGbmNode* nodes, int64_t n_nodes, const int32_t* tree_root, int32_t n_trees,
                     const int32_t* chunk_node0, const int32_t* chunk_tree0, int32_t n_chunks, int32_t max_depth,
                     double baseline, const double* X, int64_t n, int32_t n_features, double* out, int32_t* status,

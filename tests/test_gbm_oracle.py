@@ -34,22 +34,26 @@ def _test_rows(m, n, nf, seed=1):
 
 
 def walk_packed(nodes, roots, cn0, ct0, depth, baseline, X):
-    """Host walk of the packed records (VhGbmNode layout) -- the kernel's loop, for the packing test."""
+    """Host walk of the packed records (VhGbmNode layout: children chunk-local, leaves point to themselves),
+    for the packing test."""
     out = np.zeros(len(X))
+    chunk_of = np.searchsorted(ct0, np.arange(len(roots)), side="right") - 1
     for i, x in enumerate(X):
         acc = 0.0
         acc += baseline
         for t in range(len(roots)):
-            k = int(roots[t])
+            base = int(cn0[chunk_of[t]])
+            k = int(roots[t]) - base
             for _ in range(depth + 1):
-                nd = nodes[k]
+                nd = nodes[base + k]
                 if nd["is_leaf"]:
+                    assert nd["left"] == k and nd["feature"] == -1
                     break
                 v = x[nd["feature"]]
                 go_left = bool(nd["missing_left"]) if np.isnan(v) else v <= nd["v"]
                 k = int(nd["left"]) + (0 if go_left else 1)
-            assert nodes[k]["is_leaf"]
-            acc += nodes[k]["v"]
+            assert nodes[base + k]["is_leaf"]
+            acc += nodes[base + k]["v"]
         out[i] = acc
     return out
 
@@ -113,8 +117,12 @@ def test_packed_layout_equals_sklearn():
     for c in range(len(cn0) - 1):                          # whole trees: every root of a chunk lies in it
         r = roots[ct0[c]:ct0[c + 1]]
         assert np.all((r >= cn0[c]) & (r < cn0[c + 1]))
+    idx = np.arange(len(nodes))
+    base = cn0[np.searchsorted(cn0, idx, side="right") - 1]       # chunk start of every node
     inner = nodes["is_leaf"] == 0
-    assert np.all(nodes["left"][inner] > np.flatnonzero(inner))   # breadth-first: children after the parent
+    assert np.all(nodes["left"][inner] + base[inner] > idx[inner])  # breadth-first: children after the parent
+    assert np.all(nodes["left"][~inner] + base[~inner] == idx[~inner])    # a leaf points to itself
+    assert np.all(nodes["left"][inner] + 1 < np.diff(cn0)[np.searchsorted(cn0, idx[inner], side="right") - 1])
     Xt = _test_rows(m, 300, 5)
     assert np.array_equal(walk_packed(nodes, roots, cn0, ct0, depth, _O().baseline_of(m), Xt),
                           m.decision_function(Xt))

@@ -1,26 +1,29 @@
 // Gradient-boosted tree scorer: sklearn HistGradientBoosting raw predictions (one tree per iteration) in f64,
 // the device counterpart of the reference's GpuHGB (vulcan/rescore/fast.py:198-254).
 //
-// A workgroup owns up to GBM_SPW(nf) samples (one per thread). Their feature rows are staged once in LDS
-// (row stride nf | 1 doubles, so lanes reading different rows spread over the banks); the trees are staged in
-// chunks of whole trees (at most VH_GBM_CHUNK_NODES nodes, 16 B each). Every thread walks the trees of the
-// chunk for its sample GBM_WALK at a time (independent chains of LDS reads) and adds the leaf values to its f64
-// sum in tree order -- the summation order of sklearn (0 + baseline, then += tree by tree), so the result is
-// bit-identical. Walks are bounded by max_depth + 1
-// node visits and every node / feature index is checked, so a malformed model sets status bit 1 instead of
-// reading outside the staged arrays or looping.
+// A workgroup owns up to gbm_spw(nf) samples (one per thread). Their features are staged once in LDS, column-
+// major (feature f of the workgroup's row r at xs[f * spw + r], so the lanes of a wave read consecutive
+// doubles), plus one column of -inf. The trees are staged in chunks of whole trees (at most
+// VH_GBM_CHUNK_NODES nodes, 16 B each, children as chunk-local indexes). Every thread walks the trees of the
+// chunk for its sample GBM_WALK at a time and adds the leaf values to its f64 sum in tree order -- the
+// summation order of sklearn (0 + baseline, then += tree by tree), so the result is bit-identical.
+//
+// The walk has no branches per node: a leaf points to itself and its feature (0xFFFF) is clamped to the -inf
+// column, so "x <= v" holds and the walker stays on the leaf; a wave stops when none of its walkers moved
+// (masking the walkers that stopped, to spare their LDS reads, measured slower).
+// Child and feature indexes are clamped to the staged arrays (any node array stays in bounds), walks are
+// bounded by max_depth + 1 steps, and a walk that does not end on a leaf sets status bit 1.
 #include "vh_common.h"
 
 namespace {
 
-constexpr int GBM_X_BYTES = 32768;   // LDS for the staged feature rows
+constexpr int GBM_X_BYTES = 32768;   // LDS for the staged features (+ the -inf column)
 constexpr int GBM_WALK = 4;          // trees walked at once per thread
 
-__host__ __device__ inline int gbm_nfp(int nf) { return nf | 1; }
-// samples per workgroup: the largest of 256 / 128 / 64 whose rows fit GBM_X_BYTES
+// samples per workgroup: the largest of 256 / 128 / 64 whose nf + 1 columns fit GBM_X_BYTES
 __host__ __device__ inline int gbm_spw(int nf) {
-  const int row = gbm_nfp(nf) * 8;
-  return row * 256 <= GBM_X_BYTES ? 256 : row * 128 <= GBM_X_BYTES ? 128 : 64;
+  const int col = (nf + 1) * 8;
+  return col * 256 <= GBM_X_BYTES ? 256 : col * 128 <= GBM_X_BYTES ? 128 : 64;
 }
 
 __global__ __launch_bounds__(256) void gbm_kernel(const VhGbmNode* __restrict__ nodes,
@@ -34,7 +37,6 @@ __global__ __launch_bounds__(256) void gbm_kernel(const VhGbmNode* __restrict__ 
   __shared__ __align__(16) double xs[GBM_X_BYTES / 8];
   const int tid = threadIdx.x;
   const int spw = blockDim.x;
-  const int nfp = gbm_nfp(nf);
   const int64_t s0 = (int64_t)blockIdx.x * spw;
   const int ns = (int)(n - s0 < spw ? n - s0 : spw);
   {
@@ -42,16 +44,18 @@ __global__ __launch_bounds__(256) void gbm_kernel(const VhGbmNode* __restrict__ 
     const int tot = ns * nf;
     for (int i = tid; i < tot; i += spw) {
       const int r = i / nf;
-      xs[r * nfp + (i - r * nf)] = src[i];
+      xs[(i - r * nf) * spw + r] = src[i];
     }
+    xs[nf * spw + tid] = -__builtin_inf();
   }
-  const double* xr = xs + tid * nfp;
+  const double* xcol = xs + tid;
   double acc = 0.0;
   acc += baseline;
   bool bad = false;
   for (int c = 0; c < n_chunks; ++c) {
     const int c0 = chunk_node0[c], cn = chunk_node0[c + 1] - c0;
     const int t0 = chunk_tree0[c], t1 = chunk_tree0[c + 1];
+    const unsigned last = (unsigned)(cn > 0 ? cn - 1 : 0);
     __syncthreads();   // the previous chunk's walks are done (and, on the first chunk, the rows are staged)
     {
       const uint4* g = (const uint4*)(nodes + c0);
@@ -59,61 +63,35 @@ __global__ __launch_bounds__(256) void gbm_kernel(const VhGbmNode* __restrict__ 
       for (int i = tid; i < cn; i += spw) l[i] = g[i];
     }
     __syncthreads();
-    if (tid < ns) {
-      // GBM_WALK trees at once: independent chains of dependent LDS reads. A walker that has reached its leaf
-      // re-reads it (no branch around the loads); walkers past the chunk's last tree walk that tree and are
-      // not added. Child and feature indexes are checked before use; a walk that leaves the chunk or ends
-      // without a leaf marks the sample bad.
+    if (tid < ns && cn > 0) {
       for (int t = t0; t < t1; t += GBM_WALK) {
-        int li[GBM_WALK];
-#pragma unroll
-        for (int u = 0; u < GBM_WALK; ++u) li[u] = tree_root[t + u < t1 ? t + u : t1 - 1] - c0;
-        bool at_leaf[GBM_WALK];
-        double v[GBM_WALK];
+        unsigned li[GBM_WALK];
 #pragma unroll
         for (int u = 0; u < GBM_WALK; ++u) {
-          if ((unsigned)li[u] >= (unsigned)cn) {
-            li[u] = 0;
-            bad = true;
-          }
-          at_leaf[u] = false;
-          v[u] = 0.0;
+          const unsigned r = (unsigned)(tree_root[t + u < t1 ? t + u : t1 - 1] - c0);
+          li[u] = r < last ? r : last;
         }
         for (int step = 0; step < max_steps; ++step) {
-          VhGbmNode nd[GBM_WALK];
-#pragma unroll
-          for (int u = 0; u < GBM_WALK; ++u) nd[u] = sn[li[u]];
-          double x[GBM_WALK];
+          bool moved = false;
 #pragma unroll
           for (int u = 0; u < GBM_WALK; ++u) {
-            const bool fok = (unsigned)nd[u].feature < (unsigned)nf;
-            bad |= !fok && !nd[u].is_leaf;
-            x[u] = xr[fok ? nd[u].feature : 0];
+            const VhGbmNode nd = sn[li[u]];
+            const unsigned f = (uint16_t)nd.feature;
+            const double x = xcol[(f < (unsigned)nf ? f : (unsigned)nf) * spw];
+            const bool go_left = (x <= nd.v) || (x != x && nd.missing_left);
+            const unsigned nx = (unsigned)nd.left + (go_left ? 0u : 1u);
+            const unsigned nc = nx < last ? nx : last;
+            moved |= nc != li[u];
+            li[u] = nc;
           }
-          bool all = true;
-#pragma unroll
-          for (int u = 0; u < GBM_WALK; ++u) {
-            if (nd[u].is_leaf) {
-              at_leaf[u] = true;
-              v[u] = nd[u].v;
-            } else {
-              const bool go_left = (x[u] != x[u]) ? (nd[u].missing_left != 0) : (x[u] <= nd[u].v);
-              const int nx = nd[u].left - c0 + (go_left ? 0 : 1);
-              if ((unsigned)nx < (unsigned)cn) {
-                li[u] = nx;
-              } else {
-                bad = true;
-              }
-              all = false;
-            }
-          }
-          if (all) break;
+          if (!__any(moved)) break;
         }
 #pragma unroll
         for (int u = 0; u < GBM_WALK; ++u) {
+          const VhGbmNode nd = sn[li[u]];
           if (t + u < t1) {
-            bad |= !at_leaf[u];
-            acc += v[u];   // tree order
+            bad |= !nd.is_leaf;
+            acc += nd.v;   // tree order
           }
         }
       }

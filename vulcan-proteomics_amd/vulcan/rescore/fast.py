@@ -19,7 +19,8 @@ from ..gpu._abi import GBM_CHUNK_NODES, GBM_MAX_FEATURES, GBM_NODE
 def pack_trees(trees: list):
     """sklearn predictor node arrays -> (VhGbmNode records, tree roots, chunk node / tree starts, max depth).
     Each tree is renumbered breadth-first so an internal node's children are adjacent (left, left + 1); trees
-    are kept whole and cut into chunks of at most GBM_CHUNK_NODES nodes."""
+    are kept whole and cut into chunks of at most GBM_CHUNK_NODES nodes; `left` is an index inside the chunk
+    (a leaf's is its own, with feature -1)."""
     recs, roots = [], []
     chunk_node0, chunk_tree0 = [0], [0]
     base = chunk_base = 0
@@ -49,8 +50,9 @@ def pack_trees(trees: list):
         src = nodes[np.array(order)]
         leaf = src["is_leaf"] != 0
         r["v"] = np.where(leaf, src["value"], src["num_threshold"])
-        r["left"] = [base + pos[int(c)] if not lf else 0 for c, lf in zip(src["left"], leaf)]
-        r["feature"] = np.where(leaf, 0, src["feature_idx"])
+        loc = base - chunk_base                          # the tree's first node inside its chunk
+        r["left"] = [loc + (i if lf else pos[int(c)]) for i, (c, lf) in enumerate(zip(src["left"], leaf))]
+        r["feature"] = np.where(leaf, -1, src["feature_idx"])
         r["missing_left"] = src["missing_go_to_left"]
         r["is_leaf"] = leaf
         recs.append(r)
