@@ -86,3 +86,36 @@ def test_search_many_writes_per_run_tables(runs, tmp_path):
         assert got.num_rows == tbl.num_rows
         for name in ("spec_idx", "pf_row", "rank", "hyperscore", "matched_peaks"):
             assert np.array_equal(got.column(name).to_numpy(), tbl.column(name).to_numpy()), (sp.run, name)
+
+
+def test_run_and_batch_clis_write_the_same_tables(runs, tmp_path):
+    """python -m vulcan.gpu.run and python -m vulcan.gpu.batch (reference run.py / batch.py CLIs) over an index
+    written to disk: the same PSM files, and equal to one engine.search per run."""
+    import json
+
+    import pyarrow.parquet as pq
+    from vulcan.gpu import batch, run
+    from vulcan.gpu import engine as E
+    from vulcan.index.build import L1Index
+    d, paths, sps = runs
+    idx = str(tmp_path / "idx")
+    d["l1"].save(idx, d["labels"])
+    l1 = L1Index(idx)
+    params = json.dumps({k: list(v) if isinstance(v, tuple) else v for k, v in d["p"].to_dict().items()})
+    var = json.dumps(d["var"])
+    print("[gpu] run / batch CLIs", flush=True)
+    assert run.main(["--index", idx, "--stores", *paths, "--out", str(tmp_path / "r"), "--params", params,
+                     "--var", var]) == 0
+    assert batch.main(["--index", idx, "--stores", *paths, "--out", str(tmp_path / "b"), "--params", params]) == 0
+    assert json.load(open(tmp_path / "r" / "stats.json"))["passes"][0][0]["run"] == "run0"
+    gi = E.GpuIndex(l1, d["p"], d["static"], d["var"])
+    for sp in sps:
+        want, _, _ = E.search(sp, gi, l1, d["p"])
+        # batch.main binds the reference's fixed mods (C static, M-ox variable); this table has no variable
+        # sites, so its results equal those of the empty variable set
+        for out in ("r", "b"):
+            a = pq.read_table(str(tmp_path / out / f"{sp.run}.psms.parquet"))
+            assert a.num_rows == want.num_rows, out
+            for name in ("spec_idx", "pf_row", "pf_id", "hyperscore", "poisson"):
+                assert np.array_equal(a.column(name).to_numpy(), want.column(name).to_numpy(), equal_nan=True), \
+                    (out, name)

@@ -66,7 +66,8 @@ class GpuIndex:
         self.h2d = 0
 
         def up(a, dtype=None):
-            t = torch.from_numpy(np.ascontiguousarray(a)).to(d)
+            a = np.ascontiguousarray(a)
+            t = torch.from_numpy(a if a.flags.writeable else a.copy()).to(d)   # arrays of a loaded table are read-only
             self.h2d += t.numel() * t.element_size()
             return t
         self.mass32_np = np.ascontiguousarray(l1.mass32, np.float32)
