@@ -119,3 +119,28 @@ def test_run_and_batch_clis_write_the_same_tables(runs, tmp_path):
             for name in ("spec_idx", "pf_row", "pf_id", "hyperscore", "poisson"):
                 assert np.array_equal(a.column(name).to_numpy(), want.column(name).to_numpy(), equal_nan=True), \
                     (out, name)
+
+
+def test_empty_runs(runs, tmp_path):
+    """An empty run (no spectra) searched on its own and in the middle of a feed: no PSMs, and the runs around
+    it are unaffected."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu.feed import RunFeed
+    from vulcan.gpu.host import Spectra
+    d, paths, sps = runs
+    print("[gpu] empty runs", flush=True)
+    empty = Spectra("empty", np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0), np.zeros(0),
+                    np.zeros(0, np.int8), np.zeros(0, np.float32), np.zeros(0, np.float32), np.zeros(1, np.int64))
+    gi = E.GpuIndex(d["l1"], d["p"], d["static"], d["var"])
+    tbl, st, _ = E.search(empty, gi, d["l1"], d["p"])
+    assert tbl.num_rows == 0 and st.counts["n_spectra"] == 0
+    ep = str(tmp_path / "empty.vspec.arrow")
+    E.write_store(empty, ep)
+    feed = RunFeed(gi, d["p"], max_spectra=1500, max_peaks=max(int(s.offsets[-1]) for s in sps))
+    res = feed.run([paths[0], ep, paths[2]], l1=d["l1"])
+    assert [r["n_spectra"] for r in res] == [len(sps[0]), 0, len(sps[2])]
+    assert res[1]["table"].num_rows == 0 and len(res[1]["rows"]) == 0
+    for r, sp in ((res[0], sps[0]), (res[2], sps[2])):
+        want, _, dbg = E.search(sp, gi, d["l1"], d["p"])
+        assert np.array_equal(r["rows"], dbg["device"].rank1_rows(len(sp)))
+        assert r["table"].num_rows == want.num_rows

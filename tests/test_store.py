@@ -128,3 +128,22 @@ def test_load_store_into_buffers_and_threads(tmp_path):
     import pytest
     with pytest.raises(ValueError, match="do not fit"):
         E.load_store(path, out=small)
+
+
+def test_empty_and_ms1_only_stores(tmp_path):
+    from vulcan.gpu import engine as E
+    path = str(tmp_path / "empty.vspec.arrow")
+    w = S.StoreWriter(path, {"run_name": "empty"})
+    w.write({"offsets": np.zeros(1, np.int64), "mz": np.zeros(0, np.float32), "intensity": np.zeros(0, np.float32),
+             "rt_min": np.zeros(0)})
+    side = w.close()
+    assert side["n_spectra"] == 0 and side["n_peaks"] == 0 and S.counts(path) == (0, 0)
+    sp = E.load_store(path)
+    assert len(sp) == 0 and len(sp.offsets) == 1 and len(sp.mz) == 0
+    off, mz, it = _run(20, 9)
+    path1 = str(tmp_path / "ms1.vspec.arrow")
+    w = S.StoreWriter(path1, {"run_name": "ms1"})
+    w.write({"offsets": off, "mz": mz, "intensity": it, "ms_level": np.ones(20, np.uint8), "rt_min": np.arange(20.0)})
+    w.close()
+    sp1 = E.load_store(path1)
+    assert len(sp1) == 0 and sp1.offsets.tolist() == [0] and len(sp1.mz) == 0

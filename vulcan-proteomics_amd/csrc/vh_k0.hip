@@ -273,6 +273,10 @@ __device__ __forceinline__ int n_hyp_of(const DevP& P, double pmz, int8_t z, uin
   return niso * nz + extra;
 }
 
+// an empty batch still gets its leading CSR offset (hyp_off[0] / pk_off[0] = 0): a kernel, not a memset, so the
+// call stays capturable in a graph
+__global__ void k0_zero_i64_kernel(int64_t* p) { *p = 0; }
+
 __global__ __launch_bounds__(256) void hyp_count_kernel(DevP P, int n_spec, const double* __restrict__ pmz,
                                                         const int8_t* __restrict__ charge,
                                                         const uint8_t* __restrict__ valid, int64_t* hyp_off) {
@@ -478,7 +482,11 @@ int vh_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* inten
   if (!p || n_spec < 0 || !raw_off || !pk_off || !tic || !valid) return VH_E_ARG;
   if (p->max_peaks < 0 || p->max_peaks > K0_SEL) return VH_E_LIMIT;
   if (pk_capacity < (int64_t)n_spec * p->max_peaks) return VH_E_CAPACITY;
-  if (n_spec == 0) return 0;
+  if (n_spec == 0) {
+    k0_zero_i64_kernel<<<1, 1, 0, (hipStream_t)stream>>>(pk_off);
+    VH_CHECK_LAUNCH();
+    return 0;
+  }
   hipStream_t st = (hipStream_t)stream;
   DevP P = vh_make_devp(p);
   unsigned g = (unsigned)((n_spec + K0_WAVES - 1) / K0_WAVES);
@@ -506,7 +514,11 @@ int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz,
   int maxh = niso * nz + nz * (p->n_offsets + (p->open_search ? 1 : 0));
   if (maxh > VH_MAX_HYP_PER_SPEC) return VH_E_LIMIT;
   if (hyp_capacity < (int64_t)n_spec * maxh) return VH_E_CAPACITY;
-  if (n_spec == 0) return 0;
+  if (n_spec == 0) {
+    k0_zero_i64_kernel<<<1, 1, 0, (hipStream_t)stream>>>(hyp_off);
+    VH_CHECK_LAUNCH();
+    return 0;
+  }
   hipStream_t st = (hipStream_t)stream;
   DevP P = vh_make_devp(p);
   unsigned g = (unsigned)((n_spec + 255) / 256);
