@@ -511,6 +511,35 @@ def psm_features(gi: GpuIndex, n_spec: int, grp_spec, sel, nxt, report: int, h, 
     return {k: v[:n_groups * report] for k, v in cols.items()}
 
 
+def string_column(value: str, n: int) -> pa.Array:
+    """n copies of one string as an Arrow string column, built from buffers (a Python list of n strings is
+    orders of magnitude slower)."""
+    b = value.encode()
+    offsets = np.arange(n + 1, dtype=np.int64) * len(b)
+    if offsets[-1] < 2 ** 31:
+        return pa.StringArray.from_buffers(n, pa.py_buffer(offsets.astype(np.int32)), pa.py_buffer(b * n))
+    return pa.array([value] * n)
+
+
+def _to_host(cols: dict) -> dict:
+    """Device columns -> numpy arrays with one device-to-host copy (the columns packed as bytes)."""
+    names = sorted(cols, key=lambda k: -cols[k].element_size())    # widest first: every column stays aligned
+    flat = [cols[k].contiguous().view(-1) for k in names]
+    if not flat or sum(t.numel() for t in flat) == 0:
+        return {k: t.cpu().numpy() for k, t in zip(names, flat)}
+    buf = torch.cat([t.view(torch.uint8) for t in flat]).cpu().numpy()
+    out, o = {}, 0
+    for k, t in zip(names, flat):
+        nb = t.numel() * t.element_size()
+        out[k] = buf[o:o + nb].view(_NP_OF[t.dtype])
+        o += nb
+    return out
+
+
+_NP_OF = {torch.int64: np.int64, torch.int32: np.int32, torch.int16: np.int16, torch.int8: np.int8,
+          torch.uint8: np.uint8, torch.float64: np.float64, torch.float32: np.float32}
+
+
 def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched, peaks, p: SearchParams,
               report: int | None = None, return_hyp: bool = False):
     """Rank candidates per spectrum (vh_rank: hyperscore desc, row asc), compute the feature columns on the
@@ -532,10 +561,10 @@ def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched
                                k2["hyperscore"].contiguous(), k2["matched_b"].contiguous(),
                                k2["matched_y"].contiguous(), report)
         cols = psm_features(gi, n_spec, us, sel, nxt, report, h, r, k1c, k2, hyp_d, scored, matched, peaks)
-        ok = cols["rank"] > 0
-        idx = sel[ok]
-        dcols = {k: v[ok].cpu().numpy() for k, v in cols.items()}
-        hh = h[idx].cpu().numpy()
+        keep = torch.nonzero(cols["rank"] > 0).squeeze(1)
+        cols["hyp"] = h[sel[keep]]
+        dcols = _to_host({k: (v if k == "hyp" else v[keep]) for k, v in cols.items()})
+        hh = dcols.pop("hyp")
     else:
         empty = torch.zeros(0, dtype=torch.int64, device=dev)
         cols = psm_features(gi, n_spec, empty, empty, empty, report, h, r, k1c, k2, hyp_d, scored, matched, peaks)
@@ -543,7 +572,7 @@ def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched
         hh = np.zeros(0, np.int64)
     s = dcols["spec_idx"]
     rows = dcols["pf_row"]
-    cols = {"run": pa.array([sp.run] * len(s)), "spec_idx": s, "scan": sp.scan[s], "rt": sp.rt[s],
+    cols = {"run": string_column(sp.run, len(s)), "spec_idx": s, "scan": sp.scan[s], "rt": sp.rt[s],
             "rank": dcols["rank"], "pf_row": rows, "pf_id": l1.pf_id[rows]}
     for k in ("charge", "isotope_error", "expmass", "calcmass", "delta_mass_ppm", "hyperscore", "delta_next",
               "delta_best", "matched_b", "matched_y", "matched_peaks", "summed_intensity", "matched_intensity_pct",
@@ -810,16 +839,20 @@ class DeviceSearch:
         return self.c_hyp[flat].long(), self.c_row[flat].long(), self.c_cnt[flat].long(), flat
 
 
-def device_table(ds: DeviceSearch, sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int):
+def device_table(ds: DeviceSearch, sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int,
+                 host_hyp: bool = True):
     """PSM table from a DeviceSearch that has run on `sp` (candidates flattened in (spectrum, row) order).
-    Returns (table, per-PSM hypothesis index, debug dict)."""
+    Returns (table, per-PSM hypothesis index, debug dict); host_hyp=False leaves the host copy of the
+    hypotheses out of the debug dict (the table is built from the device ones either way)."""
     n = len(sp)
     dev = gi.dev
     h, r, c, flat = ds.candidates(n)
     k2 = {k: v[flat] for k, v in ds.k2.items()}
     k2["spec"] = torch.repeat_interleave(torch.arange(n, device=dev), ds.n_cand[:n].long())
-    hyp = hyp_to_host(ds.hyp, n)
-    tbl, hh = psm_table(sp, gi, l1, hyp, h, r, c, k2, ds.scored, ds.matched, ds.peaks, p, report, return_hyp=True)
+    n_h = int(ds.hyp["hyp_off"][n].item())
+    hyp_d = {k: ds.hyp[k][:n_h] for k in ("spec", "z", "iso", "pm")}
+    tbl, hh = psm_table(sp, gi, l1, hyp_d, h, r, c, k2, ds.scored, ds.matched, ds.peaks, p, report, return_hyp=True)
+    hyp = hyp_to_host(ds.hyp, n) if host_hyp else None
     stt = int(ds.status.item())
     if stt & 1:
         raise RuntimeError("k1_topk: a spectrum has more hypotheses than one wave handles")

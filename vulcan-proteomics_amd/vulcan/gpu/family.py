@@ -190,7 +190,7 @@ def family_table(sp, l1, fi: FamilyIndex, hyp, h, r, intop, spec, ukey, k2, p) -
     rows = r[idx].cpu().numpy()
     sn = s.cpu().numpy()
     return pa.table({
-        "run": pa.array([sp.run] * len(sn)),
+        "run": E.string_column(sp.run, len(sn)),
         "spec_idx": sn.astype(np.int64), "scan": sp.scan[sn],
         "fgroup": g.cpu().numpy().astype(np.int16), "grank": (rank.cpu().numpy() + 1).astype(np.int8),
         "pf_row": rows.astype(np.int64), "pf_id": l1.pf_id[rows],

@@ -115,7 +115,8 @@ class RunFeed:
                 n = len(sp)
                 res = {"run": sp.run, "n_spectra": n, "h2d_bytes": self.staged_bytes[slot]}
                 if l1 is not None:
-                    tbl, _, _ = device_table(self.ds, sp, self.gi, l1, self.p, report or self.p.report_psms)
+                    tbl, _, _ = device_table(self.ds, sp, self.gi, l1, self.p, report or self.p.report_psms,
+                                             host_hyp=False)
                     res["table"] = tbl
                     if out_dir is not None:
                         res["path"] = write_psms(tbl, out_dir, sp.run)
