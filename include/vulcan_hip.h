@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 15
+#define VH_ABI_VERSION 16
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -464,6 +464,24 @@ int vh_gbm_decision(const VhGbmNode* nodes, int64_t n_nodes, const int32_t* tree
                     const int32_t* chunk_node0, const int32_t* chunk_tree0, int32_t n_chunks, int32_t max_depth,
                     double baseline, const double* X, int64_t n, int32_t n_features, double* out, int32_t* status,
                     void* stream);
+
+/* ---- family top-2 retention (replaces the torch select_family of vulcan/gpu/family.py:90-136) ---- */
+/* Per spectrum, from the bucket-join K1's window counts (vh_k1j_pairs / vh_k1j_count, mode 1: u32 per slot):
+ * the window rows with count >= p->k1_threshold; the best p->topk of every K1 family kfam[row] by (count desc,
+ * row desc, hypothesis asc); each kept row plus its pair partner in each of the n_sets decoy sets
+ * (partners[j * n_rows + row], count -1, not shortlisted); one entry per row in row order: the spectrum's
+ * lowest hypothesis whose window holds the row (else the lowest source hypothesis), the highest count,
+ * shortlisted if any source was. mode 0: out_off[0..n_spec] = exclusive offsets of the spectra's entries (the
+ * total at out_off[n_spec]); mode 1: the entries at out_off[s] (o_row, o_hyp, o_cnt, o_top), out_off from mode
+ * 0. status (zeroed by the caller) bit 1: a spectrum exceeded 64 hypotheses, 512 rows over the threshold or
+ * 2,048 shortlist + partner entries, or a window lay beyond slot_cap (its list is then incomplete).
+ * n_rows < 2^28, 1 <= n_sets <= 8. */
+size_t vh_family_select_workspace_bytes(int32_t n_spec);
+int vh_family_select(const VhParams* p, int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo,
+                     const int64_t* h_hi, const int64_t* h_slot, int64_t slot_cap, const uint32_t* counts,
+                     const uint8_t* kfam, int64_t n_rows, const int64_t* partners, int32_t n_sets, int32_t mode,
+                     int64_t* out_off, int32_t* o_row, int32_t* o_hyp, int32_t* o_cnt, uint8_t* o_top,
+                     int32_t* status, void* ws, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,8 @@ def test_header_declares_the_hot_path():
                  "vh_mlp_forward", "vh_mlp_backward", "vh_mlp_adam", "vh_prefilter_features", "vh_prefilter_score",
                  "vh_prefilter_score_workspace_bytes", "vh_psm_features",
                  "vh_prefilter_select", "vh_index_block_count", "vh_index_build_blocks", "vh_k1j_pairs",
-                 "vh_k1j_count", "vh_k1j_topk", "vh_k1j_features", "vh_gbm_decision"):
+                 "vh_k1j_count", "vh_k1j_topk", "vh_k1j_features", "vh_gbm_decision",
+                 "vh_family_select"):
         assert need in names
 
 

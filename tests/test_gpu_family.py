@@ -77,3 +77,39 @@ def test_family_rank1_is_the_plain_search_rank1_on_a_single_decoy_replicate(mult
     pa_ = dict(zip(ps.tolist(), plain.column("pf_row").to_numpy().tolist()))
     same = np.mean([fa[s] == pa_[s] for s in common.tolist()])
     assert same > 0.97, same
+
+
+def test_family_select_kernel_equals_device_sorts(multi):
+    """vh_family_select on the join K1's window counts == select_family over the appended K1 list (torch
+    device sorts): the same (spectrum, row) entries, hypotheses, counts and shortlist flags; and the two search
+    paths give the same table."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu import family as F
+    from vulcan.index import synth
+    l1, lab, sp = multi
+    p = synth.WORKLOADS[1].search_params()
+    var = dict(synth.WORKLOADS[1].var_mods)
+    gi = E.GpuIndex(l1, p, STATIC, var)
+    fi = F.FamilyIndex.from_l1(l1, gi, lab)
+    n = len(sp)
+    print("[gpu] family select kernel vs sorts", flush=True)
+    st = E.Stats()
+    peaks = E.k0_preprocess(sp, p, gi.dev, st, gi)
+    hd = E.device_hypotheses(gi, peaks["raw"], peaks, p)
+    n_h = int(hd["hyp_off"][n].item())
+    hyp = {k: hd[k][:n_h] for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz")}
+    hyp["hyp_off"] = hd["hyp_off"][:n + 1]
+    o_h, o_r, o_c, _, _, d_spec, _ = E.run_k1(gi, hyp, peaks, p, st)
+    want = F.select_family(o_h, o_r, o_c, d_spec, hyp, fi, p, gi.dev)
+    ds = E.DeviceSearch(gi, p, n, k1_impl="join")
+    ds.front(E.upload_spectra(sp, gi.dev), n, E._marker(None))
+    ds.join_front(n, E._marker(None))
+    got = F.select_family_join(ds, fi, n)
+    assert len(got[0]) == len(want[0]) > 0
+    for a, b, name in zip(got, want, ("hyp", "row", "count", "shortlisted", "spectrum", "key")):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy()), name
+    t1, _ = F.search_family(sp, gi, l1, fi, p, select="kernel")
+    t2, _ = F.search_family(sp, gi, l1, fi, p, select="torch")
+    assert t1.num_rows == t2.num_rows > 0
+    for name in t2.column_names:
+        assert np.array_equal(t1.column(name).to_numpy(), t2.column(name).to_numpy()), name

@@ -63,6 +63,8 @@ _SIGS = {
     "vh_prefilter_score": ([PIX, P, I32, I32, P, P, P, P, P, P, P, P, P, P, P, SZ, P], I32),
     "vh_prefilter_select": ([PIX, I32, I32, P, P, P, P, P, I32, I32, F32, I32, I32, P, P, P, P, P], I32),
     "vh_gbm_decision": ([P, I64, P, I32, P, P, I32, I32, F64, P, I64, I32, P, P, P], I32),
+    "vh_family_select_workspace_bytes": ([I32], SZ),
+    "vh_family_select": ([PP, I32, P, P, P, P, I64, P, P, I64, P, I32, I32, P, P, P, P, P, P, P, SZ, P], I32),
 }
 
 ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "workspace too small", -4: "size limit exceeded"}

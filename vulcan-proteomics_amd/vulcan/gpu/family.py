@@ -49,6 +49,8 @@ class FamilyIndex:
         self.side = torch.as_tensor(np.asarray(side, np.int64), device=dev)
         self.partners = torch.stack([torch.as_tensor(np.asarray(p, np.int64)) for p in partners]).to(dev)
         self.kfam = self.fwd_fam * (self.J + 1) + (self.side + 1)
+        self.kfam_u8 = None
+        self.searches = {}
 
     @staticmethod
     def codes_from_labels(lab: dict):
@@ -138,9 +140,47 @@ def select_family(o_h, o_r, o_c, d_spec, hyp, fi: FamilyIndex, p: E.SearchParams
     return hcan, ur, cc, tk.bool(), us, uk
 
 
-def search_family(sp: E.Spectra, gi: E.GpuIndex, l1, fi: FamilyIndex, p: E.SearchParams):
+def select_family_join(ds: "E.DeviceSearch", fi: FamilyIndex, n: int):
+    """select_family on the device (vh_family_select) from a DeviceSearch whose front and join K1 counts have
+    run: the same candidates as select_family over the appended K1 list, in (spectrum, row) order."""
+    import ctypes as C
+
+    from . import _lib as K
+    L = K.lib()
+    dev = ds.gi.dev
+    j, hy = ds.join, ds.hyp
+    if fi.kfam_u8 is None:
+        fi.kfam_u8 = fi.kfam.to(torch.uint8).contiguous()
+    out_off = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    ws = torch.empty(max(int(L.vh_family_select_workspace_bytes(n)), 1), dtype=torch.uint8, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    args = (C.byref(ds.vp), n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]), K.ptr(j["h_slot"]),
+            j["slot_cap"], K.ptr(j["counts"]), K.ptr(fi.kfam_u8), ds.gi.n, K.ptr(fi.partners), fi.J)
+    K.check(L.vh_family_select(*args, 0, K.ptr(out_off), None, None, None, None, K.ptr(status), K.ptr(ws),
+                               ws.numel(), K.stream()), "family_select (offsets)")
+    tot = int(out_off[n].item())
+    o_row = torch.empty(max(tot, 1), dtype=torch.int32, device=dev)
+    o_hyp = torch.empty_like(o_row)
+    o_cnt = torch.empty_like(o_row)
+    o_top = torch.empty(max(tot, 1), dtype=torch.uint8, device=dev)
+    K.check(L.vh_family_select(*args, 1, K.ptr(out_off), K.ptr(o_row), K.ptr(o_hyp), K.ptr(o_cnt), K.ptr(o_top),
+                               K.ptr(status), K.ptr(ws), ws.numel(), K.stream()), "family_select")
+    if int(status.item()):
+        raise RuntimeError("family_select: a spectrum exceeded the per-spectrum limits (64 hypotheses, 512 rows "
+                           "over the K1 threshold, 2,048 shortlist + partner entries) or a window overflowed")
+    us = torch.repeat_interleave(torch.arange(n, device=dev), out_off[1:] - out_off[:-1])
+    ur = o_row[:tot].long()
+    return o_hyp[:tot].long(), ur, o_cnt[:tot].long(), o_top[:tot].bool(), us, (us << ROW_BITS) | ur
+
+
+def search_family(sp: E.Spectra, gi: E.GpuIndex, l1, fi: FamilyIndex, p: E.SearchParams, select: str = "kernel"):
     """Closed search with family top-2 retention. Returns (table, stats); one table row per (spectrum, retention
-    group, grank in {1, 2})."""
+    group, grank in {1, 2}). select: "kernel" (join K1 counts + vh_family_select) or "torch" (append-form K1 +
+    device sorts; the same candidates)."""
+    if select == "kernel":
+        return _search_family_kernel(sp, gi, l1, fi, p)
+    if select != "torch":
+        raise ValueError(select)
     st = E.Stats()
     dev = gi.dev
     with E.Timer(st, "k0"):
@@ -160,6 +200,38 @@ def search_family(sp: E.Spectra, gi: E.GpuIndex, l1, fi: FamilyIndex, p: E.Searc
         tbl = family_table(sp, l1, fi, hyp, h, r, intop, spec, ukey, k2, p)
     st.counts.update(n_spectra=len(sp), n_searched=int(np.asarray(peaks["valid"]).sum()), n_hypotheses=n_h,
                      n_rescored=int(len(r)), n_rows=tbl.num_rows)
+    return tbl, st
+
+
+def _search_family_kernel(sp, gi, l1, fi: FamilyIndex, p):
+    st = E.Stats()
+    dev = gi.dev
+    n = len(sp)
+    with E.Timer(st, "upload"):
+        raw = E.upload_spectra(sp, dev, st)
+    ds = fi.searches.get((id(gi), n))      # preallocated buffers, reused by searches of the same size
+    if ds is None:
+        ds = E.DeviceSearch(gi, p, n, k1_impl="join")
+        fi.searches = {(id(gi), n): ds}
+    mark = E._marker(None)
+    ds.status.zero_()
+    with E.Timer(st, "k0"):
+        ds.front(raw, n, mark)             # K0, hypotheses, processing order
+    with E.Timer(st, "k1"):
+        ds.join_front(n, mark)             # join K1: pairs + per-row window counts
+        if int(ds.status.item()) & 8:      # this batch outgrew the pair / slot capacities: refit, count again
+            ds.join["fitted"] = False
+            ds.status.zero_()
+            ds.join_front(n, mark)
+    with E.Timer(st, "topk"):
+        h, r, c, intop, spec, ukey = select_family_join(ds, fi, n)
+    n_h = int(ds.hyp["hyp_off"][n].item())
+    hyp = {k: ds.hyp[k][:n_h] for k in ("spec", "z", "iso", "lo", "hi", "maxfz")}
+    with E.Timer(st, "k2"):
+        k2 = E.run_k2(gi, h, r, hyp["spec"], hyp["maxfz"], ds.peaks, p)
+    with E.Timer(st, "features"):
+        tbl = family_table(sp, l1, fi, hyp, h, r, intop, spec, ukey, k2, p)
+    st.counts.update(n_spectra=n, n_hypotheses=n_h, n_rescored=int(len(r)), n_rows=tbl.num_rows)
     return tbl, st
 
 
