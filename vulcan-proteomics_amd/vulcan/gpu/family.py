@@ -259,17 +259,18 @@ def family_table(sp, l1, fi: FamilyIndex, hyp, h, r, intop, spec, ukey, k2, p) -
     keep = rank < 2
     idx, g, s, rank = idx[keep], g[keep], s[keep], rank[keep]
     hh = h[idx]
-    rows = r[idx].cpu().numpy()
-    sn = s.cpu().numpy()
+    d = E._to_host({"spec": s, "row": r[idx], "fgroup": g.to(torch.int16), "grank": (rank + 1).to(torch.int8),
+                    "hs": k2["hyperscore"][idx], "z": _dev_i64(hyp["z"], dev)[hh].to(torch.int8),
+                    "iso": _dev_i64(hyp["iso"], dev)[hh].to(torch.int8), "mb": k2["matched_b"][idx],
+                    "my": k2["matched_y"][idx]})
+    sn, rows = d["spec"], d["row"]
     return pa.table({
         "run": E.string_column(sp.run, len(sn)),
         "spec_idx": sn.astype(np.int64), "scan": sp.scan[sn],
-        "fgroup": g.cpu().numpy().astype(np.int16), "grank": (rank.cpu().numpy() + 1).astype(np.int8),
+        "fgroup": d["fgroup"], "grank": d["grank"],
         "pf_row": rows.astype(np.int64), "pf_id": l1.pf_id[rows],
-        "hyperscore": k2["hyperscore"][idx].cpu().numpy(),
-        "charge": _dev_i64(hyp["z"], dev)[hh].cpu().numpy().astype(np.int8),
-        "isotope_error": _dev_i64(hyp["iso"], dev)[hh].cpu().numpy().astype(np.int8),
-        "matched_b": k2["matched_b"][idx].cpu().numpy(), "matched_y": k2["matched_y"][idx].cpu().numpy(),
+        "hyperscore": d["hs"], "charge": d["z"], "isotope_error": d["iso"],
+        "matched_b": d["mb"], "matched_y": d["my"],
     })
 
 
