@@ -147,3 +147,41 @@ def test_empty_and_ms1_only_stores(tmp_path):
     w.close()
     sp1 = E.load_store(path1)
     assert len(sp1) == 0 and sp1.offsets.tolist() == [0] and len(sp1.mz) == 0
+
+
+def test_reads_a_store_written_by_the_reference_writer():
+    """tests/golden/ref_run.vspec.arrow was written by the reference's own StoreWriter (full schema, two record
+    batches, MS1 rows, null charges; tests/golden/make_store_fixture.py): this reader and load_store give back
+    the arrays it was given, and the columns this build writes have the reference's names and types."""
+    import os
+
+    from vulcan.gpu import engine as E
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    path = os.path.join(g, "ref_run.vspec.arrow")
+    want = json.load(open(os.path.join(g, "ref_run_inputs.json")))
+    t = S.read_table(path)
+    assert t.num_rows == len(want["scan"]) and len(t.column("mz").chunks) == 2
+    for f in S.FIELDS:
+        assert t.schema.field(f.name).type == f.type, f.name
+    meta = S.run_meta(path)
+    assert meta["run_name"] == "ref_run" and meta["run_id"] == S.default_run_id("ref_run")
+    assert np.array_equal(t.column("spectrum_id").to_numpy(),
+                          (np.uint64(meta["run_id"]) << np.uint64(32)) | np.arange(t.num_rows, dtype=np.uint64))
+    pk = S.peaks_numpy(t)
+    assert np.array_equal(np.diff(pk.offsets), want["peak_counts"])
+    assert np.array_equal(pk.mz, np.array(want["mz"], np.float32))
+    assert np.array_equal(pk.intensity, np.array(want["intensity"], np.float32))
+    assert S.counts(path) == (len(want["scan"]), len(want["mz"]))
+    sp = E.load_store(path)
+    lv = np.array(want["ms_level"])
+    keep = np.flatnonzero(lv == 2)
+    assert len(sp) == len(keep) and np.array_equal(sp.scan, np.array(want["scan"])[keep])
+    assert np.array_equal(sp.rt, np.array(want["rt_min"])[keep])
+    assert np.array_equal(sp.precursor_mz, np.array(want["precursor_mz"], np.float64)[keep])
+    z = np.where(np.array(want["precursor_charge_null"]), 0, np.array(want["precursor_charge"]))[keep]
+    assert np.array_equal(sp.charge, z.astype(np.int8))
+    cnt = np.array(want["peak_counts"])
+    starts = np.concatenate([[0], np.cumsum(cnt)])
+    for j, i in enumerate(keep):
+        assert np.array_equal(sp.mz[sp.offsets[j]:sp.offsets[j + 1]],
+                              np.array(want["mz"], np.float32)[starts[i]:starts[i + 1]])
