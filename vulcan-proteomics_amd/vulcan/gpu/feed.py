@@ -94,6 +94,8 @@ class RunFeed:
         that exceeds them is flagged by the pairs kernel (status 8, writes bounded) and searched again after a
         refit, so no run waits on a host read of its totals."""
         out = []
+        if not paths:
+            return out
         cur = torch.cuda.current_stream(self.gi.dev)
         with ThreadPoolExecutor(1) as ex:
             fut = ex.submit(self._load_stage, paths[0], 0)
