@@ -130,3 +130,21 @@ def test_packed_layout_equals_sklearn():
         bad = [t.copy() for t in _O().trees_of(m)]
         bad[0]["is_categorical"][0] = 1
         pack_trees(bad)
+
+
+def test_large_trees_fill_one_chunk_each_and_oversized_trees_are_refused():
+    from vulcan.rescore.fast import pack_trees
+    rng = np.random.default_rng(8)
+    X = rng.normal(size=(20000, 6))
+    y = np.sin(X[:, 0] * 3) + X[:, 1] * X[:, 2] + rng.normal(size=20000) * 0.1
+    m = sk.HistGradientBoostingRegressor(max_iter=4, max_leaf_nodes=512, min_samples_leaf=2, max_depth=None,
+                                         random_state=0).fit(X, y)
+    trees = _O().trees_of(m)
+    sizes = [len(t) for t in trees]
+    assert max(sizes) > 900                                  # close to the 1,024-node chunk
+    nodes, roots, cn0, ct0, depth = pack_trees(trees)
+    assert np.all(np.diff(cn0) <= 1024)
+    Xt = _test_rows(m, 400, 6)
+    assert np.array_equal(walk_packed(nodes, roots, cn0, ct0, depth, _O().baseline_of(m), Xt), m.predict(Xt))
+    with pytest.raises(ValueError, match="nodes"):            # a tree of 1,030 nodes cannot be staged
+        pack_trees([np.resize(trees[0], 1030)])

@@ -87,3 +87,18 @@ def test_empty_input_and_malformed_models():
     except RuntimeError as e:
         assert "malformed" in str(e)
     assert np.array_equal(GpuHGB(m).decision_function(np.zeros((10, 7))), ref)
+
+
+def test_trees_near_the_chunk_limit():
+    """512-leaf trees (about 1,000 nodes: one tree per LDS chunk), bit-identical to sklearn."""
+    from vulcan.rescore.fast import GpuHGB
+    print("[gpu] gbm: 512-leaf trees", flush=True)
+    rng = np.random.default_rng(8)
+    X = rng.normal(size=(20000, 6))
+    y = np.sin(X[:, 0] * 3) + X[:, 1] * X[:, 2] + rng.normal(size=20000) * 0.1
+    m = sk.HistGradientBoostingRegressor(max_iter=6, max_leaf_nodes=512, min_samples_leaf=2, max_depth=None,
+                                         random_state=0).fit(X, y)
+    g = GpuHGB(m)
+    assert g.n_chunks >= 5
+    Xt = _test_rows(m, 7001, 6)
+    assert np.array_equal(g.decision_function(Xt), m.predict(Xt))
