@@ -473,8 +473,8 @@ int vh_gbm_decision(const VhGbmNode* nodes, int64_t n_nodes, const int32_t* tree
  * lowest hypothesis whose window holds the row (else the lowest source hypothesis), the highest count,
  * shortlisted if any source was. mode 0: out_off[0..n_spec] = exclusive offsets of the spectra's entries (the
  * total at out_off[n_spec]); mode 1: the entries at out_off[s] (o_row, o_hyp, o_cnt, o_top), out_off from mode
- * 0. status (zeroed by the caller) bit 1: a spectrum exceeded 64 hypotheses, 512 rows over the threshold or
- * 2,048 shortlist + partner entries, or a window lay beyond slot_cap (its list is then incomplete).
+ * 0. status (zeroed by the caller) bit 1: a spectrum exceeded 64 hypotheses, 256 rows over the threshold or
+ * 1,024 shortlist + partner entries, or a window lay beyond slot_cap (its list is then incomplete).
  * n_rows < 2^28, 1 <= n_sets <= 8. */
 size_t vh_family_select_workspace_bytes(int32_t n_spec);
 int vh_family_select(const VhParams* p, int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo,

@@ -17,9 +17,9 @@
 
 namespace {
 
-constexpr int FS_E = 512;
-constexpr int FS_C = 2048;
-constexpr int FS_WAVES = 2;
+constexpr int FS_E = 256;    // rows over the K1 threshold per spectrum
+constexpr int FS_C = 1024;   // shortlist + partner entries per spectrum
+constexpr int FS_WAVES = 4;  // 10 KB of LDS per wave
 constexpr uint64_t FS_ROWMASK = (1ull << 28) - 1;
 
 struct FsArgs {

@@ -166,8 +166,8 @@ def select_family_join(ds: "E.DeviceSearch", fi: FamilyIndex, n: int):
     K.check(L.vh_family_select(*args, 1, K.ptr(out_off), K.ptr(o_row), K.ptr(o_hyp), K.ptr(o_cnt), K.ptr(o_top),
                                K.ptr(status), K.ptr(ws), ws.numel(), K.stream()), "family_select")
     if int(status.item()):
-        raise RuntimeError("family_select: a spectrum exceeded the per-spectrum limits (64 hypotheses, 512 rows "
-                           "over the K1 threshold, 2,048 shortlist + partner entries) or a window overflowed")
+        raise RuntimeError("family_select: a spectrum exceeded the per-spectrum limits (64 hypotheses, 256 rows "
+                           "over the K1 threshold, 1,024 shortlist + partner entries) or a window overflowed")
     us = torch.repeat_interleave(torch.arange(n, device=dev), out_off[1:] - out_off[:-1])
     ur = o_row[:tot].long()
     return o_hyp[:tot].long(), ur, o_cnt[:tot].long(), o_top[:tot].bool(), us, (us << ROW_BITS) | ur
