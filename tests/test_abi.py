@@ -78,6 +78,16 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert L.vh_prefilter_score(C.byref(ix), 1, 1000, 200, *[1] * 10, None, 10 ** 9, None) == -1  # no workspace
     assert L.vh_prefilter_select(C.byref(ix), 1, 2000, *[None] * 5, 16, 16, 0.0, 1, 32, *[None] * 4, None) == -1
     assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 0, 16, 0.0, 1, 32, *[None] * 4, None) == -1
+    # GBM scorer: no status word / too many features / negative depth; pointers never dereferenced
+    assert L.vh_gbm_decision(1, 10, 1, 1, 1, 1, 1, 3, 0.0, 1, 5, 7, 1, None, None) == -1
+    assert L.vh_gbm_decision(1, 10, 1, 1, 1, 1, 1, 3, 0.0, 1, 5, 64, 1, 1, None) == -4
+    assert L.vh_gbm_decision(1, 10, 1, 1, 1, 1, 1, -1, 0.0, 1, 5, 7, 1, 1, None) == -1
+    # family select: bad mode / decoy-set count / table too large / workspace short; n_spec 0 needs no inputs
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 2, 2, *[1] * 6, 1, 0, None) == -1
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 0, 0, *[1] * 6, 1, 0, None) == -1
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 1 << 28, 1, 2, 0, *[1] * 6, 1, 0, None) == -4
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 2, 0, *[1] * 6, 1, 0, None) == -3
+    assert L.vh_family_select_workspace_bytes(1000) > 0
     assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 16, 8, 0.0, 1, 32, *[None] * 4, None) == -1
     assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 8, 16, -1.0, 1, 32, *[None] * 4, None) == -1
     ixp = VhIndexView()
