@@ -302,15 +302,18 @@ def gbm_leg(a):
     equal = bool(np.array_equal(out[:chk].cpu().numpy(), m.decision_function(Xt[:chk])))
     bytes_per_launch = n * (nf * 8 + 8)                  # feature rows read + one f64 score written
     ach = bytes_per_launch / (ms * 1e-3) / 1e9
+    workload = f"gbm-{iters}-trees-{nf}-features-{n}-rows"
+    tpath = os.path.join(ROOT, "profiles", f"traffic_gbm_kernel_{workload}.json")
+    traffic = json.load(open(tpath)).get("hbm_bytes_per_launch") if os.path.exists(tpath) else None
     res = {"metric": "rows/s (gradient-boosted tree scorer, f64, bit-identical to sklearn)", "value": n / (ms * 1e-3),
            "unit": "rows/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"gbm-{iters}-trees-{nf}-features-{n}-rows", "n_rows": n, "n_features": nf,
+           "config": {"workload": workload, "n_rows": n, "n_features": nf,
                       "n_trees": g.T, "n_nodes": g.M, "node_chunks": g.n_chunks, "max_depth": g.depth,
                       "parallelism": "single-gpu"},
            "kernel_ms": ms, "tree_walks_per_s": n * g.T / (ms * 1e-3), "equal_sklearn_first_rows": equal,
            "roofline": {"bound": "hbm", "kernel": "gbm_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": bytes_per_launch,
+                        "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes_per_launch": bytes_per_launch,
                         "note": "walks are bound by dependent LDS reads (node, then feature), not by HBM"},
            "cpu_baseline": None}
     if not a.no_cpu:
