@@ -574,10 +574,39 @@ def env_modes():
             "hix": hix, "pk": pk, "hyp": hyp, "k1": k1, "var": w1.var_mods}
 
 
-def test_modes_device_pipeline_equals_oracle(env_modes):
+@pytest.fixture(scope="module")
+def env_open500():
+    """The open-search workload's parameters at full width: 1,000-protein workload-1-semantics table, 400
+    spectra with 40% precursor-only shifts U(-150, +400) Da, open window +-500 Da, top-K 100 + closure."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu._abi import make_params, max_hyp_per_spectrum, set_modes
+    from vulcan.index import synth
+    from vulcan.index.build import build_tables
+    from vulcan.ptm.search import ModeParams, mode_search_params
+    print("[gpu] building open +-500 Da env", flush=True)
+    w1, w3 = synth.WORKLOADS[1], synth.WORKLOADS[3]
+    l1, lab = build_tables(synth.proteome(1000, 103), w1.index_params())
+    sp, truth, meta = synth.spectra(l1, lab, 3, n=400, seed=35, index_params=w1.index_params())
+    mp = ModeParams(open_da=w3.open_da)
+    p = mode_search_params(w3.search_params(), mp)
+    assert p.topk == 100 and mp.open_da == (-500.0, 500.0)
+    gi = E.GpuIndex(l1, p, STATIC, w1.var_mods)
+    vp = set_modes(make_params(p, STATIC, w1.var_mods), mp.offsets, mp.open_da)
+    O = _O()
+    hix = O.HostIndex(l1)
+    pk = O.k0(vp, sp)
+    hyp = O.hypotheses(vp, sp, pk["valid"], l1.mass32, max_hyp_per_spectrum(p, 0, True))
+    k1 = O.k1_counts(vp, hix, pk, hyp)
+    print(f"[gpu] open env: {len(l1)} rows, {int(k1['win_off'][-1])} window rows", flush=True)
+    return {"l1": l1, "sp": sp, "truth": truth, "shift": meta["shift"], "mp": mp, "p": p, "gi": gi, "vp": vp,
+            "hix": hix, "pk": pk, "hyp": hyp, "k1": k1, "var": w1.var_mods}
+
+
+@pytest.mark.parametrize("which", ["env_modes", "env_open500"])
+def test_modes_device_pipeline_equals_oracle(which, request):
     """Mode hypotheses, K1 top-K over closed + offset + open windows, shifted K2 and rank 1 == the oracle."""
     from vulcan.gpu import engine as E
-    e = env_modes
+    e = request.getfixturevalue(which)
     O = _O()
     ds = E.DeviceSearch(e["gi"], e["p"], len(e["sp"]), offsets=e["mp"].offsets, open_da=e["mp"].open_da)
     ds.run(E.upload_spectra(e["sp"], e["gi"].dev))
@@ -607,6 +636,9 @@ def test_modes_device_pipeline_equals_oracle(env_modes):
         assert np.array_equal(_np(ds.k2[k])[:n * ds.stride][m], ref[k]), k
     assert np.array_equal(_np(ds.k2["ppm"])[:n * ds.stride][m], ref["ppm"], equal_nan=True)
     assert np.allclose(_np(ds.k2["hyperscore"])[:n * ds.stride][m], ref["hyperscore"], rtol=HS_RTOL, atol=0)
+    sel, _ = O.rank(e["vp"], start, nc, rows, ref["hyperscore"], ref["matched_b"], ref["matched_y"], 1)
+    flat_of = np.flatnonzero(m)
+    assert np.array_equal(_np(ds.sel)[:n], np.where(sel >= 0, flat_of[np.maximum(sel, 0)], -1))
 
 
 def test_search_modes_recovers_shifted_spectra_and_is_label_blind(env_modes):
@@ -636,11 +668,12 @@ def test_search_modes_recovers_shifted_spectra_and_is_label_blind(env_modes):
     assert ho(tbl).equals(ho(tb))
 
 
-def test_modes_join_equals_oracle(env_modes):
-    """The join K1 forced on the mode search (mass offsets + a 650 Da open window: each open hypothesis spans
-    many blocks) gives the oracle's candidate lists."""
+@pytest.mark.parametrize("which", ["env_modes", "env_open500"])
+def test_modes_join_equals_oracle(which, request):
+    """The join K1 forced on the mode search (mass offsets + a 650 Da open window, or the +-500 Da window: each
+    open hypothesis spans many blocks) gives the oracle's candidate lists."""
     from vulcan.gpu import engine as E
-    e = env_modes
+    e = request.getfixturevalue(which)
     ds = E.DeviceSearch(e["gi"], e["p"], len(e["sp"]), offsets=e["mp"].offsets, open_da=e["mp"].open_da,
                         k1_impl="join")
     ds.run(E.upload_spectra(e["sp"], e["gi"].dev))

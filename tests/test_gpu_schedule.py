@@ -145,3 +145,38 @@ def test_tune_step_picks_a_configuration_with_identical_results(uneven):
     torch.cuda.synchronize()
     assert int(obj.status.item()) == 0
     _same(obj, u["base"], u["n"])
+
+
+def test_uneven_batch_equals_oracle(uneven, proteome_small):
+    """The uneven batch (charges 2-6, 20-500 peaks, per-run calibration offsets) against the oracle: the default
+    step's candidate lists, hyperscores and rank 1, and the rank 1 of the tuned class schedule replayed as a
+    hipGraph."""
+    from oracle import oracle as O
+    from vulcan.gpu import schedule as S
+    from vulcan.gpu._abi import make_params, max_hyp_per_spectrum
+    u = uneven
+    d = proteome_small
+    n, ds = u["n"], u["base"]
+    vp = make_params(u["p"], STATIC, d["var"])
+    hix = O.HostIndex(d["l1"])
+    pk = O.k0(vp, u["sp"])
+    hyp = O.hypotheses(vp, u["sp"], pk["valid"], d["l1"].mass32, max_hyp_per_spectrum(u["p"]))
+    assert len(np.unique(np.asarray(u["sp"].charge))) == 5
+    k1 = O.k1_counts(vp, hix, pk, hyp)
+    tk = O.topk(vp, hix, hyp, k1, ds.stride)
+    nc = _np(ds.n_cand)[:n]
+    assert np.array_equal(nc, tk["n_cand"])
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    rows, hyps = _np(ds.c_row)[:n * ds.stride][m], _np(ds.c_hyp)[:n * ds.stride][m]
+    assert np.array_equal(rows, tk["row"][m]) and np.array_equal(hyps, tk["hyp"][m])
+    start = np.concatenate([[0], np.cumsum(nc)[:-1]])
+    ref = O.k2(vp, hix, pk, rows, hyps, hyp["maxfz"], grp_start=start, grp_count=nc)
+    assert np.allclose(_np(ds.k2["hyperscore"])[:n * ds.stride][m], ref["hyperscore"], rtol=1e-12, atol=0)
+    assert np.array_equal(_np(ds.k2["matched_y"])[:n * ds.stride][m], ref["matched_y"])
+    full = O.search(vp, hix, u["sp"])
+    assert np.array_equal(ds.rank1_rows(n), full["row"])
+    sched = S.ClassSchedule(u["gi"], u["p"], u["sp"])
+    g = S.GraphStep(sched, u["raw"])
+    g.replay()
+    torch.cuda.synchronize()
+    assert np.array_equal(sched.rank1_rows(n), full["row"])
