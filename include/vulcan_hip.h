@@ -158,6 +158,17 @@ int vh_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* inten
           int64_t* pk_off, float* pk_mass, float* pk_int, float* tic, uint8_t* valid, int64_t pk_capacity,
           void* ws, size_t ws_bytes, void* stream);
 
+/* vh_k0 without a workspace argument and without any allocation: the scan scratch is a static device array
+ * of the library (65,792 bytes per GPU, held until the process ends), so n_spec is limited to
+ * VH_K0_NOALLOC_MAX_SPECTRA (VH_E_LIMIT beyond). Same outputs as vh_k0. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under hipGraph capture, tight serving loops, hosts
+ * that forbid hipMalloc). Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+#define VH_K0_NOALLOC_MAX_SPECTRA (1 << 24)
+int vh_k0_noalloc(const VhParams* p, int32_t n_spec, const float* mz, const float* intensity,
+                  const int64_t* raw_off, int64_t* pk_off, float* pk_mass, float* pk_int, float* tic, uint8_t* valid,
+                  int64_t pk_capacity, void* stream);
+
 /* ---------------------------------------------------------------------------------------------- hypotheses */
 /* Precursor hypotheses (host.py:64-91) on the device: order (spectrum, charge, isotope), f32 arithmetic
  * mz = pmz - PROTON, pm = mz*z, M = pm - iso*NEUTRON, window = M + M*ppm/1e6, rows = searchsorted left/right

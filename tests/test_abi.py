@@ -113,6 +113,9 @@ def test_argument_errors_are_reported_without_a_gpu():
     vp.max_peaks = 4096
     assert L.vh_k0(C.byref(vp), 1, None, None, C.c_void_p(8), C.c_void_p(8), None, None, C.c_void_p(8),
                    C.c_void_p(8), 10, None, 0, None) == -4
+    # the allocation-free K0 refuses batches beyond its static scratch before touching the GPU
+    assert L.vh_k0_noalloc(C.byref(vp), (1 << 24) + 1, *[None] * 8, 0, None) == -4
+    assert L.vh_k0_noalloc(C.byref(vp), -1, *[None] * 8, 0, None) == -1
 
 
 def _declarations():

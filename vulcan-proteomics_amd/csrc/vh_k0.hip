@@ -13,6 +13,8 @@
 // strictly ascending (m/z-sorted input) takes a fast path: a 64-step bitwise search for the max_peaks-th key and
 // a ballot compaction in input order give the same kept set and slots without the O(n^2) passes.
 // Hypotheses restate make_hypotheses (vulcan/gpu/host.py:64-91) with the same f32 operation order.
+#include <mutex>
+
 #include "vh_common.h"
 
 #define K0_WAVES 4
@@ -425,6 +427,61 @@ __global__ __launch_bounds__(256) void order_out_kernel(int n_spec, const uint64
 
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// K0 = count kernel, inclusive scan of the per-spectrum kept counts into pk_off (scan partials in `ws`), emit
+// kernel. Shared by vh_k0 (caller workspace) and vh_k0_noalloc (the static scratch below).
+static int k0_launch(const VhParams* p, int32_t n_spec, const float* mz, const float* intensity,
+                     const int64_t* raw_off, int64_t* pk_off, float* pk_mass, float* pk_int, float* tic,
+                     uint8_t* valid, int64_t pk_capacity, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (!p || n_spec < 0 || !raw_off || !pk_off || !tic || !valid) return VH_E_ARG;
+  if (p->max_peaks < 0 || p->max_peaks > K0_SEL) return VH_E_LIMIT;
+  if (pk_capacity < (int64_t)n_spec * p->max_peaks) return VH_E_CAPACITY;
+  if (n_spec == 0) {
+    k0_zero_i64_kernel<<<1, 1, 0, st>>>(pk_off);
+    VH_CHECK_LAUNCH();
+    return 0;
+  }
+  DevP P = vh_make_devp(p);
+  unsigned g = (unsigned)((n_spec + K0_WAVES - 1) / K0_WAVES);
+  k0_count_kernel<<<g, 64 * K0_WAVES, 0, st>>>(P, n_spec, mz, raw_off, pk_off);
+  VH_CHECK_LAUNCH();
+  int rc = vh_scan_inclusive_i64(pk_off + 1, n_spec, ws, ws_bytes, st);
+  if (rc) return rc;
+  k0_emit_kernel<<<g, 64 * K0_WAVES, 0, st>>>(P, n_spec, mz, intensity, raw_off, pk_off, pk_mass, pk_int, tic,
+                                              valid);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+// Scratch of vh_k0_noalloc: the scan partials of up to VH_K0_NOALLOC_MAX_SPECTRA spectra (one i64 per 2,048
+// spectra + 256 bytes of alignment slack, the size vh_scan_workspace_bytes returns for that count). One copy per
+// GPU, alive until the process ends; every call on a GPU reuses it (the partials are fully rewritten by each
+// scan, so nothing needs clearing), hence calls that overlap on one GPU race.
+#define K0_SCRATCH_BYTES ((size_t)(VH_K0_NOALLOC_MAX_SPECTRA / 2048) * 8 + 256)
+static __device__ __align__(256) unsigned char scratch[K0_SCRATCH_BYTES];
+
+#define K0_MAX_DEVICES 64
+static std::mutex k0_scratch_mu;
+static void* k0_scratch_ptr[K0_MAX_DEVICES];
+
+// Device address of `scratch` on the current device, looked up once per device with that device current. The
+// host-side symbol is only a stand-in: kernels and scans get this address, never the array's name.
+static int k0_scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= K0_MAX_DEVICES) return VH_E_LIMIT;
+  std::lock_guard<std::mutex> lock(k0_scratch_mu);
+  if (!k0_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!ptr) return VH_E_ARG;
+    k0_scratch_ptr[dev] = ptr;
+  }
+  *out = k0_scratch_ptr[dev];
+  return 0;
+}
+
 extern "C" {
 
 size_t vh_spectrum_order_workspace_bytes(int32_t n_spec) {
@@ -479,25 +536,20 @@ int vh_spectrum_order_work(int32_t n_spec, const int64_t* hyp_off, const int64_t
 int vh_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* intensity, const int64_t* raw_off,
           int64_t* pk_off, float* pk_mass, float* pk_int, float* tic, uint8_t* valid, int64_t pk_capacity,
           void* ws, size_t ws_bytes, void* stream) {
-  if (!p || n_spec < 0 || !raw_off || !pk_off || !tic || !valid) return VH_E_ARG;
-  if (p->max_peaks < 0 || p->max_peaks > K0_SEL) return VH_E_LIMIT;
-  if (pk_capacity < (int64_t)n_spec * p->max_peaks) return VH_E_CAPACITY;
-  if (n_spec == 0) {
-    k0_zero_i64_kernel<<<1, 1, 0, (hipStream_t)stream>>>(pk_off);
-    VH_CHECK_LAUNCH();
-    return 0;
-  }
-  hipStream_t st = (hipStream_t)stream;
-  DevP P = vh_make_devp(p);
-  unsigned g = (unsigned)((n_spec + K0_WAVES - 1) / K0_WAVES);
-  k0_count_kernel<<<g, 64 * K0_WAVES, 0, st>>>(P, n_spec, mz, raw_off, pk_off);
-  VH_CHECK_LAUNCH();
-  int rc = vh_scan_inclusive_i64(pk_off + 1, n_spec, ws, ws_bytes, st);
+  return k0_launch(p, n_spec, mz, intensity, raw_off, pk_off, pk_mass, pk_int, tic, valid, pk_capacity, ws,
+                   ws_bytes, (hipStream_t)stream);
+}
+
+int vh_k0_noalloc(const VhParams* p, int32_t n_spec, const float* mz, const float* intensity,
+                  const int64_t* raw_off, int64_t* pk_off, float* pk_mass, float* pk_int, float* tic, uint8_t* valid,
+                  int64_t pk_capacity, void* stream) {
+  if (n_spec < 0) return VH_E_ARG;
+  if (n_spec > VH_K0_NOALLOC_MAX_SPECTRA) return VH_E_LIMIT;
+  void* ws = nullptr;
+  const int rc = k0_scratch_address(&ws);
   if (rc) return rc;
-  k0_emit_kernel<<<g, 64 * K0_WAVES, 0, st>>>(P, n_spec, mz, intensity, raw_off, pk_off, pk_mass, pk_int, tic,
-                                              valid);
-  VH_CHECK_LAUNCH();
-  return 0;
+  return k0_launch(p, n_spec, mz, intensity, raw_off, pk_off, pk_mass, pk_int, tic, valid, pk_capacity, ws,
+                   K0_SCRATCH_BYTES, (hipStream_t)stream);
 }
 
 int vh_hypotheses(const VhParams* p, int32_t n_spec, const double* precursor_mz, const int8_t* charge,
