@@ -9,7 +9,12 @@ exact rescoring -> rank 1. The index build is timed separately and not part of a
 
 Prints ONE JSON line (rank 0) with the metric, per-kernel HIP-event times, the roofline of the dominant kernel
 (algorithmic bytes per launch / its average duration) and the CPU baseline (the C++ oracle's search, OpenMP, on a
-bounded sample of the same workload).
+bounded sample of the same workload). The default line also carries:
+  * recall_vs_exhaustive: the shortlist's rank 1 against K2 over every window row on the first 5,000 spectra;
+  * prefilter: the metric's plain vs learned comparison on the phospho-scale workload (36M peptidoforms, 200k
+    spectra, index top-200): plain, learned (adaptive keep), learned literal top-16, the K1-count rule at 16 and at
+    the learned path's rescoring budget -- spectra/s, top-1 agreement with plain, rescored candidates per
+    spectrum, the MLP's MFMA roofline and a bounded-sample CPU baseline.
 """
 from __future__ import annotations
 
@@ -64,11 +69,19 @@ MLP_FLOP_PER_CANDIDATE = 2 * (8 * 256 + 256 * 256 + 256)   # 135,680
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
-    """Plain (index top-K + closure -> K2) vs learned prefilter (index top-K -> MLP -> best keep_min, then every
-    candidate within margin of the best score up to keep, + closure -> K2) vs the K1-count rule (index top-16 +
-    closure), on the same spectra; the network is trained first on separately seeded spectra of the same workload
-    (labels from the plain K2)."""
+def _take(sp, n):
+    from vulcan.gpu.host import Spectra
+    off = sp.offsets[: n + 1]
+    return Spectra(sp.run, sp.scan[:n], sp.index[:n], sp.rt[:n], sp.precursor_mz[:n], sp.charge[:n],
+                   sp.mz[: off[-1]], sp.intensity[: off[-1]], off.copy())
+
+
+def prefilter_leg(a, w, l1, lab, gi, p, raw, n, sp=None):
+    """Plain vs learned prefilter on the same spectra. Plain: index top-K (200) + closure -> K2. Learned: index
+    top-K -> MLP -> the best keep_min by score, then every candidate within `margin` of the best up to `keep`, +
+    closure -> K2 (the default rule); also the literal top-16 by score. Ablations at the same index top-K: the
+    K1-count rule at K = 16 (the configuration's literal budget) and at the learned path's mean kept count. The
+    network is trained first on separately seeded spectra of the same workload (labels from the plain K2)."""
     import numpy as np
     import torch
 
@@ -78,7 +91,7 @@ def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
     pp = PF.PrefilterParams(index_topk=p.topk, keep=a.keep, keep_min=a.keep_min, margin=a.margin,
                             steps=a.train_steps)
     t0 = time.perf_counter()
-    sp_tr, _, _ = synth.spectra(l1, lab, a.workload, n=a.n_train, seed=900 + a.workload)
+    sp_tr, _, _ = synth.spectra(l1, lab, w.key, n=a.n_train, seed=900 + w.key)
     raw_tr = E.upload_spectra(sp_tr, gi.dev)
     x, y, _, _ = PF.training_data(gi, p, raw_tr, pp)
     torch.cuda.synchronize()
@@ -89,14 +102,11 @@ def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
     train_s = [v for t, v in model.history if t == "train_s"][0]
     log(f"trained {pp.steps} steps in {train_s:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
     del raw_tr, x, y
-    q_plain = type(p)(**{**p.to_dict(), "topk": pp.index_topk, "pair_closure": True})
-    q_rule = type(p)(**{**p.to_dict(), "topk": 16, "pair_closure": True})
-    out = {}
-    rows = {}
-    for name, mk in (("plain", lambda: E.DeviceSearch(gi, q_plain, n)),
-                     ("learned", lambda: PF.LearnedSearch(gi, p, n, model, pp)),
-                     ("count_rule", lambda: E.DeviceSearch(gi, q_rule, n))):
-        obj = mk()
+    q = lambda k: type(p)(**{**p.to_dict(), "topk": k, "pair_closure": True})  # noqa: E731
+    pp16 = PF.PrefilterParams(**{**pp.__dict__, "keep": 16, "keep_min": 16, "margin": 0.0})
+    out, rows = {}, {}
+
+    def measure(name, obj):
         dt, st = time_steps(obj, raw, a.steps, a.warmup)
         if int(obj.status.item()) != 0:
             raise RuntimeError(f"{name}: k1 status {int(obj.status.item())}")
@@ -104,28 +114,85 @@ def prefilter_leg(a, w, l1, lab, gi, p, raw, n):
         nc = obj.n_cand[:n].cpu().numpy()
         out[name] = {"spectra_per_s": n / dt, "ms_per_step": dt * 1e3, "stage_ms": st,
                      "rescored_per_spectrum": float(nc.mean())}
-        if name == "learned":
-            scored = int(obj.lists.n_cand[:n].sum().item())
-            mlp_s = st["mlp"] * 1e-3
-            fl = MLP_FLOP_PER_CANDIDATE * scored / mlp_s / 1e12
-            out[name]["mlp_roofline"] = {"bound": "mfma", "achieved": fl, "peak": MFMA_BF16_PEAK_TFLOPS,
-                                         "unit": "TFLOP/s", "frac": fl / MFMA_BF16_PEAK_TFLOPS,
-                                         "candidates_scored": scored,
-                                         "flop_per_candidate": MLP_FLOP_PER_CANDIDATE}
-            learned_obj = obj
-        else:
-            del obj
-        log(f"{name}: {n / dt:,.0f} spectra/s ({dt * 1e3:.2f} ms/step)")
-    ag = {k: float(np.mean(rows[k][rows["plain"] >= 0] == rows["plain"][rows["plain"] >= 0]))
-          for k in ("learned", "count_rule")}
+        log(f"{name}: {n / dt:,.0f} spectra/s ({dt * 1e3:.2f} ms/step), {nc.mean():.1f} rescored per spectrum")
+        return st
+
+    plain = E.DeviceSearch(gi, q(pp.index_topk), n)
+    measure("plain", plain)
+    plain_vp = plain.vp
+    del plain
+    ls = PF.LearnedSearch(gi, p, n, model, pp)
+    st = measure("learned", ls)
+    scored = int(ls.lists.n_cand[:n].sum().item())
+    fl = MLP_FLOP_PER_CANDIDATE * scored / (st["mlp"] * 1e-3) / 1e12
+    out["learned"]["mlp_roofline"] = {"bound": "mfma", "kernel": "prefilter_score_kernel", "achieved": fl,
+                                      "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                      "frac": fl / MFMA_BF16_PEAK_TFLOPS, "candidates_scored": scored,
+                                      "flop_per_candidate": MLP_FLOP_PER_CANDIDATE}
+    del ls
+    measure("learned_top16", PF.LearnedSearch(gi, p, n, model, pp16))
+    measure("count_rule_16", E.DeviceSearch(gi, q(16), n))
+    k_eq = max(1, int(round(out["learned"]["rescored_per_spectrum"] / 2)))
+    measure("count_rule_budget", E.DeviceSearch(gi, q(k_eq), n))
+    out["count_rule_budget"]["topk"] = k_eq
+    ref = rows["plain"]
+    ag = {k: PF.agreement(ref, v) for k, v in rows.items() if k != "plain"}
+    for k, v in ag.items():
+        out[k]["top1_agreement"] = v
     res = {"index_topk": pp.index_topk, "keep_max": pp.keep, "keep_min": pp.keep_min, "margin": pp.margin,
-           "pair_closure": True,
-           "top1_agreement": ag["learned"], "count_rule_top1_agreement": ag["count_rule"],
-           "speedup_vs_plain": out["learned"]["spectra_per_s"] / out["plain"]["spectra_per_s"],
+           "pair_closure": True, "top1_agreement": ag["learned"], "speedup_vs_plain":
+           out["learned"]["spectra_per_s"] / out["plain"]["spectra_per_s"],
+           "meets_agreement_bar": ag["learned"] >= 0.995,
            "train": {"spectra": a.n_train, "steps": pp.steps, "batch": pp.batch, "train_s": train_s,
                      "loss_first": losses[0], "loss_last": losses[-1], "labels": "K2 hyperscore - spectrum best"},
            **out}
-    return res, learned_obj
+    res["cpu_baseline"] = None
+    if sp is not None and not a.no_cpu:
+        log("prefilter workload cpu baseline (plain top-200 + closure) ...")
+        cb, cres, ncpu = cpu_baseline(plain_vp, l1, sp, a.cpu_seconds / 2)
+        res["cpu_baseline"] = cb
+        res["plain_rank1_equal_cpu_sample"] = bool(np.array_equal(ref[:ncpu], cres["row"][:ncpu]))
+    return res
+
+
+def prefilter_block(a):
+    """The phospho-scale prefilter workload (index top-200, STY phospho <= 3 + Met-ox <= 2, 200k spectra of
+    charges 2-6 / 20-500 peaks) built and compared as prefilter_leg does, for the default line."""
+    import torch
+
+    from vulcan.gpu import engine as E
+    from vulcan.index import synth
+    w = synth.WORKLOADS[4]
+    t0 = time.perf_counter()
+    l1, lab = synth.build_index(4)
+    sp, _, _ = synth.spectra(l1, lab, 4, n=a.prefilter_spectra)
+    log(f"prefilter workload: {len(l1)} peptidoforms, {len(sp)} spectra ({time.perf_counter() - t0:.1f} s)")
+    p = w.search_params()
+    gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
+    raw = E.upload_spectra(sp, gi.dev)
+    res = prefilter_leg(a, w, l1, lab, gi, p, raw, len(sp), sp)
+    res["workload"] = w.name
+    res["n_spectra"] = len(sp)
+    res["n_peptidoforms"] = len(l1)
+    res["n_postings"] = gi.n_postings
+    del gi, raw
+    torch.cuda.empty_cache()
+    return res
+
+
+def recall_vs_exhaustive(gi, l1, p, sp, rows, n_sub: int):
+    """Shortlist recall: over the first n_sub spectra, the fraction whose exhaustive rank 1 (K2 over every
+    window row, search_exhaustive) is the shortlist's rank 1 (`rows`)."""
+    import numpy as np
+
+    from vulcan.gpu import engine as E
+    sub = _take(sp, min(n_sub, len(sp)))
+    t0 = time.perf_counter()
+    tbl, _, dbg = E.search_exhaustive(sub, gi, l1, p)
+    ex = dict(zip(tbl.column("spec_idx").to_pylist(), tbl.column("pf_row").to_pylist()))
+    hit = sum(int(rows[s] == r) for s, r in ex.items())
+    return {"spectra": len(sub), "with_exhaustive_psm": len(ex), "window_rows_rescored": int(len(dbg["r"])),
+            "recall": hit / max(len(ex), 1), "seconds": time.perf_counter() - t0}
 
 
 def uneven_leg(a, w, l1, gi, p, sp, raw, n):
@@ -258,6 +325,44 @@ def cpu_baseline(vp, l1, sp, target_s: float):
             "sample": f"first {n} of {len(sp)} spectra of the same workload, {dt:.1f} s, OpenMP C++ oracle "
                       f"(K0, hypotheses, per-row K1, top-K + closure, K2, rank 1)",
             "cpu_model": _cpu_model()}, res, n
+
+
+def cpu_baseline_modes(vp, l1, sp, maxh: int, target_s: float):
+    """The oracle's component functions in pipeline order for the open / mass-offset search (K0, mode hypotheses,
+    per-row K1 counts of every window row, top-K + closure, shifted K2, rank 1), OpenMP inside each, on a
+    bounded prefix of the same spectra."""
+    import numpy as np
+
+    from oracle import oracle as O
+    hix = O.HostIndex(l1)
+
+    def run(sub):
+        pk = O.k0(vp, sub)
+        hyp = O.hypotheses(vp, sub, pk["valid"], l1.mass32, maxh)
+        k1 = O.k1_counts(vp, hix, pk, hyp)
+        stride = 2 * vp.topk if vp.pair_closure else vp.topk
+        tk = O.topk(vp, hix, hyp, k1, stride)
+        nc = tk["n_cand"]
+        m = (np.arange(stride)[None, :] < nc[:, None]).ravel()
+        rows, hyps = tk["row"][m], tk["hyp"][m]
+        shift = np.where(hyp["mode"][hyps] == 0, np.float32(0),
+                         (hyp["M"][hyps] - l1.mass32[rows]).astype(np.float32)).astype(np.float32)
+        start = np.concatenate([[0], np.cumsum(nc)[:-1]]).astype(np.int64)
+        k2 = O.k2(vp, hix, pk, rows, hyps, hyp["maxfz"], grp_start=start, grp_count=nc, shift=shift)
+        sel, _ = O.rank(vp, start, nc, rows, k2["hyperscore"], k2["matched_b"], k2["matched_y"], 1)
+        return np.where(sel >= 0, rows[np.maximum(sel, 0)], -1), int(k1["win_off"][-1])
+    n0 = min(20, len(sp))
+    t0 = time.perf_counter()
+    run(_take(sp, n0))
+    dt = time.perf_counter() - t0
+    n = int(min(len(sp), max(n0, n0 * target_s / max(dt, 1e-6))))
+    t0 = time.perf_counter()
+    rows, n_win = run(_take(sp, n))
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "spectra/s", "cores": int(O.lib().or_max_threads()), "kind": "port",
+            "sample": f"first {n} of {len(sp)} spectra of the same workload, {dt:.1f} s, {n_win} window rows; "
+                      f"OpenMP C++ oracle functions in pipeline order (K0, mode hypotheses, per-row K1, top-K + "
+                      f"closure, shifted K2, rank 1)", "cpu_model": _cpu_model()}, rows, n
 
 
 def gbm_leg(a):
@@ -406,6 +511,12 @@ def main():
     ap.add_argument("--keep-min", type=int, default=8, help="prefilter: always keep the best this many by score")
     ap.add_argument("--margin", type=float, default=3.0,
                     help="prefilter: also keep candidates scoring within this margin of the best")
+    ap.add_argument("--no-prefilter", action="store_true",
+                    help="default workload: leave out the plain vs learned comparison on the phospho workload")
+    ap.add_argument("--prefilter-spectra", type=int, default=200_000,
+                    help="spectra of the phospho workload in the default line's prefilter comparison")
+    ap.add_argument("--recall-spectra", type=int, default=5000,
+                    help="spectra whose shortlist rank 1 is checked against exhaustive rescoring (0: skip)")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--n-train", type=int, default=20000, help="prefilter: training spectra (own seed)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -492,7 +603,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
     if a.prefilter or w.leg == "prefilter":
-        res, ls = prefilter_leg(a, w, l1, lab, gi, p, raw, n)
+        res = prefilter_leg(a, w, l1, lab, gi, p, raw, n, sp)
         L = res["learned"]
         st = L["stage_ms"]
         mr = L["mlp_roofline"]
@@ -507,7 +618,7 @@ def main():
                           "parallelism": "single-gpu"},
                "stage_ms": st, "index_build_s": gi.build_s,
                "roofline": dict(mr, kernel="prefilter_score", traffic=None),
-               "prefilter": res, "cpu_baseline": None}
+               "prefilter": res, "cpu_baseline": res.pop("cpu_baseline", None)}
         print(json.dumps(out), flush=True)
         return
     ds = E.DeviceSearch(gi, p, n, **modes)
@@ -555,14 +666,26 @@ def main():
            "counts": ab, "top1_truth_recovery": recovery}
     if modes:
         out["config"]["open_da"] = list(modes["open_da"])
-    if not a.no_cpu and not modes:
+    out["cpu_baseline"] = None
+    if not a.no_cpu:
         log("cpu baseline ...")
-        cb, res, ncpu = cpu_baseline(ds.vp, l1, sp, a.cpu_seconds)
+        if modes:
+            from vulcan.gpu._abi import max_hyp_per_spectrum
+            cb, crow, ncpu = cpu_baseline_modes(ds.vp, l1, sp, max_hyp_per_spectrum(p, 0, True), a.cpu_seconds)
+        else:
+            cb, res, ncpu = cpu_baseline(ds.vp, l1, sp, a.cpu_seconds)
+            crow = res["row"]
         out["cpu_baseline"] = cb
         if not a.no_check:
-            out["rank1_equal_cpu_sample"] = bool(np.array_equal(best[:ncpu], res["row"][:ncpu]))
-    else:
-        out["cpu_baseline"] = None
+            out["rank1_equal_cpu_sample"] = bool(np.array_equal(best[:ncpu], crow[:ncpu]))
+    if not modes and a.recall_spectra > 0:
+        log("shortlist recall vs exhaustive rescoring ...")
+        out["recall_vs_exhaustive"] = recall_vs_exhaustive(gi, l1, p, sp, best, a.recall_spectra)
+        log(f"recall {out['recall_vs_exhaustive']['recall']:.4f}")
+    if a.workload == 1 and not a.no_prefilter:
+        del ds, gi, raw
+        torch.cuda.empty_cache()
+        out["prefilter"] = prefilter_block(a)
     print(json.dumps(out), flush=True)
 
 

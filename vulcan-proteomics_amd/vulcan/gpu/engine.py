@@ -139,6 +139,11 @@ class GpuIndex:
         return self._bin["postings"]
 
     @property
+    def n_slices(self) -> int:
+        """Posting directories of this index: one (the whole index is resident)."""
+        return 1
+
+    @property
     def slices(self):
         self.tile_view()
         return [Slice(0, self.n_buckets, self._bin["offsets"], self._bin["postings"])]

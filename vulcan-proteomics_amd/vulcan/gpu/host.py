@@ -1,9 +1,9 @@
 """Torch-free host-side pieces of the closed search: constants, SearchParams, Spectra, precursor hypotheses.
 
-Mirrors vulcan/gpu/host.py:10-105 of the reference (same names, fields and arithmetic), so decode workers can
-build search-ready arrays without importing torch. The device path computes the same hypotheses with the
-`vh_hypotheses` HIP kernel (include/vulcan_hip.h); this numpy version is kept for callers that precompute
-hypotheses on the host (reference `pre_hypotheses`, host.py:94-105).
+Same names, fields and arithmetic as the reference's vulcan/gpu/host.py:10-105, so decode workers can build
+search-ready arrays without importing torch. The device path computes the same hypotheses with the
+`vh_hypotheses` HIP kernel (include/vulcan_hip.h); this numpy version serves callers that precompute hypotheses
+on the host (`pre_hypotheses`).
 """
 from __future__ import annotations
 
@@ -88,44 +88,56 @@ class Spectra:
         return len(self.scan)
 
 
+def _hypothesis_grid(spec: np.ndarray, charge: np.ndarray, p: SearchParams):
+    """(spectrum, charge, isotope) of every precursor hypothesis, grouped by spectrum, charge ascending, then
+    isotope ascending: a spectrum with a known charge (> 0) tries that charge only, the others every charge of
+    p.precursor_charge; every charge takes every isotope error of p.isotope_errors."""
+    z_lo, z_hi = (int(x) for x in p.precursor_charge)
+    i_lo, i_hi = (int(x) for x in p.isotope_errors)
+    n_iso = max(i_hi - i_lo + 1, 0)
+    z_known = charge[spec].astype(np.int64)
+    per = np.where(z_known > 0, 1, max(z_hi - z_lo + 1, 0)) * n_iso
+    total = int(per.sum())
+    if total == 0:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, np.int64)
+    start = np.cumsum(per) - per
+    j = np.arange(total) - np.repeat(start, per)           # position inside the spectrum's block
+    zk = np.repeat(z_known, per)
+    z = np.where(zk > 0, zk, z_lo + j // n_iso)
+    return np.repeat(spec, per), z, i_lo + j % n_iso
+
+
+def _precursor_window(pmz: np.ndarray, z: np.ndarray, iso: np.ndarray, p: SearchParams):
+    """Sage's f32 precursor arithmetic, one rounding per operation: m/z - proton, times charge, minus the isotope
+    shift, then the ppm bounds m + (m * ppm) / 1e6."""
+    f32 = np.float32
+    neutral = pmz.astype(f32) - PROTON                       # f32 - f32
+    pm = neutral * z.astype(f32)
+    M = pm - iso.astype(f32) * NEUTRON
+    bounds = [M + (M * f32(t)) / f32(1e6) for t in p.precursor_ppm]
+    return pm.astype(f32), M.astype(f32), bounds[0].astype(f32), bounds[1].astype(f32)
+
+
 def make_hypotheses(sp: Spectra, valid: np.ndarray, idx_mass32: np.ndarray, p: SearchParams):
-    """Precursor hypotheses (spectrum, charge, isotope) and candidate row windows, Sage f32 arithmetic
-    (host.py:64-91): order (spectrum, charge, isotope); window rows by searchsorted left/right on mass32."""
-    spec = np.nonzero(np.asarray(valid, bool) & np.isfinite(sp.precursor_mz))[0]
-    zi = sp.charge[spec].astype(np.int64)
-    known = zi > 0
-    zr = np.arange(p.precursor_charge[0], p.precursor_charge[1] + 1)
-    isos = np.arange(p.isotope_errors[0], p.isotope_errors[1] + 1)
-    parts_s, parts_z, parts_i = [], [], []
-    for iso in isos:
-        parts_s.append(spec[known]); parts_z.append(zi[known]); parts_i.append(np.full(known.sum(), iso))
-        for z in zr:
-            parts_s.append(spec[~known]); parts_z.append(np.full((~known).sum(), z))
-            parts_i.append(np.full((~known).sum(), iso))
-    s = np.concatenate(parts_s) if parts_s else np.zeros(0, np.int64)
-    z = np.concatenate(parts_z) if parts_z else np.zeros(0, np.int64)
-    iso = np.concatenate(parts_i) if parts_i else np.zeros(0, np.int64)
-    o = np.lexsort((iso, z, s))
-    s, z, iso = s[o], z[o], iso[o]
-    mz = (sp.precursor_mz[s].astype(np.float32) - PROTON).astype(np.float32)
-    pm = (mz * z.astype(np.float32)).astype(np.float32)
-    M = (pm - (iso.astype(np.float32) * NEUTRON).astype(np.float32)).astype(np.float32)
-    plo, phi = np.float32(p.precursor_ppm[0]), np.float32(p.precursor_ppm[1])
-    lo = (M + (M * plo) / np.float32(1e6)).astype(np.float32)
-    hi = (M + (M * phi) / np.float32(1e6)).astype(np.float32)
-    rlo = np.searchsorted(idx_mass32, lo, side="left")
-    rhi = np.searchsorted(idx_mass32, hi, side="right")
+    """Precursor hypotheses and their candidate row windows (host.py:64-91): every valid spectrum with a finite
+    precursor, grouped by spectrum, charge, isotope; the window is the rows whose mass32 lies in [lo, hi]
+    (searchsorted left / right on the mass-sorted table)."""
+    spec = np.flatnonzero(np.asarray(valid, bool) & np.isfinite(sp.precursor_mz))
+    s, z, iso = _hypothesis_grid(spec, np.asarray(sp.charge), p)
+    pm, M, lo, hi = _precursor_window(np.asarray(sp.precursor_mz, np.float64)[s], z, iso, p)
     return {"spec": s.astype(np.int32), "z": z.astype(np.int8), "iso": iso.astype(np.int8), "pm": pm, "M": M,
-            "lo": rlo.astype(np.int64), "hi": rhi.astype(np.int64), "maxfz": p.maxfz_of(z)}
+            "lo": np.searchsorted(idx_mass32, lo, side="left").astype(np.int64),
+            "hi": np.searchsorted(idx_mass32, hi, side="right").astype(np.int64), "maxfz": p.maxfz_of(z)}
 
 
 def pre_hypotheses(sp: Spectra, gi, p: SearchParams):
-    """Host hypotheses computable before K0 for the global peak view (host.py:94-105)."""
-    assert p.peak_mode == "global"
-    mz = sp.mz
-    ok = (mz >= p.fragment_min_mz) & (mz <= p.fragment_max_mz) & ~np.isnan(mz)
-    cs = np.zeros(len(ok) + 1, np.int64)
-    np.cumsum(ok, out=cs[1:])
-    n_in = cs[sp.offsets[1:]] - cs[sp.offsets[:-1]]
+    """(valid, hypotheses) computable on the host before K0, e.g. while a run is loaded (host.py:94-105): with
+    the global peak view a spectrum is valid when min(#peaks with fragment_min_mz <= m/z <= fragment_max_mz,
+    max_peaks) >= min_peaks. `engine.search(pre_hyp=...)` uses them only when this mask equals K0's."""
+    if p.peak_mode != "global":
+        raise ValueError("pre_hypotheses needs the global peak view")
+    mz = np.asarray(sp.mz)
+    kept = np.flatnonzero((mz >= p.fragment_min_mz) & (mz <= p.fragment_max_mz))   # NaN compares false
+    n_in = np.diff(np.searchsorted(kept, np.asarray(sp.offsets)))
     valid = np.minimum(n_in, p.max_peaks) >= p.min_peaks
     return valid, make_hypotheses(sp, valid, gi.mass32_np, p)
