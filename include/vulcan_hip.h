@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 16
+#define VH_ABI_VERSION 17
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -116,8 +116,8 @@ typedef struct VhBlockView {
 
 /* ---------------------------------------------------------------------------------------------- misc */
 int vh_version(void);
-/* sizeof(VhParams) (which = 0), VhIndexView (1), VhBlockView (2), VhK2Cols (3), VhPsmCols (4) or VhGbmNode (5),
- * to check a binding's struct layout. */
+/* sizeof(VhParams) (which = 0), VhIndexView (1), VhBlockView (2), VhK2Cols (3), VhPsmCols (4), VhGbmNode (5) or
+ * VhFamilyCols (6), to check a binding's struct layout. */
 size_t vh_abi_sizeof(int which);
 /* Scratch bytes of the prefix scans used inside vh_k0 / vh_hypotheses / vh_frag_count for n items. */
 size_t vh_scan_workspace_bytes(int64_t n);
@@ -311,7 +311,8 @@ int vh_k1j_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, co
  * (ties: the later peak); matched counts, summed intensities, intensity-weighted ppm, longest consecutive
  * ladders; hyperscore = ln((sb+1)(sy+1)) + lnfact(mb) + lnfact(my) (255 if not finite).
  * Groups of candidates share a spectrum: group g has spectrum grp_spec[g] (NULL: g), first candidate
- * grp_start[g] (NULL: g*stride) and grp_count[g] candidates. maxfz of a candidate = h_maxfz[c_hyp[c]].
+ * grp_start[g] (NULL: g*stride) and grp_count[g] candidates (grp_count NULL: grp_start holds n_groups + 1
+ * offsets and group g has grp_start[g + 1] - grp_start[g]). maxfz of a candidate = h_maxfz[c_hyp[c]].
  * c_shift (NULL = closed search) = observed - candidate mass: a candidate with a nonzero shift scores shifted +
  * unshifted ions and the delta is placed on the residue that maximises the hyperscore (vk_k2_shift,
  * kernels.cu:291-363, 461-472; peptides up to 63 residues). One wave64 per group; `order` (NULL =
@@ -470,29 +471,85 @@ typedef struct {
  * VH_GBM_CHUNK_NODES nodes. max_depth: the deepest leaf's depth (walks stop after max_depth + 1 nodes).
  * status (zeroed by the caller): bit 1 = a walk did not end on a leaf within max_depth + 1 steps (a malformed
  * model: indexes beyond the chunk / features beyond n_features are clamped, so every read stays in bounds, but
- * the sample's sum is then wrong). */
+ * the sample's sum is then wrong); bit 2 = the chunk table itself is malformed (a chunk of more than
+ * VH_GBM_CHUNK_NODES nodes, node ranges outside [0, n_nodes) or not increasing, tree ranges outside
+ * [0, n_trees)): it is clamped into bounds, so every read stays in bounds, and the sums are then wrong. */
 int vh_gbm_decision(const VhGbmNode* nodes, int64_t n_nodes, const int32_t* tree_root, int32_t n_trees,
                     const int32_t* chunk_node0, const int32_t* chunk_tree0, int32_t n_chunks, int32_t max_depth,
                     double baseline, const double* X, int64_t n, int32_t n_features, double* out, int32_t* status,
                     void* stream);
 
-/* ---- family top-2 retention (replaces the torch select_family of vulcan/gpu/family.py:90-136) ---- */
-/* Per spectrum, from the bucket-join K1's window counts (vh_k1j_pairs / vh_k1j_count, mode 1: u32 per slot):
- * the window rows with count >= p->k1_threshold; the best p->topk of every K1 family kfam[row] by (count desc,
- * row desc, hypothesis asc); each kept row plus its pair partner in each of the n_sets decoy sets
- * (partners[j * n_rows + row], count -1, not shortlisted); one entry per row in row order: the spectrum's
- * lowest hypothesis whose window holds the row (else the lowest source hypothesis), the highest count,
- * shortlisted if any source was. mode 0: out_off[0..n_spec] = exclusive offsets of the spectra's entries (the
- * total at out_off[n_spec]); mode 1: the entries at out_off[s] (o_row, o_hyp, o_cnt, o_top), out_off from mode
- * 0. status (zeroed by the caller) bit 1: a spectrum exceeded 64 hypotheses, 256 rows over the threshold or
- * 1,024 shortlist + partner entries, or a window lay beyond slot_cap (its list is then incomplete).
- * n_rows < 2^28, 1 <= n_sets <= 8. */
+/* ---- family top-2 retention (replaces the torch select_family / family_table of vulcan/gpu/family.py:90-203) -- */
+/* Shortlist per spectrum, from the bucket-join K1's window counts (vh_k1j_pairs / vh_k1j_count, mode 1: u32 per
+ * slot): the window rows with count >= p->k1_threshold; the best p->topk of every K1 family kfam[row] by (count
+ * desc, row desc, hypothesis asc); each kept row plus its pair partner in each of the n_sets decoy sets
+ * (partners[j * n_rows + row], count -1, not shortlisted); one entry per row in row order: the spectrum's lowest
+ * hypothesis whose window holds the row (else the lowest source hypothesis), the highest count (clamped at
+ * 32,767), shortlisted if any source was.
+ * mode 0: out_off[s + 1] = the spectrum's entry count (out_off[0] = 0); a spectrum whose rows over the threshold
+ *   (more than 256) or shortlist + partner entries (more than 1,024) do not fit one wave's LDS gets count 0 and
+ *   is appended to `wide` (wide[0], zeroed by the caller, = the number listed; wide[1 + i] = the spectra): the
+ *   vh_family_wide_* calls below compute those spectra and write their counts;
+ * mode 2: out_off[1..n_spec] scanned into exclusive offsets (the total at out_off[n_spec]); needs ws;
+ * mode 1: the entries of the spectra that fit at out_off[s] (o_row, o_hyp, o_cnt, o_top); vh_family_wide_copy
+ *   writes the wide ones.
+ * status (zeroed by the caller) bit 1: a spectrum has more than 64 hypotheses, or a window lay beyond slot_cap
+ * (its list is then incomplete). n_rows < 2^28, 1 <= n_sets <= 8. */
 size_t vh_family_select_workspace_bytes(int32_t n_spec);
 int vh_family_select(const VhParams* p, int32_t n_spec, const int64_t* hyp_off, const int64_t* h_lo,
                      const int64_t* h_hi, const int64_t* h_slot, int64_t slot_cap, const uint32_t* counts,
                      const uint8_t* kfam, int64_t n_rows, const int64_t* partners, int32_t n_sets, int32_t mode,
-                     int64_t* out_off, int32_t* o_row, int32_t* o_hyp, int32_t* o_cnt, uint8_t* o_top,
+                     int64_t* out_off, int32_t* o_row, int32_t* o_hyp, int32_t* o_cnt, uint8_t* o_top, int32_t* wide,
                      int32_t* status, void* ws, size_t ws_bytes, void* stream);
+/* Wide path, for the n_wide spectra wide_spec[0..n_wide) that mode 0 listed. 1. vh_family_wide_stats: w_off
+ * (n_wide + 1) = exclusive offsets of their rows over the threshold, rmin[q] = the lowest such row, gstats[0] =
+ * the largest count (clamped at 32,767), gstats[1] = the largest row span. 2. vh_family_wide: the same shortlist
+ * rule as vh_family_select through two device radix sorts (keys packed with field widths fitted to cmax / span /
+ * n_kfam, the number of K1 families; batches of at most 8,192 spectra); w_off_host = a host copy of w_off;
+ * writes each spectrum's entries packed into uniq (capacity (n_sets + 1) * w_off[n_wide]) at uniq_off[q],
+ * uniq_cnt[q] of them, and the count to out_off[wide_spec[q] + 1] (before vh_family_select mode 2).
+ * 3. vh_family_wide_copy (after mode 2): the packed entries into o_row / o_hyp / o_cnt / o_top at out_off. */
+size_t vh_family_wide_stats_workspace_bytes(int32_t n_wide);
+int vh_family_wide_stats(const VhParams* p, int32_t n_wide, const int32_t* wide_spec, const int64_t* hyp_off,
+                         const int64_t* h_lo, const int64_t* h_hi, const int64_t* h_slot, int64_t slot_cap,
+                         const uint32_t* counts, int64_t n_rows, int64_t* w_off, int64_t* rmin, uint64_t* gstats,
+                         void* ws, size_t ws_bytes, void* stream);
+size_t vh_family_wide_workspace_bytes(int64_t n_entries, int32_t n_sets);
+int vh_family_wide(const VhParams* p, int32_t n_wide, const int32_t* wide_spec, const int64_t* hyp_off,
+                   const int64_t* h_lo, const int64_t* h_hi, const int64_t* h_slot, int64_t slot_cap,
+                   const uint32_t* counts, const uint8_t* kfam, int32_t n_kfam, int64_t n_rows,
+                   const int64_t* partners, int32_t n_sets, const int64_t* w_off, const int64_t* w_off_host,
+                   const int64_t* rmin, uint64_t cmax, uint64_t span, uint64_t* uniq, int64_t* uniq_off,
+                   int64_t* uniq_cnt, int64_t* out_off, void* ws, size_t ws_bytes, void* stream);
+int vh_family_wide_copy(int32_t n_wide, const int32_t* wide_spec, const int64_t* hyp_off, const int64_t* uniq_off,
+                        const int64_t* uniq_cnt, const uint64_t* uniq, const int64_t* out_off, int32_t* o_row,
+                        int32_t* o_hyp, int32_t* o_cnt, uint8_t* o_top, void* stream);
+/* Top-2 table (family_table, family.py:160-203): per spectrum s, over its shortlist [cand_off[s],
+ * cand_off[s + 1]) (rows ascending, vh_family_select's output order; c_hyp = the entries' hypotheses) with K2
+ * results hs / mb / my per entry: an entry with mb + my >= p->min_matched_peaks is admissible in decoy context j
+ * when it is shortlisted or the set-j partner of its row is a shortlisted entry of the spectrum; forward rows
+ * (side < 0) join group (fwd_fam * n_sets + j) * 2, rows of decoy set j group (fwd_fam * n_sets + j) * 2 + 1; per
+ * group the best two by (hyperscore desc, row asc). mode 0 / 2 / 1 as vh_family_select: counts, offsets, then
+ * the table rows in (group, rank) order into `cols`: the entry's index, group, rank (1 or 2), row, hypothesis,
+ * hyperscore, matched b / y and the hypothesis' charge / isotope (h_z / h_iso). n_fwd_fam * n_sets * 2 <= 256. */
+typedef struct VhFamilyCols {
+  int64_t* idx;
+  int16_t* group;
+  int8_t* rank;
+  int32_t* row;
+  int32_t* hyp;
+  double* hs;
+  int16_t* mb;
+  int16_t* my;
+  int8_t* z;
+  int8_t* iso;
+} VhFamilyCols;
+size_t vh_family_top2_workspace_bytes(int32_t n_spec);
+int vh_family_top2(const VhParams* p, int32_t n_spec, const int64_t* cand_off, const int32_t* c_row,
+                   const int32_t* c_hyp, const uint8_t* c_top, const double* hs, const int16_t* mb, const int16_t* my,
+                   const int8_t* h_z, const int8_t* h_iso, const int8_t* fwd_fam, const int8_t* side, int64_t n_rows,
+                   const int64_t* partners, int32_t n_sets, int32_t n_fwd_fam, int32_t mode, int64_t* out_off,
+                   const VhFamilyCols* cols, void* ws, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -45,7 +45,9 @@ def test_abi_layout_and_version():
     assert L.vh_abi_sizeof(4) == C.sizeof(VhPsmCols) == 24 * 8
     from vulcan.gpu._abi import GBM_NODE
     assert L.vh_abi_sizeof(5) == GBM_NODE.itemsize == 16
-    assert L.vh_abi_sizeof(6) == 0
+    from vulcan.gpu._abi import VhFamilyCols
+    assert L.vh_abi_sizeof(6) == C.sizeof(VhFamilyCols) == 80
+    assert L.vh_abi_sizeof(7) == 0
     m = re.search(r"#define VH_ABI_VERSION (\d+)", open(HEADER).read())
     assert L.vh_version() == int(m.group(1))
 
@@ -82,12 +84,24 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert L.vh_gbm_decision(1, 10, 1, 1, 1, 1, 1, 3, 0.0, 1, 5, 7, 1, None, None) == -1
     assert L.vh_gbm_decision(1, 10, 1, 1, 1, 1, 1, 3, 0.0, 1, 5, 64, 1, 1, None) == -4
     assert L.vh_gbm_decision(1, 10, 1, 1, 1, 1, 1, -1, 0.0, 1, 5, 7, 1, 1, None) == -1
-    # family select: bad mode / decoy-set count / table too large / workspace short; n_spec 0 needs no inputs
-    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 2, 2, *[1] * 6, 1, 0, None) == -1
-    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 0, 0, *[1] * 6, 1, 0, None) == -1
-    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 1 << 28, 1, 2, 0, *[1] * 6, 1, 0, None) == -4
-    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 2, 0, *[1] * 6, 1, 0, None) == -3
+    # family select: bad mode / decoy-set count / table too large / scan workspace short / no wide list;
+    # pointers are never dereferenced (every call returns before a launch)
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 2, 3, *[1] * 7, 1, 0, None) == -1
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 0, 0, *[1] * 7, 1, 0, None) == -1
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 1 << 28, 1, 2, 0, *[1] * 7, 1, 0, None) == -4
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 2, 2, *[1] * 7, 1, 0, None) == -3
+    assert L.vh_family_select(C.byref(vp), 4, *[1] * 4, 10, 1, 1, 100, 1, 2, 0, *[1] * 5, None, 1, 1, 0,
+                              None) == -1
     assert L.vh_family_select_workspace_bytes(1000) > 0
+    # wide path: workspace short for the listed rows; top-2 table: too many retention groups
+    import numpy as np
+    w_host = np.array([0, 5000], np.int64)
+    assert L.vh_family_wide(C.byref(vp), 1, *[1] * 5, 10, 1, 1, 9, 100, 1, 2, 1, w_host.ctypes.data, 1, 10, 10,
+                            *[1] * 4, 1, 16, None) == -3
+    assert L.vh_family_wide_workspace_bytes(5000, 2) > 2 * 5000 * 8 + 2 * 15000 * 8
+    assert L.vh_family_top2(C.byref(vp), 4, *[1] * 11, 100, 1, 8, 17, 0, 1, None, 1, 0, None) == -4
+    assert L.vh_family_top2(C.byref(vp), 4, *[1] * 11, 100, 1, 2, 3, 3, 1, None, 1, 0, None) == -1
+    assert L.vh_family_top2(C.byref(vp), 4, *[1] * 11, 100, 1, 2, 3, 1, 1, None, 1, 0, None) == -1   # no columns
     assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 16, 8, 0.0, 1, 32, *[None] * 4, None) == -1
     assert L.vh_prefilter_select(C.byref(ix), 1, 200, *[None] * 5, 8, 16, -1.0, 1, 32, *[None] * 4, None) == -1
     ixp = VhIndexView()

@@ -69,3 +69,31 @@ def test_device_hypotheses_equal_the_reference_rule():
             else:
                 assert np.array_equal(g.astype(np.int64), w.astype(np.int64)), k
 
+
+
+def test_pre_hypotheses_validity_equals_k0(small_plain):
+    """pre_hypotheses' host validity rule (global peak view: min(#in-range peaks, max_peaks) >= min_peaks) equals
+    the oracle K0's mask, NaN m/z and boundary peaks included, and its hypotheses are make_hypotheses' on it."""
+    from oracle import oracle as O
+    from vulcan.gpu._abi import make_params
+    from vulcan.gpu.host import pre_hypotheses
+    d = small_plain
+    sp = d["sp"]
+    rng = np.random.default_rng(5)
+    mz = sp.mz.copy()
+    spec_of = np.repeat(np.arange(len(sp)), np.diff(sp.offsets))
+    drop = rng.uniform(0.0, 1.0, len(sp))[spec_of]                 # per-spectrum NaN share: some fall below min_peaks
+    mz[rng.random(len(mz)) < drop] = np.nan
+    mz[rng.random(len(mz)) < 0.01] = np.float32(d["p"].fragment_min_mz)
+    mz[rng.random(len(mz)) < 0.01] = np.float32(d["p"].fragment_max_mz)
+    sp2 = Spectra(sp.run, sp.scan, sp.index, sp.rt, sp.precursor_mz, sp.charge, mz, sp.intensity, sp.offsets)
+
+    class _Gi:
+        mass32_np = d["l1"].mass32
+    valid, hyp = pre_hypotheses(sp2, _Gi, d["p"])
+    pk = O.k0(make_params(d["p"], d["static"], d["var"]), sp2)
+    assert np.array_equal(valid, pk["valid"].astype(bool))
+    assert 0 < valid.sum() < len(valid)
+    ref = make_hypotheses(sp2, valid, d["l1"].mass32, d["p"])
+    for k in ref:
+        assert np.array_equal(hyp[k], ref[k]), k

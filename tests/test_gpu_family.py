@@ -79,37 +79,89 @@ def test_family_rank1_is_the_plain_search_rank1_on_a_single_decoy_replicate(mult
     assert same > 0.97, same
 
 
-def test_family_select_kernel_equals_device_sorts(multi):
-    """vh_family_select on the join K1's window counts == select_family over the appended K1 list (torch
-    device sorts): the same (spectrum, row) entries, hypotheses, counts and shortlist flags; and the two search
-    paths give the same table."""
+def _oracle_shortlist(l1, fi_np, sp, p, var):
+    """The oracle's K0 / hypotheses / per-row K1 counts and its family shortlist (oracle/family.py)."""
+    from oracle import family as OF
+    from oracle import oracle as O
+    from vulcan.gpu._abi import make_params, max_hyp_per_spectrum
+    vp = make_params(p, STATIC, var)
+    hix = O.HostIndex(l1)
+    pk = O.k0(vp, sp)
+    hyp = O.hypotheses(vp, sp, pk["valid"], l1.mass32, max_hyp_per_spectrum(p))
+    k1 = O.k1_counts(vp, hix, pk, hyp)
+    fwd, side, partners = fi_np
+    kfam = fwd * (len(partners) + 1) + side + 1
+    sl = OF.shortlist(hyp, k1, kfam, partners, p.topk, p.k1_threshold)
+    return vp, hix, pk, hyp, sl
+
+
+@pytest.mark.parametrize("case", ["default", "wide"])
+def test_family_shortlist_and_table_equal_oracle(multi, case):
+    """vh_family_select (and, for spectra beyond one wave's LDS, the wide path) == the oracle's shortlist over the
+    per-row K1 counts (rows, canonical hypotheses, counts, shortlisted flags per spectrum); the top-2 table ==
+    the oracle's top-2 over the oracle's K2 of that shortlist. "wide": the reference's defaults top-K 500 and
+    K1 threshold 1 with a 300 ppm precursor window and 0.5 Da fragments, so about a quarter of the spectra have
+    more than 256 rows over the threshold."""
+    from oracle import family as OF
+    from oracle import oracle as O
     from vulcan.gpu import engine as E
     from vulcan.gpu import family as F
     from vulcan.index import synth
     l1, lab, sp = multi
     p = synth.WORKLOADS[1].search_params()
+    if case == "wide":
+        p.topk, p.k1_threshold, p.precursor_ppm, p.fragment_ppm = 500, 1, (-300.0, 300.0), (-0.5, 0.5)
     var = dict(synth.WORKLOADS[1].var_mods)
     gi = E.GpuIndex(l1, p, STATIC, var)
     fi = F.FamilyIndex.from_l1(l1, gi, lab)
+    fwd, side, _ = F.family_codes(lab)
+    partners = np.stack(l1.partners)
+    vp, hix, pk, hyp, want = _oracle_shortlist(l1, (fwd, side, partners), sp, p, var)
     n = len(sp)
-    print("[gpu] family select kernel vs sorts", flush=True)
-    st = E.Stats()
-    peaks = E.k0_preprocess(sp, p, gi.dev, st, gi)
-    hd = E.device_hypotheses(gi, peaks["raw"], peaks, p)
-    n_h = int(hd["hyp_off"][n].item())
-    hyp = {k: hd[k][:n_h] for k in ("spec", "z", "iso", "pm", "M", "lo", "hi", "maxfz")}
-    hyp["hyp_off"] = hd["hyp_off"][:n + 1]
-    o_h, o_r, o_c, _, _, d_spec, _ = E.run_k1(gi, hyp, peaks, p, st)
-    want = F.select_family(o_h, o_r, o_c, d_spec, hyp, fi, p, gi.dev)
+    print(f"[gpu] family vs oracle ({case}): {len(want['row'])} shortlist entries", flush=True)
     ds = E.DeviceSearch(gi, p, n, k1_impl="join")
     ds.front(E.upload_spectra(sp, gi.dev), n, E._marker(None))
     ds.join_front(n, E._marker(None))
     got = F.select_family_join(ds, fi, n)
-    assert len(got[0]) == len(want[0]) > 0
-    for a, b, name in zip(got, want, ("hyp", "row", "count", "shortlisted", "spectrum", "key")):
-        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy()), name
-    t1, _ = F.search_family(sp, gi, l1, fi, p, select="kernel")
-    t2, _ = F.search_family(sp, gi, l1, fi, p, select="torch")
-    assert t1.num_rows == t2.num_rows > 0
-    for name in t2.column_names:
-        assert np.array_equal(t1.column(name).to_numpy(), t2.column(name).to_numpy()), name
+    if case == "wide":
+        assert got.n_wide > 50
+    else:
+        assert got.n_wide == 0
+    assert np.array_equal(got.off.cpu().numpy(), want["off"])
+    for name, dv in (("row", got.row), ("hyp", got.hyp), ("cnt", got.cnt)):
+        assert np.array_equal(dv.cpu().numpy().astype(np.int64), want[name]), name
+    assert np.array_equal(got.top.cpu().numpy().astype(bool), want["top"])
+    # the table: oracle K2 over the oracle shortlist, oracle top-2
+    off = want["off"]
+    k2 = O.k2(vp, hix, pk, want["row"], want["hyp"], hyp["maxfz"], grp_start=off[:-1], grp_count=np.diff(off))
+    t2 = OF.top2(want, k2["hyperscore"], k2["matched_b"], k2["matched_y"], fwd, side, partners, p.min_matched_peaks)
+    tbl, st = F.search_family(sp, gi, l1, fi, p)
+    assert st.counts["n_wide"] == got.n_wide
+    assert len(t2["idx"]) == tbl.num_rows > 0
+    assert np.array_equal(tbl.column("spec_idx").to_numpy(), t2["spec"])
+    assert np.array_equal(tbl.column("fgroup").to_numpy().astype(np.int64), t2["group"])
+    assert np.array_equal(tbl.column("grank").to_numpy().astype(np.int64), t2["rank"])
+    assert np.array_equal(tbl.column("pf_row").to_numpy(), want["row"][t2["idx"]])
+    assert np.allclose(tbl.column("hyperscore").to_numpy(), k2["hyperscore"][t2["idx"]], rtol=1e-12, atol=0)
+    assert np.array_equal(tbl.column("charge").to_numpy(), hyp["z"][want["hyp"][t2["idx"]]])
+
+
+def test_family_search_parameters_change_between_calls(multi):
+    """Two family searches on the same index and batch size with different parameters (top-K, precursor
+    tolerance) each equal a search with fresh buffers: the step buffers are not reused across parameters."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu import family as F
+    from vulcan.index import synth
+    l1, lab, sp = multi
+    var = dict(synth.WORKLOADS[1].var_mods)
+    p1 = synth.WORKLOADS[1].search_params()
+    p2 = type(p1)(**{**p1.to_dict(), "topk": 7, "precursor_ppm": (-5.0, 5.0)})
+    gi = E.GpuIndex(l1, p1, STATIC, var)
+    fi = F.FamilyIndex.from_l1(l1, gi, lab)
+    a1, _ = F.search_family(sp, gi, l1, fi, p1)
+    a2, _ = F.search_family(sp, gi, l1, fi, p2)
+    for p, a in ((p1, a1), (p2, a2)):
+        fresh = F.FamilyIndex.from_l1(l1, gi, lab)
+        b, _ = F.search_family(sp, gi, l1, fresh, p)
+        assert a.equals(b)
+    assert not a1.equals(a2)

@@ -102,3 +102,27 @@ def test_trees_near_the_chunk_limit():
     assert g.n_chunks >= 5
     Xt = _test_rows(m, 7001, 6)
     assert np.array_equal(g.decision_function(Xt), m.predict(Xt))
+
+
+def test_malformed_chunk_table_is_clamped_and_reported():
+    """A chunk table whose chunk exceeds the LDS chunk size, runs past the node array or names trees outside the
+    model: the kernel clamps it into bounds (every read stays inside the arrays) and sets status bit 2."""
+    from vulcan.rescore.fast import GpuHGB
+    m = _clf(iters=20)
+    X = np.zeros((64, 7))
+    for bad in ("oversized", "past_end", "trees"):
+        g = GpuHGB(m)
+        cn0 = g.chunk_node0.cpu().numpy().copy()
+        ct0 = g.chunk_tree0.cpu().numpy().copy()
+        if bad == "oversized":
+            cn0[-1] = cn0[-2] + 5000                 # last chunk: 5,000 nodes (limit 1,024)
+        elif bad == "past_end":
+            cn0[-1] = g.M + 700                      # beyond the node array
+        else:
+            ct0[-1] = g.T + 40                       # trees beyond the model
+        g.chunk_node0 = torch.from_numpy(cn0).cuda()
+        g.chunk_tree0 = torch.from_numpy(ct0).cuda()
+        print(f"[gpu] gbm malformed chunk table: {bad}", flush=True)
+        with pytest.raises(RuntimeError, match="chunk table"):
+            g.decision_function(X)
+    assert np.array_equal(GpuHGB(m).decision_function(X), m.decision_function(X))

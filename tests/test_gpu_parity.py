@@ -434,6 +434,22 @@ def test_search_end_to_end(env0):
     assert st.counts["n_rescored"] == int(ref["n_cand"].sum())
 
 
+def test_search_with_host_hypotheses(env0):
+    """search(pre_hyp=...): host hypotheses whose validity mask equals K0's replace the device ones
+    (pre_hyp_used 1), a mismatched mask falls back to the device hypotheses (-1); the table is the same."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu.host import pre_hypotheses
+    sp = env0.d["sp"]
+    base, st0, _ = E.search(sp, env0.gi, env0.l1, env0.p)
+    assert st0.counts["pre_hyp_used"] == 0
+    pre = pre_hypotheses(sp, env0.gi, env0.p)
+    tbl, st, _ = E.search(sp, env0.gi, env0.l1, env0.p, pre_hyp=pre)
+    assert st.counts["pre_hyp_used"] == 1 and tbl.equals(base)
+    bad = (~pre[0], pre[1])
+    tbl2, st2, _ = E.search(sp, env0.gi, env0.l1, env0.p, pre_hyp=bad)
+    assert st2.counts["pre_hyp_used"] == -1 and tbl2.equals(base)
+
+
 def test_search_report5_matches_oracle_ranking(env1):
     from vulcan.gpu import engine as E
     tbl, _, dbg = E.search(env1.d["sp"], env1.gi, env1.l1, env1.p, report=5)

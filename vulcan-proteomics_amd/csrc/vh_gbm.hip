@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void gbm_kernel(const VhGbmNode* __restrict__ 
                                                   const int32_t* __restrict__ tree_root,
                                                   const int32_t* __restrict__ chunk_node0,
                                                   const int32_t* __restrict__ chunk_tree0, int n_chunks,
-                                                  double baseline, const double* __restrict__ X, int64_t n,
+                                                  int64_t n_nodes, int n_trees, double baseline, const double* __restrict__ X, int64_t n,
                                                   int nf, int max_steps, double* __restrict__ out,
                                                   int32_t* status) {
   __shared__ __align__(16) VhGbmNode sn[VH_GBM_CHUNK_NODES];
@@ -52,9 +52,22 @@ __global__ __launch_bounds__(256) void gbm_kernel(const VhGbmNode* __restrict__ 
   double acc = 0.0;
   acc += baseline;
   bool bad = false;
+  bool clamped = false;
   for (int c = 0; c < n_chunks; ++c) {
-    const int c0 = chunk_node0[c], cn = chunk_node0[c + 1] - c0;
-    const int t0 = chunk_tree0[c], t1 = chunk_tree0[c + 1];
+    // a malformed chunk table (a chunk beyond the node array or larger than the LDS chunk, trees outside
+    // [0, n_trees)) is clamped into bounds and reported through status bit 2
+    int64_t c0l = chunk_node0[c], c1l = chunk_node0[c + 1];
+    int64_t t0l = chunk_tree0[c], t1l = chunk_tree0[c + 1];
+    const int64_t c0c = c0l < 0 ? 0 : (c0l > n_nodes ? n_nodes : c0l);
+    int64_t cnl = c1l - c0c;
+    cnl = cnl < 0 ? 0 : cnl;
+    cnl = cnl > VH_GBM_CHUNK_NODES ? VH_GBM_CHUNK_NODES : cnl;
+    cnl = c0c + cnl > n_nodes ? n_nodes - c0c : cnl;
+    const int64_t t0c = t0l < 0 ? 0 : (t0l > n_trees ? n_trees : t0l);
+    const int64_t t1c = t1l < t0c ? t0c : (t1l > n_trees ? n_trees : t1l);
+    clamped |= c0c != c0l || c0c + cnl != c1l || t0c != t0l || t1c != t1l;
+    const int c0 = (int)c0c, cn = (int)cnl;
+    const int t0 = (int)t0c, t1 = (int)t1c;
     const unsigned last = (unsigned)(cn > 0 ? cn - 1 : 0);
     __syncthreads();   // the previous chunk's walks are done (and, on the first chunk, the rows are staged)
     {
@@ -101,6 +114,7 @@ __global__ __launch_bounds__(256) void gbm_kernel(const VhGbmNode* __restrict__ 
     out[s0 + tid] = acc;
     if (bad) atomicOr(status, 1);
   }
+  if (clamped && tid == 0) atomicOr(status, 2);
 }
 
 }  // namespace
@@ -117,7 +131,7 @@ extern "C" int vh_gbm_decision(const VhGbmNode* nodes, int64_t n_nodes, const in
   const int spw = gbm_spw(n_features);
   const int64_t blocks = (n + spw - 1) / spw;
   hipLaunchKernelGGL(gbm_kernel, dim3((unsigned)blocks), dim3(spw), 0, (hipStream_t)stream, nodes, tree_root,
-                     chunk_node0, chunk_tree0, n_trees ? n_chunks : 0, baseline, X, n, n_features, max_depth + 1,
+                     chunk_node0, chunk_tree0, n_trees ? n_chunks : 0, n_nodes, n_trees, baseline, X, n, n_features, max_depth + 1,
                      out, status);
   VH_CHECK_LAUNCH();
   return 0;

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
   const int g = order ? order[wv] : wv;
   const int s = grp_spec ? grp_spec[g] : g;
   const int64_t c0 = grp_start ? grp_start[g] : (int64_t)g * stride;
-  const int nc = grp_count[g];
+  const int nc = grp_count ? grp_count[g] : (int)(grp_start[g + 1] - c0);
   const int64_t q0 = pk_off[s];
   int np = (int)(pk_off[s + 1] - q0);
   np = np < VH_K2_MAX_PEAKS ? np : VH_K2_MAX_PEAKS;
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(64) void cand_shift_kernel(int n_groups, const int6
                                                         const float* __restrict__ mass32, float* c_shift) {
   const int g = blockIdx.x;
   const int64_t c0 = grp_start ? grp_start[g] : (int64_t)g * stride;
-  const int nc = grp_count[g];
+  const int nc = grp_count ? grp_count[g] : (int)(grp_start[g + 1] - c0);
   for (int j = threadIdx.x; j < nc; j += blockDim.x) {
     const int64_t c = c0 + j;
     const int h = c_hyp[c];
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void rank_kernel(DevP P, int n_groups, const i
   const int g = blockIdx.x * (blockDim.x >> 6) + w;
   if (g >= n_groups) return;
   const int64_t c0 = grp_start ? grp_start[g] : (int64_t)g * stride;
-  const int nc = grp_count[g];
+  const int nc = grp_count ? grp_count[g] : (int)(grp_start[g + 1] - c0);
   RKey prev = {0.0, 0, -2};   // idx -2: nothing taken yet
   for (int i = 0; i <= report; ++i) {
     RKey best = {0.0, 0, -1};
@@ -446,7 +446,7 @@ int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int3
           const int32_t* c_hyp, const int8_t* h_maxfz, const float* c_shift, const int64_t* pk_off,
           const float* pk_mass, const float* pk_int, double* o_hs, float* o_sb, float* o_sy, float* o_ppm,
           int16_t* o_mb, int16_t* o_my, int16_t* o_lb, int16_t* o_ly, uint32_t* queue, void* stream) {
-  if (!p || !ix || n_groups < 0 || !grp_count) return VH_E_ARG;
+  if (!p || !ix || n_groups < 0 || (!grp_count && !grp_start)) return VH_E_ARG;
   if (!grp_start && stride <= 0) return VH_E_ARG;
   if (n_groups == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
@@ -477,7 +477,7 @@ int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int3
 int vh_candidate_shift(int32_t n_groups, const int64_t* grp_start, const int32_t* grp_count, int32_t stride,
                        const int32_t* c_row, const int32_t* c_hyp, const uint8_t* h_mode, const float* h_M,
                        const float* mass32, float* c_shift, void* stream) {
-  if (n_groups < 0 || !grp_count || (!grp_start && stride <= 0)) return VH_E_ARG;
+  if (n_groups < 0 || (!grp_count && !grp_start) || (!grp_start && stride <= 0)) return VH_E_ARG;
   if (n_groups == 0) return 0;
   if (!c_row || !c_hyp || !h_mode || !h_M || !mass32 || !c_shift) return VH_E_ARG;
   cand_shift_kernel<<<(unsigned)n_groups, 64, 0, (hipStream_t)stream>>>(n_groups, grp_start, grp_count, stride, c_row,
@@ -515,7 +515,7 @@ int vh_psm_features(const VhIndexView* ix, int32_t n_groups, const int32_t* grp_
 int vh_rank(const VhParams* p, int32_t n_groups, const int64_t* grp_start, const int32_t* grp_count, int32_t stride,
             const int32_t* c_row, const double* hs, const int16_t* mb, const int16_t* my, int32_t report,
             int64_t* o_sel, double* o_next, void* stream) {
-  if (!p || n_groups < 0 || !grp_count || report < 1 || !o_sel || !o_next) return VH_E_ARG;
+  if (!p || n_groups < 0 || (!grp_count && !grp_start) || report < 1 || !o_sel || !o_next) return VH_E_ARG;
   if (!grp_start && stride <= 0) return VH_E_ARG;
   if (n_groups == 0) return 0;
   hipStream_t st = (hipStream_t)stream;

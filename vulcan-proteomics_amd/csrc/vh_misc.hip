@@ -201,6 +201,7 @@ size_t vh_abi_sizeof(int which) {
     case 3: return sizeof(VhK2Cols);
     case 4: return sizeof(VhPsmCols);
     case 5: return sizeof(VhGbmNode);
+    case 6: return sizeof(VhFamilyCols);
     default: return 0;
   }
 }

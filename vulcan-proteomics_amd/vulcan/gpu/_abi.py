@@ -49,6 +49,14 @@ class VhPsmCols(C.Structure):
     _fields_ = [(k, P) for k in PSM_COLS]
 
 
+FAMILY_COLS = ("idx", "group", "rank", "row", "hyp", "hs", "mb", "my", "z", "iso")
+
+
+class VhFamilyCols(C.Structure):
+    """Output columns of vh_family_top2 (mode 1)."""
+    _fields_ = [(k, P) for k in FAMILY_COLS]
+
+
 GBM_NODE = np.dtype([("v", "<f8"), ("left", "<i4"), ("feature", "<i2"), ("missing_left", "u1"), ("is_leaf", "u1")])
 GBM_CHUNK_NODES = 1024    # VH_GBM_CHUNK_NODES
 GBM_MAX_FEATURES = 63     # VH_GBM_MAX_FEATURES

@@ -12,7 +12,7 @@ import ctypes as C
 import torch
 
 from .. import _build
-from ._abi import VhBlockView, VhIndexView, VhK2Cols, VhParams, VhPsmCols
+from ._abi import VhBlockView, VhFamilyCols, VhIndexView, VhK2Cols, VhParams, VhPsmCols
 
 P = C.c_void_p
 I32, I64, SZ, F32, F64 = C.c_int32, C.c_int64, C.c_size_t, C.c_float, C.c_double
@@ -21,6 +21,7 @@ PIX = C.POINTER(VhIndexView)
 PBV = C.POINTER(VhBlockView)
 PK2 = C.POINTER(VhK2Cols)
 PPSM = C.POINTER(VhPsmCols)
+PFAM = C.POINTER(VhFamilyCols)
 
 _SIGS = {
     "vh_version": ([], I32),
@@ -65,7 +66,15 @@ _SIGS = {
     "vh_prefilter_select": ([PIX, I32, I32, P, P, P, P, P, I32, I32, F32, I32, I32, P, P, P, P, P], I32),
     "vh_gbm_decision": ([P, I64, P, I32, P, P, I32, I32, F64, P, I64, I32, P, P, P], I32),
     "vh_family_select_workspace_bytes": ([I32], SZ),
-    "vh_family_select": ([PP, I32, P, P, P, P, I64, P, P, I64, P, I32, I32, P, P, P, P, P, P, P, SZ, P], I32),
+    "vh_family_select": ([PP, I32, P, P, P, P, I64, P, P, I64, P, I32, I32, P, P, P, P, P, P, P, P, SZ, P], I32),
+    "vh_family_wide_stats_workspace_bytes": ([I32], SZ),
+    "vh_family_wide_stats": ([PP, I32, P, P, P, P, P, I64, P, I64, P, P, P, P, SZ, P], I32),
+    "vh_family_wide_workspace_bytes": ([I64, I32], SZ),
+    "vh_family_wide": ([PP, I32, P, P, P, P, P, I64, P, P, I32, I64, P, I32, P, P, P, C.c_uint64, C.c_uint64, P, P,
+                        P, P, P, SZ, P], I32),
+    "vh_family_wide_copy": ([I32, P, P, P, P, P, P, P, P, P, P, P], I32),
+    "vh_family_top2_workspace_bytes": ([I32], SZ),
+    "vh_family_top2": ([PP, I32, P, P, P, P, P, P, P, P, P, P, P, I64, P, I32, I32, I32, P, PFAM, P, SZ, P], I32),
 }
 
 ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "workspace too small", -4: "size limit exceeded"}

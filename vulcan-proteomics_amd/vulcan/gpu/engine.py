@@ -757,22 +757,58 @@ class DeviceSearch:
 
     def front(self, raw: dict, n: int, mark):
         """K0 peak view, precursor hypotheses, processing order."""
+        self.peak_view(raw, n, mark)
+        self.hypotheses(raw, n, mark)
+        self.processing_order(n, mark)
+
+    def peak_view(self, raw: dict, n: int, mark):
+        """vh_k0 into self.peaks."""
         L = K.lib()
-        gi, vp = self.gi, self.vp
-        st = K.stream()
-        pk, hy = self.peaks, self.hyp
+        pk = self.peaks
         mark("k0")
-        K.check(L.vh_k0(C.byref(vp), n, K.ptr(raw["mz"]), K.ptr(raw["int"]), K.ptr(raw["off"]), K.ptr(pk["off"]),
-                        K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(pk["tic"]), K.ptr(pk["valid_d"]),
-                        pk["mass"].numel(), K.ptr(pk["ws"]), pk["ws"].numel(), st), "k0")
+        K.check(L.vh_k0(C.byref(self.vp), n, K.ptr(raw["mz"]), K.ptr(raw["int"]), K.ptr(raw["off"]),
+                        K.ptr(pk["off"]), K.ptr(pk["mass"]), K.ptr(pk["int"]), K.ptr(pk["tic"]),
+                        K.ptr(pk["valid_d"]), pk["mass"].numel(), K.ptr(pk["ws"]), pk["ws"].numel(), K.stream()), "k0")
         mark("k0")
+
+    def hypotheses(self, raw: dict, n: int, mark):
+        """vh_hypotheses into self.hyp (closed, mass-offset and open hypotheses of the step's parameters)."""
+        L = K.lib()
+        gi, pk, hy = self.gi, self.peaks, self.hyp
         mark("hyp")
-        K.check(L.vh_hypotheses(C.byref(vp), n, K.ptr(raw["pmz"]), K.ptr(raw["z"]), K.ptr(pk["valid_d"]),
+        K.check(L.vh_hypotheses(C.byref(self.vp), n, K.ptr(raw["pmz"]), K.ptr(raw["z"]), K.ptr(pk["valid_d"]),
                                 K.ptr(gi.mass32), gi.n, K.ptr(hy["hyp_off"]), K.ptr(hy["spec"]), K.ptr(hy["z"]),
                                 K.ptr(hy["iso"]), K.ptr(hy["pm"]), K.ptr(hy["M"]), K.ptr(hy["lo"]), K.ptr(hy["hi"]),
                                 K.ptr(hy["maxfz"]), K.ptr(hy["mode"]), K.ptr(hy["off"]), hy["spec"].numel(),
-                                K.ptr(hy["ws"]), hy["ws"].numel(), st), "hypotheses")
+                                K.ptr(hy["ws"]), hy["ws"].numel(), K.stream()), "hypotheses")
         mark("hyp")
+
+    def load_hypotheses(self, hyp: dict, n: int):
+        """Closed hypotheses computed on the host (host.make_hypotheses / pre_hypotheses layout, grouped by
+        spectrum) copied into self.hyp instead of vh_hypotheses."""
+        if self.modes:
+            raise ValueError("host hypotheses are closed hypotheses; this step searches mass-offset / open modes")
+        s = np.asarray(hyp["spec"], np.int64)
+        nh = len(s)
+        if nh > self.hyp["spec"].numel():
+            raise ValueError("more host hypotheses than the step's capacity")
+        if nh and (np.any(np.diff(s) < 0) or s[0] < 0 or s[-1] >= n):
+            raise ValueError("host hypotheses must be grouped by spectrum and name spectra of this batch")
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(np.bincount(s, minlength=n)[:n], out=off[1:])
+        hy, dev = self.hyp, self.gi.dev
+        hy["hyp_off"][:n + 1].copy_(torch.from_numpy(off))
+        if nh:
+            for k, dt in (("spec", np.int32), ("z", np.int8), ("iso", np.int8), ("pm", np.float32),
+                          ("M", np.float32), ("lo", np.int64), ("hi", np.int64), ("maxfz", np.int8)):
+                hy[k][:nh].copy_(torch.from_numpy(np.ascontiguousarray(hyp[k], dt)).to(dev))
+            hy["mode"][:nh].zero_()
+            hy["off"][:nh].fill_(float("nan"))
+
+    def processing_order(self, n: int, mark):
+        """The spectra's processing order for the K1 / K2 launches (self.order)."""
+        L = K.lib()
+        hy, pk, st = self.hyp, self.peaks, K.stream()
         mark("order")
         if self.order_mode == "mass":
             K.check(L.vh_spectrum_order(n, self.gi.n, K.ptr(hy["hyp_off"]), K.ptr(hy["lo"]), K.ptr(self.order),
@@ -867,24 +903,39 @@ def device_table(ds: DeviceSearch, sp: Spectra, gi: GpuIndex, l1, p: SearchParam
 
 
 def search(sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int | None = None, pre_hyp=None):
-    """Full closed search of one run (reference engine.py:435-464). Returns (PSM table, stats, debug)."""
+    """Full closed search of one run (reference engine.py:435-464). Returns (PSM table, stats, debug).
+    pre_hyp: optional (valid, hypotheses) from host.pre_hypotheses; they replace the device hypotheses when
+    `valid` equals K0's validity mask (counts["pre_hyp_used"] = 1; -1 when given but mismatched, 0 when not
+    given), so the results are the same either way."""
     st = Stats()
     report = p.report_psms if report is None else report
     ru0 = resource.getrusage(resource.RUSAGE_SELF)
     t0 = time.perf_counter()
     dev = gi.dev
-    raw = upload_spectra(sp, dev, st)
-    ds = DeviceSearch(gi, p, len(sp), report=1)
-    with Timer(st, "search_device"):
-        ds.run(raw)
     n = len(sp)
+    raw = upload_spectra(sp, dev, st)
+    ds = DeviceSearch(gi, p, n, report=1)
+    mark = _marker(None)
+    with Timer(st, "search_device"):
+        ds.peak_view(raw, n, mark)
+        used = 0
+        if pre_hyp is not None:
+            valid_k0 = ds.peaks["valid_d"][:n].cpu().numpy().astype(bool)
+            used = 1 if np.array_equal(np.asarray(pre_hyp[0], bool), valid_k0) else -1
+        if used == 1:
+            ds.load_hypotheses(pre_hyp[1], n)
+        else:
+            ds.hypotheses(raw, n, mark)
+        ds.processing_order(n, mark)
+        ds.candidate_stage(n, mark)
+        ds.back(n, mark)
     with Timer(st, "features"):
         tbl, _, dbg = device_table(ds, sp, gi, l1, p, report)
     ru1 = resource.getrusage(resource.RUSAGE_SELF)
     st.wall_s["total"] = time.perf_counter() - t0
     valid = ds.peaks["valid_d"][:n].cpu().numpy().astype(bool)
     st.counts.update(n_spectra=n, n_searched=int(valid.sum()), n_hypotheses=len(dbg["hyp"]["spec"]),
-                     n_rescored=int(len(dbg["r"])), n_psms=tbl.num_rows)
+                     n_rescored=int(len(dbg["r"])), n_psms=tbl.num_rows, pre_hyp_used=used)
     st.counts["cpu_core_s"] = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
     peaks = dict(ds.peaks)
     peaks["valid"] = valid

@@ -106,7 +106,11 @@ class GpuHGB:
         return out
 
     def check(self) -> None:
-        if int(self.status.item()):
+        st = int(self.status.item())
+        if st & 2:
+            raise RuntimeError("gbm_decision: malformed chunk table (a chunk beyond the node array, larger than "
+                               f"{GBM_CHUNK_NODES} nodes, or trees outside the model)")
+        if st:
             raise RuntimeError("gbm_decision: malformed tree (a walk left its chunk or found no leaf)")
 
     def decision_function(self, X, chunk=100_000) -> np.ndarray:
