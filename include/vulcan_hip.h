@@ -305,6 +305,13 @@ int vh_k1j_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, co
                     uint32_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored, unsigned long long* hyp_matched,
                     int32_t* status, uint32_t* queue, void* stream);
 
+/* Candidates sorted by spectrum (non-decreasing h_spec[c_hyp[c]]): o_spec[c] = the candidate's spectrum (o_spec
+ * may be NULL) and o_off[s] = the first candidate of spectrum s, s = 0..n_spec (o_off[n_spec] = n_cand), the
+ * grouping vh_k2 / vh_rank take with grp_count NULL. Reference: the per-spectrum grouping run_k2 and psm_table
+ * derive from the sorted candidate list (engine.py:315-351, 366-385). */
+int vh_candidate_spectra(int64_t n_cand, const int32_t* c_hyp, const int32_t* h_spec, int32_t n_spec,
+                         int32_t* o_spec, int64_t* o_off, void* stream);
+
 /* ---------------------------------------------------------------------------------------------- K2 */
 /* Exact b/y rescoring (k2_kernel, kernels.cu:263-376): all b_k / y_k (k = 0..L-2) at fragment charges
  * 1..maxfz-1 against the spectrum's first 512 mass-sorted peaks; most intense peak in the tolerance window
@@ -377,6 +384,7 @@ typedef struct VhPsmCols {
   int32_t* k1_count;
   int32_t* scored_candidates;
   double* poisson;
+  int32_t* hyp;               /* the PSM's hypothesis (index into the hypothesis arrays) */
 } VhPsmCols;
 
 /* The feature columns of psm_table (vulcan/gpu/engine.py:361-432) for the vh_rank selections sel / next_hs of
@@ -419,6 +427,12 @@ int vh_mlp_backward(const void* blob, int64_t n, const float* x, const uint16_t*
  * theta -= lr * (m1 c1) / (sqrt(m2 c2) + eps), with c1 = 1/(1-b1^t), c2 = 1/(1-b2^t) given by the caller. */
 int vh_mlp_adam(float* theta, const float* grad, float* m1, float* m2, int64_t n, float lr, float beta1,
                 float beta2, float eps, float c1, float c2, void* stream);
+
+/* Count-only feature source: for the filled slots of index top-K lists (stride slots per spectrum, n_cand
+ * filled) zero the b-ion counts c_nb and intensity pseudo-logs c_iq, and s_maxcnt[s] = the spectrum's best count
+ * (0 for an empty list). No reference counterpart (the learned prefilter is new). */
+int vh_prefilter_count_features(int32_t n_spec, int32_t stride, const int32_t* n_cand, const int32_t* c_cnt,
+                                int16_t* c_nb, uint32_t* c_iq, int32_t* s_maxcnt, void* stream);
 
 /* Raw features of every candidate slot of vh_k1_features' lists (stride slots per spectrum): out[slot][8] =
  * (count, count / s_maxcnt (0 if s_maxcnt is 0), b-ion count c_nb, b-ion intensity pseudo-log (c_iq >> 16) / 256,

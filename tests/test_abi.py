@@ -42,7 +42,7 @@ def test_abi_layout_and_version():
     assert L.vh_abi_sizeof(1) == C.sizeof(VhIndexView)
     assert L.vh_abi_sizeof(2) == C.sizeof(VhBlockView)
     assert L.vh_abi_sizeof(3) == C.sizeof(VhK2Cols) == 64
-    assert L.vh_abi_sizeof(4) == C.sizeof(VhPsmCols) == 24 * 8
+    assert L.vh_abi_sizeof(4) == C.sizeof(VhPsmCols) == 25 * 8
     from vulcan.gpu._abi import GBM_NODE
     assert L.vh_abi_sizeof(5) == GBM_NODE.itemsize == 16
     from vulcan.gpu._abi import VhFamilyCols

@@ -454,11 +454,11 @@ def test_search_report5_matches_oracle_ranking(env1):
     from vulcan.gpu import engine as E
     tbl, _, dbg = E.search(env1.d["sp"], env1.gi, env1.l1, env1.p, report=5)
     O = _O()
-    k2 = {k: _np(v) for k, v in dbg["k2"].items()}
+    k2 = E.flat_candidates(dbg["device"], len(env1.d["sp"]))
     spec = k2["spec"]
     us, cnt = np.unique(spec, return_counts=True)
     start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
-    rows = _np(dbg["r"]).astype(np.int32)
+    rows = k2["r"].astype(np.int32)
     sel, nxt = O.rank(env1.vp, start, cnt, rows, k2["hyperscore"], k2["matched_b"], k2["matched_y"], 5)
     want = [(int(spec[i]), int(rows[i])) for i in sel if i >= 0]
     got = list(zip(tbl.column("spec_idx").to_pylist(), tbl.column("pf_row").to_pylist()))
@@ -475,18 +475,18 @@ def test_psm_features_equal_oracle(env1):
     report = 5
     tbl, _, dbg = E.search(env1.d["sp"], env1.gi, env1.l1, env1.p, report=report)
     O = _O()
-    k2 = {k: _np(v) for k, v in dbg["k2"].items()}
+    k2 = E.flat_candidates(dbg["device"], len(env1.d["sp"]))
     spec = k2["spec"]
     us, cnt = np.unique(spec, return_counts=True)
     start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
-    rows = _np(dbg["r"]).astype(np.int32)
+    rows = k2["r"].astype(np.int32)
     sel, nxt = O.rank(env1.vp, start, cnt, rows, k2["hyperscore"], k2["matched_b"], k2["matched_y"], report)
     hyp = dbg["hyp"]
     n = len(env1.d["sp"])
     n_h = len(hyp["spec"])
     hyp_off = np.concatenate([[0], np.cumsum(np.bincount(np.asarray(hyp["spec"]), minlength=n)[:n])])
     ds = dbg["device"]
-    ref = OP.features(sel, nxt, us, rows, _np(dbg["h"]), _np(dbg["k1c"]),
+    ref = OP.features(sel, nxt, us, rows, k2["h"], k2["k1c"],
                       {"hs": k2["hyperscore"], "sb": k2["summed_b"], "sy": k2["summed_y"], "ppm": k2["ppm"],
                        "mb": k2["matched_b"], "my": k2["matched_y"], "lb": k2["longest_b"], "ly": k2["longest_y"]},
                       hyp_off, hyp["z"], hyp["iso"], hyp["pm"], _np(ds.scored)[:n_h], _np(ds.matched)[:n_h],

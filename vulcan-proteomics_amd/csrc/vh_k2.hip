@@ -437,9 +437,42 @@ __global__ __launch_bounds__(256) void psm_kernel(DevIdx X, int n_groups, const 
   O.k1_count[q] = c_cnt[c];
   O.scored_candidates[q] = (int32_t)sc;
   O.poisson[q] = log10(pois);
+  O.hyp[q] = h;
+}
+
+// candidates sorted by spectrum: each candidate's spectrum, and per spectrum the first candidate (lower bound)
+__global__ __launch_bounds__(256) void cand_spec_kernel(int64_t n, const int32_t* __restrict__ c_hyp,
+                                                        const int32_t* __restrict__ h_spec, int32_t* o_spec) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) o_spec[c] = h_spec[c_hyp[c]];
+}
+
+__global__ __launch_bounds__(256) void spec_off_kernel(int64_t n, const int32_t* __restrict__ c_hyp,
+                                                       const int32_t* __restrict__ h_spec, int32_t n_spec,
+                                                       int64_t* o_off) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > n_spec) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (h_spec[c_hyp[mid]] < s) lo = mid + 1;
+    else hi = mid;
+  }
+  o_off[s] = lo;
 }
 
 extern "C" {
+
+int vh_candidate_spectra(int64_t n_cand, const int32_t* c_hyp, const int32_t* h_spec, int32_t n_spec,
+                         int32_t* o_spec, int64_t* o_off, void* stream) {
+  if (n_cand < 0 || n_spec < 0 || !o_off || (n_cand && (!c_hyp || !h_spec))) return VH_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (o_spec && n_cand)
+    cand_spec_kernel<<<(unsigned)((n_cand + 255) / 256), 256, 0, st>>>(n_cand, c_hyp, h_spec, o_spec);
+  spec_off_kernel<<<(unsigned)((n_spec + 256) / 256), 256, 0, st>>>(n_cand, c_hyp, h_spec, n_spec, o_off);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
 
 int vh_k2(const VhParams* p, const VhIndexView* ix, int32_t n_groups, const int32_t* order, const int32_t* grp_spec,
           const int64_t* grp_start, const int32_t* grp_count, int32_t stride, const int32_t* c_row,

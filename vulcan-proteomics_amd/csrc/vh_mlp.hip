@@ -746,6 +746,30 @@ static int bw_splits(int64_t n) {
   return (int)(s < 1 ? 1 : (s > 16 ? 16 : s));
 }
 
+// Count-only features of index top-K lists: zero b-ion counts and intensity pseudo-logs of the filled slots and
+// the best count per spectrum (wave per spectrum).
+__global__ __launch_bounds__(256) void count_features_kernel(int n_spec, int stride, const int32_t* __restrict__ n_cand,
+                                                             const int32_t* __restrict__ cnt, int16_t* nb,
+                                                             uint32_t* iq, int32_t* maxcnt) {
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6), lane = lane_id();
+  if (s >= n_spec) return;
+  const int nc = n_cand[s] < stride ? n_cand[s] : stride;
+  const int64_t b = (int64_t)s * stride;
+  int m = 0;
+  for (int i = lane; i < nc; i += 64) {
+    const int c = cnt[b + i];
+    m = c > m ? c : m;
+    nb[b + i] = 0;
+    iq[b + i] = 0u;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int x = __shfl_xor(m, o);
+    m = x > m ? x : m;
+  }
+  if (lane == 0) maxcnt[s] = m;
+}
+
 extern "C" {
 
 int vh_mlp_pack(const float* theta, const float* mu, const float* isd, void* blob, void* stream) {
@@ -836,6 +860,17 @@ int vh_mlp_adam(float* theta, const float* grad, float* m1, float* m2, int64_t n
   if (n == 0) return 0;
   adam_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(theta, grad, m1, m2, n, lr, beta1, beta2,
                                                                           eps, c1, c2);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
+
+int vh_prefilter_count_features(int32_t n_spec, int32_t stride, const int32_t* n_cand, const int32_t* c_cnt,
+                                int16_t* c_nb, uint32_t* c_iq, int32_t* s_maxcnt, void* stream) {
+  if (n_spec < 0 || stride < 1) return VH_E_ARG;
+  if (n_spec == 0) return 0;
+  if (!n_cand || !c_cnt || !c_nb || !c_iq || !s_maxcnt) return VH_E_ARG;
+  count_features_kernel<<<(unsigned)((n_spec + 3) / 4), 256, 0, (hipStream_t)stream>>>(n_spec, stride, n_cand, c_cnt,
+                                                                                       c_nb, c_iq, s_maxcnt);
   VH_CHECK_LAUNCH();
   return 0;
 }

@@ -42,7 +42,7 @@ class VhK2Cols(C.Structure):
 PSM_COLS = ("spec_idx", "pf_row", "rank", "charge", "isotope_error", "expmass", "calcmass", "delta_mass_ppm",
             "hyperscore", "delta_next", "delta_best", "matched_b", "matched_y", "matched_peaks", "summed_intensity",
             "matched_intensity_pct", "average_ppm", "longest_b", "longest_y", "longest_y_pct", "peptide_len",
-            "k1_count", "scored_candidates", "poisson")
+            "k1_count", "scored_candidates", "poisson", "hyp")
 
 
 class VhPsmCols(C.Structure):

@@ -45,6 +45,7 @@ _SIGS = {
     "vh_k1_brute": ([PP, PIX, I64, P, P, P, P, P, P, P, P], I32),
     "vh_k2": ([PP, PIX, I32, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_candidate_shift": ([I32, P, P, I32, P, P, P, P, P, P, P], I32),
+    "vh_candidate_spectra": ([I64, P, P, I32, P, P, P], I32),
     "vh_rank": ([PP, I32, P, P, I32, P, P, P, P, I32, P, P, P], I32),
     "vh_psm_features": ([PIX, I32, P, I32, P, P, P, P, P, PK2, P, P, P, P, P, P, P, PPSM, P], I32),
     "vh_k1_features": ([PP, PIX, I32, P, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P], I32),
@@ -60,6 +61,7 @@ _SIGS = {
     "vh_mlp_workspace_bytes": ([I64], SZ),
     "vh_mlp_backward": ([P, I64, P, P, P, P, P, P, P, SZ, P], I32),
     "vh_mlp_adam": ([P, P, P, P, I64, F32, F32, F32, F32, F32, F32, P], I32),
+    "vh_prefilter_count_features": ([I32, I32, P, P, P, P, P, P], I32),
     "vh_prefilter_features": ([PIX, I32, I32, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_prefilter_score_workspace_bytes": ([I32, I32], SZ),
     "vh_prefilter_score": ([PIX, P, I32, I32, P, P, P, P, P, P, P, P, P, P, P, SZ, P], I32),

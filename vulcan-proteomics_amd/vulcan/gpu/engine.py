@@ -437,24 +437,34 @@ def _k2_call(gi, vp, n_groups, grp_spec, grp_start, grp_count, stride, row, hyp,
                     K.ptr(out["longest_y"]), K.ptr(queue), K.stream()), "k2")
 
 
+def candidate_spectra(h: torch.Tensor, d_spec: torch.Tensor, n_spec: int):
+    """Candidates sorted by spectrum (hypothesis h[c] of spectrum d_spec[h[c]]): (spectrum per candidate, i32;
+    first candidate per spectrum, i64 n_spec + 1) by vh_candidate_spectra."""
+    dev = h.device
+    n = len(h)
+    spec = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    off = torch.empty(n_spec + 1, dtype=torch.int64, device=dev)
+    K.check(K.lib().vh_candidate_spectra(n, K.ptr(h), K.ptr(d_spec), n_spec, K.ptr(spec), K.ptr(off), K.stream()),
+            "candidate_spectra")
+    return spec[:n], off
+
+
 def run_k2(gi: GpuIndex, h, r, d_spec, d_maxfz, peaks, p: SearchParams, shift: torch.Tensor | None = None):
     """K2 over candidates sorted by spectrum (reference engine.py:315-351). shift (optional, f32 per candidate):
     observed - candidate mass; nonzero entries get shifted + unshifted ions with the delta localized on the best
-    residue (open / mass-offset modes). Returns dict of device tensors."""
-    dev = gi.dev
-    spec = d_spec[h].long()
+    residue (open / mass-offset modes). Returns a dict of device tensors, one value per candidate, plus "spec"
+    (its spectrum) and "off" (first candidate per spectrum)."""
     n = len(r)
-    out = _k2_out(n, dev)
+    n_spec = peaks["off"].numel() - 1
+    hh = h.to(torch.int32).contiguous()
+    spec, off = candidate_spectra(hh, d_spec.to(torch.int32).contiguous(), n_spec)
+    out = _k2_out(n, gi.dev)
     if n:
-        us, cnts = torch.unique_consecutive(spec, return_counts=True)
-        start = torch.zeros(len(us), dtype=torch.int64, device=dev)
-        start[1:] = torch.cumsum(cnts, 0)[:-1]
         sh = None if shift is None else shift.to(torch.float32).contiguous()
-        _k2_call(gi, gi.params_for(p), len(us), us.to(torch.int32).contiguous(), start,
-                 cnts.to(torch.int32).contiguous(), 0, r.to(torch.int32).contiguous(), h.to(torch.int32).contiguous(),
+        _k2_call(gi, gi.params_for(p), n_spec, None, off, None, 0, r.to(torch.int32).contiguous(), hh,
                  d_maxfz.contiguous(), peaks, out, shift=sh)
     out = {k: v[:n] for k, v in out.items()}
-    out["spec"] = spec
+    out["spec"], out["off"] = spec, off
     return out
 
 
@@ -467,7 +477,8 @@ def lnfact(n):
 
 
 def rank_groups(gi, p: SearchParams, n_groups, grp_start, grp_count, stride, row, hs, mb, my, report):
-    """vh_rank: per group the top `report` candidate indices and the next hyperscore (device)."""
+    """vh_rank: per group the top `report` candidate indices and the next hyperscore (device). grp_count None:
+    grp_start holds n_groups + 1 offsets."""
     L = K.lib()
     dev = gi.dev
     sel = torch.empty(max(n_groups * report, 1), dtype=torch.int64, device=dev)
@@ -479,40 +490,33 @@ def rank_groups(gi, p: SearchParams, n_groups, grp_start, grp_count, stride, row
     return sel[:n_groups * report], nxt[:n_groups * report]
 
 
-def psm_features(gi: GpuIndex, n_spec: int, grp_spec, sel, nxt, report: int, h, r, k1c, k2, hyp, scored, matched,
-                 peaks) -> dict:
+_PSM_TYPES = {"spec_idx": torch.int64, "pf_row": torch.int64, "rank": torch.int32, "charge": torch.int8,
+              "isotope_error": torch.int8, "hyperscore": torch.float64, "delta_next": torch.float64,
+              "delta_best": torch.float64, "poisson": torch.float64, "matched_b": torch.int16,
+              "matched_y": torch.int16, "matched_peaks": torch.int16, "longest_b": torch.int16,
+              "longest_y": torch.int16, "peptide_len": torch.int16, "k1_count": torch.int32,
+              "scored_candidates": torch.int32, "hyp": torch.int32}
+
+
+def psm_features(gi: GpuIndex, n_groups: int, grp_spec, sel, nxt, report: int, c_hyp, c_row, c_cnt, k2: dict,
+                 hyp_off, h_z, h_iso, h_pm, scored, matched, tic) -> dict:
     """vh_psm_features: the psm_table feature columns (engine.py:361-432) of vh_rank's selections, computed on
-    the device, dense [n_groups * report] (rank 0 = empty slot). hyp: device hypotheses (spec, z, iso, pm)."""
+    the device, dense [n_groups * report] (rank 0 = empty slot). Candidate arrays (c_hyp / c_row / c_cnt and the
+    K2 columns) are indexed by the selections; the hypothesis arrays by c_hyp. All device tensors."""
     from ._abi import PSM_COLS, VhK2Cols, VhPsmCols
     dev = gi.dev
-    n_groups = len(grp_spec)
     q = max(n_groups * report, 1)
-    types = {"spec_idx": torch.int64, "pf_row": torch.int64, "rank": torch.int32, "charge": torch.int8,
-             "isotope_error": torch.int8, "hyperscore": torch.float64, "delta_next": torch.float64,
-             "delta_best": torch.float64, "poisson": torch.float64, "matched_b": torch.int16, "matched_y": torch.int16,
-             "matched_peaks": torch.int16, "longest_b": torch.int16, "longest_y": torch.int16,
-             "peptide_len": torch.int16, "k1_count": torch.int32, "scored_candidates": torch.int32}
-    cols = {k: torch.empty(q, dtype=types.get(k, torch.float32), device=dev) for k in PSM_COLS}
+    cols = {k: torch.empty(q, dtype=_PSM_TYPES.get(k, torch.float32), device=dev) for k in PSM_COLS}
     if n_groups == 0:
         return {k: v[:0] for k, v in cols.items()}
-    hs_spec = hyp["spec"].long()
-    hyp_off = torch.zeros(n_spec + 1, dtype=torch.int64, device=dev)
-    hyp_off[1:] = torch.cumsum(torch.bincount(hs_spec, minlength=n_spec)[:n_spec], 0)
     kc = {"hs": k2["hyperscore"], "sb": k2["summed_b"], "sy": k2["summed_y"], "ppm": k2["ppm"],
           "mb": k2["matched_b"], "my": k2["matched_y"], "lb": k2["longest_b"], "ly": k2["longest_y"]}
-    kc = {k: v.contiguous() for k, v in kc.items()}
     k2c = VhK2Cols(**{k: K.ptr(v) for k, v in kc.items()})
     oc = VhPsmCols(**{k: K.ptr(v) for k, v in cols.items()})
-    args = {"grp": grp_spec.to(torch.int32).contiguous(), "row": r.to(torch.int32).contiguous(),
-            "hyp": h.to(torch.int32).contiguous(), "cnt": k1c.to(torch.int32).contiguous(),
-            "z": hyp["z"].to(torch.int8).contiguous(), "iso": hyp["iso"].to(torch.int8).contiguous(),
-            "pm": hyp["pm"].to(torch.float32).contiguous(), "sc": scored.to(torch.int32).contiguous(),
-            "mt": matched.to(torch.int64).contiguous(), "tic": peaks["tic"].to(torch.float32).contiguous()}
-    K.check(K.lib().vh_psm_features(C.byref(gi.view), n_groups, K.ptr(args["grp"]), report, K.ptr(sel), K.ptr(nxt),
-                                    K.ptr(args["row"]), K.ptr(args["hyp"]), K.ptr(args["cnt"]), C.byref(k2c),
-                                    K.ptr(hyp_off), K.ptr(args["z"]), K.ptr(args["iso"]), K.ptr(args["pm"]),
-                                    K.ptr(args["sc"]), K.ptr(args["mt"]), K.ptr(args["tic"]), C.byref(oc),
-                                    K.stream()), "psm_features")
+    K.check(K.lib().vh_psm_features(C.byref(gi.view), n_groups, K.ptr(grp_spec), report, K.ptr(sel), K.ptr(nxt),
+                                    K.ptr(c_row), K.ptr(c_hyp), K.ptr(c_cnt), C.byref(k2c), K.ptr(hyp_off),
+                                    K.ptr(h_z), K.ptr(h_iso), K.ptr(h_pm), K.ptr(scored), K.ptr(matched),
+                                    K.ptr(tic), C.byref(oc), K.stream()), "psm_features")
     return {k: v[:n_groups * report] for k, v in cols.items()}
 
 
@@ -545,50 +549,65 @@ _NP_OF = {torch.int64: np.int64, torch.int32: np.int32, torch.int16: np.int16, t
           torch.uint8: np.uint8, torch.float64: np.float64, torch.float32: np.float32}
 
 
-def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched, peaks, p: SearchParams,
-              report: int | None = None, return_hyp: bool = False):
-    """Rank candidates per spectrum (vh_rank: hyperscore desc, row asc), compute the feature columns on the
-    device (vh_psm_features) and build the PSM table with the reference's columns (engine.py:361-432,
-    PSM_SCHEMA.md section 1); the host only gathers the run's scan / rt and the index's pf_id / missed / n_var."""
-    report = p.report_psms if report is None else report
-    dev = gi.dev
-    spec = k2["spec"]
-    n = len(spec)
-    n_spec = len(sp)
-    hyp_d = hyp if isinstance(hyp.get("spec"), torch.Tensor) else {
-        k: torch.as_tensor(np.ascontiguousarray(hyp[k]), device=dev) for k in ("spec", "z", "iso", "pm")}
-    if n:
-        us, cnts = torch.unique_consecutive(spec, return_counts=True)
-        start = torch.zeros(len(us), dtype=torch.int64, device=dev)
-        start[1:] = torch.cumsum(cnts, 0)[:-1]
-        row32 = r.to(torch.int32).contiguous()
-        sel, nxt = rank_groups(gi, p, len(us), start, cnts.to(torch.int32).contiguous(), 0, row32,
-                               k2["hyperscore"].contiguous(), k2["matched_b"].contiguous(),
-                               k2["matched_y"].contiguous(), report)
-        cols = psm_features(gi, n_spec, us, sel, nxt, report, h, r, k1c, k2, hyp_d, scored, matched, peaks)
-        keep = torch.nonzero(cols["rank"] > 0).squeeze(1)
-        cols["hyp"] = h[sel[keep]]
-        dcols = _to_host({k: (v if k == "hyp" else v[keep]) for k, v in cols.items()})
-        hh = dcols.pop("hyp")
-    else:
-        empty = torch.zeros(0, dtype=torch.int64, device=dev)
-        cols = psm_features(gi, n_spec, empty, empty, empty, report, h, r, k1c, k2, hyp_d, scored, matched, peaks)
-        dcols = {k: v.cpu().numpy() for k, v in cols.items()}
-        hh = np.zeros(0, np.int64)
-    s = dcols["spec_idx"]
-    rows = dcols["pf_row"]
-    cols = {"run": string_column(sp.run, len(s)), "spec_idx": s, "scan": sp.scan[s], "rt": sp.rt[s],
-            "rank": dcols["rank"], "pf_row": rows, "pf_id": l1.pf_id[rows]}
-    for k in ("charge", "isotope_error", "expmass", "calcmass", "delta_mass_ppm", "hyperscore", "delta_next",
+_PSM_ORDER = ("charge", "isotope_error", "expmass", "calcmass", "delta_mass_ppm", "hyperscore", "delta_next",
               "delta_best", "matched_b", "matched_y", "matched_peaks", "summed_intensity", "matched_intensity_pct",
-              "average_ppm", "longest_b", "longest_y", "longest_y_pct", "peptide_len"):
-        cols[k] = dcols[k]
+              "average_ppm", "longest_b", "longest_y", "longest_y_pct", "peptide_len")
+
+
+def _table_from_dense(sp: Spectra, l1, dense: dict):
+    """PSM table (PSM_SCHEMA.md column order) from the dense device feature columns: one packed copy to the
+    host, empty rank slots dropped there. Returns (table, hypothesis per PSM row)."""
+    d = _to_host(dense)
+    keep = d["rank"] > 0
+    d = {k: v[keep] for k, v in d.items()}
+    s, rows = d["spec_idx"], d["pf_row"]
+    cols = {"run": string_column(sp.run, len(s)), "spec_idx": s, "scan": sp.scan[s], "rt": sp.rt[s],
+            "rank": d["rank"], "pf_row": rows, "pf_id": l1.pf_id[rows]}
+    for k in _PSM_ORDER:
+        cols[k] = d[k]
     cols["missed_cleavages"] = l1.missed[rows]
     cols["n_var"] = l1.n_var[rows]
     for k in ("k1_count", "scored_candidates", "poisson"):
-        cols[k] = dcols[k]
-    tbl = pa.table(cols)
-    return (tbl, hh) if return_hyp else tbl
+        cols[k] = d[k]
+    return pa.table(cols), d["hyp"].astype(np.int64)
+
+
+def _hyp_offsets(hyp: dict, n_spec: int, dev) -> torch.Tensor:
+    if "hyp_off" in hyp:
+        off = hyp["hyp_off"]
+        return off[:n_spec + 1] if isinstance(off, torch.Tensor) else torch.as_tensor(np.asarray(off)[:n_spec + 1],
+                                                                                    device=dev)
+    spec = hyp["spec"].cpu().numpy() if isinstance(hyp["spec"], torch.Tensor) else np.asarray(hyp["spec"])
+    off = np.zeros(n_spec + 1, np.int64)
+    np.cumsum(np.bincount(spec, minlength=n_spec)[:n_spec], out=off[1:])
+    return torch.as_tensor(off, device=dev)
+
+
+def psm_table(sp: Spectra, gi: GpuIndex, l1, hyp, h, r, k1c, k2, scored, matched, peaks, p: SearchParams,
+              report: int | None = None, return_hyp: bool = False):
+    """Rank the candidates of each spectrum (vh_rank: hyperscore desc, row asc), compute the feature columns on
+    the device (vh_psm_features) and build the PSM table with the reference's columns (engine.py:361-432,
+    PSM_SCHEMA.md section 1); the host only drops empty slots and gathers scan / rt / pf_id / missed / n_var.
+    Candidates (h, r, k1c and the K2 outputs) are sorted by spectrum; hyp: device or host hypotheses."""
+    report = p.report_psms if report is None else report
+    dev = gi.dev
+    n_spec = len(sp)
+    hd = {k: (hyp[k] if isinstance(hyp[k], torch.Tensor) else torch.as_tensor(np.ascontiguousarray(hyp[k]),
+                                                                               device=dev))
+          for k in ("spec", "z", "iso", "pm")}
+    hd = {k: v.to(t).contiguous() for (k, v), t in zip(hd.items(), (torch.int32, torch.int8, torch.int8,
+                                                                    torch.float32))}
+    hh = h.to(torch.int32).contiguous()
+    off = k2["off"] if "off" in k2 else candidate_spectra(hh, hd["spec"], n_spec)[1]
+    row32 = r.to(torch.int32).contiguous()
+    sel, nxt = rank_groups(gi, p, n_spec, off, None, 0, row32, k2["hyperscore"].contiguous(),
+                           k2["matched_b"].contiguous(), k2["matched_y"].contiguous(), report)
+    dense = psm_features(gi, n_spec, None, sel, nxt, report, hh, row32, k1c.to(torch.int32).contiguous(),
+                         {k: v.contiguous() for k, v in k2.items()}, _hyp_offsets(hyp, n_spec, dev), hd["z"],
+                         hd["iso"], hd["pm"], scored.to(torch.int32).contiguous(),
+                         matched.to(torch.int64).contiguous(), peaks["tic"].to(torch.float32).contiguous())
+    tbl, hyp_of_row = _table_from_dense(sp, l1, dense)
+    return (tbl, hyp_of_row) if return_hyp else tbl
 
 
 def write_psms(tbl: pa.Table, out_dir: str, run: str | None = None) -> str:
@@ -871,35 +890,44 @@ class DeviceSearch:
         rows = self.c_row[sel.clamp(min=0)].long()        # gathered on the device: n values cross to the host
         return torch.where(sel >= 0, rows, -1).cpu().numpy()
 
-    def candidates(self, n: int):
-        """Flatten the per-spectrum candidate lists into (h, r, c) sorted by (spectrum, row) (device)."""
-        cnt = self.n_cand[:n].long()
-        spec = torch.repeat_interleave(torch.arange(n, device=self.gi.dev), cnt)
-        pos = torch.arange(len(spec), device=self.gi.dev) - torch.repeat_interleave(torch.cumsum(cnt, 0) - cnt, cnt)
-        flat = spec * self.stride + pos
-        return self.c_hyp[flat].long(), self.c_row[flat].long(), self.c_cnt[flat].long(), flat
+    def n_rescored(self, n: int) -> int:
+        return int(self.n_cand[:n].sum().item())
+
+
+def flat_candidates(ds: DeviceSearch, n: int) -> dict:
+    """Host copy of a DeviceSearch's candidate lists flattened in (spectrum, row) order: "h", "r", "k1c",
+    "spec" and the K2 columns (for checks and debugging; one packed copy, flattened with numpy)."""
+    cols = {"h": ds.c_hyp, "r": ds.c_row, "k1c": ds.c_cnt, **ds.k2}
+    d = _to_host({k: v[:n * ds.stride] for k, v in cols.items()})
+    nc = ds.n_cand[:n].cpu().numpy().astype(np.int64)
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    out = {k: v[m] for k, v in d.items()}
+    out["spec"] = np.repeat(np.arange(n), nc)
+    return out
 
 
 def device_table(ds: DeviceSearch, sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int,
                  host_hyp: bool = True):
-    """PSM table from a DeviceSearch that has run on `sp` (candidates flattened in (spectrum, row) order).
-    Returns (table, per-PSM hypothesis index, debug dict); host_hyp=False leaves the host copy of the
-    hypotheses out of the debug dict (the table is built from the device ones either way)."""
+    """PSM table from a DeviceSearch that has run on `sp`, straight from its per-spectrum candidate lists (no
+    flattening): vh_rank for `report` ranks (the step's own rank-1 selection when report is 1), vh_psm_features,
+    empty slots dropped on the host. Returns (table, hypothesis per PSM row, debug dict with the host hypotheses
+    unless host_hyp=False and the rescored candidate count)."""
     n = len(sp)
-    dev = gi.dev
-    h, r, c, flat = ds.candidates(n)
-    k2 = {k: v[flat] for k, v in ds.k2.items()}
-    k2["spec"] = torch.repeat_interleave(torch.arange(n, device=dev), ds.n_cand[:n].long())
-    n_h = int(ds.hyp["hyp_off"][n].item())
-    hyp_d = {k: ds.hyp[k][:n_h] for k in ("spec", "z", "iso", "pm")}
-    tbl, hh = psm_table(sp, gi, l1, hyp_d, h, r, c, k2, ds.scored, ds.matched, ds.peaks, p, report, return_hyp=True)
-    hyp = hyp_to_host(ds.hyp, n) if host_hyp else None
     stt = int(ds.status.item())
     if stt & 1:
         raise RuntimeError("k1_topk: a spectrum has more hypotheses than one wave handles")
     if stt & 8:
         raise RuntimeError("k1j: pair / slot capacity exceeded")
-    return tbl, hh, {"h": h, "r": r, "k1c": c, "k2": k2, "hyp": hyp}
+    if report == ds.report:
+        sel, nxt = ds.sel[:n * report], ds.nxt[:n * report]
+    else:
+        sel, nxt = rank_groups(gi, p, n, None, ds.n_cand, ds.stride, ds.c_row, ds.k2["hyperscore"],
+                               ds.k2["matched_b"], ds.k2["matched_y"], report)
+    hy = ds.hyp
+    dense = psm_features(gi, n, None, sel, nxt, report, ds.c_hyp, ds.c_row, ds.c_cnt, ds.k2, hy["hyp_off"],
+                         hy["z"], hy["iso"], hy["pm"], ds.scored, ds.matched, ds.peaks["tic"])
+    tbl, hh = _table_from_dense(sp, l1, dense)
+    return tbl, hh, {"hyp": hyp_to_host(hy, n) if host_hyp else None, "n_rescored": ds.n_rescored(n)}
 
 
 def search(sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int | None = None, pre_hyp=None):
@@ -935,7 +963,7 @@ def search(sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int | None = 
     st.wall_s["total"] = time.perf_counter() - t0
     valid = ds.peaks["valid_d"][:n].cpu().numpy().astype(bool)
     st.counts.update(n_spectra=n, n_searched=int(valid.sum()), n_hypotheses=len(dbg["hyp"]["spec"]),
-                     n_rescored=int(len(dbg["r"])), n_psms=tbl.num_rows, pre_hyp_used=used)
+                     n_rescored=dbg["n_rescored"], n_psms=tbl.num_rows, pre_hyp_used=used)
     st.counts["cpu_core_s"] = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
     peaks = dict(ds.peaks)
     peaks["valid"] = valid
@@ -978,24 +1006,32 @@ def k1_brute(gi: GpuIndex, hyp, peaks, h: torch.Tensor, r: torch.Tensor, p: Sear
 def search_exhaustive(sp: Spectra, gi: GpuIndex, l1, p: SearchParams, report: int | None = None,
                       spec_subset: np.ndarray | None = None, batch: int = 20_000_000):
     """K2 over EVERY candidate in the precursor windows (no K1 shortlist): the recall reference
-    (engine.py:500-523)."""
+    (engine.py:500-523). Candidates are rescored in spectrum-aligned batches of at most `batch` rows into one
+    output buffer."""
     st = Stats()
     dev = gi.dev
     peaks = k0_preprocess(sp, p, dev, st, gi)
     hd = device_hypotheses(gi, peaks["raw"], peaks, p)
-    hyp = hyp_to_host(hd)
+    hyp = hyp_to_host(hd, len(sp))
     o_h, o_r, o_c, scored, matched, d_spec, d_maxfz = run_k1(gi, hd, peaks, p, st)
     h, r = window_candidates(hyp, dev, spec_subset)
-    spec = d_spec[h].long()
-    parts, i, n = [], 0, len(h)
-    while i < n:
-        j = min(n, i + batch)
-        if j < n:
-            j = int(torch.searchsorted(spec, spec[j - 1], side="right"))
-        parts.append(run_k2(gi, h[i:j], r[i:j], d_spec, d_maxfz, peaks, p))
-        i = j
-    k2 = {k: torch.cat([q[k] for q in parts]) for k in parts[0]} if parts else run_k2(gi, h, r, d_spec, d_maxfz,
-                                                                                       peaks, p)
-    c = torch.full((len(h),), -1, dtype=torch.int64, device=dev)
-    tbl = psm_table(sp, gi, l1, hyp, h, r, c, k2, scored, matched, peaks, p, report)
+    n = len(h)
+    hh, rr = h.to(torch.int32).contiguous(), r.to(torch.int32).contiguous()
+    n_spec = len(sp)
+    spec, off = candidate_spectra(hh, d_spec.to(torch.int32).contiguous(), n_spec)
+    k2 = _k2_out(n, dev)
+    off_h = off.cpu().numpy()
+    vp = gi.params_for(p)
+    s0 = 0
+    while s0 < n_spec:                                   # spectra [s0, s1) hold at most `batch` candidates
+        s1 = max(int(np.searchsorted(off_h, off_h[s0] + batch, side="right")) - 1, s0 + 1)
+        s1 = min(s1, n_spec)
+        if off_h[s1] > off_h[s0]:       # group g = spectrum s0 + g: offsets and peak CSR viewed from s0
+            pk = {"off": peaks["off"][s0:], "mass": peaks["mass"], "int": peaks["int"]}
+            _k2_call(gi, vp, s1 - s0, None, off[s0:], None, 0, rr, hh, d_maxfz.contiguous(), pk, k2)
+        s0 = s1
+    k2 = {k: v[:n] for k, v in k2.items()}
+    k2["spec"], k2["off"] = spec, off
+    c = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    tbl = psm_table(sp, gi, l1, hyp, hh, rr, c, k2, scored, matched, peaks, p, report)
     return tbl, st, {"peaks": peaks, "hyp": hyp, "h": h, "r": r, "k2": k2}

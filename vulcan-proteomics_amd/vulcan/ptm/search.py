@@ -97,6 +97,6 @@ def search_modes(sp, gi: E.GpuIndex, l1, p: SearchParams, mp: ModeParams, report
         tbl = tbl.append_column("hyp_offset", pa.array(hyp["off"][hh], pa.float32()))
         tbl = tbl.append_column("mass_shift", pa.array(ms, pa.float32()))
     st.wall_s["total"] = time.perf_counter() - t0
-    st.counts.update(n_spectra=len(sp), n_hypotheses=len(hyp["spec"]), n_rescored=int(len(dbg["r"])),
+    st.counts.update(n_spectra=len(sp), n_hypotheses=len(hyp["spec"]), n_rescored=dbg["n_rescored"],
                      n_psms=tbl.num_rows)
     return tbl, st

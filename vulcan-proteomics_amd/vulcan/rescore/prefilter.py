@@ -233,13 +233,11 @@ class FeatureLists:
             self._count_only(n)
 
     def _count_only(self, n: int):
-        """Count-only features: zero b-ion counts / intensity fields, best count per spectrum (device ops only)."""
-        self.nb.zero_()
-        self.iq.zero_()
-        S = self.stride
-        cnt = self.cnt[:n * S].view(n, S)
-        valid = torch.arange(S, device=cnt.device)[None, :] < self.n_cand[:n, None]
-        self.maxcnt[:n] = torch.where(valid, cnt, torch.zeros_like(cnt)).amax(1)
+        """Count-only features: zero b-ion counts / intensity fields, best count per spectrum
+        (vh_prefilter_count_features)."""
+        K.check(K.lib().vh_prefilter_count_features(n, self.stride, K.ptr(self.n_cand), K.ptr(self.cnt),
+                                                    K.ptr(self.nb), K.ptr(self.iq), K.ptr(self.maxcnt),
+                                                    K.stream()), "prefilter_count_features")
 
     def features(self, ds: DeviceSearch, n: int) -> torch.Tensor:
         out = torch.empty((max(n, 1) * self.stride, 8), dtype=torch.float32, device=ds.gi.dev)
