@@ -501,7 +501,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=None, help="override rows per index bucket")
     ap.add_argument("--bin-ppm", type=float, default=None, help="override directory bin width")
     ap.add_argument("--tile", type=int, default=None, help="override K1 LDS tile")
-    ap.add_argument("--k1", choices=("auto", "tile", "join"), default=None,
+    ap.add_argument("--k1", choices=("auto", "tile", "join", "wide"), default=None,
                     help="K1 implementation (default: the workload's search parameters, auto)")
     ap.add_argument("--block-q", type=int, default=None, help="join K1: log2 postings per index block")
     ap.add_argument("--block-rows", type=int, default=None, help="join K1: log2 max rows per index block")
@@ -634,6 +634,9 @@ def main():
     recovery = float(np.mean(best[sig] == truth[sig]))
     ab = algorithmic_bytes(ds, gi, n, l1)
     kernels = ("k1_topk", "k2")
+    if ds.k1_impl == "wide":
+        ab["k1w_topk"] = ab["k1_topk"]
+        kernels = ("k1w_topk", "k2")
     if ds.k1_impl == "join":
         ab.update(algorithmic_bytes_join(ds, gi, n))
         stage_ms["k1"] = sum(stage_ms[k] for k in ("k1j_pairs", "k1j_count", "k1j_select"))

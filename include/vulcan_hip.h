@@ -305,6 +305,19 @@ int vh_k1j_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, co
                     uint32_t* c_iq, int32_t* s_maxcnt, uint32_t* hyp_scored, unsigned long long* hyp_matched,
                     int32_t* status, uint32_t* queue, void* stream);
 
+/* Wide-window K1 for open search: the lists of vh_k1_topk (same count, threshold, top-K, closure and dedupe
+ * rules, same outputs) by one workgroup per spectrum walking its windows one index bucket at a time with an LDS
+ * histogram of the bucket; each (peak, fragment charge) lookup reads one directory cell per bin. `ix` must be a
+ * bin-major view with buckets of 2^VH_K1W_BUCKET_LOG2 rows (vh_index_build with that bucket_log2). topk <= 1024,
+ * max_peaks <= 512. No reference counterpart beyond the semantics (the reference runs its K1 on open windows in
+ * per-spectrum batches, vulcan/ptm/search.py:119-175). */
+#define VH_K1W_BUCKET_LOG2 14
+int vh_k1w_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order, const int64_t* pk_off,
+                const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const uint8_t* h_mode,
+                const int64_t* h_lo, const int64_t* h_hi, int32_t cand_stride, int32_t* n_cand, int32_t* c_row,
+                int32_t* c_hyp, int32_t* c_cnt, uint32_t* hyp_scored, unsigned long long* hyp_matched,
+                int32_t* status, void* stream);
+
 /* Candidates sorted by spectrum (non-decreasing h_spec[c_hyp[c]]): o_spec[c] = the candidate's spectrum (o_spec
  * may be NULL) and o_off[s] = the first candidate of spectrum s, s = 0..n_spec (o_off[n_spec] = n_cand), the
  * grouping vh_k2 / vh_rank take with grp_count NULL. Reference: the per-spectrum grouping run_k2 and psm_table

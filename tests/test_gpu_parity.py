@@ -684,14 +684,16 @@ def test_search_modes_recovers_shifted_spectra_and_is_label_blind(env_modes):
     assert ho(tbl).equals(ho(tb))
 
 
+@pytest.mark.parametrize("impl", ["join", "tile"])
 @pytest.mark.parametrize("which", ["env_modes", "env_open500"])
-def test_modes_join_equals_oracle(which, request):
-    """The join K1 forced on the mode search (mass offsets + a 650 Da open window, or the +-500 Da window: each
-    open hypothesis spans many blocks) gives the oracle's candidate lists."""
+def test_modes_join_equals_oracle(which, impl, request):
+    """The join K1 (each open hypothesis spans many blocks) and the tile K1 forced on the mode search (mass
+    offsets + a 650 Da open window, or the +-500 Da window) give the oracle's candidate lists, as the default
+    wide-window K1 does."""
     from vulcan.gpu import engine as E
     e = request.getfixturevalue(which)
     ds = E.DeviceSearch(e["gi"], e["p"], len(e["sp"]), offsets=e["mp"].offsets, open_da=e["mp"].open_da,
-                        k1_impl="join")
+                        k1_impl=impl)
     ds.run(E.upload_spectra(e["sp"], e["gi"].dev))
     torch.cuda.synchronize()
     assert int(ds.status.item()) == 0
@@ -705,3 +707,53 @@ def test_modes_join_equals_oracle(which, request):
     nh = len(e["hyp"]["spec"])
     assert np.array_equal(_np(ds.scored)[:nh].astype(np.uint32), e["k1"]["scored"])
     assert np.array_equal(_np(ds.matched)[:nh].astype(np.uint64), e["k1"]["matched"])
+
+
+@pytest.mark.parametrize("case", ["unknown_charges", "u32_fallback"])
+def test_wide_k1_edge_cases_equal_oracle(env_open500, case):
+    """Wide-window K1 vs the oracle where its bucket walk has extra paths: spectra without a known charge (open
+    and closed hypotheses of several maxfz over one bucket: counts added charge level by level), and a 1.5 Da
+    fragment tolerance that pushes a bucket's matches past 2^16 (the bucket is counted again in u32 sub-chunks
+    after the first level's offers)."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu._abi import make_params, max_hyp_per_spectrum, set_modes
+    from vulcan.gpu.host import Spectra
+    e = env_open500
+    O = _O()
+    sp = e["sp"]
+    n = 120
+    off = sp.offsets[:n + 1]
+    z = sp.charge[:n].copy()
+    p = type(e["p"])(**e["p"].to_dict())
+    if case == "unknown_charges":
+        z[::2] = 0
+    else:
+        p.fragment_ppm = (-1.5, 1.5)
+    sub = Spectra(sp.run, sp.scan[:n], sp.index[:n], sp.rt[:n], sp.precursor_mz[:n], z, sp.mz[:off[-1]],
+                  sp.intensity[:off[-1]], off.copy())
+    gi = e["gi"] if case == "unknown_charges" else E.GpuIndex(e["l1"], p, STATIC, e["var"])
+    vp = set_modes(make_params(p, STATIC, e["var"]), e["mp"].offsets, e["mp"].open_da)
+    pk = O.k0(vp, sub)
+    hyp = O.hypotheses(vp, sub, pk["valid"], e["l1"].mass32, max_hyp_per_spectrum(p, 0, True))
+    k1 = O.k1_counts(vp, e["hix"], pk, hyp)
+    print(f"[gpu] wide K1 {case}: {int(k1['counts'].sum())} matches over {len(hyp['spec'])} hypotheses", flush=True)
+    ds = E.DeviceSearch(gi, p, n, open_da=e["mp"].open_da, k1_impl="wide")
+    ds.run(E.upload_spectra(sub, gi.dev))
+    torch.cuda.synchronize()
+    assert int(ds.status.item()) == 0
+    tk = O.topk(vp, e["hix"], hyp, k1, ds.stride)
+    nc = _np(ds.n_cand)[:n]
+    assert np.array_equal(nc, tk["n_cand"])
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    for dv, k in ((ds.c_row, "row"), (ds.c_hyp, "hyp"), (ds.c_cnt, "cnt")):
+        assert np.array_equal(_np(dv)[:n * ds.stride][m], tk[k][m]), k
+    nh = len(hyp["spec"])
+    assert np.array_equal(_np(ds.scored)[:nh].astype(np.uint32), k1["scored"])
+    assert np.array_equal(_np(ds.matched)[:nh].astype(np.uint64), k1["matched"])
+    if case == "u32_fallback":     # some bucket of some spectrum really held 2^16 matches or more
+        w = np.maximum(hyp["hi"] - hyp["lo"], 0)
+        rows = np.repeat(hyp["lo"], w) + (np.arange(int(w.sum())) - np.repeat(k1["win_off"][:-1], w))
+        spec = np.repeat(hyp["spec"], w)
+        open_h = np.repeat(hyp["mode"] == 2, w)
+        tot = np.bincount(spec[open_h] * 100000 + (rows[open_h] >> 15), weights=k1["counts"][open_h])
+        assert tot.max() >= 65536

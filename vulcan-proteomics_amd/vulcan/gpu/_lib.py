@@ -42,6 +42,7 @@ _SIGS = {
     "vh_spectrum_order_work": ([I32, P, P, P, P, P, I32, P, P, SZ, P], I32),
     "vh_k1": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I64, P, P, P, P, P, P, P], I32),
     "vh_k1_topk": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P, P], I32),
+    "vh_k1w_topk": ([PP, PIX, I32, P, P, P, P, P, P, P, P, I32, P, P, P, P, P, P, P, P], I32),
     "vh_k1_brute": ([PP, PIX, I64, P, P, P, P, P, P, P, P], I32),
     "vh_k2": ([PP, PIX, I32, P, P, P, P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P], I32),
     "vh_candidate_shift": ([I32, P, P, I32, P, P, P, P, P, P, P], I32),

@@ -23,6 +23,7 @@ from ._abi import VhBlockView, VhIndexView, index_geometry, make_params, max_hyp
 from .host import NEUTRON, PROTON, SAGE_MONO, SearchParams, Spectra, make_hypotheses, pre_hypotheses  # noqa: F401
 
 ROW_BITS = 28   # peptidoform rows < 2^28 (engine.py:35 of the reference)
+WIDE_BUCKET_LOG2 = 14   # VH_K1W_BUCKET_LOG2: bucket rows of the wide-window K1's index view
 
 
 def _u8(t):
@@ -127,6 +128,31 @@ class GpuIndex:
             self._bin = {"offsets": off, "postings": post, "build_s": time.perf_counter() - t0}
             self.device_bytes += off.numel() * 4 + post.numel() * 4
         return self.view
+
+    def wide_view(self) -> VhIndexView:
+        """Bin-major view with buckets of 2^VH_K1W_BUCKET_LOG2 rows for the wide-window (open search) K1
+        (vh_k1w_topk), built on first use with the same bins as the tile view."""
+        if getattr(self, "_wide", None) is None:
+            t0 = time.perf_counter()
+            L = K.lib()
+            d, st = self.dev, K.stream()
+            v = VhIndexView()
+            C.memmove(C.byref(v), C.byref(self.view), C.sizeof(VhIndexView))
+            v.bucket_log2 = WIDE_BUCKET_LOG2
+            v.n_buckets = max(1, (self.n + (1 << WIDE_BUCKET_LOG2) - 1) >> WIDE_BUCKET_LOG2)
+            nb, nn = v.n_buckets, self.n_bins
+            off = torch.empty(nb * nn + 1, dtype=torch.int32, device=d)
+            post = torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d)
+            wsb = int(L.vh_index_workspace_bytes(self.n_postings, nb, nn))
+            ws = torch.empty(wsb, dtype=torch.uint8, device=d)
+            K.check(L.vh_index_build(C.byref(self.vp), C.byref(v), K.ptr(self.post_off), K.ptr(off), K.ptr(post),
+                                     K.ptr(ws), wsb, st), "index_build (wide)")
+            v.offsets, v.postings = K.ptr(off), K.ptr(post)
+            torch.cuda.synchronize()
+            del ws
+            self._wide = {"view": v, "offsets": off, "postings": post, "build_s": time.perf_counter() - t0}
+            self.device_bytes += off.numel() * 4 + post.numel() * 4
+        return self._wide["view"]
 
     @property
     def offsets(self) -> torch.Tensor:
@@ -643,16 +669,21 @@ class DeviceSearch:
                  k1_threshold_open: int = 1, persistent: bool = False, order_mode: str = "mass",
                  band_log2: int = 28, k1_impl: str | None = None, join_features: bool = False):
         self.gi, self.p, self.n_spec, self.report = gi, p, n_spec, report
-        # k1_impl: "tile" (vh_k1_topk, spectrum-major) or "join" (vh_k1j_*, block-major); "auto" picks join
-        # unless an open-search window is on (its windows span 10^5 rows: per-row slots would not pay)
+        # k1_impl: "tile" (vh_k1_topk, spectrum-major), "join" (vh_k1j_*, block-major) or "wide" (vh_k1w_topk,
+        # workgroup per spectrum over 2^14-row buckets); "auto" picks join, and wide when an open-search window is
+        # on (its windows span 10^5 - 10^6 rows: per-row slots would not pay)
         impl = k1_impl or p.k1_impl
         if impl == "auto":
-            impl = "tile" if open_da is not None else "join"
-        if impl not in ("tile", "join"):
+            impl = "wide" if open_da is not None else "join"
+        if impl not in ("tile", "join", "wide"):
             raise ValueError(impl)
+        if impl == "wide" and persistent:
+            raise ValueError("the wide-window K1 runs as a grid launch (no persistent mode)")
         self.k1_impl = impl
         if impl == "tile":
             gi.tile_view()     # the bin-major layout is built here, never inside a (captured) step
+        if impl == "wide":
+            gi.wide_view()
         # persistent: K1 and K2 run as resident waves pulling spectra from an atomic work counter.
         # order_mode: "mass" (first window row, vh_spectrum_order), "work" (mass band of 2^band_log2 rows, then
         # estimated work descending, vh_spectrum_order_work) or "fixed" (self.order is set by the caller)
@@ -854,6 +885,15 @@ class DeviceSearch:
                                   K.ptr(self.matched), K.ptr(self.status), K.ptr(self.queue), K.stream()),
                     "k1j_topk")
             mark("k1j_select")
+            return
+        if self.k1_impl == "wide":
+            mark("k1w_topk")
+            K.check(L.vh_k1w_topk(C.byref(vp), C.byref(gi.wide_view()), n, K.ptr(self.order), K.ptr(pk["off"]),
+                                  K.ptr(pk["mass"]), K.ptr(hy["hyp_off"]), K.ptr(hy["maxfz"]), K.ptr(hy["mode"]),
+                                  K.ptr(hy["lo"]), K.ptr(hy["hi"]), self.stride, K.ptr(self.n_cand),
+                                  K.ptr(self.c_row), K.ptr(self.c_hyp), K.ptr(self.c_cnt), K.ptr(self.scored),
+                                  K.ptr(self.matched), K.ptr(self.status), K.stream()), "k1w_topk")
+            mark("k1w_topk")
             return
         mark("k1_topk")
         K.check(L.vh_k1_topk(C.byref(vp), C.byref(gi.tile_view()), n, K.ptr(self.order), K.ptr(pk["off"]),

@@ -52,8 +52,9 @@ class SearchParams:
     k2_threads: int = 128                   # kept for signature compatibility (K2 runs one wave64 per spectrum)
     max_slice_entries: int = 2 ** 32 - 1    # postings per index (u32 offsets)
     xcd_group: int = 32                     # K1/K2 blocks per XCD per scheduling group (0 = plain block order)
-    k1_impl: str = "auto"                   # K1: "tile" (spectrum-major), "join" (block-major), "auto" (join
-                                            # unless an open-search window is on)
+    k1_impl: str = "auto"                   # K1: "tile" (spectrum-major), "join" (block-major), "wide"
+                                            # (workgroup per spectrum over 2^14-row buckets), "auto" (join, wide
+                                            # when an open-search window is on)
     block_postings_log2: int = 12           # block layout: ~2^q postings per block (LDS-staged by the join K1)
     block_rows_log2: int = 10               # block layout: at most 2^rb rows per block
 
