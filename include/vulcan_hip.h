@@ -308,9 +308,11 @@ int vh_k1j_features(const VhParams* p, const VhIndexView* ix, int32_t n_spec, co
 /* Wide-window K1 for open search: the lists of vh_k1_topk (same count, threshold, top-K, closure and dedupe
  * rules, same outputs) by one workgroup per spectrum walking its windows one index bucket at a time with an LDS
  * histogram of the bucket; each (peak, fragment charge) lookup reads one directory cell per bin. `ix` must be a
- * bin-major view with buckets of 2^VH_K1W_BUCKET_LOG2 rows (vh_index_build with that bucket_log2). topk <= 1024,
- * max_peaks <= 512. No reference counterpart beyond the semantics (the reference runs its K1 on open windows in
- * per-spectrum batches, vulcan/ptm/search.py:119-175). */
+ * bin-major view with buckets of 2^VH_K1W_BUCKET_LOG2 rows (vh_index_build with that bucket_log2), bin_shift
+ * <= 15, fewer than 2^32 postings and `ix->postings` 16-byte aligned (postings are read in aligned groups of
+ * four; nothing past n_postings is read). topk <= 1024, max_peaks <= 512. VH_E_ARG otherwise. No reference
+ * counterpart beyond the semantics (the reference runs its K1 on open windows in per-spectrum batches,
+ * vulcan/ptm/search.py:119-175). */
 #define VH_K1W_BUCKET_LOG2 14
 int vh_k1w_topk(const VhParams* p, const VhIndexView* ix, int32_t n_spec, const int32_t* order, const int64_t* pk_off,
                 const float* pk_mass, const int64_t* hyp_off, const int8_t* h_maxfz, const uint8_t* h_mode,

@@ -26,6 +26,8 @@
 //     count >= the hypothesis' K1 threshold; per-hypothesis scored / matched sums are LDS atomics.
 // At the end the four buffers (each a superset of its wave's top K) are merged and one wave runs the shared
 // final stage (top-K, closure, dedupe by row). Results do not depend on the bucket, cell or strip order.
+#include <type_traits>
+
 #include "vh_topk.h"
 
 #define KW_WAVES 4
@@ -47,6 +49,7 @@ struct KwArgs {
   const uint8_t* h_mode;
   const int64_t* h_lo;
   const int64_t* h_hi;
+  uint32_t n_postings;
   int hist_words;   // u32 words of the LDS histogram
   int region_bytes; // bytes of the histogram / merge region
 };
@@ -89,6 +92,20 @@ __device__ __forceinline__ bool kw_in(uint32_t pv, uint32_t lohi) {
   return k >= (lohi & 0xFFFFu) && k <= (lohi >> 16);
 }
 
+// The count-of-counts bound: hcc[c] = keys (row, hypothesis) offered with count c (c >= 63 in hcc[63]) among the
+// keys whose count exceeded the bound in force when their strip was read; the largest c with at least K tallied
+// keys of count >= c (0 if none). The bound only rises, so the tally is complete for every c above it (and the
+// bound itself had K keys), and a key of count below it has K keys of larger count: not in the spectrum's top K.
+__device__ __forceinline__ int kw_tstar(const uint32_t* hcc, int K, int lane) {
+  uint32_t x = hcc[63 - lane];   // suffix sums over c = 63 .. 0
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  const uint64_t ok = ballot(x >= (uint32_t)K);   // lane l: keys of count >= 63 - l
+  return ok ? 63 - (int)__builtin_ctzll(ok) : 0;
+}
+
 __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, KwArgs A, K1Out O) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint32_t* hist = (uint32_t*)smem;
@@ -99,11 +116,13 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
   // prunes with it (a lower bound of the spectrum's K-th key)
   uint64_t* gthr = (uint64_t*)(wl_n + 1);
   uint32_t* hsc = (uint32_t*)(wl_n + 2);
-  int* sh = (int*)(hsc + 64);                             // [1] clamped, [2..5] buffer fills, [6..7] slots
+  uint32_t* hcc = hsc + 64;                               // keys tallied by count (63: 63 or more)
+  int* sh = (int*)(hcc + 64);                             // [1] clamped, [2..5] buffer fills, [6..7] slots
   float* pm = (float*)(sh + 8);                           // the spectrum's peak masses
   uint32_t* wl_pos = (uint32_t*)(pm + VH_K2_MAX_PEAKS);   // cell list: first position (KW_WORK + 1: end mark)
-  uint32_t* wl_e0 = wl_pos + KW_WORK + 1;                 //   first posting
-  uint32_t* wl_lohi = wl_e0 + KW_WORK;                    //   key bits range
+  uint32_t* wl_e0 = wl_pos + KW_WORK + 1;                 //   postings [e0, e1)
+  uint32_t* wl_e1 = wl_e0 + KW_WORK;
+  uint32_t* wl_lohi = wl_e1 + KW_WORK;                    //   key bits range
   const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
   const int wv = xcd_band_position(P.xcd_group);
   if (wv >= A.n_spec) return;   // the whole workgroup leaves
@@ -124,6 +143,7 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
   if (tid < 64) {
     hsc[tid] = 0u;
     hmt[tid] = 0ull;
+    hcc[tid] = 0u;
   }
   if (tid < 8) sh[tid] = 0;
   if (tid == 0) *gthr = 0ull;
@@ -149,6 +169,7 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
   uint64_t* mybuf = bufs + (size_t)w * O.capb;
   int nbuf = 0;
   uint64_t thresh = 0;
+  int tstar = 0;   // count bound: K keys of the spectrum have a count >= tstar
   bool clamped = false;
   __syncthreads();   // peak masses in LDS
   // ---- slots: lookups i = (peak i / nzt, fragment charge 1 + i % nzt), each with its bins in order, numbered
@@ -258,11 +279,13 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
         else atomicAdd(&hist[rr], 1u);
       };
       auto add_cell = [&](uint32_t e0, uint32_t e1, uint32_t lh) {
-        const unsigned long long old = atomicAdd(wl_n, (1ull << 32) | (unsigned long long)(e1 - e0));
+        const uint32_t groups = (((e1 + 3u) & ~3u) - (e0 & ~3u)) >> 2;   // 16-byte groups holding [e0, e1)
+        const unsigned long long old = atomicAdd(wl_n, (1ull << 32) | (unsigned long long)groups);
         const uint32_t idx = (uint32_t)(old >> 32), pos = (uint32_t)old;
         if (idx < KW_WORK) {
           wl_pos[idx] = pos;
           wl_e0[idx] = e0;
+          wl_e1[idx] = e1;
           wl_lohi[idx] = lh;
         } else {   // list full: the owner counts its cell (the first such cell marks the list's end)
           if (idx == KW_WORK) wl_pos[KW_WORK] = pos;
@@ -290,7 +313,8 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
         }
       }
       __syncthreads();
-      // (B) posting m of the listed cells' sequence to thread m mod 256
+      // (B) 16-byte group m of the listed cells' sequence to thread m mod 256: one 128-bit load of four postings
+      // (aligned to four entries; entries outside the cell are skipped)
       const unsigned long long wn = *wl_n;
       const uint32_t n_all = (uint32_t)(wn >> 32);
       const uint32_t n = n_all < KW_WORK ? n_all : KW_WORK;
@@ -304,16 +328,33 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
             at[u] = wl_pos[at[u] + half] <= m0 + (uint32_t)(u * KW_THREADS) ? at[u] + half : at[u];
           len -= half;
         }
-        uint32_t pv[4], lh[4];
+        uint4 pv[4];
+        uint32_t lh[4], eb[4], ea[4], ee[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const uint32_t m = m0 + (uint32_t)(u * KW_THREADS);
           lh[u] = wl_lohi[at[u]];
-          pv[u] = m < total ? X.postings[wl_e0[at[u]] + (m - wl_pos[at[u]])] : 0u;
+          ea[u] = wl_e0[at[u]];
+          ee[u] = m < total ? wl_e1[at[u]] : 0u;   // an empty range past the sequence
+          eb[u] = (ea[u] & ~3u) + 4u * (m - wl_pos[at[u]]);
+          if (m >= total) pv[u] = make_uint4(0u, 0u, 0u, 0u);
+          else if (eb[u] + 4u <= A.n_postings) pv[u] = *(const uint4*)&X.postings[eb[u]];
+          else {   // the index's last group: entries past its end are not read
+            pv[u].x = X.postings[eb[u]];
+            pv[u].y = eb[u] + 1u < ee[u] ? X.postings[eb[u] + 1u] : 0u;
+            pv[u].z = eb[u] + 2u < ee[u] ? X.postings[eb[u] + 2u] : 0u;
+            pv[u].w = eb[u] + 3u < ee[u] ? X.postings[eb[u] + 3u] : 0u;
+          }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (m0 + (uint32_t)(u * KW_THREADS) < total && kw_in(pv[u], lh[u])) visit(pv[u]);
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t v[4] = {pv[u].x, pv[u].y, pv[u].z, pv[u].w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const uint32_t e = eb[u] + (uint32_t)t;
+            if (e >= ea[u] && e < ee[u] && kw_in(v[t], lh[u])) visit(v[t]);
+          }
+        }
       }
       return (uint32_t)wn;
     };
@@ -370,34 +411,39 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
           const int a = qlo > r0 ? qlo : r0, b = qhi < r0 + nr ? qhi : r0 + nr;
           const bool whole = a == r0 && b == r0 + nr;   // the window covers the chunk: no row masks
           uint32_t sc = 0, mt = 0;   // per lane: < 2^32 (a chunk holds fewer postings)
-          const int rpw = packed ? 2 : 1;                               // rows per word
+          auto strips = [&](auto PK, auto WH) {   // specialised on the packed and whole flags (uniform)
+          constexpr bool pk = decltype(PK)::value, wh = decltype(WH)::value;
+          constexpr int rpw = pk ? 2 : 1;                               // rows per word
           const int wa = (a - r0) / rpw, wb = (b - 1 - r0) / rpw + 1;  // words holding rows [a, b)
           const int W0 = wa & ~3, W1 = (wb + 3) & ~3;                   // in whole 16-byte groups
           const uint64_t qlow = (uint64_t)(63 - q) << 14;
+          const int t_tally = tstar + 1 > qthr ? tstar + 1 : (qthr > 1 ? qthr : 1);
           for (int k = w; W1 - 256 * k > W0; k += KW_WAVES) {
             const int wl = W1 - 256 * (k + 1) + 4 * lane;   // the lane's words wl .. wl + 3
             const bool act = wl >= W0;
             uint4 v4 = act ? *(const uint4*)&hist[wl] : make_uint4(0u, 0u, 0u, 0u);
             uint32_t word[4] = {v4.x, v4.y, v4.z, v4.w};
-            if (!whole) {   // rows outside [a, b) count 0 here
+            if (!wh) {   // rows outside [a, b) count 0 here
 #pragma unroll
               for (int u = 0; u < 4; ++u) {
                 const int rl = r0 + (wl + u) * rpw;   // the word's lower row
-                if (packed)
+                if (pk)
                   word[u] &= ((rl >= a && rl < b) ? 0x0000FFFFu : 0u) | ((rl + 1 >= a && rl + 1 < b) ? 0xFFFF0000u : 0u);
                 else
                   word[u] = (rl >= a && rl < b) ? word[u] : 0u;
               }
             }
             uint32_t cmax = 0;
-            if (packed) {
+            if (pk) {   // per word: non-zero halves, their sum and maximum, as packed u16 operations
               kw_u16x2 cm = {0, 0};
+              const uint32_t ones = 0x00010001u;
 #pragma unroll
               for (int u = 0; u < 4; ++u) {
-                const uint32_t x = word[u];
-                sc += __popc((((x & 0x7FFF7FFFu) + 0x7FFF7FFFu) | x) & 0x80008000u);   // non-zero halves
-                mt += (x & 0xFFFFu) + (x >> 16);
-                cm = __builtin_elementwise_max(cm, __builtin_bit_cast(kw_u16x2, x));
+                uint32_t nz;   // min(half, 1) per half (the compiler splits the vector min into compares)
+                asm("v_pk_min_u16 %0, %1, %2" : "=v"(nz) : "v"(word[u]), "v"(ones));
+                sc = __builtin_amdgcn_sad_u16(nz, 0u, sc);
+                mt = __builtin_amdgcn_sad_u16(word[u], 0u, mt);
+                cm = __builtin_elementwise_max(cm, __builtin_bit_cast(kw_u16x2, word[u]));
               }
               cmax = cm.x > cm.y ? cm.x : cm.y;
             } else {
@@ -417,6 +463,16 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
                                (cmax > tc || (cmax == tc && (((uint64_t)(uint32_t)rtop << 20) | qlow) >
                                                                 (thresh & 0xFFFFFFFFFFFFull)));
             clamped |= (int)cmax >= qthr && cmax > 32767u;   // rows outside the window count 0 here
+            if (!ballot(act && ((int)cmax >= t_tally || qthr <= 0))) continue;
+            // keys of count >= the tally bound go into the count-of-counts (rows outside the window read 0)
+            if (act && (int)cmax >= t_tally) {
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const uint32_t cA = pk ? word[u] >> 16 : word[u], cB = word[u] & 0xFFFFu;
+                if ((int)cA >= t_tally) atomicAdd(&hcc[cA < 63u ? cA : 63u], 1u);
+                if (pk && (int)cB >= t_tally) atomicAdd(&hcc[cB < 63u ? cB : 63u], 1u);
+              }
+            }
             if (!ballot(could)) continue;
             // the lane's rows that pass the threshold and beat the K-th key (bit 2 u: the word's higher row,
             // bit 2 u + 1: its lower row), offered one a lane per round
@@ -425,11 +481,11 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
             for (int u = 0; u < 4; ++u) {
               const int wi = wl + u;
               const int rA = r0 + wi * rpw + rpw - 1, rB = r0 + wi * rpw;   // the word's higher and lower row
-              const uint32_t cA = packed ? word[u] >> 16 : word[u], cB = word[u] & 0xFFFFu;
+              const uint32_t cA = pk ? word[u] >> 16 : word[u], cB = word[u] & 0xFFFFu;
               const uint64_t kA = ((uint64_t)(cA < 32767u ? cA : 32767u) << 48) | ((uint64_t)(uint32_t)rA << 20) | qlow;
               const uint64_t kB = ((uint64_t)(cB < 32767u ? cB : 32767u) << 48) | ((uint64_t)(uint32_t)rB << 20) | qlow;
               if (act && rA >= a && rA < b && (int)cA >= qthr && kA > thresh) cand |= 1u << (2 * u);
-              if (packed && act && rB >= a && rB < b && (int)cB >= qthr && kB > thresh) cand |= 2u << (2 * u);
+              if (pk && act && rB >= a && rB < b && (int)cB >= qthr && kB > thresh) cand |= 2u << (2 * u);
             }
             for (;;) {
               const bool has = cand != 0u;
@@ -441,11 +497,19 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
 #pragma unroll
               for (int v = 1; v < 4; ++v) x = u == v ? word[v] : x;
               const bool hi_row = (bit & 1) == 0;
-              const uint32_t c = hi_row ? (packed ? x >> 16 : x) : (x & 0xFFFFu);
+              const uint32_t c = hi_row ? (pk ? x >> 16 : x) : (x & 0xFFFFu);
               const int row = r0 + (wl + u) * rpw + (hi_row ? rpw - 1 : 0);
               topk_offer<1>(has, c, row, q, 0, mybuf, nullptr, O.capb, K, nbuf, thresh, clamped);
             }
             if (lane == 0 && thresh > g) atomicMax((unsigned long long*)gthr, (unsigned long long)thresh);
+          }
+          };
+          if (packed) {
+            if (whole) strips(std::true_type(), std::true_type());
+            else strips(std::true_type(), std::false_type());
+          } else {
+            if (whole) strips(std::false_type(), std::true_type());
+            else strips(std::false_type(), std::false_type());
           }
           sc = wave_sum(sc);
           const unsigned long long mts = wave_sum((unsigned long long)mt);
@@ -458,6 +522,8 @@ __global__ __launch_bounds__(KW_THREADS, 3) void k1w_kernel(DevP P, DevIdx X, Kw
         if (m == 1) break;
       }
       __syncthreads();   // offers done before the histogram is cleared again
+      tstar = kw_tstar(hcc, K, lane);
+      if (((uint64_t)tstar << 48) > thresh) thresh = (uint64_t)tstar << 48;
       if (redo) {
         packed = false;   // same bucket, u32 sub-chunks
         continue;
@@ -497,7 +563,8 @@ extern "C" int vh_k1w_topk(const VhParams* p, const VhIndexView* ix, int32_t n_s
                            uint32_t* hyp_scored, unsigned long long* hyp_matched, int32_t* status, void* stream) {
   if (!p || !ix || !ix->offsets || (ix->n_postings && !ix->postings)) return VH_E_ARG;
   if (n_spec < 0 || !n_cand || !c_row || !c_hyp || !c_cnt || !hyp_scored || !hyp_matched || !status) return VH_E_ARG;
-  if (ix->bucket_log2 != VH_K1W_BUCKET_LOG2 || ix->bin_shift < 4 || ix->bin_shift > 16) return VH_E_ARG;
+  if (ix->bucket_log2 != VH_K1W_BUCKET_LOG2 || ix->bin_shift < 4 || ix->bin_shift > 15) return VH_E_ARG;
+  if ((uintptr_t)ix->postings % 16u || ix->n_postings >= (1LL << 32)) return VH_E_ARG;   // 128-bit group loads
   if (ix->n_rows >= (1LL << 28)) return VH_E_LIMIT;
   if (p->topk < 1 || p->topk > 1024 || (p->pair_closure && !ix->partner)) return VH_E_ARG;
   if (p->max_peaks > VH_K2_MAX_PEAKS) return VH_E_LIMIT;
@@ -521,7 +588,7 @@ extern "C" int vh_k1w_topk(const VhParams* p, const VhIndexView* ix, int32_t n_s
   O.clos_n = clos;
   // region: the histogram during the walk, then the merged buffers (4 capb) and the closure keys (clos)
   const int merge_bytes = (KW_WAVES * capb + clos) * 8;
-  const int tail = KW_WAVES * capb * 8 + 64 * 8 + 16 + 64 * 4 + 8 * 4 + VH_K2_MAX_PEAKS * 4 + (3 * KW_WORK + 1) * 4;
+  const int tail = KW_WAVES * capb * 8 + 64 * 8 + 16 + 64 * 4 + 8 * 4 + VH_K2_MAX_PEAKS * 4 + (4 * KW_WORK + 1) * 4 + 64 * 4;
   int dev = 0, lds_max = 65536;
   if (hipGetDevice(&dev) == hipSuccess)
     hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
@@ -532,7 +599,8 @@ extern "C" int vh_k1w_topk(const VhParams* p, const VhIndexView* ix, int32_t n_s
   const int stage_bytes = (KW_SPT * KW_THREADS * 9 + 15) & ~15;   // slot staging (dir, key bits, charge)
   if (region < stage_bytes) region = stage_bytes;
   if (region + tail > lds_max) return VH_E_LIMIT;
-  KwArgs A{n_spec, order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, region / 4, region};
+  KwArgs A{n_spec, order, pk_off, pk_mass, hyp_off, h_maxfz, h_mode, h_lo, h_hi, (uint32_t)ix->n_postings, region / 4,
+           region};
   const size_t lds = (size_t)region + (size_t)tail;
   hipStream_t st = (hipStream_t)stream;
   if (lds > 65536) hipFuncSetAttribute((const void*)k1w_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
