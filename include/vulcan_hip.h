@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 17
+#define VH_ABI_VERSION 18
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -71,7 +71,14 @@ typedef struct VhParams {
   int32_t open_search;
   float open_lo, open_hi;
   int32_t k1_threshold_open;  /* K1 append threshold of open hypotheses */
-  int32_t reserved[8];
+  /* K0 "window" peak view (engine.py:182-183, 195-196; host.py:33-35): peak_window_k > 0 keeps the top
+   * peak_window_k peaks of each floor(m/z / peak_window_th) window (f32 division) by the global view's selection
+   * order, with no total cap; max_peaks is then the per-spectrum capacity of the output (at most
+   * peak_window_k x the number of windows in [fmin, fmax], which the caller must allow; the kept peaks are
+   * clamped to it). 0 = the global view (top max_peaks). */
+  int32_t peak_window_k;
+  float peak_window_th;
+  int32_t reserved[6];
 } VhParams;
 
 /* Device view of the peptidoform table (vulcan/index/build.py:13-21) and the fragment index.

@@ -3,7 +3,7 @@
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library, as the checker and as
 // the multi-core CPU baseline. It restates the reference semantics function by function (citations are
 // files of pedrobeltrao/vulcan-proteomics):
-//   or_k0           vulcan/gpu/engine.py:171-210        (k0_preprocess, global peak view)
+//   or_k0           vulcan/gpu/engine.py:171-210        (k0_preprocess, global and window peak views)
 //   or_hypotheses   vulcan/gpu/host.py:64-91             (make_hypotheses, numpy f32 arithmetic)
 //   or_k1_brute     vulcan/gpu/kernels.cu:195-232        (k1_brute_kernel: the definition of the K1 count)
 //   or_k1_counts    vulcan/gpu/kernels.cu:112-190        (k1_kernel counts, scored, matched) computed per row
@@ -68,14 +68,29 @@ int k0_one(const Ctx& c, const float* mz, const float* it, int64_t n, float* om,
     if (!isnan(m) && m >= p.fmin && m <= p.fmax) idx.push_back(i);
     mass[i] = m - PROTON;
   }
-  // selection order: intensity desc (NaN first), mass desc, input order (engine.py:187-189, stable sorts)
+  // selection order: intensity desc (NaN first), mass desc, input order (engine.py:187-189, stable sorts); the
+  // window view groups by floor(m/z / window_th) (f32 division) first and keeps the first window_k of each group
+  // (engine.py:185-193), the global view the first max_peaks
+  const bool win = p.peak_window_k > 0;
+  auto wnd = [&](int64_t i) -> int64_t { return win ? (int64_t)floorf(mz[i] / p.peak_window_th) : 0; };
   std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+    if (wnd(a) != wnd(b)) return wnd(a) < wnd(b);
     if (!eq_nan(it[a], it[b])) return gt_nan_first(it[a], it[b]);
     if (mass[a] != mass[b]) return mass[a] > mass[b];
     return a < b;
   });
-  int64_t k = std::min<int64_t>((int64_t)idx.size(), p.max_peaks);
-  idx.resize(k);
+  if (win) {
+    std::vector<int64_t> kept;
+    for (size_t j = 0, first = 0; j < idx.size(); ++j) {
+      if (j > 0 && wnd(idx[j]) != wnd(idx[j - 1])) first = j;
+      if ((int64_t)(j - first) < p.peak_window_k) kept.push_back(idx[j]);
+    }
+    idx.swap(kept);
+    if ((int64_t)idx.size() > p.max_peaks) idx.resize(p.max_peaks);   // the output capacity (a caller limit)
+  } else if ((int64_t)idx.size() > p.max_peaks) {
+    idx.resize(p.max_peaks);
+  }
+  const int64_t k = (int64_t)idx.size();
   // final order: mass asc, ties keep the selection order (engine.py:196-197)
   std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return mass[a] < mass[b]; });
   double t = 0.0;
@@ -457,8 +472,24 @@ int or_k0(const VhParams* p, int32_t n_spec, const float* mz, const float* it, c
           int64_t* pk_off, float* pk_mass, float* pk_int, float* tic, uint8_t* valid, int64_t cap) {
   Ctx c(p);
   std::vector<int64_t> cnt(n_spec);
+  if (p->peak_window_k < 0 || (p->peak_window_k > 0 && !(p->peak_window_th > 0.f))) return VH_E_ARG;
 #pragma omp parallel for schedule(dynamic, 256)
   for (int32_t s = 0; s < n_spec; ++s) {
+    if (p->peak_window_k > 0) {   // kept count of the window view: sum over windows of min(in window, k)
+      std::vector<int64_t> w;
+      for (int64_t i = raw_off[s]; i < raw_off[s + 1]; ++i) {
+        float m = mz[i];
+        if (!isnan(m) && m >= p->fmin && m <= p->fmax) w.push_back((int64_t)floorf(m / p->peak_window_th));
+      }
+      std::sort(w.begin(), w.end());
+      int64_t c = 0;
+      for (size_t j = 0, first = 0; j < w.size(); ++j) {
+        if (j > 0 && w[j] != w[j - 1]) first = j;
+        c += (int64_t)(j - first) < p->peak_window_k ? 1 : 0;
+      }
+      cnt[s] = std::min<int64_t>(c, p->max_peaks);
+      continue;
+    }
     int64_t n_in = 0;
     for (int64_t i = raw_off[s]; i < raw_off[s + 1]; ++i) {
       float m = mz[i];

@@ -434,6 +434,28 @@ def test_search_end_to_end(env0):
     assert st.counts["n_rescored"] == int(ref["n_cand"].sum())
 
 
+@pytest.mark.parametrize("impl", ["join", "tile"])
+def test_search_window_peak_view_equals_oracle(env0, impl):
+    """The K0 window view (top 4 peaks per 37.5 Th window, no total cap: up to 200 peaks a spectrum) through
+    the whole search: rank-1 rows and hyperscores equal the oracle's search with the same view."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu._abi import make_params
+    O = _O()
+    p = type(env0.p)(**env0.p.to_dict())
+    p.peak_mode, p.window_th, p.window_k, p.min_peaks, p.k1_impl = "window", 37.5, 4, 10, impl
+    vp = make_params(p, env0.d["static"], env0.d["var"])
+    assert vp.max_peaks == 200 and vp.peak_window_k == 4
+    gi = E.GpuIndex(env0.l1, p, env0.d["static"], env0.d["var"])
+    tbl, st, _ = E.search(env0.d["sp"], gi, env0.l1, p)
+    ref = O.search(vp, env0.hix, env0.d["sp"])
+    got = dict(zip(tbl.column("spec_idx").to_pylist(), tbl.column("pf_row").to_pylist()))
+    want = {i: int(r) for i, r in enumerate(ref["row"]) if r >= 0}
+    assert got == want and len(want) > 100
+    hs = dict(zip(tbl.column("spec_idx").to_pylist(), tbl.column("hyperscore").to_pylist()))
+    assert all(abs(hs[i] - ref["hs"][i]) <= HS_RTOL * abs(ref["hs"][i]) for i in want)
+    assert st.counts["n_rescored"] == int(ref["n_cand"].sum())
+
+
 def test_search_with_host_hypotheses(env0):
     """search(pre_hyp=...): host hypotheses whose validity mask equals K0's replace the device ones
     (pre_hyp_used 1), a mismatched mask falls back to the device hypotheses (-1); the table is the same."""

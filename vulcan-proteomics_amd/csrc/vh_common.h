@@ -28,6 +28,8 @@ struct DevP {
   int open_search;
   float open_lo, open_hi;
   int k1_threshold_open;
+  int peak_window_k;       // > 0: K0 window view (top peak_window_k per floor(m/z / peak_window_th))
+  float peak_window_th;
 };
 
 struct DevIdx {

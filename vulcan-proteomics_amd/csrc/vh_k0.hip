@@ -1,6 +1,6 @@
 // K0 peak view and precursor hypotheses on the device.
 //
-// K0 restates k0_preprocess (vulcan/gpu/engine.py:171-210, global view): keep fmin <= m/z <= fmax (not NaN),
+// K0 restates k0_preprocess (vulcan/gpu/engine.py:171-210; the window view is at k0w_*): keep fmin <= m/z <= fmax (not NaN),
 // mass = m/z - PROTON, the top max_peaks peaks by (intensity desc with NaN first, mass desc, input order),
 // then mass ascending (ties keep that order); TIC = f64 sum; valid = kept >= min_peaks.
 // One wave64 per spectrum: in-range peaks are compacted (ballot prefix, order preserved) and held in registers,
@@ -429,6 +429,99 @@ static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // K0 = count kernel, inclusive scan of the per-spectrum kept counts into pk_off (scan partials in `ws`), emit
 // kernel. Shared by vh_k0 (caller workspace) and vh_k0_noalloc (the static scratch below).
+// ---- window view (peak_window_k > 0; engine.py:182-196 with peak_mode "window"): a peak is kept when fewer than
+// peak_window_k in-range peaks of its window floor(m/z / peak_window_th) (f32 division) precede it in the selection
+// order of the global view; there is no total cap. The kept peaks are then ordered by mass, ties by (window,
+// selection order), the order the reference's stable sorts leave. One wave per spectrum, O(n^2) comparisons over
+// its raw peaks (the global view's register paths do not apply: the rank counts only the peak's own window). The
+// emit pass keeps the kept flags of the first K0W_BITS raw peaks in LDS and recomputes the flag of later ones.
+#define K0W_BITS 16384
+
+__device__ __forceinline__ int k0w_window(float m, const DevP& P) { return (int)floorf(m / P.peak_window_th); }
+
+__device__ bool k0w_kept(const DevP& P, const float* __restrict__ mz, const float* __restrict__ it, int n, int i) {
+  const float mi = mz[i];
+  if (!inrange(mi, P)) return false;
+  const int wi = k0w_window(mi, P);
+  const float ii = it[i], ai = mi - VH_PROTON;
+  int r = 0;
+  for (int j = 0; j < n && r < P.peak_window_k; ++j) {
+    const float mj = mz[j];
+    if (j == i || !inrange(mj, P) || k0w_window(mj, P) != wi) continue;
+    r += sel_before(it[j], mj - VH_PROTON, j, ii, ai, i) ? 1 : 0;
+  }
+  return r < P.peak_window_k;
+}
+
+__global__ __launch_bounds__(256) void k0w_count_kernel(DevP P, int n_spec, const float* __restrict__ mz,
+                                                        const float* __restrict__ it,
+                                                        const int64_t* __restrict__ raw_off, int64_t* pk_off) {
+  const int s = blockIdx.x * K0_WAVES + (threadIdx.x >> 6);
+  if (blockIdx.x == 0 && threadIdx.x == 0) pk_off[0] = 0;
+  if (s >= n_spec) return;
+  const int64_t a = raw_off[s];
+  const int n = (int)(raw_off[s + 1] - a);
+  int c = 0;
+  for (int i = lane_id(); i < n; i += 64) c += k0w_kept(P, mz + a, it + a, n, i) ? 1 : 0;
+  c = wave_sum(c);
+  if (lane_id() == 0) pk_off[s + 1] = c < P.max_peaks ? c : P.max_peaks;
+}
+
+__global__ __launch_bounds__(256) void k0w_emit_kernel(DevP P, int n_spec, const float* __restrict__ mz,
+                                                       const float* __restrict__ it,
+                                                       const int64_t* __restrict__ raw_off,
+                                                       const int64_t* __restrict__ pk_off, float* pk_mass,
+                                                       float* pk_int, float* tic, uint8_t* valid) {
+  __shared__ uint32_t s_kept[K0_WAVES][K0W_BITS / 32];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  const int s = blockIdx.x * K0_WAVES + w;
+  if (s >= n_spec) return;
+  const int64_t a = raw_off[s];
+  const int n = (int)(raw_off[s + 1] - a);
+  const float* m = mz + a;
+  const float* v = it + a;
+  uint32_t* kb = s_kept[w];
+  const int nb = n < K0W_BITS ? n : K0W_BITS;
+  for (int i0 = 0; i0 < nb; i0 += 64) {   // kept flags of the first K0W_BITS peaks, one ballot per 64
+    const int i = i0 + lane;
+    const uint64_t bal = ballot(i < nb && k0w_kept(P, m, v, n, i));
+    if (lane == 0) {
+      kb[i0 >> 5] = (uint32_t)bal;
+      if ((i0 >> 5) + 1 < K0W_BITS / 32) kb[(i0 >> 5) + 1] = (uint32_t)(bal >> 32);
+    }
+  }
+  wave_sync();
+  auto kept = [&](int j) -> bool { return j < K0W_BITS ? ((kb[j >> 5] >> (j & 31)) & 1u) : k0w_kept(P, m, v, n, j); };
+  const int64_t o = pk_off[s];
+  const int K = (int)(pk_off[s + 1] - o);
+  double t = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    if (!kept(i)) continue;
+    const float mi = m[i], ai = mi - VH_PROTON, ii = v[i];
+    const int wi = k0w_window(mi, P);
+    int slot = 0;   // kept peaks before it in (mass, window, selection order)
+    for (int j = 0; j < n; ++j) {
+      if (j == i || !kept(j)) continue;
+      const float aj = m[j] - VH_PROTON;
+      if (aj < ai) ++slot;
+      else if (aj == ai) {
+        const int wj = k0w_window(m[j], P);
+        slot += (wj < wi || (wj == wi && sel_before(v[j], aj, j, ii, ai, i))) ? 1 : 0;
+      }
+    }
+    if (slot < K) {
+      pk_mass[o + slot] = ai;
+      pk_int[o + slot] = ii;
+      t += (double)ii;
+    }
+  }
+  t = wave_sum(t);
+  if (lane == 0) {
+    tic[s] = (float)t;
+    valid[s] = K >= P.min_peaks;
+  }
+}
+
 static int k0_launch(const VhParams* p, int32_t n_spec, const float* mz, const float* intensity,
                      const int64_t* raw_off, int64_t* pk_off, float* pk_mass, float* pk_int, float* tic,
                      uint8_t* valid, int64_t pk_capacity, void* ws, size_t ws_bytes, hipStream_t st) {
@@ -440,8 +533,19 @@ static int k0_launch(const VhParams* p, int32_t n_spec, const float* mz, const f
     VH_CHECK_LAUNCH();
     return 0;
   }
+  if (p->peak_window_k < 0 || (p->peak_window_k > 0 && !(p->peak_window_th > 0.f))) return VH_E_ARG;
   DevP P = vh_make_devp(p);
   unsigned g = (unsigned)((n_spec + K0_WAVES - 1) / K0_WAVES);
+  if (p->peak_window_k > 0) {
+    k0w_count_kernel<<<g, 64 * K0_WAVES, 0, st>>>(P, n_spec, mz, intensity, raw_off, pk_off);
+    VH_CHECK_LAUNCH();
+    int rc = vh_scan_inclusive_i64(pk_off + 1, n_spec, ws, ws_bytes, st);
+    if (rc) return rc;
+    k0w_emit_kernel<<<g, 64 * K0_WAVES, 0, st>>>(P, n_spec, mz, intensity, raw_off, pk_off, pk_mass, pk_int, tic,
+                                                 valid);
+    VH_CHECK_LAUNCH();
+    return 0;
+  }
   k0_count_kernel<<<g, 64 * K0_WAVES, 0, st>>>(P, n_spec, mz, raw_off, pk_off);
   VH_CHECK_LAUNCH();
   int rc = vh_scan_inclusive_i64(pk_off + 1, n_spec, ws, ws_bytes, st);

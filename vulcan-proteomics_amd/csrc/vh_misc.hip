@@ -36,6 +36,8 @@ DevP vh_make_devp(const VhParams* p) {
   d.open_lo = p->open_lo;
   d.open_hi = p->open_hi;
   d.k1_threshold_open = p->k1_threshold_open;
+  d.peak_window_k = p->peak_window_k;
+  d.peak_window_th = p->peak_window_th;
   return d;
 }
 

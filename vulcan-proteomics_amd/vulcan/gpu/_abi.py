@@ -21,7 +21,7 @@ class VhParams(C.Structure):
                 ("max_fragment_charge", I32), ("max_peaks", I32), ("min_peaks", I32), ("min_matched_peaks", I32),
                 ("topk", I32), ("k1_threshold", I32), ("pair_closure", I32), ("tile", I32), ("xcd_group", I32),
                 ("n_offsets", I32), ("offsets", F32 * 8), ("open_search", I32), ("open_lo", F32), ("open_hi", F32),
-                ("k1_threshold_open", I32), ("reserved", I32 * 8)]
+                ("k1_threshold_open", I32), ("peak_window_k", I32), ("peak_window_th", F32), ("reserved", I32 * 6)]
 
 
 class VhIndexView(C.Structure):
@@ -80,8 +80,8 @@ def mod_arrays(static_mods: dict, var_mods: dict):
 def make_params(p: SearchParams, static_mods: dict, var_mods: dict) -> VhParams:
     if p.fragment_unit not in ("ppm", "da"):
         raise ValueError("fragment_unit must be 'ppm' or 'da'")
-    if p.peak_mode != "global":
-        raise ValueError("only the global peak view is implemented")
+    if p.peak_mode not in ("global", "window"):
+        raise ValueError("peak_mode must be 'global' or 'window'")
     stat, var, nterm = mod_arrays(static_mods, var_mods)
     v = VhParams()
     v.mono[:] = SAGE_MONO.tolist()
@@ -96,7 +96,9 @@ def make_params(p: SearchParams, static_mods: dict, var_mods: dict) -> VhParams:
     v.iso_lo, v.iso_hi = int(p.isotope_errors[0]), int(p.isotope_errors[1])
     v.z_lo, v.z_hi = int(p.precursor_charge[0]), int(p.precursor_charge[1])
     v.max_fragment_charge = int(p.max_fragment_charge or 0)
-    v.max_peaks, v.min_peaks = int(p.max_peaks), int(p.min_peaks)
+    v.max_peaks, v.min_peaks = peak_capacity(p), int(p.min_peaks)
+    if p.peak_mode == "window":
+        v.peak_window_k, v.peak_window_th = int(p.window_k), float(np.float32(p.window_th))
     v.min_matched_peaks = int(p.min_matched_peaks)
     v.topk = int(p.topk)
     v.k1_threshold = int(p.k1_threshold)
@@ -105,6 +107,26 @@ def make_params(p: SearchParams, static_mods: dict, var_mods: dict) -> VhParams:
     v.xcd_group = int(p.xcd_group)
     v.k1_threshold_open = int(p.k1_threshold)
     return v
+
+
+def peak_capacity(p: SearchParams) -> int:
+    """Kept peaks per spectrum at most: max_peaks in the global view; in the window view (no total cap,
+    engine.py:182-198) window_k per floor(m/z / window_th) window of [fragment_min_mz, fragment_max_mz]
+    (f32 division, as K0 computes it). The device K0 and K2 hold at most 512 peaks per spectrum."""
+    if p.peak_mode != "window":
+        return int(p.max_peaks)
+    th = np.float32(p.window_th)
+    if not (p.window_k > 0 and th > 0):
+        raise ValueError("the window view needs window_k > 0 and window_th > 0")
+    w0 = int(np.floor(np.float32(p.fragment_min_mz) / th))
+    w1 = int(np.floor(np.float32(p.fragment_max_mz) / th))
+    if w1 >= 1000:
+        raise ValueError("window view: fragment_max_mz / window_th >= 1000 would mix spectra in the reference's "
+                         "grouping key (spectrum * 1000 + window, engine.py:185)")
+    cap = (w1 - w0 + 1) * int(p.window_k)
+    if cap > 512:
+        raise ValueError(f"window view keeps up to {cap} peaks per spectrum; at most 512 are supported")
+    return cap
 
 
 def set_modes(v: VhParams, offsets=(), open_da=None, k1_threshold_open: int = 1) -> VhParams:

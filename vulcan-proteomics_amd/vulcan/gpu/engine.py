@@ -19,7 +19,8 @@ import pyarrow as pa
 import torch
 
 from . import _lib as K
-from ._abi import VhBlockView, VhIndexView, index_geometry, make_params, max_hyp_per_spectrum, set_modes
+from ._abi import (VhBlockView, VhIndexView, index_geometry, make_params, max_hyp_per_spectrum, peak_capacity,
+                   set_modes)
 from .host import NEUTRON, PROTON, SAGE_MONO, SearchParams, Spectra, make_hypotheses, pre_hypotheses  # noqa: F401
 
 ROW_BITS = 28   # peptidoform rows < 2^28 (engine.py:35 of the reference)
@@ -280,7 +281,7 @@ def upload_spectra(sp: Spectra, dev, stats: Stats | None = None) -> dict:
 def _k0_dev(vp, raw, p: SearchParams, dev, out=None):
     L = K.lib()
     n = raw["n"]
-    cap = max(n * p.max_peaks, 1)
+    cap = max(n * peak_capacity(p), 1)
     o = out or {"off": torch.empty(n + 1, dtype=torch.int64, device=dev),
                 "mass": torch.empty(cap, dtype=torch.float32, device=dev),
                 "int": torch.empty(cap, dtype=torch.float32, device=dev),
@@ -703,8 +704,8 @@ class DeviceSearch:
         cap_c = n * self.stride
         ws = int(L.vh_scan_workspace_bytes(n))
         e = lambda m, dt: torch.empty(m, dtype=dt, device=dev)  # noqa: E731
-        self.peaks = {"off": e(n + 1, torch.int64), "mass": e(n * p.max_peaks, torch.float32),
-                      "int": e(n * p.max_peaks, torch.float32), "tic": e(n, torch.float32),
+        self.peaks = {"off": e(n + 1, torch.int64), "mass": e(n * peak_capacity(p), torch.float32),
+                      "int": e(n * peak_capacity(p), torch.float32), "tic": e(n, torch.float32),
                       "valid_d": e(n, torch.uint8), "ws": e(ws, torch.uint8)}
         self.hyp = {"hyp_off": e(n + 1, torch.int64), "spec": e(cap_h, torch.int32), "z": e(cap_h, torch.int8),
                     "iso": e(cap_h, torch.int8), "pm": e(cap_h, torch.float32), "M": e(cap_h, torch.float32),

@@ -42,7 +42,7 @@ class SearchParams:
     k1_threshold: int = 1
     pair_closure: bool = True
     report_psms: int = 1
-    peak_mode: str = "global"               # only "global" (Sage) is implemented on the device
+    peak_mode: str = "global"               # "global" (Sage: top max_peaks) | "window" (top window_k per window_th)
     window_th: float = 100.0
     window_k: int = 15
     # index layout
