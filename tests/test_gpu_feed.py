@@ -59,6 +59,33 @@ def test_feed_equals_per_run_search(runs, tmp_path):
         assert pq.read_table(r["path"]).num_rows == tbl.num_rows
 
 
+def test_feed_equals_oracle_search_per_store(runs):
+    """The feed's rank-1 rows and its PSM tables' rank-1 rows and hyperscores equal the oracle's whole search
+    (or_search: K0, hypotheses, per-row K1, top-K + closure, K2, rank) on each loaded store's spectra."""
+    from oracle import oracle as O
+    from vulcan.gpu import engine as E
+    from vulcan.gpu._abi import make_params
+    from vulcan.gpu.feed import RunFeed
+    d, paths, sps = runs
+    print("[gpu] feed vs the oracle's search: 3 runs", flush=True)
+    gi = E.GpuIndex(d["l1"], d["p"], d["static"], d["var"])
+    feed = RunFeed(gi, d["p"], max_spectra=1500, max_peaks=max(int(s.offsets[-1]) for s in sps))
+    res = feed.run(paths, l1=d["l1"])
+    torch.cuda.synchronize()
+    vp = make_params(d["p"], d["static"], d["var"])
+    hix = O.HostIndex(d["l1"])
+    for r, sp in zip(res, sps):
+        ref = O.search(vp, hix, sp)
+        assert np.array_equal(r["rows"], ref["row"].astype(r["rows"].dtype)), r["run"]
+        t = r["table"]
+        one = t.column("rank").to_numpy() == 1
+        s, row, hs = (t.column(k).to_numpy()[one] for k in ("spec_idx", "pf_row", "hyperscore"))
+        want = np.flatnonzero(ref["row"] >= 0)
+        assert np.array_equal(s, want) and np.array_equal(row, ref["row"][want]), r["run"]
+        assert np.allclose(hs, ref["hs"][want], rtol=1e-12, atol=0), r["run"]
+        assert len(want) > 0.5 * len(sp)
+
+
 def test_feed_rejects_oversized_run(runs):
     from vulcan.gpu import engine as E
     from vulcan.gpu.feed import RunFeed
