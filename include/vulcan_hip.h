@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VH_ABI_VERSION 18
+#define VH_ABI_VERSION 19
 
 #define VH_E_ARG (-1)        /* bad argument (null pointer, negative size, unsupported setting) */
 #define VH_E_CAPACITY (-2)   /* an output buffer is smaller than the call needs */
@@ -156,6 +156,13 @@ int vh_index_block_count(const VhIndexView* ix, const int64_t* post_off, int32_t
 int vh_index_build_blocks(const VhParams* p, const VhIndexView* ix, const int64_t* post_off, int32_t q_log2,
                           int32_t row_log2, int64_t n_blocks, int32_t* row0, uint32_t* off, uint32_t* key,
                           uint16_t* row, void* ws, size_t ws_bytes, void* stream);
+/* Row-range slices (engine.py:59-65, 113-148: an index of more postings than one slice holds is cut into
+ * consecutive row ranges, all resident): a slice's block layout is built on the view of its rows (n_rows = r1 -
+ * r0, the per-row pointers advanced by r0; post_off from vh_frag_count on that view), then its row0 (n_blocks +
+ * 1 entries) is moved to table rows by adding r0. The join K1 runs vh_k1j_pairs and vh_k1j_count once per slice
+ * into the same per-window slot array (slices partition the rows, so every slot is written once) and one
+ * vh_k1j_topk over it. */
+int vh_index_rebase_rows(int32_t* row0, int64_t n, int32_t base, void* stream);
 
 /* ---------------------------------------------------------------------------------------------- K0 */
 /* Peak view (engine.py:171-210): keep fmin <= m/z <= fmax (not NaN), mass = m/z - PROTON (f32), top

@@ -351,6 +351,43 @@ def test_join_pairs_and_counts_equal_oracle(which, request):
     assert np.array_equal(_np(j["counts"])[:tot], env.k1["counts"])
 
 
+def test_join_row_range_slices_equal_oracle(env1):
+    """An index cut into row-range slices (max_slice_entries a fraction of its postings, as an index beyond
+    2^32 postings is cut): every slice's block starts are table rows and its blocks cover its rows once; the
+    join K1 over the slices gives every window slot the oracle's count and the oracle's top-K lists; the
+    capacities are fitted per slice."""
+    from vulcan.gpu import engine as E
+    env = env1
+    p = type(env.p)(**env.p.to_dict())
+    p.max_slice_entries = env.gi.n_postings // 3 + 1000
+    gi = E.GpuIndex(env.l1, p, env.d["static"], env.d["var"])
+    sl = gi.block_slices()
+    assert gi.n_slices >= 3
+    r = [(b["r0"], b["r1"]) for b in sl]
+    assert r[0][0] == 0 and r[-1][1] == gi.n and all(a[1] == b_[0] for a, b_ in zip(r[:-1], r[1:]))
+    po = _np(gi.post_off)
+    for b in sl:
+        row0 = _np(b["row0"]).astype(np.int64)
+        assert row0[0] == b["r0"] and row0[-1] == b["r1"] and np.all(np.diff(row0) > 0)
+        assert b["n_postings"] == po[b["r1"]] - po[b["r0"]] <= p.max_slice_entries
+    with pytest.raises(ValueError, match="use block_slices"):
+        gi.blocks()
+    n = len(env.d["sp"])
+    ds = E.DeviceSearch(gi, p, n, k1_impl="join")
+    ds._join_alloc(ds.join, 16, 16)   # too small for every slice: fitted on the first run
+    ds.run(E.upload_spectra(env.d["sp"], gi.dev))
+    torch.cuda.synchronize()
+    assert int(ds.status.item()) == 0
+    tot = int(env.k1["win_off"][-1])
+    assert np.array_equal(_np(ds.join["counts"])[:tot], env.k1["counts"])
+    tk = _O().topk(env.vp, env.hix, env.hyp, env.k1, ds.stride)
+    nc = _np(ds.n_cand)[:n]
+    assert np.array_equal(nc, tk["n_cand"])
+    m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+    for dv, k in ((ds.c_row, "row"), (ds.c_hyp, "hyp"), (ds.c_cnt, "cnt")):
+        assert np.array_equal(_np(dv)[:n * ds.stride][m], tk[k][m]), k
+
+
 def test_join_capacity_fit_and_overflow_flag(env1):
     """Pair / slot buffers far too small: the first run grows them to the batch (status clean, lists equal to
     the tile K1); a later batch that needs more than the fitted capacity raises status bit 8."""

@@ -191,6 +191,11 @@ __global__ void copy_tail_kernel(const uint32_t* src, uint32_t* dst, int64_t a, 
   if (i < n) dst[i] = src[i];
 }
 
+__global__ void add_i32_kernel(int32_t* a, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] += v;
+}
+
 extern "C" {
 
 int vh_version(void) { return VH_ABI_VERSION; }
@@ -209,6 +214,15 @@ size_t vh_abi_sizeof(int which) {
 }
 
 size_t vh_scan_workspace_bytes(int64_t n) { return scan_ws(n, sizeof(int64_t)); }
+
+int vh_index_rebase_rows(int32_t* row0, int64_t n, int32_t base, void* stream) {
+  if (n < 0 || (n && !row0)) return VH_E_ARG;
+  if (n == 0 || base == 0) return 0;
+  const unsigned g = (unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+  add_i32_kernel<<<g, 256, 0, (hipStream_t)stream>>>(row0, n, base);
+  VH_CHECK_LAUNCH();
+  return 0;
+}
 
 int vh_copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, void* stream) {
   if (n < 0 || (n && (!src || !dst))) return VH_E_ARG;

@@ -34,6 +34,7 @@ _SIGS = {
     "vh_index_blocks_workspace_bytes": ([I64, I64], SZ),
     "vh_index_block_count": ([PIX, P, I32, I32, P, P, SZ, P], I32),
     "vh_index_build_blocks": ([PP, PIX, P, I32, I32, I64, P, P, P, P, P, SZ, P], I32),
+    "vh_index_rebase_rows": ([P, I64, I32, P], I32),
     "vh_k0": ([PP, I32, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),
     "vh_k0_noalloc": ([PP, I32, P, P, P, P, P, P, P, P, I64, P], I32),
     "vh_hypotheses": ([PP, I32, P, P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, I64, P, SZ, P], I32),

@@ -95,8 +95,6 @@ class GpuIndex:
         K.check(L.vh_frag_count(C.byref(self.vp), C.byref(v), K.ptr(self.post_off), K.ptr(ws), ws.numel(), st),
                 "frag_count")
         self.n_postings = int(self.post_off[-1].item())
-        if self.n_postings >= 2 ** 32 or self.n_postings > params.max_slice_entries:
-            raise ValueError("more than 2^32 - 1 postings")
         v.n_postings = self.n_postings
         torch.cuda.synchronize()
         self._bin = None
@@ -109,10 +107,16 @@ class GpuIndex:
         if params.k1_impl == "tile":
             self.tile_view()
 
+    def _u32_postings(self, what: str):
+        if self.n_postings >= 2 ** 32:
+            raise ValueError(f"{what}: {self.n_postings} postings; the bin-major views address postings with u32 "
+                             "offsets (the join K1's row-range slices hold any number)")
+
     def tile_view(self) -> VhIndexView:
         """The index view with the bin-major postings and directory the spectrum-major (tile) K1 reads
         (vh_index_build), built on first use: the bucket-join K1 reads the block layout instead (blocks())."""
         if self._bin is None:
+            self._u32_postings("tile view")
             t0 = time.perf_counter()
             L = K.lib()
             v, d, st = self.view, self.dev, K.stream()
@@ -134,6 +138,7 @@ class GpuIndex:
         """Bin-major view with buckets of 2^VH_K1W_BUCKET_LOG2 rows for the wide-window (open search) K1
         (vh_k1w_topk), built on first use with the same bins as the tile view."""
         if getattr(self, "_wide", None) is None:
+            self._u32_postings("wide view")
             t0 = time.perf_counter()
             L = K.lib()
             d, st = self.dev, K.stream()
@@ -167,51 +172,97 @@ class GpuIndex:
 
     @property
     def n_slices(self) -> int:
-        """Posting directories of this index: one (the whole index is resident)."""
-        return 1
+        """Row-range slices of the join K1's block index (reference engine.py:59-65, 113-148)."""
+        return len(self.block_slices())
 
     @property
     def slices(self):
         self.tile_view()
         return [Slice(0, self.n_buckets, self._bin["offsets"], self._bin["postings"])]
 
-    def blocks(self) -> VhBlockView:
+    def _row_view(self, r0: int, r1: int) -> VhIndexView:
+        """The index view of rows [r0, r1): per-row pointers advanced by r0 (residues are addressed through
+        seq_off, partner rows stay table rows)."""
+        v = VhIndexView()
+        C.memmove(C.byref(v), C.byref(self.view), C.sizeof(VhIndexView))
+        v.n_rows = r1 - r0
+        v.seq_off = K.ptr(self.seq_off) + 8 * r0
+        v.length = K.ptr(self.length) + r0
+        v.mask = K.ptr(self.mask) + 8 * r0
+        v.mass32 = K.ptr(self.mass32) + 4 * r0
+        v.partner = K.ptr(self.partner) + 8 * r0
+        return v
+
+    def block_slices(self) -> list:
         """Block layout of the index for the bucket-join K1 (vh_index_build_blocks), built on first use:
         consecutive rows cut into blocks of ~2^block_postings_log2 postings (at most 2^block_rows_log2 rows),
-        postings key-sorted inside a block."""
+        postings key-sorted inside a block. Rows are cut into consecutive ranges (slices) of at most
+        max_slice_entries postings, each with its own block layout (u32 offsets) and its block starts in table
+        rows (vh_index_rebase_rows); all slices stay resident."""
         if self._blocks is not None:
-            return self._blocks["view"]
+            return self._blocks
         t0 = time.perf_counter()
         L = K.lib()
         d, st = self.dev, K.stream()
         q, rb = int(self.p.block_postings_log2), int(self.p.block_rows_log2)
-        wsb = int(L.vh_index_blocks_workspace_bytes(self.n, self.n_postings))
-        ws = torch.empty(wsb, dtype=torch.uint8, device=d)
-        stats = torch.zeros(3, dtype=torch.int64, device=d)
-        K.check(L.vh_index_block_count(C.byref(self.view), K.ptr(self.post_off), q, rb, K.ptr(stats), K.ptr(ws), wsb,
-                                       st), "index_block_count")
-        nb, max_rows, max_post = (int(x) for x in stats.tolist())
-        b = {"row0": torch.empty(nb + 1, dtype=torch.int32, device=d),
-             "off": torch.empty(nb + 1, dtype=torch.int32, device=d),
-             "key": torch.empty(max(self.n_postings, 1), dtype=torch.int32, device=d),
-             "row": torch.empty(max(self.n_postings, 1), dtype=torch.int16, device=d)}
-        K.check(L.vh_index_build_blocks(C.byref(self.vp), C.byref(self.view), K.ptr(self.post_off), q, rb, nb,
-                                        K.ptr(b["row0"]), K.ptr(b["off"]), K.ptr(b["key"]), K.ptr(b["row"]),
-                                        K.ptr(ws), wsb, st), "index_build_blocks")
-        torch.cuda.synchronize()
-        del ws
-        v = VhBlockView()
-        v.n_blocks, v.max_rows, v.max_postings = nb, max_rows, max_post
-        v.row0, v.off, v.key, v.row = (K.ptr(b[k]) for k in ("row0", "off", "key", "row"))
-        b["view"] = v
-        b["build_s"] = time.perf_counter() - t0
-        self._blocks = b
-        self.device_bytes += sum(b[k].numel() * b[k].element_size() for k in ("row0", "off", "key", "row"))
-        return v
+        cap = min(int(self.p.max_slice_entries), 2 ** 32 - 1)
+        if cap < 1:
+            raise ValueError("max_slice_entries must be positive")
+        bounds = [0]
+        while bounds[-1] < self.n:
+            r0 = bounds[-1]
+            target = torch.tensor([int(self.post_off[r0].item()) + cap], dtype=torch.int64, device=d)
+            r1 = int(torch.searchsorted(self.post_off, target, right=True).item()) - 1
+            bounds.append(min(max(r1, r0 + 1), self.n))   # a row holds < 2^16 postings: one row always fits
+        out = []
+        for r0, r1 in zip(bounds[:-1], bounds[1:]):
+            if r0 == 0 and r1 == self.n:
+                sv, po = self.view, self.post_off
+            else:
+                sv = self._row_view(r0, r1)
+                po = torch.empty(r1 - r0 + 1, dtype=torch.int64, device=d)
+                ws = torch.empty(int(L.vh_scan_workspace_bytes(r1 - r0)), dtype=torch.uint8, device=d)
+                K.check(L.vh_frag_count(C.byref(self.vp), C.byref(sv), K.ptr(po), K.ptr(ws), ws.numel(), st),
+                        "frag_count (slice)")
+                sv.n_postings = int(po[-1].item())
+            npost = int(sv.n_postings)
+            wsb = int(L.vh_index_blocks_workspace_bytes(r1 - r0, npost))
+            ws = torch.empty(wsb, dtype=torch.uint8, device=d)
+            stats = torch.zeros(3, dtype=torch.int64, device=d)
+            K.check(L.vh_index_block_count(C.byref(sv), K.ptr(po), q, rb, K.ptr(stats), K.ptr(ws), wsb, st),
+                    "index_block_count")
+            nb, max_rows, max_post = (int(x) for x in stats.tolist())
+            b = {"row0": torch.empty(nb + 1, dtype=torch.int32, device=d),
+                 "off": torch.empty(nb + 1, dtype=torch.int32, device=d),
+                 "key": torch.empty(max(npost, 1), dtype=torch.int32, device=d),
+                 "row": torch.empty(max(npost, 1), dtype=torch.int16, device=d)}
+            K.check(L.vh_index_build_blocks(C.byref(self.vp), C.byref(sv), K.ptr(po), q, rb, nb, K.ptr(b["row0"]),
+                                            K.ptr(b["off"]), K.ptr(b["key"]), K.ptr(b["row"]), K.ptr(ws), wsb, st),
+                    "index_build_blocks")
+            K.check(L.vh_index_rebase_rows(K.ptr(b["row0"]), nb + 1, r0, st), "index_rebase_rows")
+            torch.cuda.synchronize()
+            del ws
+            v = VhBlockView()
+            v.n_blocks, v.max_rows, v.max_postings = nb, max_rows, max_post
+            v.row0, v.off, v.key, v.row = (K.ptr(b[k]) for k in ("row0", "off", "key", "row"))
+            b.update(view=v, r0=r0, r1=r1, n_postings=npost)
+            self.device_bytes += sum(b[k].numel() * b[k].element_size() for k in ("row0", "off", "key", "row"))
+            out.append(b)
+        for b in out:
+            b["build_s"] = time.perf_counter() - t0
+        self._blocks = out
+        return out
+
+    def blocks(self) -> VhBlockView:
+        """The block view of a one-slice index (block_slices() for any number of slices)."""
+        sl = self.block_slices()
+        if len(sl) != 1:
+            raise ValueError(f"the index has {len(sl)} row-range slices: use block_slices()")
+        return sl[0]["view"]
 
     def block_arrays(self) -> dict:
         self.blocks()
-        return self._blocks
+        return self._blocks[0]
 
     def params_for(self, p: SearchParams):
         """VhParams for a SearchParams that shares this index's residue tables (topk / tile / tolerance may
@@ -727,11 +778,12 @@ class DeviceSearch:
 
     # ---------------------------------------------------------------------------------------- bucket-join K1
     def _join_buffers(self, cap_h: int, maxh: int, feat: bool = False) -> dict:
-        """Pair / slot buffers of the join K1. Capacities start from the index's mean precursor window (rows
-        within the precursor tolerance of each row's mass, averaged over the table) and are fitted to the batch
-        on the first run (one host read of the totals)."""
+        """Pair / slot buffers of the join K1: one per-window slot array for the batch, pair lists per row-range
+        slice of the index. Capacities start from the index's mean precursor window (rows within the precursor
+        tolerance of each row's mass, averaged over the table) and are fitted to the batch on the first run (one
+        host read of the totals)."""
         gi, dev = self.gi, self.gi.dev
-        bv = gi.blocks()
+        sl = gi.block_slices()
         L = K.lib()
         m = gi.mass32
         p = self.p
@@ -739,43 +791,49 @@ class DeviceSearch:
         hi = m * (1.0 + p.precursor_ppm[1] * 1e-6)
         rows = (torch.searchsorted(m, hi, right=True) - torch.searchsorted(m, lo)).float()
         mean_rows = float(rows.mean().item()) if len(m) else 1.0
-        rows_per_block = max(gi.n / max(bv.n_blocks, 1), 1.0)
+        nb_tot = max(sum(b["view"].n_blocks for b in sl), 1)
+        rows_per_block = max(gi.n / nb_tot, 1.0)
         n_h = max(self.n_spec, 1) * min(maxh, 4)       # typical hypotheses per spectrum; fitted on the first run
         slot_cap = int(n_h * (1.5 * mean_rows + 8)) + 4096
         pair_cap = int(n_h * (1.5 * mean_rows / rows_per_block + 2)) + 4096
         j = {"feat": feat, "cap_h": cap_h,
              "s_qe": torch.empty(max(self.n_spec, 1), dtype=torch.int32, device=dev) if feat else None,
              "h_slot": torch.empty(cap_h + 1, dtype=torch.int64, device=dev),
-             "blk_pairs": torch.empty(bv.n_blocks + 1, dtype=torch.int32, device=dev),
-             "ws": torch.empty(int(L.vh_k1j_workspace_bytes(bv.n_blocks, cap_h)), dtype=torch.uint8, device=dev),
+             "slices": [{"bv": b["view"], "blk_pairs": torch.empty(b["view"].n_blocks + 1, dtype=torch.int32,
+                                                                   device=dev),
+                         "ws": torch.empty(int(L.vh_k1j_workspace_bytes(b["view"].n_blocks, cap_h)),
+                                           dtype=torch.uint8, device=dev)} for b in sl],
              "fitted": False}
-        self._join_alloc(j, slot_cap, pair_cap)
+        self._join_alloc(j, slot_cap, [max(pair_cap * b["view"].n_blocks // nb_tot, 4096) for b in sl])
         return j
 
-    def _join_alloc(self, j: dict, slot_cap: int, pair_cap: int):
+    def _join_alloc(self, j: dict, slot_cap: int, pair_cap):
+        """(Re)allocate the slot array and the pair lists (pair_cap: one capacity for every slice, or a list)."""
         dev = self.gi.dev
-        j["slot_cap"], j["pair_cap"] = slot_cap, pair_cap
-        j["pairs"] = torch.empty((max(pair_cap, 1), 4), dtype=torch.int64, device=dev)    # 32-byte records
-        if j["feat"]:
-            j["counts"] = torch.empty(max(slot_cap, 1), dtype=torch.int64, device=dev)
-        else:
-            j["counts"] = torch.empty(max(slot_cap, 1), dtype=torch.int32, device=dev)
+        caps = pair_cap if isinstance(pair_cap, (list, tuple)) else [pair_cap] * len(j["slices"])
+        j["slot_cap"] = slot_cap
+        for sj, c in zip(j["slices"], caps):
+            sj["pair_cap"] = c
+            sj["pairs"] = torch.empty((max(c, 1), 4), dtype=torch.int64, device=dev)    # 32-byte records
+        j["counts"] = torch.empty(max(slot_cap, 1), dtype=torch.int64 if j["feat"] else torch.int32, device=dev)
+        s0 = j["slices"][0]   # the one-slice names
+        j["pair_cap"], j["pairs"], j["blk_pairs"] = s0["pair_cap"], s0["pairs"], s0["blk_pairs"]
 
     def _join_pairs(self, n: int):
         L = K.lib()
         j, hy, pk = self.join, self.hyp, self.peaks
-        bv = self.gi.blocks()
-        K.check(L.vh_k1j_pairs(C.byref(bv), n, K.ptr(hy["hyp_off"]), K.ptr(hy["spec"]), K.ptr(hy["lo"]),
-                               K.ptr(hy["hi"]), K.ptr(hy["maxfz"]), K.ptr(pk["off"]), K.ptr(j["s_qe"]), j["cap_h"],
-                               K.ptr(j["h_slot"]), j["slot_cap"], K.ptr(j["blk_pairs"]), K.ptr(j["pairs"]),
-                               j["pair_cap"], K.ptr(self.status), K.ptr(j["ws"]), j["ws"].numel(), K.stream()),
-                "k1j_pairs")
+        for sj in j["slices"]:   # every call writes the same window slot offsets (h_slot)
+            K.check(L.vh_k1j_pairs(C.byref(sj["bv"]), n, K.ptr(hy["hyp_off"]), K.ptr(hy["spec"]), K.ptr(hy["lo"]),
+                                   K.ptr(hy["hi"]), K.ptr(hy["maxfz"]), K.ptr(pk["off"]), K.ptr(j["s_qe"]),
+                                   j["cap_h"], K.ptr(j["h_slot"]), j["slot_cap"], K.ptr(sj["blk_pairs"]),
+                                   K.ptr(sj["pairs"]), sj["pair_cap"], K.ptr(self.status), K.ptr(sj["ws"]),
+                                   sj["ws"].numel(), K.stream()), "k1j_pairs")
 
     def join_front(self, n: int, mark):
-        """Pairs + per-block counts of the join K1 (fits the pair / slot capacities on the first call)."""
+        """Pairs + per-block counts of the join K1, slice by slice into one slot array (fits the pair / slot
+        capacities on the first call)."""
         L = K.lib()
-        j, hy, pk = self.join, self.hyp, self.peaks
-        bv = self.gi.blocks()
+        j, pk = self.join, self.peaks
         mark("k1j_pairs")
         if j["feat"]:
             K.check(L.vh_k1j_iq_shift(n, K.ptr(pk["off"]), K.ptr(pk["int"]), K.ptr(j["s_qe"]), K.stream()),
@@ -783,17 +841,20 @@ class DeviceSearch:
         self._join_pairs(n)
         if not j["fitted"]:
             tot_slots = int(j["h_slot"][j["cap_h"]].item())
-            tot_pairs = int(j["blk_pairs"][bv.n_blocks].item())
-            if tot_slots > j["slot_cap"] or tot_pairs > j["pair_cap"]:
-                self._join_alloc(j, max(tot_slots * 5 // 4, j["slot_cap"]), max(tot_pairs * 5 // 4, j["pair_cap"]))
+            tot_pairs = [int(sj["blk_pairs"][sj["bv"].n_blocks].item()) for sj in j["slices"]]
+            if tot_slots > j["slot_cap"] or any(t > sj["pair_cap"] for t, sj in zip(tot_pairs, j["slices"])):
+                self._join_alloc(j, max(tot_slots * 5 // 4, j["slot_cap"]),
+                                 [max(t * 5 // 4, sj["pair_cap"]) for t, sj in zip(tot_pairs, j["slices"])])
                 self.status.zero_()
                 self._join_pairs(n)
             j["fitted"] = True
         mark("k1j_pairs")
         mark("k1j_count")
-        K.check(L.vh_k1j_count(C.byref(self.vp), C.byref(bv), 2 if j["feat"] else 1, K.ptr(j["blk_pairs"]),
-                               K.ptr(j["pairs"]), j["pair_cap"], K.ptr(pk["mass"]), K.ptr(pk["int"]),
-                               K.ptr(j["counts"]), K.ptr(self.status), K.stream()), "k1j_count")
+        for sj in j["slices"]:
+            K.check(L.vh_k1j_count(C.byref(self.vp), C.byref(sj["bv"]), 2 if j["feat"] else 1,
+                                   K.ptr(sj["blk_pairs"]), K.ptr(sj["pairs"]), sj["pair_cap"], K.ptr(pk["mass"]),
+                                   K.ptr(pk["int"]), K.ptr(j["counts"]), K.ptr(self.status), K.stream()),
+                    "k1j_count")
         mark("k1j_count")
 
     def run(self, raw: dict, ev: dict | None = None):

@@ -50,7 +50,7 @@ class SearchParams:
     bin_ppm: float = 80.0                   # directory bin width; rounded down to a power-of-two f32 bit bin
     tile: int = 512                         # K1 LDS histogram slots per wave (rows per tile)
     k2_threads: int = 128                   # kept for signature compatibility (K2 runs one wave64 per spectrum)
-    max_slice_entries: int = 2 ** 32 - 1    # postings per index (u32 offsets)
+    max_slice_entries: int = 2 ** 31 - 1    # postings per row-range slice of the join K1's block index
     xcd_group: int = 32                     # K1/K2 blocks per XCD per scheduling group (0 = plain block order)
     k1_impl: str = "auto"                   # K1: "tile" (spectrum-major), "join" (block-major), "wide"
                                             # (workgroup per spectrum over 2^14-row buckets), "auto" (join, wide
