@@ -188,16 +188,20 @@ def read_labels(path: str) -> dict:
 
 
 def build_tables(proteins, params: dict | None = None, shuffle_rows_seed: int | None = None,
-                 swap_labels: bool = False):
+                 swap_labels: bool = False, progress=None):
     """Build the L1 table in memory. Returns (L1Index, labels dict). shuffle_rows_seed / swap_labels exist only
-    for the label-blindness test (reference build.py:80-83): the index side must come out bit-identical."""
+    for the label-blindness test (reference build.py:80-83): the index side must come out bit-identical.
+    progress: optional callable(str), told when each stage ends (digest, decoys, expansion, sort, assembly)."""
     p = default_params()
     p.update(params or {})
     t0 = time.perf_counter()
+    say = progress or (lambda msg: None)
     e = p["enzyme"]
     targets, t_missed = digest(proteins, e)
+    say(f"digest: {len(targets)} target peptides ({time.perf_counter() - t0:.1f} s)")
     t_missed_of = dict(zip(targets, t_missed.tolist()))
     dec, n_shuf = D.make_decoys(targets, p["decoy_seed"])
+    say(f"decoys: {len(dec)} ({time.perf_counter() - t0:.1f} s)")
     fw = [t for t in targets if t in dec]
     dc = [dec[t][0] for t in fw]
     perms = [dec[t][1] for t in fw]
@@ -219,6 +223,7 @@ def build_tables(proteins, params: dict | None = None, shuffle_rows_seed: int | 
     r = _core.expand_pairs(a_seq, pr, p["static_mods"], p["variable_mods"], p["max_variable_mods"],
                            p["peptide_min_mass"], p["peptide_max_mass"], p["id_seed"], p.get("variable_mod_caps"))
     n_pairs = len(r["seq"])
+    say(f"expansion: {2 * n_pairs} peptidoforms ({time.perf_counter() - t0:.1f} s)")
     seq_i = r["seq"].astype(np.int64)
     lens = np.array([len(s) for s in a_seq], np.int64)
     a_off = np.zeros(len(a_seq) + 1, np.int64)
@@ -236,7 +241,9 @@ def build_tables(proteins, params: dict | None = None, shuffle_rows_seed: int | 
     missed = np.concatenate([missed_seq[seq_i], missed_seq[seq_i]])
     if len(np.unique(pf_id)) != len(pf_id):
         raise RuntimeError("peptidoform id collision")
+    say(f"columns ({time.perf_counter() - t0:.1f} s)")
     srt = np.lexsort((pf_id, mass32))
+    say(f"sort ({time.perf_counter() - t0:.1f} s)")
     inv = np.empty_like(srt)
     inv[srt] = np.arange(len(srt))
     partner_unsorted = np.concatenate([np.arange(n_pairs) + n_pairs, np.arange(n_pairs)])
