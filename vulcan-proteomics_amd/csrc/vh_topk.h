@@ -9,12 +9,11 @@
 // Keep the best K keys of the buffer (keys are unique: (row, local hypothesis) occurs once per spectrum), return
 // the new fill and set the pruning threshold (the K-th largest key once K are kept, else 0); pay (feature mode):
 // payloads moved with the keys. The kept keys stay in buffer order (unsorted): the K-th largest key T is found by
-// a bitwise search over bits 62..14 (key bits 13..0 and 63 are zero; each step counts the keys >= the candidate
-// with one LDS pass over the buffer), then keys >= T are ballot-compacted in place (every read of a chunk happens
-// before its writes, and writes never pass the read position).
+// a bitwise search over bits 62..14 (key bits 13..0 and 63 are zero; each step counts the keys >= the candidate:
+// from registers for buffers of up to 512 keys, else with one LDS pass), then keys >= T are ballot-compacted in
+// place (every read of a chunk happens before its writes, and writes never pass the read position).
 __device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int capb, int nbuf, int K,
                                             uint64_t& thresh) {
-  (void)capb;
   const int lane = lane_id();
   if (nbuf <= K) {
     uint64_t mn = ~0ULL;
@@ -27,12 +26,30 @@ __device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int ca
     return nbuf;
   }
   uint64_t T = 0;
-  for (int bit = 62; bit >= 14; --bit) {
-    const uint64_t c = T | (1ULL << bit);
-    int cnt = 0;
-    for (int i = lane; i < nbuf; i += 64) cnt += buf[i] >= c ? 1 : 0;
-    cnt = wave_sum(cnt);
-    if (cnt >= K) T = c;
+  constexpr int KR = 8;   // keys per lane held in registers (buffers of up to 512 keys)
+  if (capb <= 64 * KR) {
+    // the buffer's keys in registers; each pass counts the keys >= the candidate with one compare per register
+    // (the compare mask is the ballot) and a scalar popcount
+    const int nk = (nbuf + 63) >> 6;
+    uint64_t k[KR];
+#pragma unroll
+    for (int j = 0; j < KR; ++j) k[j] = (j < nk && j * 64 + lane < nbuf) ? buf[j * 64 + lane] : 0ULL;
+    for (int bit = 62; bit >= 14; --bit) {
+      const uint64_t c = T | (1ULL << bit);
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < KR; ++j)
+        if (j < nk) cnt += __popcll(ballot(k[j] >= c));
+      if (cnt >= K) T = c;
+    }
+  } else {
+    for (int bit = 62; bit >= 14; --bit) {
+      const uint64_t c = T | (1ULL << bit);
+      int cnt = 0;
+      for (int i = lane; i < nbuf; i += 64) cnt += buf[i] >= c ? 1 : 0;
+      cnt = wave_sum(cnt);
+      if (cnt >= K) T = c;
+    }
   }
   int n = 0;
   for (int i0 = 0; i0 < nbuf; i0 += 64) {
