@@ -46,6 +46,36 @@ def _stages(ev):
     return out
 
 
+def time_with_features(ds, raw, n, steps):
+    """The step followed by the PSM feature columns of rank 1 on the device (vh_psm_features over the step's
+    candidate lists, dense; the host table assembly is left out): (seconds per step, feature ms per step)."""
+    import torch
+
+    from vulcan.gpu import engine as E
+    hy = ds.hyp
+
+    def feats():
+        return E.psm_features(ds.gi, n, None, ds.sel[:n], ds.nxt[:n], 1, ds.c_hyp, ds.c_row, ds.c_cnt, ds.k2,
+                              hy["hyp_off"], hy["z"], hy["iso"], hy["pm"], ds.scored, ds.matched,
+                              ds.peaks["tic"])
+    ds.run(raw)
+    feats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ds.run(raw)
+        feats()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(torch.cuda.current_stream())
+    for _ in range(steps):
+        feats()
+    b.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    return dt, a.elapsed_time(b) / steps
+
+
 def time_steps(obj, raw, steps, warmup):
     """Warm up, time `steps` runs (wall clock between synchronisations, no instrumentation in the timed loop),
     then run `steps` more with per-stage HIP events for the stage breakdown."""
@@ -629,6 +659,9 @@ def main():
                  ("block_postings_log2", a.block_q), ("block_rows_log2", a.block_rows)):
         if v is not None:
             setattr(p, k, v)
+    torch.zeros(1, device=dev)   # HIP runtime and library initialisation stay out of index_build_s
+    K.lib()
+    torch.cuda.synchronize()
     gi = E.GpuIndex(l1, p, STATIC, w.var_mods)
     log(f"index: {gi.n_postings} postings, {gi.n_buckets} buckets x {gi.n_bins} bins, build {gi.build_s:.3f} s")
     raw = E.upload_spectra(sp, dev)
@@ -725,6 +758,11 @@ def main():
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                         "algorithmic_bytes_per_launch": ab[dom]},
            "counts": ab, "top1_truth_recovery": recovery}
+    if ds.report == 1:
+        dtf, fms = time_with_features(ds, raw, n, a.steps)
+        out["with_psm_features"] = {"spectra_per_s": n / dtf, "ms_per_step": dtf * 1e3, "feature_ms": fms,
+                                    "note": "the step + rank-1 PSM feature columns on the device "
+                                            "(vh_psm_features); host table assembly not included"}
     if modes:
         out["config"]["open_da"] = list(modes["open_da"])
     out["cpu_baseline"] = None
