@@ -614,7 +614,21 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs A) {
   int above = 0;   // keys with score_ord > T
   const int mode = nt >= n ? 0 : (nt == within ? 1 : 2);   // 0: all, 1: the prefix {score >= thr}, 2: search
   if (mode == 2) {
-    {
+    constexpr int SR = 4;   // lists of up to 256 slots: the score_ords in registers, counted by ballot
+    if (n <= 64 * SR) {
+      uint32_t kr[SR];   // 0 past the fill: never counted (every candidate c below has a bit set)
+#pragma unroll
+      for (int j = 0; j < SR; ++j) kr[j] = j * 64 + lane < n ? (uint32_t)(keys[j * 64 + lane] >> 32) : 0u;
+      for (int bit = 31; bit >= 0; --bit) {
+        const uint32_t c = T | (1u << bit);
+        int k = 0;
+#pragma unroll
+        for (int j = 0; j < SR; ++j) k += __popcll(ballot(kr[j] >= c));
+        if (k >= nt) T = c;
+      }
+#pragma unroll
+      for (int j = 0; j < SR; ++j) above += __popcll(ballot(kr[j] > T));
+    } else {
       for (int bit = 31; bit >= 0; --bit) {
         const uint32_t c = T | (1u << bit);
         int k = 0;
