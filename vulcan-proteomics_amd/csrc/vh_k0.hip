@@ -38,7 +38,16 @@ __global__ __launch_bounds__(256) void k0_count_kernel(DevP P, int n_spec, const
   if (s >= n_spec) return;
   int64_t a = raw_off[s], b = raw_off[s + 1];
   int c = 0;
-  for (int64_t i = a + lane_id(); i < b; i += 64) c += inrange(mz[i], P);
+  for (int64_t i0 = a; i0 < b; i0 += 4 * 64) {   // four chunks' loads in flight together
+    float m[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t i = i0 + u * 64 + lane_id();
+      m[u] = i < b ? mz[i] : __builtin_nanf("");
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c += inrange(m[u], P);
+  }
   c = wave_sum(c);
   if (lane_id() == 0) pk_off[s + 1] = c < P.max_peaks ? c : P.max_peaks;
 }
@@ -74,16 +83,25 @@ __global__ __launch_bounds__(256) void k0_emit_kernel(DevP P, int n_spec, const 
   float* li = s_i[w];
   const int64_t base = raw_off[s];
   const int n = (int)(raw_off[s + 1] - base);
-  // compact in-range peaks (order preserved)
+  // compact in-range peaks (order preserved); the loads of up to K0_REG chunks are issued together
   int n_in = 0;
-  for (int i0 = 0; i0 < n; i0 += 64) {
-    int i = i0 + lane;
-    float m = i < n ? mz[base + i] : 0.f;
-    bool ok = i < n && inrange(m, P);
-    uint64_t bal = ballot(ok);
-    int pos = n_in + __popcll(bal & lanemask_lt());
-    if (ok && pos < K0_LDS) { lm[pos] = m - VH_PROTON; li[pos] = it[base + i]; }
-    n_in += __popcll(bal);
+  for (int i00 = 0; i00 < n; i00 += 64 * K0_REG) {
+    float mv[K0_REG], iv[K0_REG];
+#pragma unroll
+    for (int c = 0; c < K0_REG; ++c) {
+      const int i = i00 + c * 64 + lane;
+      mv[c] = i < n ? mz[base + i] : 0.f;
+      iv[c] = i < n ? it[base + i] : 0.f;
+    }
+#pragma unroll
+    for (int c = 0; c < K0_REG; ++c) {
+      const int i = i00 + c * 64 + lane;
+      const bool ok = i < n && inrange(mv[c], P);
+      const uint64_t bal = ballot(ok);
+      const int pos = n_in + __popcll(bal & lanemask_lt());
+      if (ok && pos < K0_LDS) { lm[pos] = mv[c] - VH_PROTON; li[pos] = iv[c]; }
+      n_in += __popcll(bal);
+    }
   }
   wave_sync();
   const int K = n_in < P.max_peaks ? n_in : P.max_peaks;
