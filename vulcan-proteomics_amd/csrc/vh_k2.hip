@@ -143,8 +143,22 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
   np = np < VH_K2_MAX_PEAKS ? np : VH_K2_MAX_PEAKS;
   float2* pk = s_p[w];
   uint16_t* tb = s_t[w];
-  for (int j = lane; j <= np; j += 64)
-    pk[j] = j < np ? make_float2(pk_mass[q0 + j], pk_int[q0 + j]) : make_float2(__builtin_nanf(""), 0.f);
+  {
+    // all chunks' loads in flight before the stores; pk[np] = (NaN, 0) ends every scan
+    constexpr int PC = (VH_K2_MAX_PEAKS + 63) / 64;
+    float2 v[PC];
+#pragma unroll
+    for (int u = 0; u < PC; ++u) {
+      const int j = u * 64 + lane;
+      v[u] = j < np ? make_float2(pk_mass[q0 + j], pk_int[q0 + j]) : make_float2(__builtin_nanf(""), 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < PC; ++u) {
+      const int j = u * 64 + lane;
+      if (j <= np) pk[j] = v[u];
+    }
+    if (np == PC * 64 && lane == 0) pk[np] = make_float2(__builtin_nanf(""), 0.f);
+  }
   wave_sync();
   // coarse table over [first peak, last peak]
   const float base = np ? pk[0].x : 0.f;
