@@ -76,6 +76,7 @@ __global__ __launch_bounds__(256) void k0_emit_kernel(DevP P, int n_spec, const 
                                                       const int64_t* __restrict__ pk_off, float* pk_mass, float* pk_int,
                                                       float* tic, uint8_t* valid) {
   __shared__ float s_m[K0_WAVES][K0_LDS], s_i[K0_WAVES][K0_LDS];
+  __shared__ uint32_t s_h[K0_WAVES][256];
   const int w = threadIdx.x >> 6, lane = lane_id();
   int s = blockIdx.x * K0_WAVES + w;
   if (s >= n_spec) return;
@@ -134,14 +135,49 @@ __global__ __launch_bounds__(256) void k0_emit_kernel(DevP P, int n_spec, const 
       } else {
         // the intensity half of T first (32 passes); its low (mass) half is needed only when the peaks whose
         // intensity key equals that half are more than the places left: otherwise T is the smallest of them
+        // Ti = the K-th largest intensity key, by a radix select over its four bytes (high first): per byte, an
+        // LDS histogram of the keys that share the bytes fixed so far, then the largest byte t whose keys >= t
+        // (plus those already above the prefix) number K or more
         uint32_t Ti = 0;
-        for (int bit = 31; bit >= 0; --bit) {
-          const uint32_t cand = Ti | (1u << bit);
-          int cnt = 0;
+        int above = 0;
+        uint32_t* hist = s_h[w];
+        for (int d = 3; d >= 0; --d) {
+          const int sh = 8 * d;
+          const uint32_t hm = d == 3 ? 0u : (0xFFFFFFFFu << (sh + 8));
+#pragma unroll
+          for (int q = 0; q < 4; ++q) hist[q * 64 + lane] = 0u;
+          wave_sync();
 #pragma unroll
           for (int c = 0; c < K0_REG; ++c)
-            if (c < nch) cnt += __popcll(ballot(c * 64 + lane < n_in && ik[c] >= cand));
-          if (cnt >= K) Ti = cand;
+            if (c < nch && c * 64 + lane < n_in && (ik[c] & hm) == Ti) atomicAdd(&hist[(ik[c] >> sh) & 255u], 1u);
+          wave_sync();
+          uint32_t h4[4];   // this lane's bytes 4 lane .. 4 lane + 3
+#pragma unroll
+          for (int q = 0; q < 4; ++q) h4[q] = hist[4 * lane + q];
+          uint32_t suf = h4[0] + h4[1] + h4[2] + h4[3];   // -> keys with byte >= 4 lane
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_down((int)suf, o);
+            if (lane + o < 64) suf += y;
+          }
+          int tb = -1;   // largest byte of this lane with above + #(byte >= it) >= K (the condition is monotone)
+          uint32_t run = suf;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (above + (int)run >= K) tb = 4 * lane + q;
+            run -= h4[q];
+          }
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            const int y = __shfl_xor(tb, o);
+            tb = y > tb ? y : tb;
+          }
+          int gt = 0;   // keys of this prefix with byte > tb
+#pragma unroll
+          for (int q = 0; q < 4; ++q) gt += 4 * lane + q > tb ? (int)h4[q] : 0;
+          above += wave_sum(gt);
+          Ti |= (uint32_t)tb << sh;
+          wave_sync();   // the next byte's histogram reuses the bins
         }
         int gti = 0, eqi = 0;
         uint32_t mlo = 0xFFFFFFFFu;
