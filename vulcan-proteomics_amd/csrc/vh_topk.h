@@ -25,6 +25,10 @@ __device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int ca
     thresh = nbuf == K ? mn : 0ULL;
     return nbuf;
   }
+  // The search starts at the highest bit set in any key (the bits above are zero in all of them) and stops at the
+  // first candidate with exactly K keys >= it: keys are unique, so keeping the keys >= T keeps exactly the K
+  // largest for any such T (the largest of them is the K-th key itself; a smaller one only prunes later offers
+  // less tightly).
   uint64_t T = 0;
   constexpr int KR = 8;   // keys per lane held in registers (buffers of up to 512 keys)
   if (capb <= 64 * KR) {
@@ -32,15 +36,23 @@ __device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int ca
     // (the compare mask is the ballot) and a scalar popcount
     const int nk = (nbuf + 63) >> 6;
     uint64_t k[KR];
+    uint64_t any = 0;
 #pragma unroll
-    for (int j = 0; j < KR; ++j) k[j] = (j < nk && j * 64 + lane < nbuf) ? buf[j * 64 + lane] : 0ULL;
-    for (int bit = 62; bit >= 14; --bit) {
+    for (int j = 0; j < KR; ++j) {
+      k[j] = (j < nk && j * 64 + lane < nbuf) ? buf[j * 64 + lane] : 0ULL;
+      any |= k[j];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) any |= __shfl_xor(any, o);
+    const int top = any ? 63 - __builtin_clzll(any) : 14;
+    for (int bit = top < 62 ? top : 62; bit >= 14; --bit) {
       const uint64_t c = T | (1ULL << bit);
       int cnt = 0;
 #pragma unroll
       for (int j = 0; j < KR; ++j)
         if (j < nk) cnt += __popcll(ballot(k[j] >= c));
       if (cnt >= K) T = c;
+      if (cnt == K) break;   // uniform: cnt is a wave-wide count
     }
   } else {
     for (int bit = 62; bit >= 14; --bit) {
@@ -49,6 +61,7 @@ __device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int ca
       for (int i = lane; i < nbuf; i += 64) cnt += buf[i] >= c ? 1 : 0;
       cnt = wave_sum(cnt);
       if (cnt >= K) T = c;
+      if (cnt == K) break;
     }
   }
   int n = 0;
