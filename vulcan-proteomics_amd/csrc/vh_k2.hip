@@ -143,8 +143,9 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
   np = np < VH_K2_MAX_PEAKS ? np : VH_K2_MAX_PEAKS;
   float2* pk = s_p[w];
   uint16_t* tb = s_t[w];
-  {
-    // all chunks' loads in flight before the stores; pk[np] = (NaN, 0) ends every scan
+  // the peaks' chunk loads in flight before their stores: all of them at once in the closed kernel, four chunks
+  // at a time in the shifted one (short of registers); pk[np] = (NaN, 0) ends every scan
+  if constexpr (!SHIFT) {
     constexpr int PC = (VH_K2_MAX_PEAKS + 63) / 64;
     float2 v[PC];
 #pragma unroll
@@ -158,6 +159,20 @@ __global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups,
       if (j <= np) pk[j] = v[u];
     }
     if (np == PC * 64 && lane == 0) pk[np] = make_float2(__builtin_nanf(""), 0.f);
+  } else {
+    for (int j0 = 0; j0 <= np; j0 += 4 * 64) {
+      float2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * 64 + lane;
+        v[u] = j < np ? make_float2(pk_mass[q0 + j], pk_int[q0 + j]) : make_float2(__builtin_nanf(""), 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * 64 + lane;
+        if (j <= np) pk[j] = v[u];
+      }
+    }
   }
   wave_sync();
   // coarse table over [first peak, last peak]
