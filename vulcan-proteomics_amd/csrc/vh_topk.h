@@ -7,11 +7,12 @@
 #include "vh_common.h"
 
 // Keep the best K keys of the buffer (keys are unique: (row, local hypothesis) occurs once per spectrum), return
-// the new fill and set the pruning threshold (the K-th largest key once K are kept, else 0); pay (feature mode):
-// payloads moved with the keys. The kept keys stay in buffer order (unsorted): the K-th largest key T is found by
-// a bitwise search over bits 62..14 (key bits 13..0 and 63 are zero; each step counts the keys >= the candidate:
-// from registers for buffers of up to 512 keys, else with one LDS pass), then keys >= T are ballot-compacted in
-// place (every read of a chunk happens before its writes, and writes never pass the read position).
+// the new fill and set the pruning threshold T (once K are kept: a value with exactly K keys >= it, at most the
+// K-th largest key; else 0); pay (feature mode): payloads moved with the keys. The kept keys stay in buffer order
+// (unsorted): T is found by a bitwise search over key bits 62..14 (bits 13..0 and 63 are zero; each step counts
+// the keys >= the candidate: from registers for buffers of up to 512 keys, else with one LDS pass), then keys >= T
+// are ballot-compacted in place (every read of a chunk happens before its writes, and writes never pass the read
+// position).
 __device__ __forceinline__ int topk_compact(uint64_t* buf, uint64_t* pay, int capb, int nbuf, int K,
                                             uint64_t& thresh) {
   const int lane = lane_id();
