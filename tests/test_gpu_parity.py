@@ -586,6 +586,53 @@ def test_deterministic_rerun(env1):
     assert np.array_equal(ra[5][:n], rb[5][:n])
 
 
+def _spectra_range(sp, a: int, b: int):
+    from vulcan.gpu.host import Spectra
+    off = sp.offsets[a:b + 1]
+    return Spectra(sp.run, sp.scan[a:b], sp.index[a:b], sp.rt[a:b], sp.precursor_mz[a:b], sp.charge[a:b],
+                   sp.mz[off[0]:off[-1]], sp.intensity[off[0]:off[-1]], off - off[0])
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_pipeline_steps_equal_oracle(env1, depth):
+    """Two batches of different sizes dealt alternately to a SearchPipeline, no synchronisation between
+    submissions: every resident step's rank 1 and candidate counts equal the oracle's whole search of its batch,
+    and its candidate lists equal one DeviceSearch run of the same batch."""
+    from vulcan.gpu import engine as E
+    sp = env1.d["sp"]
+    n = len(sp)
+    cut = n * 3 // 5
+    parts = [(0, cut), (cut, n)]
+    ref = _O().search(env1.vp, env1.hix, sp)
+    raws = [E.upload_spectra(_spectra_range(sp, a, b), env1.gi.dev) for a, b in parts]
+    solo = []
+    for raw in raws:
+        ds = E.DeviceSearch(env1.gi, env1.p, n)
+        ds.run(raw)
+        solo.append(ds)
+    pipe = E.SearchPipeline(env1.gi, env1.p, n, depth=depth)
+    n_steps = 2 * depth + 1
+    for i in range(n_steps):
+        pipe.run(raws[i % 2])
+    pipe.synchronize()
+    torch.cuda.synchronize()
+    for i in range(n_steps - depth, n_steps):
+        a, b = parts[i % 2]
+        ds = pipe.slot(i)
+        assert int(ds.status.item()) == 0
+        m = b - a
+        assert np.array_equal(ds.rank1_rows(m), ref["row"][a:b]), i
+        nc = _np(ds.n_cand)[:m]
+        assert np.array_equal(nc, ref["n_cand"][a:b]), i
+        one = solo[i % 2]
+        mk = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
+        for x, y in ((ds.c_row, one.c_row), (ds.c_hyp, one.c_hyp), (ds.c_cnt, one.c_cnt),
+                     (ds.k2["hyperscore"], one.k2["hyperscore"])):
+            assert np.array_equal(_np(x)[:m * ds.stride][mk], _np(y)[:m * ds.stride][mk]), i
+    with pytest.raises(ValueError, match="not resident"):
+        pipe.slot(n_steps - depth - 1)
+
+
 def test_label_blindness_gpu():
     from vulcan.gpu import engine as E
     from vulcan.index import synth

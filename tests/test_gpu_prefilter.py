@@ -349,3 +349,14 @@ def test_learned_search_keeps_the_plain_rank1_better_than_the_count_rule(trained
     # the learned path rescored at most 2 * keep candidates per spectrum
     assert int(_np(ls.n_cand)[:n].max()) <= 2 * pp.keep
     assert (r_learn >= 0).sum() > 0.5 * n
+    # the learned search as SearchPipeline slots, several steps in flight: the same lists and rank 1
+    pipe = E.SearchPipeline(gi, p, n, depth=2, make=lambda: PF.LearnedSearch(gi, p, n, t["model"], pp))
+    for _ in range(5):
+        pipe.run(raw)
+    pipe.synchronize()
+    nc = _np(ls.n_cand)[:n]
+    mk = (np.arange(ls.stride)[None, :] < nc[:, None]).ravel()
+    for s in pipe.slots:
+        assert np.array_equal(_np(s.n_cand)[:n], nc)
+        assert np.array_equal(_np(s.c_row)[:n * ls.stride][mk], _np(ls.c_row)[:n * ls.stride][mk])
+        assert np.array_equal(s.rank1_rows(n), r_learn)

@@ -147,6 +147,30 @@ def test_tune_step_picks_a_configuration_with_identical_results(uneven):
     _same(obj, u["base"], u["n"])
 
 
+@pytest.mark.parametrize("kind", ["classes", "persistent_work"])
+def test_pipeline_of_tuned_steps_equals_default(uneven, kind):
+    """Slots built by step_factory from a tuned step (a ClassSchedule with its table; a persistent, work-ordered
+    DeviceSearch), several steps in flight with no synchronisation between submissions: every slot's lists,
+    scores and rank 1 equal the default eager step's."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu import schedule as S
+    u = uneven
+    if kind == "classes":
+        obj = S.ClassSchedule(u["gi"], u["p"], u["sp"], table={0: {"xcd_group": 0, "persistent": False}})
+    else:
+        obj = E.DeviceSearch(u["gi"], u["p"], u["n"], persistent=True, order_mode="work")
+    pipe = E.SearchPipeline(u["gi"], u["p"], u["n"], depth=3, make=S.step_factory(u["gi"], u["sp"], obj))
+    assert all(type(s) is type(obj) for s in pipe.slots)
+    if kind == "classes":
+        assert all(s.table == obj.table for s in pipe.slots)
+    for _ in range(7):
+        pipe.run(u["raw"])
+    pipe.synchronize()
+    for s in pipe.slots:
+        assert int(s.status.item()) == 0
+        _same(s, u["base"], u["n"])
+
+
 def test_uneven_batch_equals_oracle(uneven, proteome_small):
     """The uneven batch (charges 2-6, 20-500 peaks, per-run calibration offsets) against the oracle: the default
     step's candidate lists, hyperscores and rank 1, and the rank 1 of the tuned class schedule replayed as a

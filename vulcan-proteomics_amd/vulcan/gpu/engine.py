@@ -996,6 +996,69 @@ class DeviceSearch:
         return int(self.n_cand[:n].sum().item())
 
 
+class SearchPipeline:
+    """Several closed-search steps in flight at once, for a stream of batches (runs, or slices of one run).
+
+    `depth` DeviceSearch slots, each with a stream of its own (default priority); step i runs on slot
+    i mod depth. A step touches only its slot's buffers and the read-only index, so consecutive steps overlap
+    on the device: one step's short launches (K0, hypotheses, order, pair lists, rank: about 40 kernels whose
+    grids drain in microseconds) and the ramp and tail of its count / select / K2 grids run beside another
+    step's long launches. Each step's results are those of DeviceSearch.run on the same batch. The reference's
+    cohort runner overlaps runs the same way, one GPU-stage thread per CUDA stream (`--search-threads`,
+    vulcan/cohort.py:665-745); here one host thread enqueues every slot, since a step never waits on the host.
+
+    Ordering contract: a step starts after the work already enqueued on the caller's current stream when
+    `run` is called (inputs written there are visible to it); its inputs must stay unchanged, and its results
+    stay in its slot, until the step ends (`wait(step)` / `synchronize()`) -- the slot is rewritten by step
+    + depth. No device memory is allocated per step."""
+
+    def __init__(self, gi: GpuIndex, p: SearchParams, n_spec: int, depth: int = 3, make=None, **kw):
+        """make: builds one slot (a DeviceSearch or a subclass: ClassSchedule, LearnedSearch); default
+        DeviceSearch(gi, p, n_spec, **kw). Three slots measured fastest on the proteome-scale batch (two and
+        four are slower: the device runs four hardware queues per process)."""
+        if depth < 1:
+            raise ValueError("depth must be at least 1")
+        self.depth = depth
+        make = make or (lambda: DeviceSearch(gi, p, n_spec, **kw))
+        self.slots = [make() for _ in range(depth)]
+        self.streams = [torch.cuda.Stream(device=gi.dev) for _ in range(depth)]
+        self.done = [None] * depth
+        self.n_steps = 0
+
+    def run(self, raw: dict, ev: dict | None = None) -> int:
+        """Enqueue one step over `raw` on the next slot; returns its step number."""
+        i = self.n_steps
+        k = i % self.depth
+        ready = torch.cuda.Event()
+        ready.record()
+        s = self.streams[k]
+        s.wait_event(ready)
+        with torch.cuda.stream(s):
+            self.slots[k].run(raw, ev)
+            done = torch.cuda.Event()
+            done.record()
+        self.done[k] = done
+        self.n_steps += 1
+        return i
+
+    def slot(self, step: int) -> DeviceSearch:
+        """The DeviceSearch holding `step`'s results (the step must still be resident)."""
+        if not (self.n_steps - self.depth <= step < self.n_steps):
+            raise ValueError(f"step {step} is not resident (steps {max(self.n_steps - self.depth, 0)}"
+                             f"..{self.n_steps - 1} are)")
+        return self.slots[step % self.depth]
+
+    def wait(self, step: int) -> DeviceSearch:
+        """Make the caller's current stream wait for `step`'s end; returns its slot."""
+        ds = self.slot(step)
+        torch.cuda.current_stream().wait_event(self.done[step % self.depth])
+        return ds
+
+    def synchronize(self):
+        for s in self.streams:
+            s.synchronize()
+
+
 def flat_candidates(ds: DeviceSearch, n: int) -> dict:
     """Host copy of a DeviceSearch's candidate lists flattened in (spectrum, row) order: "h", "r", "k1c",
     "spec" and the K2 columns (for checks and debugging; one packed copy, flattened with numpy)."""

@@ -211,6 +211,19 @@ def tune_step(gi: GpuIndex, p: SearchParams, sp: Spectra, raw: dict, candidates=
     return best[1], out
 
 
+def step_factory(gi: GpuIndex, sp: Spectra, obj):
+    """A callable that builds a fresh step object configured like `obj` (a tune_step result: a ClassSchedule
+    with its table, or a DeviceSearch with its launch form, order and XCD grouping), for the slots of an
+    engine.SearchPipeline."""
+    if isinstance(obj, ClassSchedule):
+        table = dict(obj.table)
+        return lambda: ClassSchedule(gi, obj.p, sp, table=table, report=obj.report, k1_impl=obj.k1_impl)
+    q = SearchParams(**obj.p.to_dict())
+    persistent = obj.queue is not None
+    return lambda: DeviceSearch(gi, q, len(sp), report=obj.report, persistent=persistent,
+                                order_mode=obj.order_mode, band_log2=obj.band_log2, k1_impl=obj.k1_impl)
+
+
 def save_table(path: str, table: dict):
     with open(path, "w") as f:
         json.dump({str(k): v for k, v in table.items()}, f, indent=1)
