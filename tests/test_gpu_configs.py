@@ -24,6 +24,13 @@ def _O():
     return O
 
 
+def _take(sp, n):
+    from vulcan.gpu.host import Spectra
+    off = sp.offsets[: n + 1]
+    return Spectra(sp.run, sp.scan[:n], sp.index[:n], sp.rt[:n], sp.precursor_mz[:n], sp.charge[:n],
+                   sp.mz[: off[-1]], sp.intensity[: off[-1]], off.copy())
+
+
 def _flat(ds, n):
     nc = _np(ds.n_cand)[:n]
     m = (np.arange(ds.stride)[None, :] < nc[:, None]).ravel()
@@ -117,6 +124,37 @@ def test_phospho_search_equals_oracle(phospho, impl):
     truth = e["truth"]
     sig = truth >= 0
     print(f"[gpu] phospho {impl}: truth recovery {np.mean(full['row'][sig] == truth[sig]):.3f}", flush=True)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_join_counts_of_spectra_beyond_256_peaks_equal_oracle(phospho, mode):
+    """max_peaks 512 on the phospho spectra (20-500 raw peaks): the count kernel holds a spectrum's first 256
+    peaks in registers and loads the rest on demand; window counts (mode 1) and the per-ion feature counts
+    (mode 2, the learned path's K1) equal the oracle's on 1,000 spectra, a share of them beyond 256 kept peaks."""
+    from vulcan.gpu import engine as E
+    from vulcan.gpu._abi import make_params, max_hyp_per_spectrum
+    e = phospho
+    O = _O()
+    p = type(e["p"])(**{**e["p"].to_dict(), "max_peaks": 512})
+    vp = make_params(p, STATIC, e["w"].var_mods)
+    sub = _take(e["sp"], 1000)
+    n = len(sub)
+    pk = O.k0(vp, sub)
+    kept = np.diff(pk["off"])
+    assert (kept > 256).sum() > 20
+    hyp = O.hypotheses(vp, sub, pk["valid"], e["l1"].mass32, max_hyp_per_spectrum(p))
+    k1 = O.k1_counts(vp, e["hix"], pk, hyp)
+    ds = E.DeviceSearch(e["gi"], p, n, k1_impl="join", join_features=mode == 2)
+    mark = E._marker(None)
+    ds.front(E.upload_spectra(sub, e["gi"].dev), n, mark)     # K0, hypotheses, order
+    ds.join_front(n, mark)                                    # pairs + block counts
+    torch.cuda.synchronize()
+    assert int(ds.status.item()) == 0
+    tot = int(k1["win_off"][-1])
+    got = _np(ds.join["counts"])[:tot]
+    if mode == 2:
+        got = (got.astype(np.uint64) >> np.uint64(43)).astype(np.uint32)
+    assert np.array_equal(got, k1["counts"])
 
 
 @pytest.fixture(scope="module")
