@@ -13,7 +13,8 @@
 // row asc (then candidate index, the stable-sort tie order).
 #include "vh_common.h"
 
-#define K2_WAVES 4
+// waves per workgroup: two measured faster than four and than one (cfg1 K2 0.839 / 0.830 -> 0.813 ms)
+#define K2_WAVES 2
 #define K2_LNF 256
 #define K2_CELLS 1024   // coarse peak lookup table per spectrum (u16 first-peak index per mass cell)
 
@@ -108,7 +109,7 @@ __device__ __forceinline__ void ion_alt_sums(const PeakView& V, const DevP& P, f
 }
 
 template <bool SHIFT>
-__global__ __launch_bounds__(256) void k2_kernel(DevP P, DevIdx X, int n_groups, const int32_t* __restrict__ order,
+__global__ __launch_bounds__(64 * K2_WAVES) void k2_kernel(DevP P, DevIdx X, int n_groups, const int32_t* __restrict__ order,
                                                  const int32_t* __restrict__ grp_spec,
                                                  const int64_t* __restrict__ grp_start,
                                                  const int32_t* __restrict__ grp_count, int stride,
