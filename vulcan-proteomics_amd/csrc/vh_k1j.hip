@@ -581,8 +581,11 @@ static int select_geometry(const VhParams* p, int mode, K1Out& O, int& waves) {
   O.clos_n = j_pow2_at_least((2 * K) > 64 ? 2 * K : 64);
   O.hist_bytes = 0;
   O.wave_bytes = (int)(((size_t)O.capb * (mode == 2 ? 16 : 8) + (size_t)O.clos_n * 8 + 15) & ~(size_t)15);
-  waves = 65536 / O.wave_bytes;
-  waves = waves < 1 ? 1 : (waves > 4 ? 4 : waves);
+  // one wave (spectrum) per workgroup: a workgroup keeps its CU slot until its slowest wave ends, and the
+  // spectra's select times vary, so single-wave workgroups free the slot as each spectrum finishes (measured:
+  // 0.352 -> 0.315 ms on the proteome batch at K = 50, 0.986 -> 0.775 ms on the phospho batch at K = 200,
+  // against four waves per workgroup)
+  waves = 1;
   return O.wave_bytes > 65536 ? VH_E_LIMIT : 0;
 }
 

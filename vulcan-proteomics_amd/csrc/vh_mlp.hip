@@ -976,8 +976,10 @@ int vh_prefilter_select(const VhIndexView* ix, int32_t n_spec, int32_t stride_in
   A.n_cand = n_cand; A.c_row = c_row; A.c_hyp = c_hyp; A.c_cnt = c_cnt; A.score = score;
   A.partner = ix->partner;
   A.out_n = out_n; A.out_row = out_row; A.out_hyp = out_hyp; A.out_cnt = out_cnt;
-  int waves = 65536 / A.wave_bytes;
-  waves = waves < 1 ? 1 : (waves > 4 ? 4 : waves);
+  // one wave (spectrum) per workgroup, as the index top-K select: a workgroup holds its CU slot until its
+  // slowest spectrum ends
+  const int waves = 1;
+  if (A.wave_bytes > 65536) return VH_E_LIMIT;
   const unsigned g = (unsigned)((n_spec + waves - 1) / waves);
   select_kernel<<<g, 64 * waves, (size_t)A.wave_bytes * waves, (hipStream_t)stream>>>(A);
   VH_CHECK_LAUNCH();
