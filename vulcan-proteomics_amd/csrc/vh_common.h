@@ -79,9 +79,22 @@ static inline unsigned xcd_band_grid(int64_t n_blocks, int c) {
 
 #define VH_CHECK_LAUNCH() do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return (int)e_; } while (0)
 
+// x / 1e6 rounded to nearest, as the IEEE division gives it, without the division's instruction sequence
+// (scale, reciprocal, four fmas, fixup): q0 = x * RN(1e-6), the residual x - q0 * 1e6 exactly by an fma, and one
+// correction q0 + residual * RN(1e-6) (Markstein's correction step). Compared on the host with the division for
+// every f32 x (tests/div1e6_check.c): equal for every finite |x| >= 2^-90; smaller |x| (quotients near the
+// subnormal range), zeros, infinities and NaN take the division.
+__device__ __forceinline__ float div_1e6(float x) {
+  const float ax = fabsf(x);
+  if (!(ax >= 0x1p-90f && ax <= 0x1.fffffep127f)) return x / 1000000.0f;
+  const float r = 1e-6f;
+  const float q0 = x * r;
+  return fmaf(fmaf(-q0, 1000000.0f, x), r, q0);
+}
+
 // kernels.cu:36-38: ppm bound = m + (m*tol)/1e6 (each op rounded), Da bound = m + tol
 __device__ __forceinline__ float tol_bound(float m, float tol, int da) {
-  return da ? (m + tol) : (m + (m * tol) / 1000000.0f);
+  return da ? (m + tol) : (m + div_1e6(m * tol));
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -202,20 +202,19 @@ __global__ __launch_bounds__(64 * K2_WAVES) void k2_kernel(DevP P, DevIdx X, int
     float bs_init = 0.f, yu_init = 0.f;   // sum_k is[0][k] and sum_k iu[1][k] (k ascending)
     int nbs_init = 0, nyu_init = 0;
     // one fragment window [a, bd] at m/z mz, its scan starting at peak j (= q): the best peak is matched (ppm,
-    // count, intensity, ladder run)
+    // count, intensity, ladder run); its mass and intensity are kept from the scan (no second LDS read)
     auto window = [&](float mz, float a, float bd, int j, float2 q, int kind, int k) {
-      float mx = 0.f;
-      int best = -1;
+      float mx = 0.f, bx = 0.f;
+      bool hit = false;
       while (q.x <= bd) {   // the NaN sentinel ends every scan
-        if (q.x >= a && q.y >= mx) { mx = q.y; best = j; }
+        if (q.x >= a && q.y >= mx) { mx = q.y; bx = q.x; hit = true; }
         q = pk[++j];
       }
-      if (best >= 0) {
-        const float2 b = pk[best];
-        const float num = (b.y * fabsf(mz - b.x)) * 2.0e6f;
-        ppm = ppm + num / (mz + b.x);
-        if (kind == 0) { mb += 1; sb = sb + b.y; run_matched(rb, k); }
-        else { my += 1; sy = sy + b.y; run_matched(ry, k); }
+      if (hit) {
+        const float num = (mx * fabsf(mz - bx)) * 2.0e6f;
+        ppm = ppm + num / (mz + bx);
+        if (kind == 0) { mb += 1; sb = sb + mx; run_matched(rb, k); }
+        else { my += 1; sy = sy + mx; run_matched(ry, k); }
       }
     };
     auto match = [&](float mz, int kind, int k) {
@@ -235,11 +234,13 @@ __global__ __launch_bounds__(64 * K2_WAVES) void k2_kernel(DevP P, DevIdx X, int
           const float m2 = cum * 0.5f;
           const float a1 = tol_bound(cum, P.tol_lo, P.tol_da), b1 = tol_bound(cum, P.tol_hi, P.tol_da);
           const float a2 = tol_bound(m2, P.tol_lo, P.tol_da), b2 = tol_bound(m2, P.tol_hi, P.tol_da);
-          // both charges' lookups (cell start, first peak) are issued before either window is scanned
+          // both charges' lookups (cell start, first peak) are issued before either window is scanned; a charge
+          // at or above maxfz gets an empty window (high bound -inf: its scan stops at the first peak), so the
+          // lookups take no branch on maxfz and the two chains of LDS reads overlap
           const int j1 = V.cell_start(a1), j2 = V.cell_start(a2);
           const float2 q1 = pk[j1], q2 = pk[j2];
-          if (maxfz > 1) window(cum, a1, b1, j1, q1, kind, k);
-          if (maxfz > 2) window(m2, a2, b2, j2, q2, kind, k);
+          window(cum, a1, maxfz > 1 ? b1 : -INFINITY, j1, q1, kind, k);
+          window(m2, a2, maxfz > 2 ? b2 : -INFINITY, j2, q2, kind, k);
           for (int z = 3; z < maxfz; ++z) match(cum / (float)z, kind, k);
           continue;
         }
