@@ -73,16 +73,17 @@ struct PeakView {
 };
 
 // One tolerance window at m/z `mz`: the most intense peak (ties -> later peak) updates the running (mx, best)
-// shared across the unshifted/shifted windows and the window's own (mx_a, best_a) (kernels.cu:312-329).
+// shared across the unshifted/shifted windows, with its mass best_x, and the window's own (mx_a, best_a)
+// (kernels.cu:312-329). mx / mx_a are the best peaks' intensities, so callers need not read them again.
 __device__ __forceinline__ void window_scan(const PeakView& V, const DevP& P, float mz, float& mx, int& best,
-                                            float& best_mz, float& mx_a, int& best_a) {
+                                            float& best_mz, float& best_x, float& mx_a, int& best_a) {
   const float a = tol_bound(mz, P.tol_lo, P.tol_da);
   const float bd = tol_bound(mz, P.tol_hi, P.tol_da);
   for (int j = V.cell_start(a);; ++j) {
     const float2 q = V.pk[j];
     if (!(q.x <= bd)) break;
     if (q.x < a) continue;
-    if (q.y >= mx) { mx = q.y; best = j; best_mz = mz; }
+    if (q.y >= mx) { mx = q.y; best = j; best_mz = mz; best_x = q.x; }
     if (q.y >= mx_a) { mx_a = q.y; best_a = j; }
   }
 }
@@ -93,16 +94,16 @@ __device__ __forceinline__ void ion_alt_sums(const PeakView& V, const DevP& P, f
                                              float& iu, int& nu, float& is, int& ns) {
   iu = 0.f; is = 0.f; nu = 0; ns = 0;
   for (int z = 1; z < maxfz; ++z) {
-    float mx = 0.f, bmz = 0.f;
+    float mx = 0.f, bmz = 0.f, bx = 0.f;
     int best = -1;
     for (int alt = 0; alt < 2; ++alt) {
       float mx_a = 0.f;
       int best_a = -1;
       const float base = alt == 0 ? cum : cum + shift;
-      window_scan(V, P, div_charge(base, z), mx, best, bmz, mx_a, best_a);
+      window_scan(V, P, div_charge(base, z), mx, best, bmz, bx, mx_a, best_a);
       if (best_a >= 0) {
-        if (alt == 0) { iu = iu + V.pk[best_a].y; nu += 1; }
-        else { is = is + V.pk[best_a].y; ns += 1; }
+        if (alt == 0) { iu = iu + mx_a; nu += 1; }
+        else { is = is + mx_a; ns += 1; }
       }
     }
   }
@@ -248,19 +249,19 @@ __global__ __launch_bounds__(64 * K2_WAVES) void k2_kernel(DevP P, DevIdx X, int
         int nu = 0, ns = 0;
         for (int z = 1; z < maxfz; ++z) {
           int best = -1;
-          float mx = 0.f, mz = 0.f;
+          float mx = 0.f, mz = 0.f, bx = 0.f;
           for (int alt = 0; alt < n_alt; ++alt) {
             float mx_a = 0.f;
             int best_a = -1;
             const float b = alt == 0 ? cum : cum + shift;
-            window_scan(V, P, div_charge(b, z), mx, best, mz, mx_a, best_a);
+            window_scan(V, P, div_charge(b, z), mx, best, mz, bx, mx_a, best_a);
             if (best_a >= 0) {
-              if (alt == 0) { iu = iu + pk[best_a].y; nu += 1; }
-              else { is = is + pk[best_a].y; ns += 1; }
+              if (alt == 0) { iu = iu + mx_a; nu += 1; }
+              else { is = is + mx_a; ns += 1; }
             }
           }
           if (best >= 0) {
-            const float pmass = pk[best].x, pint = pk[best].y;
+            const float pmass = bx, pint = mx;
             const float num = (pint * fabsf(mz - pmass)) * 2.0e6f;
             ppm = ppm + num / (mz + pmass);
             if (kind == 0) { mb += 1; sb = sb + pint; run_matched(rb, k); }
